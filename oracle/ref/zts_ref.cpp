@@ -1,0 +1,123 @@
+// zts_ref.cpp -- driver around the reference's own assembly code, compiled
+// together with the reference sources (see oracle/ref/Makefile).  It is test
+// infrastructure: it generates the golden impedance fixtures and validates the
+// C restatement in oracle/bem_oracle.c.  It never ships and never runs on the
+// product path.
+//
+// Input (text, stdin):
+//   omega
+//   nbody
+//   per body: eps_re eps_im mu_re mu_im nv nt
+//             nv lines "x y z", nt lines "a b c" (0-based vertex ids)
+// Output (binary, file named by argv[1]; the reference's mesher prints to stdout):
+//   per body: int64 ne, then ne x 4 int64 {p1, p2, t1, t2} of the reference's
+//             ShapeMesh edge table (body-local ids), in Edge::index order
+//   then complex128 column-major matrices:
+//     Zts(top, top, exterior g)   (2E x 2E, E = total RWGs, body 1 first)
+//     Zts(body_b, body_b, g_b)    for each body b
+//
+// Reference entry points used: ShapeMesh::ShapeMesh (src/ShapeMesh.cpp:821),
+// RWG::RWG (src/RWG.cpp:5), RWGTree protected ctor (src/RWGTree.cpp:590),
+// GreensFunction (src/GreensFunction.cpp:11), Zts (src/Impedance.cpp:388).
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+#include <string>
+#include "ShapeMesh.h"
+#include "RWGTree.h"
+#include "Impedance.h"
+#include "Options.h"
+#include "ReferenceCountingPointer.h"
+
+struct FlatTree : public RWGTree {
+  FlatTree(int n, PointEdgeVectorMap &m, Dielectric *d, GreensFunction *g) : RWGTree(n, m, d, g) {}
+};
+
+static FILE *out;
+
+static void write_matrix(ComplexDenseMatrix *Z) {
+  int m = Z->m, n = Z->n;
+  std::vector<double> col(2 * (size_t)m);
+  for (int j = 0; j < n; j++) {
+    for (int i = 0; i < m; i++) {
+      Complex z = (*Z)(i, j);
+      col[2 * i] = real(z);
+      col[2 * i + 1] = imag(z);
+    }
+    fwrite(col.data(), sizeof(double), col.size(), out);
+  }
+}
+
+int main(int argc, char **argv) {
+  if (argc != 2) return 2;
+  out = fopen(argv[1], "wb");
+  if (!out) return 2;
+  initReferenceCountingPointerLock();
+  double omega;
+  int nbody;
+  if (scanf("%lf %d", &omega, &nbody) != 2) return 2;
+  Options options;
+  PointEdgeVectorMap eap;
+  std::vector<ShapeMesh *> meshes;
+  std::vector<std::vector<RWG *> > rwgs(nbody);
+  std::vector<Complex> eps(nbody), mu(nbody);
+  for (int b = 0; b < nbody; b++) {
+    double er, ei, mr, mi;
+    int nv, nt;
+    if (scanf("%lf %lf %lf %lf %d %d", &er, &ei, &mr, &mi, &nv, &nt) != 6) return 2;
+    eps[b] = Complex(er, ei);
+    mu[b] = Complex(mr, mi);
+    Shape *shape = new Shape();
+    for (int i = 0; i < nv; i++) {
+      double x, y, z;
+      if (scanf("%lf %lf %lf", &x, &y, &z) != 3) return 2;
+      shape->coord.push_back(Vector3(x, y, z));
+    }
+    for (int i = 0; i < nt; i++) {
+      int a, c, d;
+      if (scanf("%d %d %d", &a, &c, &d) != 3) return 2;
+      char buf[96];
+      snprintf(buf, sizeof buf, "%d %d %d -1 ", a, c, d);
+      shape->faceIndex.push_back(FaceIndex(buf));
+    }
+    ShapeMesh *m = new ShapeMesh(*shape, options.sharpThresh, options.concaveThresh,
+                                 options.relativeCostAngleConcave, NULL, NULL);
+    meshes.push_back(m);
+    Array<Point> &pts = *m->refinedPoint[0];
+    Array<Edge> &edges = *m->refinedEdge[0];
+    Array<Triangle> &tris = *m->refinedTriangle[0];
+    int64_t ne = edges.n;
+    fwrite(&ne, sizeof ne, 1, out);
+    for (int i = 0; i < edges.n; i++) {
+      Edge *e = edges + i;
+      int64_t rec[4] = {(int64_t)(e->p1 - (pts + 0)), (int64_t)(e->p2 - (pts + 0)),
+                        (int64_t)(e->t1 - (tris + 0)), (int64_t)(e->t2 - (tris + 0))};
+      fwrite(rec, sizeof(int64_t), 4, out);
+      rwgs[b].push_back(new RWG(e));
+    }
+  }
+  size_t total = 0;
+  for (int b = 0; b < nbody; b++) total += rwgs[b].size();
+  GreensFunction gext(Complex(1., 0.), Complex(1., 0.), Complex(omega, 0.), true);
+  std::vector<GreensFunction *> gb(nbody);
+  std::vector<FlatTree *> sub(nbody);
+  FlatTree top((int)total, eap, NULL, &gext);
+  for (int b = 0; b < nbody; b++) {
+    gb[b] = new GreensFunction(eps[b], mu[b], Complex(omega, 0.), false);
+    sub[b] = new FlatTree((int)rwgs[b].size(), eap, NULL, gb[b]);
+    for (size_t i = 0; i < rwgs[b].size(); i++) {
+      sub[b]->rwg[sub[b]->rwgs++] = rwgs[b][i];
+      top.rwg[top.rwgs++] = rwgs[b][i];
+    }
+  }
+  ComplexDenseMatrix *W = Zts(&top, &top, &gext, options);
+  write_matrix(W);
+  delete W;
+  for (int b = 0; b < nbody; b++) {
+    ComplexDenseMatrix *G = Zts(sub[b], sub[b], gb[b], options);
+    write_matrix(G);
+    delete G;
+  }
+  fclose(out);
+  return 0;
+}
