@@ -1,0 +1,117 @@
+// Reference-shaped C++ API of the hot path, implemented over libbem_hip's C ABI.
+//
+// A driver written against the reference's headers for this path
+//   src/Impedance.h:20-21      Zdense / Zts
+//   src/HeatTransfer.h:33      CalcHeatTransfer
+//   src/FrequencySweep.h:9-13  FrequencySweep
+//   src/GreensFunction.h:7-28  GreensFunction (init, eps, mu, omega, k, Zeps, Zmu)
+//   src/util/Dielectric.h:7-13 Dielectric (eps(eV), mu(eV), setTemperature)
+//   src/Options.h              Options (farThresh6/10/15/21)
+//   src/util/DenseMatrix.h     ComplexDenseMatrix (column-major, operator()(i,j), lda)
+// compiles against this header and links libbem_hip.so instead.  RWGTree is
+// built from vertex/triangle arrays (mesh ingestion from X3D is not part of
+// this library); a two-body tree is RWGTree(s, t) with s, t the bodies, as
+// the reference's top tree holds sub[0] = body 1 and sub[1] = body 2.
+//
+// Error behaviour follows the reference: no return codes on these calls.
+// Where the reference would abort (dimension mismatch) or silently continue
+// (LAPACK info ignored), this layer throws bem::Error carrying the C ABI's
+// message, which an unchanged driver turns into an abort.
+#pragma once
+
+#include <complex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bem_hip.h"
+
+typedef double Real;
+typedef std::complex<double> Complex;
+
+namespace bem {
+struct Error : public std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+// Process-wide device context used by the reference-shaped calls.
+bem_ctx *context();
+}  // namespace bem
+
+class Options {
+ public:
+  Options();
+  Real farThresh6, farThresh10, farThresh15, farThresh21;
+};
+
+class Dielectric {
+ public:
+  Dielectric() {}
+  virtual ~Dielectric() {}
+  virtual Complex eps(Real eV) = 0;
+  virtual Complex mu(Real eV) = 0;
+  virtual void setTemperature(Real T) {}
+};
+
+class ConstantDielectric : public Dielectric {
+ public:
+  ConstantDielectric(Complex e, Complex m) : e_(e), m_(m) {}
+  Complex eps(Real) { return e_; }
+  Complex mu(Real) { return m_; }
+
+ private:
+  Complex e_, m_;
+};
+
+class GreensFunction {
+ public:
+  explicit GreensFunction(bool bExternal);
+  GreensFunction(Complex eps, Complex mu, Complex omega, bool bExternal = false);
+  void init(Complex eps, Complex mu, Complex omega);
+  bool isExternal() { return bExternal; }
+  bool bExternal;
+  Complex Zeps, Zmu;
+  Real normk;
+  Complex ik, k2, k, ksquared, eps, mu, omega;
+};
+
+class ComplexDenseMatrix {
+ public:
+  ComplexDenseMatrix(int m, int n) : m(m), n(n), lda(m), p((size_t)m * n) {}
+  Complex &operator()(int i, int j) { return p[(size_t)j * lda + i]; }
+  const Complex &operator()(int i, int j) const { return p[(size_t)j * lda + i]; }
+  int m, n, lda;
+  std::vector<Complex> p;
+};
+
+class RWGTree {
+ public:
+  // One body: closed triangle mesh (xyz nv x 3, tri nt x 3), its medium and a
+  // translation (the reference applies the gap to body 1's shape).
+  RWGTree(const std::vector<double> &xyz, const std::vector<int> &tri, Dielectric *diel,
+          const Real shift[3] = nullptr);
+  // Two-body top tree over bodies s (body 1) and t (body 2).
+  RWGTree(RWGTree *s, RWGTree *t);
+  ~RWGTree();
+  int rwgs;
+  RWGTree *sub[2];
+  int subs;
+  GreensFunction *g;
+  Dielectric *diel;
+  // library handles
+  bem_body *body;
+  Real shift[3];
+
+ private:
+  bool owns_g_;
+};
+
+ComplexDenseMatrix *Zts(RWGTree *t, RWGTree *s, GreensFunction *g, Options &options);
+void Zdense(RWGTree *t, RWGTree *s, ComplexDenseMatrix *dense, GreensFunction *gExt, Options &options);
+Real CalcHeatTransfer(Real a, Real eV, Real T1, Real T2, RWGTree *top, RWGTree *s, RWGTree *t, Options &options);
+
+class FrequencySweep {
+ public:
+  explicit FrequencySweep(char *file);
+  std::vector<Real> f;
+};

@@ -1,0 +1,648 @@
+// FP64 RWG/PMCHWT impedance assembly on gfx950.
+//
+// The reference evaluates one triangle pair at a time into a 3x3 tile of
+// L (single-layer) and K (double-layer) integrals and scatters the tile into
+// Z (src/Impedance.cpp:276-385).  Here the pair space of a tree is split by
+// class and each class gets its own launch shape:
+//   far_g6_kernel   thread per pair over the dense pair grid (16x16 pairs per
+//                   256-thread block, triangle data staged in LDS); computes
+//                   every 6-point pair, queues 36-point pairs, skips close ones
+//   pair_block_kernel<36,36>   one 64-lane block per queued 36-point pair
+//   pair_block_kernel<105,91>  one 128-lane block per close pair (self or
+//                   shared vertex): smooth Gauss105 x Gauss91 quadrature plus
+//                   the analytic singular part and the neighbour edge term
+// Tiles land in a structure-of-arrays buffer (36 real planes); gather_kernel
+// then forms the 2x2 [J,M] blocks of Z for every RWG pair in a fixed order
+// (deterministic: no floating-point atomics anywhere).
+//
+// Algebra.  The reference accumulates, per outer point and source half j,
+//   Gphi_j = sum w g (r'-p0_j)/2,  DGxphi_j = sum w grad g x (r'-p0_j)/2
+// (src/Impedance.cpp:93-104).  Both are affine in p0_j, so the inner loop
+// keeps only the four j-independent moments
+//   G0 = sum w g, G1 = sum w g rb, D0 = sum w grad g, D1 = sum w grad g x rb
+// with rb = r' - o (o = the source frame vertex, keeping magnitudes local),
+// and Gphi_j = (G1 - G0 pb_j)/2, DGxphi_j = (D1 - D0 x pb_j)/2, pb_j = p0_j - o.
+// The outer step (src/Impedance.cpp:154-167, :194-250) is then applied per
+// (i, j) exactly as written there.
+#include "bem_common.h"
+
+#define BEM_QUAD_NO_HOST_TABLES
+#include "quadrature_tables.h"
+
+__constant__ double c_quad[BEM_QUAD_TOTAL_POINTS][3] = BEM_QUAD_ABW_INIT;
+__constant__ double c_line15[15][2] = BEM_LINE15_INIT;
+
+namespace {
+
+struct C2 {
+  double re, im;
+};
+
+__device__ __forceinline__ C2 cmk(double r, double i) { C2 c; c.re = r; c.im = i; return c; }
+__device__ __forceinline__ C2 cmul(C2 a, C2 b) { return cmk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+
+__device__ __forceinline__ double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - a[2] * b[1];
+  c[1] = a[2] * b[0] - a[0] * b[2];
+  c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// Pair classification with the reference's exact arithmetic (no contraction):
+// same body and a shared vertex -> close; else d*12 > emax^2 -> 6-point rule.
+__device__ int classify_pair(const TriGeom &A, const TriGeom &B, bool same_tri) {
+#pragma clang fp contract(off)
+  if (A.body == B.body) {
+    if (same_tri) return BEM_CLASS_CLOSE;
+    for (int i = 0; i < 3; i++)
+      if (A.vid[i] == B.vid[0] || A.vid[i] == B.vid[1] || A.vid[i] == B.vid[2]) return BEM_CLASS_CLOSE;
+  }
+  double vx = A.centre[0] - B.centre[0];
+  double vy = A.centre[1] - B.centre[1];
+  double vz = A.centre[2] - B.centre[2];
+  double d = vx * vx + vy * vy + vz * vz;
+  double em = A.emax > B.emax ? A.emax : B.emax;
+  double emax2 = em * em;
+  if (d * 12.0 > emax2) return BEM_CLASS_G6;
+  return BEM_CLASS_G36;
+}
+
+// Quadrature point v0 + a v01 + b v02 (left to right, as TriangleQuadrature.h).
+__device__ __forceinline__ void quad_point(const TriGeom &T, int q, double *v) {
+#pragma clang fp contract(off)
+  double a = c_quad[q][0], b = c_quad[q][1];
+  for (int c = 0; c < 3; c++) {
+    double t = T.p0[c] + a * T.v01[c];
+    v[c] = t + b * T.v02[c];
+  }
+}
+
+// Inner moments (see file comment).
+struct Inner {
+  C2 G0;
+  double G1r[3], G1i[3];
+  double D0r[3], D0i[3];
+  double D1r[3], D1i[3];
+};
+
+__device__ __forceinline__ void inner_zero(Inner &m) {
+  m.G0 = cmk(0, 0);
+  for (int c = 0; c < 3; c++) { m.G1r[c] = m.G1i[c] = m.D0r[c] = m.D0i[c] = m.D1r[c] = m.D1i[c] = 0.0; }
+}
+
+// One inner point: Green's function (regular src/GreensFunction.cpp:41-52 or
+// smooth :54-69) at d = r - r', weighted moment update.
+template <bool SMOOTH>
+__device__ __forceinline__ void inner_point(const Medium &md, const double *r, const double *rp, const double *ob,
+                                            double w, Inner &m) {
+  double d[3] = {r[0] - rp[0], r[1] - rp[1], r[2] - rp[2]};
+  double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+  double R = sqrt(R2);
+  double ar = md.ik_re * R, ai = md.ik_im * R;  // ikR
+  double ex = exp(ar);
+  double sn, cs;
+  sincos(ai, &sn, &cs);
+  double er = ex * cs, ei = ex * sn;            // exp(ikR)
+  double invR = 1.0 / R;
+  C2 g, q;
+  if (!SMOOTH) {
+    g = cmk(er * invR, ei * invR);
+    // q = g (1 - ikR) / R^2
+    C2 t = cmul(g, cmk(1.0 - ar, -ai));
+    double invR2 = invR * invR;
+    q = cmk(t.re * invR2, t.im * invR2);
+  } else {
+    // g = (e - 1)/R - k2 R ; q = (e (1 - ikR) + k2 R^2 - 1) / R^3
+    g = cmk((er - 1.0) * invR - md.k2_re * R, ei * invR - md.k2_im * R);
+    C2 t = cmul(cmk(er, ei), cmk(1.0 - ar, -ai));
+    double invR3 = invR * invR * invR;
+    q = cmk((t.re + md.k2_re * R2 - 1.0) * invR3, (t.im + md.k2_im * R2) * invR3);
+  }
+  double wgr = w * g.re, wgi = w * g.im;
+  double wqr = w * q.re, wqi = w * q.im;
+  double rb[3] = {rp[0] - ob[0], rp[1] - ob[1], rp[2] - ob[2]};
+  double cr[3];
+  cross3(d, rb, cr);
+  m.G0.re += wgr;
+  m.G0.im += wgi;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    m.G1r[c] += wgr * rb[c];
+    m.G1i[c] += wgi * rb[c];
+    m.D0r[c] += wqr * d[c];
+    m.D0i[c] += wqi * d[c];
+    m.D1r[c] += wqr * cr[c];
+    m.D1i[c] += wqi * cr[c];
+  }
+}
+
+// Far outer step (src/Impedance.cpp:154-167): L_ij += w (k^2 phi_i.Gphi_j - G),
+// K_ij -= w phi_i.DGxphi_j, phi_i = (r - p0_i)/2.
+__device__ __forceinline__ void outer_far(const Medium &md, const double *r, double w, const Inner &m,
+                                          const double (*p0t)[3], int nt, const double (*pbs)[3], int ns,
+                                          double *Lr, double *Li, double *Kr, double *Ki) {
+  for (int i = 0; i < 3; i++) {
+    if (i >= nt) break;
+    double phi[3] = {0.5 * (r[0] - p0t[i][0]), 0.5 * (r[1] - p0t[i][1]), 0.5 * (r[2] - p0t[i][2])};
+    C2 a = cmk(dot3(phi, m.G1r), dot3(phi, m.G1i));
+    C2 c = cmk(dot3(phi, m.D1r), dot3(phi, m.D1i));
+    for (int j = 0; j < 3; j++) {
+      if (j >= ns) break;
+      double b = dot3(phi, pbs[j]);
+      double e[3];
+      cross3(pbs[j], phi, e);
+      C2 gp = cmk(0.5 * (a.re - b * m.G0.re), 0.5 * (a.im - b * m.G0.im));
+      C2 kg = cmul(cmk(md.ksq_re, md.ksq_im), gp);
+      int ij = 3 * i + j;
+      Lr[ij] += w * (kg.re - m.G0.re);
+      Li[ij] += w * (kg.im - m.G0.im);
+      double dr = 0.5 * (c.re - dot3(m.D0r, e));
+      double di = 0.5 * (c.im - dot3(m.D0i, e));
+      Kr[ij] -= w * dr;
+      Ki[ij] -= w * di;
+    }
+  }
+}
+
+// EHNonSmoothIntegral (src/Source.cpp:16-86) on the frame half of triangle S.
+__device__ void eh_nonsmooth(const TriGeom &S, const double *rt, double &ISn3h, double &ISn1, double &IS1,
+                             double *IL1, double *IL3) {
+  double tol = 1e-6 * S.elen;
+  double p0[3], p1[3], p2[3];
+  for (int c = 0; c < 3; c++) { p0[c] = rt[c] - S.p0[c]; p1[c] = rt[c] - S.p1[c]; p2[c] = rt[c] - S.p2[c]; }
+  double R02 = dot3(p0, p0), R0 = sqrt(R02);
+  double R12 = dot3(p1, p1), R1 = sqrt(R12);
+  double R22 = dot3(p2, p2), R2 = sqrt(R22);
+  double i0 = 1. / R0, i1 = 1. / R1, i2 = 1. / R2;
+  double a0[3] = {i0 * p0[0], i0 * p0[1], i0 * p0[2]};
+  double a1[3] = {i1 * p1[0], i1 * p1[1], i1 * p1[2]};
+  double a2[3] = {i2 * p2[0], i2 * p2[1], i2 * p2[2]};
+  double a12[3] = {a1[0] + a2[0], a1[1] + a2[1], a1[2] + a2[2]};
+  double x = 1.0 + dot3(a0, a12) + dot3(a1, a2);
+  double cx[3];
+  cross3(a1, a2, cx);
+  double tp = dot3(a0, cx);
+  double y = sqrt(tp * tp);
+  double h = dot3(p0, S.n);
+  ISn3h = (h < 0. ? -2. : 2.) * atan2(y, x);
+  double num, den;
+  double s010 = S.normv01i * dot3(S.v01, p0);
+  double s011 = S.normv01i * dot3(S.v01, p1);
+  num = R1 - s011; den = R0 - s010;
+  double L01 = fabs(num) < tol || fabs(den) < tol ? 0. : log(num / den);
+  double s121 = S.normv12i * dot3(S.v12, p1);
+  double s122 = S.normv12i * dot3(S.v12, p2);
+  num = R2 - s122; den = R1 - s121;
+  double L12 = fabs(num) < tol || fabs(den) < tol ? 0. : log(num / den);
+  double s202 = S.normv20i * dot3(S.v20, p2);
+  double s200 = S.normv20i * dot3(S.v20, p0);
+  num = R0 - s200; den = R2 - s202;
+  double L20 = fabs(num) < tol || fabs(den) < tol ? 0. : log(num / den);
+  double t01 = dot3(S.m01, p0), t12 = dot3(S.m12, p1), t20 = dot3(S.m20, p2);
+  ISn1 = -h * ISn3h - (t01 * L01 + t12 * L12 + t20 * L20);
+  double R02_01 = R02 - s010 * s010;
+  double IL1_01 = 0.5 * (R02_01 * L01 - (R1 * s011 - R0 * s010));
+  double R02_12 = R12 - s121 * s121;
+  double IL1_12 = 0.5 * (R02_12 * L12 - (R2 * s122 - R1 * s121));
+  double R02_20 = R22 - s202 * s202;
+  double IL1_20 = 0.5 * (R02_20 * L20 - (R0 * s200 - R2 * s202));
+  double IL3_01 = 0.75 * R02_01 * IL1_01 - 0.25 * (R12 * R1 * s011 - R02 * R0 * s010);
+  double IL3_12 = 0.75 * R02_12 * IL1_12 - 0.25 * (R22 * R2 * s122 - R12 * R1 * s121);
+  double IL3_20 = 0.75 * R02_20 * IL1_20 - 0.25 * (R02 * R0 * s200 - R22 * R2 * s202);
+  IS1 = BEM_ONETHIRD * (h * h * ISn1 - (t01 * IL1_01 + t12 * IL1_12 + t20 * IL1_20));
+  for (int c = 0; c < 3; c++) {
+    IL1[c] = IL1_01 * S.m01[c] + IL1_12 * S.m12[c] + IL1_20 * S.m20[c];
+    IL3[c] = IL3_01 * S.m01[c] + IL3_12 * S.m12[c] + IL3_20 * S.m20[c];
+  }
+}
+
+// LKEdgeIntegral (src/Impedance.cpp:13-73) on the frame half of triangle T.
+__device__ void lk_edge(const TriGeom &T, const double *rs, double &ISn1, double *IL1) {
+  double p0[3], p1[3], p2[3];
+  for (int c = 0; c < 3; c++) { p0[c] = rs[c] - T.p0[c]; p1[c] = rs[c] - T.p1[c]; p2[c] = rs[c] - T.p2[c]; }
+  double R02 = dot3(p0, p0), R0 = sqrt(R02);
+  double R12 = dot3(p1, p1), R1 = sqrt(R12);
+  double R22 = dot3(p2, p2), R2 = sqrt(R22);
+  double tol = 1e-8 * T.elen;
+  double i0 = 1. / R0, i1 = 1. / R1, i2 = 1. / R2;
+  double a0[3] = {i0 * p0[0], i0 * p0[1], i0 * p0[2]};
+  double a1[3] = {i1 * p1[0], i1 * p1[1], i1 * p1[2]};
+  double a2[3] = {i2 * p2[0], i2 * p2[1], i2 * p2[2]};
+  double h = dot3(p0, T.n);
+  double a12[3] = {a1[0] + a2[0], a1[1] + a2[1], a1[2] + a2[2]};
+  double x = 1.0 + dot3(a0, a12) + dot3(a1, a2);
+  double cx[3];
+  cross3(a1, a2, cx);
+  double tp = dot3(a0, cx);
+  double y = sqrt(tp * tp);
+  double ISn3h = (h < 0 ? -2. : 2.) * atan2(y, x);
+  double t01 = dot3(T.m01, p0), t12 = dot3(T.m12, p1), t20 = dot3(T.m20, p2);
+  double s010 = T.normv01i * dot3(T.v01, p0);
+  double s011 = T.normv01i * dot3(T.v01, p1);
+  double num, den;
+  num = R1 - s011; den = R0 - s010;
+  double L01 = fabs(num) < tol || fabs(den) < tol ? 0. : log(num / den);
+  double s121 = T.normv12i * dot3(T.v12, p1);
+  double s122 = T.normv12i * dot3(T.v12, p2);
+  num = R2 - s122; den = R1 - s121;
+  double L12 = fabs(num) < tol || fabs(den) < tol ? 0. : log(num / den);
+  double s202 = T.normv20i * dot3(T.v20, p2);
+  double s200 = T.normv20i * dot3(T.v20, p0);
+  num = R0 - s200; den = R2 - s202;
+  double L20 = fabs(num) < tol || fabs(den) < tol ? 0. : log(num / den);
+  ISn1 = -h * ISn3h - (t01 * L01 + t12 * L12 + t20 * L20);
+  double R02_01 = R02 - s010 * s010;
+  double IL1_01 = 0.5 * (R02_01 * L01 - (R1 * s011 - R0 * s010));
+  double R02_12 = R12 - s121 * s121;
+  double IL1_12 = 0.5 * (R02_12 * L12 - (R2 * s122 - R1 * s121));
+  double R02_20 = R22 - s202 * s202;
+  double IL1_20 = 0.5 * (R02_20 * L20 - (R0 * s200 - R2 * s202));
+  for (int c = 0; c < 3; c++) IL1[c] = IL1_01 * T.m01[c] + IL1_12 * T.m12[c] + IL1_20 * T.m20[c];
+}
+
+// Close-pair outer step (src/Impedance.cpp:194-250) for one outer point v.
+__device__ void outer_close(const Medium &md, const TriGeom &S, const double *v, double w, const Inner &m,
+                            const double (*p0t)[3], int nt, const double (*p0s)[3], const double (*pbs)[3], int ns,
+                            double *Lr, double *Li, double *Kr, double *Ki) {
+  double ISn3h, ISn1, IS1, IL1[3], IL3[3];
+  eh_nonsmooth(S, v, ISn3h, ISn1, IS1, IL1, IL3);
+  double Ai2 = 0.5 * S.Ai;
+  // IL13 = Ai2 (IL1 + (k2/3) IL3), ISn11 = Ai2 (ISn1 + k2 IS1)
+  C2 c13 = cmk(BEM_ONETHIRD * md.k2_re, BEM_ONETHIRD * md.k2_im);
+  double IL13r[3], IL13i[3];
+  for (int c = 0; c < 3; c++) {
+    IL13r[c] = Ai2 * (IL1[c] + c13.re * IL3[c]);
+    IL13i[c] = Ai2 * (c13.im * IL3[c]);
+  }
+  C2 ISn11 = cmk(Ai2 * (ISn1 + md.k2_re * IS1), Ai2 * (md.k2_im * IS1));
+  double vp[3] = {v[0] - S.p0[0], v[1] - S.p0[1], v[2] - S.p0[2]};
+  double h = dot3(vp, S.n);
+  C2 L2 = cmk(2.0 * ISn11.re + m.G0.re, 2.0 * ISn11.im + m.G0.im);
+  C2 ksq = cmk(md.ksq_re, md.ksq_im);
+  bool coplanar = fabs(h) < 1e-8 * S.elen;
+  double rho[3] = {v[0] - h * S.n[0], v[1] - h * S.n[1], v[2] - h * S.n[2]};
+  // K31 = (Ai2 k2) u, u = IL1 - h ISn1 n ; ISn3hn = Ai2 ISn3h n
+  C2 ak2 = cmk(Ai2 * md.k2_re, Ai2 * md.k2_im);
+  double hI = h * ISn1;
+  double u[3] = {IL1[0] - hI * S.n[0], IL1[1] - hI * S.n[1], IL1[2] - hI * S.n[2]};
+  double s3 = Ai2 * ISn3h;
+  double ISn3hn[3] = {s3 * S.n[0], s3 * S.n[1], s3 * S.n[2]};
+  const double *base = coplanar ? v : rho;
+  for (int i = 0; i < 3; i++) {
+    if (i >= nt) break;
+    double phi[3] = {0.5 * (v[0] - p0t[i][0]), 0.5 * (v[1] - p0t[i][1]), 0.5 * (v[2] - p0t[i][2])};
+    C2 a = cmk(dot3(phi, m.G1r), dot3(phi, m.G1i));
+    C2 c = cmk(dot3(phi, m.D1r), dot3(phi, m.D1i));
+    C2 dIL13 = cmk(dot3(phi, IL13r), dot3(phi, IL13i));
+    for (int j = 0; j < 3; j++) {
+      if (j >= ns) break;
+      double b = dot3(phi, pbs[j]);
+      C2 gp = cmk(0.5 * (a.re - b * m.G0.re), 0.5 * (a.im - b * m.G0.im));
+      double bj[3] = {base[0] - p0s[j][0], base[1] - p0s[j][1], base[2] - p0s[j][2]};
+      double pb = dot3(phi, bj);
+      C2 k21 = cmk(dIL13.re + ISn11.re * pb, dIL13.im + ISn11.im * pb);
+      C2 kg = cmul(ksq, cmk(gp.re + k21.re, gp.im + k21.im));
+      int ij = 3 * i + j;
+      Lr[ij] += w * (kg.re - L2.re);
+      Li[ij] += w * (kg.im - L2.im);
+      if (!coplanar) {
+        double e[3];
+        cross3(pbs[j], phi, e);
+        double dr = 0.5 * (c.re - dot3(m.D0r, e));
+        double di = 0.5 * (c.im - dot3(m.D0i, e));
+        // vts_ij x ISn3hn, vts_ij = p0s_j - p0t_i
+        double vts[3] = {p0s[j][0] - p0t[i][0], p0s[j][1] - p0t[i][1], p0s[j][2] - p0t[i][2]};
+        double vx[3];
+        cross3(vts, ISn3hn, vx);
+        double t1 = dot3(phi, vx);
+        // K41_j = (Ai2 k2) (u x (v - p0s_j))
+        double vj[3] = {v[0] - p0s[j][0], v[1] - p0s[j][1], v[2] - p0s[j][2]};
+        double ux[3];
+        cross3(u, vj, ux);
+        double t2 = dot3(phi, ux);
+        Kr[ij] -= w * (dr + t1 + ak2.re * t2);
+        Ki[ij] -= w * (di + ak2.im * t2);
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ void store_tile(double *tiles, size_t npairs, size_t idx, const double *Lr, const double *Li,
+                                           const double *Kr, const double *Ki) {
+#pragma unroll
+  for (int q = 0; q < 9; q++) {
+    tiles[(size_t)(2 * q) * npairs + idx] = Lr[q];
+    tiles[(size_t)(2 * q + 1) * npairs + idx] = Li[q];
+    tiles[(size_t)(18 + 2 * q) * npairs + idx] = Kr[q];
+    tiles[(size_t)(18 + 2 * q + 1) * npairs + idx] = Ki[q];
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Dense pair grid: every 6-point pair computed here; 36-point pairs queued.
+// Block = 16 x 16 pairs.  tri_lo/tri_hi bound the test rows (row-stripe
+// launches) and the source columns cover [col_lo, col_hi).
+#define FAR_TB 16
+__global__ void __launch_bounds__(256) far_g6_kernel(const TriGeom *__restrict__ tg, int T, Medium md,
+                                                     int row_lo, int row_hi, int col_lo, int col_hi,
+                                                     double *__restrict__ tiles, size_t npairs,
+                                                     int2 *__restrict__ q36, unsigned int *__restrict__ q36_count,
+                                                     unsigned int q36_cap) {
+  __shared__ double s_out[FAR_TB][6][4];   // outer points + weight
+  __shared__ double s_in[FAR_TB][6][4];    // inner points + weight
+  __shared__ double s_tp0[FAR_TB][3][3];   // test half p0
+  __shared__ double s_spb[FAR_TB][3][3];   // source half p0 - frame p0
+  __shared__ double s_sob[FAR_TB][3];      // source frame p0
+  __shared__ int s_tnh[FAR_TB], s_snh[FAR_TB];
+
+  int tx = threadIdx.x % FAR_TB, ty = threadIdx.x / FAR_TB;
+  int a0 = row_lo + blockIdx.y * FAR_TB;
+  int b0 = col_lo + blockIdx.x * FAR_TB;
+  // stage: thread (ty, tx) with tx < 6 loads quadrature point tx of test row ty / source col ty
+  {
+    int t = threadIdx.x;
+    if (t < FAR_TB * 6) {
+      int k = t / 6, q = t % 6;
+      int a = a0 + k;
+      if (a < row_hi) {
+        double v[3];
+        quad_point(tg[a], BEM_QUAD_OFF_6 + q, v);
+        s_out[k][q][0] = v[0]; s_out[k][q][1] = v[1]; s_out[k][q][2] = v[2];
+        s_out[k][q][3] = c_quad[BEM_QUAD_OFF_6 + q][2];
+      }
+    } else if (t < 2 * FAR_TB * 6) {
+      int u = t - FAR_TB * 6;
+      int k = u / 6, q = u % 6;
+      int b = b0 + k;
+      if (b < col_hi) {
+        double v[3];
+        quad_point(tg[b], BEM_QUAD_OFF_6 + q, v);
+        s_in[k][q][0] = v[0]; s_in[k][q][1] = v[1]; s_in[k][q][2] = v[2];
+        s_in[k][q][3] = c_quad[BEM_QUAD_OFF_6 + q][2];
+      }
+    } else if (t < 2 * FAR_TB * 6 + FAR_TB) {
+      int k = t - 2 * FAR_TB * 6;
+      int a = a0 + k;
+      if (a < row_hi) {
+        const TriGeom &A = tg[a];
+        s_tnh[k] = A.nh;
+        for (int i = 0; i < 3; i++)
+          for (int c = 0; c < 3; c++) s_tp0[k][i][c] = A.hp0[i][c];
+      }
+    } else if (t < 2 * FAR_TB * 6 + 2 * FAR_TB) {
+      int k = t - 2 * FAR_TB * 6 - FAR_TB;
+      int b = b0 + k;
+      if (b < col_hi) {
+        const TriGeom &B = tg[b];
+        s_snh[k] = B.nh;
+        for (int c = 0; c < 3; c++) s_sob[k][c] = B.p0[c];
+        for (int j = 0; j < 3; j++)
+          for (int c = 0; c < 3; c++) s_spb[k][j][c] = B.hp0[j][c] - B.p0[c];
+      }
+    }
+  }
+  __syncthreads();
+  int a = a0 + ty, b = b0 + tx;
+  if (a >= row_hi || b >= col_hi) return;
+  int cls = classify_pair(tg[a], tg[b], a == b);
+  if (cls == BEM_CLASS_CLOSE) return;
+  if (cls != BEM_CLASS_G6) {
+    unsigned int slot = atomicAdd(q36_count, 1u);
+    if (slot < q36_cap) q36[slot] = make_int2(a, b);
+    return;
+  }
+  double Lr[9], Li[9], Kr[9], Ki[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) Lr[q] = Li[q] = Kr[q] = Ki[q] = 0.0;
+  int nt = s_tnh[ty], ns = s_snh[tx];
+  const double *ob = s_sob[tx];
+  for (int o = 0; o < 6; o++) {
+    const double *r = s_out[ty][o];
+    Inner m;
+    inner_zero(m);
+#pragma unroll 2
+    for (int q = 0; q < 6; q++) inner_point<false>(md, r, s_in[tx][q], ob, s_in[tx][q][3], m);
+    outer_far(md, r, s_out[ty][o][3], m, s_tp0[ty], nt, s_spb[tx], ns, Lr, Li, Kr, Ki);
+  }
+  store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)T + (size_t)b, Lr, Li, Kr, Ki);
+}
+
+// Block reduction of 36 accumulators; result valid in thread 0.
+template <int NT>
+__device__ __forceinline__ void block_reduce36(double *acc, double *red /* [NT/64][36] */) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int q = 0; q < 36; q++) {
+    double x = acc[q];
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) x += __shfl_down(x, off, 64);
+    acc[q] = x;
+  }
+  if (NT > 64) {
+    if (lane == 0)
+      for (int q = 0; q < 36; q++) red[wv * 36 + q] = acc[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 0; q < 36; q++) {
+        double s = red[q];
+        for (int w2 = 1; w2 < NT / 64; w2++) s += red[w2 * 36 + q];
+        acc[q] = s;
+      }
+    }
+  }
+}
+
+// One block per listed pair.  CLOSE: Gauss105 outer x Gauss91 smooth inner +
+// analytic terms + neighbour edge correction; otherwise 36 x 36 regular.
+template <int NOUT, int NIN, bool CLOSE, int NT>
+__global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restrict__ tg, int T, Medium md,
+                                                        const int2 *__restrict__ pairs,
+                                                        const unsigned int *__restrict__ count_dev,
+                                                        unsigned int count_host, int row_lo, int row_hi,
+                                                        double *__restrict__ tiles, size_t npairs) {
+  __shared__ double s_in[NIN][4];
+  __shared__ double s_red[(NT / 64) * 36];
+  __shared__ double s_tp0[3][3], s_sp0[3][3], s_spb[3][3];
+  const int offo = CLOSE ? BEM_QUAD_OFF_105 : BEM_QUAD_OFF_36;
+  const int offi = CLOSE ? BEM_QUAD_OFF_91 : BEM_QUAD_OFF_36;
+  unsigned int count = count_host;
+  if (count_dev && *count_dev < count_host) count = *count_dev;   // queue may have overflowed: never read past it
+  for (unsigned int p = blockIdx.x; p < count; p += gridDim.x) {
+    int2 pr = pairs[p];
+    int a = pr.x, b = pr.y;
+    if (a < row_lo || a >= row_hi) continue;   // uniform per block
+    const TriGeom &A = tg[a];
+    const TriGeom &B = tg[b];
+    __syncthreads();
+    for (int t = threadIdx.x; t < NIN; t += NT) {
+      double v[3];
+      quad_point(B, offi + t, v);
+      s_in[t][0] = v[0]; s_in[t][1] = v[1]; s_in[t][2] = v[2];
+      s_in[t][3] = c_quad[offi + t][2];
+    }
+    if (threadIdx.x < 9) {
+      int i = threadIdx.x / 3, c = threadIdx.x % 3;
+      s_tp0[i][c] = A.hp0[i][c];
+      s_sp0[i][c] = B.hp0[i][c];
+      s_spb[i][c] = B.hp0[i][c] - B.p0[c];
+    }
+    __syncthreads();
+    double acc[36];
+#pragma unroll
+    for (int q = 0; q < 36; q++) acc[q] = 0.0;
+    double *Lr = acc, *Li = acc + 9, *Kr = acc + 18, *Ki = acc + 27;
+    int nt = A.nh, ns = B.nh;
+    for (int o = threadIdx.x; o < NOUT; o += NT) {
+      double r[3];
+      quad_point(A, offo + o, r);
+      double w = c_quad[offo + o][2];
+      Inner m;
+      inner_zero(m);
+      for (int q = 0; q < NIN; q++) inner_point<CLOSE>(md, r, s_in[q], B.p0, s_in[q][3], m);
+      if (CLOSE)
+        outer_close(md, B, r, w, m, s_tp0, nt, s_sp0, s_spb, ns, Lr, Li, Kr, Ki);
+      else
+        outer_far(md, r, w, m, s_tp0, nt, s_spb, ns, Lr, Li, Kr, Ki);
+    }
+    if (CLOSE && a != b) {
+      // neighbour edge correction (src/Impedance.cpp:253-274, :301-324):
+      // 15-point line rule on each of the 3 source edges, LKEdgeIntegral on
+      // the test frame; each point's contribution to K_ij is linear, so it is
+      // added to this thread's accumulator before the block reduction.
+      for (int e = threadIdx.x; e < 45; e += NT) {
+        int edge = e / 15, q = e % 15;
+        const double *e0 = edge == 0 ? B.p0 : (edge == 1 ? B.p1 : B.p2);
+        const double *ev = edge == 0 ? B.v01 : (edge == 1 ? B.v12 : B.v20);
+        const double *me = edge == 0 ? B.m01 : (edge == 1 ? B.m12 : B.m20);
+        double nvi = edge == 0 ? B.normv01i : (edge == 1 ? B.normv12i : B.normv20i);
+        double tq = c_line15[q][0], wq = c_line15[q][1];
+        double v[3];
+        for (int c = 0; c < 3; c++) v[c] = e0[c] + tq * ev[c];
+        double ISn1, IL1[3];
+        lk_edge(A, v, ISn1, IL1);
+        double vp[3] = {v[0] - A.p0[0], v[1] - A.p0[1], v[2] - A.p0[2]};
+        double h = dot3(vp, A.n);
+        double rho[3] = {v[0] - h * A.n[0], v[1] - h * A.n[1], v[2] - h * A.n[2]};
+        double cc = 0.125 * A.Ai * B.Ai;
+        for (int i = 0; i < 3; i++) {
+          if (i >= nt) break;
+          double kv[3];
+          for (int c = 0; c < 3; c++) kv[c] = wq * (IL1[c] + ISn1 * (rho[c] - s_tp0[i][c]));
+          for (int j = 0; j < 3; j++) {
+            if (j >= ns) break;
+            double vv[3] = {s_tp0[i][0] - s_sp0[j][0], s_tp0[i][1] - s_sp0[j][1], s_tp0[i][2] - s_sp0[j][2]};
+            double vm[3];
+            cross3(vv, me, vm);
+            Kr[3 * i + j] += cc * (dot3(vm, kv) / nvi);
+          }
+        }
+      }
+    }
+    block_reduce36<NT>(acc, s_red);
+    if (threadIdx.x == 0) {
+      // acc layout Lr(9) Li(9) Kr(9) Ki(9) -> planes (L re/im interleaved, then K)
+      double Lrr[9], Lii[9], Krr[9], Kii[9];
+      for (int q = 0; q < 9; q++) { Lrr[q] = acc[q]; Lii[q] = acc[9 + q]; Krr[q] = acc[18 + q]; Kii[q] = acc[27 + q]; }
+      store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)T + (size_t)b, Lrr, Lii, Krr, Kii);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Gather (src/Impedance.cpp:365-384): for RWG pair (i, j) sum the four half
+// pairs with c = l_i l_j / 4pi and the half sign, then
+//   Z(2i,2j) = Zeps L, Z(2i,2j+1) = K, Z(2i+1,2j) = -K, Z(2i+1,2j+1) = Zmu L.
+// With ADD the values are added to Z (W = Z_ext + Z_int on the diagonal blocks);
+// Zcopy (optional) receives the same values (the separate G_b matrices).
+// Rows are RWGs [r_lo, r_hi) of the tree; the tile buffer holds test rows
+// [row_lo, row_hi) of triangles.
+__global__ void gather_kernel(const double *__restrict__ tiles, size_t npairs, int T, int row_lo,
+                              const RwgRec *__restrict__ rw, int r_lo, int r_hi, int c_lo, int c_hi, Medium md,
+                              double2 *__restrict__ Z, int64_t ldz, int zr0, int zc0, int add,
+                              double2 *__restrict__ Zcopy, int64_t ldc, int cr0, int cc0) {
+  int i = r_lo + blockIdx.x * blockDim.x + threadIdx.x;
+  int j = c_lo + blockIdx.y;
+  if (i >= r_hi || j >= c_hi) return;
+  RwgRec ri = rw[i], rj = rw[j];
+  double Lr = 0, Li = 0, Kr = 0, Ki = 0;
+  int ta[2] = {ri.tp, ri.tn}, sa[2] = {ri.sp, ri.sn};
+  int tb[2] = {rj.tp, rj.tn}, sb[2] = {rj.sp, rj.sn};
+  for (int hi = 0; hi < 2; hi++)
+    for (int hj = 0; hj < 2; hj++) {
+      double sg = (hi == hj) ? 1.0 : -1.0;
+      size_t idx = (size_t)(ta[hi] - row_lo) * (size_t)T + (size_t)tb[hj];
+      int q = 3 * sa[hi] + sb[hj];
+      Lr += sg * tiles[(size_t)(2 * q) * npairs + idx];
+      Li += sg * tiles[(size_t)(2 * q + 1) * npairs + idx];
+      Kr += sg * tiles[(size_t)(18 + 2 * q) * npairs + idx];
+      Ki += sg * tiles[(size_t)(18 + 2 * q + 1) * npairs + idx];
+    }
+  double c = BEM_ONEOVER4PI * ri.len * rj.len;
+  Lr *= c; Li *= c; Kr *= c; Ki *= c;
+  double2 zee = make_double2(md.Zeps_re * Lr - md.Zeps_im * Li, md.Zeps_re * Li + md.Zeps_im * Lr);
+  double2 zmm = make_double2(md.Zmu_re * Lr - md.Zmu_im * Li, md.Zmu_re * Li + md.Zmu_im * Lr);
+  double2 kp = make_double2(Kr, Ki), km = make_double2(-Kr, -Ki);
+  int64_t zi = zr0 + 2 * (int64_t)(i - r_lo), zj = zc0 + 2 * (int64_t)(j - c_lo);
+  double2 *z0 = Z + zj * ldz + zi;        // column 2j
+  double2 *z1 = Z + (zj + 1) * ldz + zi;  // column 2j+1
+  if (add) {
+    double2 t;
+    t = z0[0]; z0[0] = make_double2(t.x + zee.x, t.y + zee.y);
+    t = z0[1]; z0[1] = make_double2(t.x + km.x, t.y + km.y);
+    t = z1[0]; z1[0] = make_double2(t.x + kp.x, t.y + kp.y);
+    t = z1[1]; z1[1] = make_double2(t.x + zmm.x, t.y + zmm.y);
+  } else {
+    z0[0] = zee; z0[1] = km; z1[0] = kp; z1[1] = zmm;
+  }
+  if (Zcopy) {
+    int64_t ci = cr0 + 2 * (int64_t)(i - r_lo), cj = cc0 + 2 * (int64_t)(j - c_lo);
+    double2 *y0 = Zcopy + cj * ldc + ci;
+    double2 *y1 = Zcopy + (cj + 1) * ldc + ci;
+    y0[0] = zee; y0[1] = km; y1[0] = kp; y1[1] = zmm;
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+extern "C" {
+
+int bem_launch_far_g6(const TriGeom *tg, int T, const Medium *md, int row_lo, int row_hi, int col_lo, int col_hi,
+                      double *tiles, size_t npairs, int2 *q36, unsigned int *q36_count, unsigned int q36_cap,
+                      hipStream_t st) {
+  dim3 grid((col_hi - col_lo + FAR_TB - 1) / FAR_TB, (row_hi - row_lo + FAR_TB - 1) / FAR_TB);
+  if (grid.x == 0 || grid.y == 0) return 0;
+  hipLaunchKernelGGL(far_g6_kernel, grid, dim3(256), 0, st, tg, T, *md, row_lo, row_hi, col_lo, col_hi, tiles,
+                     npairs, q36, q36_count, q36_cap);
+  return (int)hipGetLastError();
+}
+
+int bem_launch_far_g36(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, const unsigned int *count_dev,
+                       unsigned int max_count, int row_lo, int row_hi, double *tiles, size_t npairs, hipStream_t st) {
+  unsigned int blocks = max_count < 4096u ? max_count : 4096u;
+  if (blocks == 0) return 0;
+  hipLaunchKernelGGL((pair_block_kernel<36, 36, false, 64>), dim3(blocks), dim3(64), 0, st, tg, T, *md, pairs,
+                     count_dev, max_count, row_lo, row_hi, tiles, npairs);
+  return (int)hipGetLastError();
+}
+
+int bem_launch_close(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, unsigned int count, int row_lo,
+                     int row_hi, double *tiles, size_t npairs, hipStream_t st) {
+  unsigned int blocks = count < 65535u ? count : 65535u;
+  if (blocks == 0) return 0;
+  hipLaunchKernelGGL((pair_block_kernel<105, 91, true, 128>), dim3(blocks), dim3(128), 0, st, tg, T, *md, pairs,
+                     (const unsigned int *)nullptr, count, row_lo, row_hi, tiles, npairs);
+  return (int)hipGetLastError();
+}
+
+int bem_launch_gather(const double *tiles, size_t npairs, int T, int row_lo, const RwgRec *rw, int r_lo, int r_hi,
+                      int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0, int add,
+                      double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st) {
+  if (r_hi <= r_lo || c_hi <= c_lo) return 0;
+  dim3 grid((r_hi - r_lo + 127) / 128, c_hi - c_lo);
+  hipLaunchKernelGGL(gather_kernel, grid, dim3(128), 0, st, tiles, npairs, T, row_lo, rw, r_lo, r_hi, c_lo, c_hi,
+                     *md, Z, ldz, zr0, zc0, add, Zcopy, ldc, cr0, cc0);
+  return (int)hipGetLastError();
+}
+
+}  // extern "C"

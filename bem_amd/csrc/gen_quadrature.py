@@ -70,16 +70,20 @@ def main():
     for n in TRI_RULES:
         out.append("#define BEM_QUAD_OFF_%d %d\n" % (n, offs))
         offs += n
-    out.append("\nstatic const double bem_quad_abw[BEM_QUAD_TOTAL_POINTS][3] = {\n")
+    out.append("\n#define BEM_QUAD_ABW_INIT { \\\n")
     for n in TRI_RULES:
-        out.append("  // %d-point rule\n" % n)
+        out.append("  /* %d-point rule */ \\\n" % n)
         for a, b, w in rules[n]:
-            out.append("  {%r, %r, %r},\n" % (a, b, w))
-    out.append("};\n\n")
-    out.append("static const double bem_line15_tw[15][2] = {\n")
+            out.append("  {%r, %r, %r}, \\\n" % (a, b, w))
+    out.append("}\n\n")
+    out.append("#define BEM_LINE15_INIT { \\\n")
     for t, w in line15:
-        out.append("  {%r, %r},\n" % (t, w))
-    out.append("};\n")
+        out.append("  {%r, %r}, \\\n" % (t, w))
+    out.append("}\n\n")
+    out.append("#ifndef BEM_QUAD_NO_HOST_TABLES\n")
+    out.append("static const double bem_quad_abw[BEM_QUAD_TOTAL_POINTS][3] = BEM_QUAD_ABW_INIT;\n")
+    out.append("static const double bem_line15_tw[15][2] = BEM_LINE15_INIT;\n")
+    out.append("#endif\n")
     with open(OUT, "w") as f:
         f.write("".join(out))
     print("wrote", OUT)

@@ -1,0 +1,340 @@
+"""ctypes binding of libbem_hip.so (include/bem_hip.h).
+
+Names follow the reference's interface where one exists:
+  Context.zts(...)               Zts (src/Impedance.cpp:388-413)
+  Context.calc_heat_transfer(...) CalcHeatTransfer (src/HeatTransfer.cpp:8-59)
+  Context.sweep(...)             the per-frequency loop over FrequencySweep::f
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+BEM_OK = 0
+BEM_E_INVALID = -1
+BEM_E_DEVICE = -2
+BEM_E_NOMEM = -3
+BEM_E_SINGULAR = -4
+BEM_E_MESH = -5
+BEM_E_UNSUPPORTED = -6
+BEM_PREC_FP64 = 0
+BEM_PREC_MIXED = 1
+BEM_ASM_DEVICE_PTR = 1
+
+# src/util/SI.cpp:3-9
+SI_HBAR = 1.05457148e-34
+SI_KB = 1.3806503e-23
+SI_EV = 1.60217646e-19
+SI_C = 2.99792458e8
+
+
+class BemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"bem error {code}: {msg}")
+        self.code = code
+
+
+class _Cfg(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("lu_block", ctypes.c_int), ("hbm_budget", ctypes.c_size_t)]
+
+
+class _Req(ctypes.Structure):
+    _fields_ = [("eV", ctypes.c_double), ("a", ctypes.c_double), ("T1", ctypes.c_double), ("T2", ctypes.c_double),
+                ("gap", ctypes.c_double * 3), ("eps1", ctypes.c_double * 2), ("mu1", ctypes.c_double * 2),
+                ("eps2", ctypes.c_double * 2), ("mu2", ctypes.c_double * 2), ("precision", ctypes.c_int),
+                ("flags", ctypes.c_int)]
+
+
+class _Res(ctypes.Structure):
+    _fields_ = [("P", ctypes.c_double), ("ReS", ctypes.c_double), ("ImS", ctypes.c_double),
+                ("theta1", ctypes.c_double), ("theta2", ctypes.c_double), ("omega", ctypes.c_double),
+                ("rel_err_est", ctypes.c_double), ("iters", ctypes.c_int), ("status", ctypes.c_int),
+                ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double),
+                ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+
+def lib_path():
+    return os.path.join(HERE, "libbem_hip.so")
+
+
+_lib = None
+
+# every symbol include/bem_hip.h declares
+EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_device_bytes", "bem_body_upload",
+           "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
+           "bem_sweep", "bem_zgetrf", "bem_zgetrs", "bem_zgemm", "bem_device_alloc", "bem_device_free",
+           "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings"]
+
+
+def load_library():
+    """Load libbem_hip.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = lib_path()
+    if not os.path.exists(p):
+        raise RuntimeError(f"{p} is missing: build it with __graft_entry__.build() or make -C bem_amd/csrc")
+    L = ctypes.CDLL(p)
+    vp, i32, i64, dbl, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_size_t
+    P = ctypes.POINTER
+    L.bem_ctx_create.argtypes = [P(vp), P(_Cfg)]
+    L.bem_ctx_destroy.argtypes = [vp]
+    L.bem_ctx_destroy.restype = None
+    L.bem_last_error.argtypes = [vp]
+    L.bem_last_error.restype = ctypes.c_char_p
+    L.bem_ctx_device_bytes.argtypes = [vp]
+    L.bem_ctx_device_bytes.restype = sz
+    L.bem_body_upload.argtypes = [vp, vp, i32, vp, i32, P(vp)]
+    L.bem_body_edges.argtypes = [vp, vp, P(i32)]
+    L.bem_body_free.argtypes = [vp]
+    L.bem_body_free.restype = None
+    L.bem_assemble.argtypes = [vp, i32, vp, vp, vp, vp, dbl, vp, i64, i32]
+    L.bem_pair_histogram.argtypes = [vp, i32, vp, vp, vp]
+    L.bem_spectral_power.argtypes = [vp, vp, vp, P(_Req), P(_Res)]
+    L.bem_sweep.argtypes = [vp, vp, vp, P(_Req), i32, P(_Res)]
+    L.bem_zgetrf.argtypes = [vp, vp, i64, i32, vp]
+    L.bem_zgetrs.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32]
+    L.bem_zgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
+    L.bem_device_alloc.argtypes = [vp, sz, P(vp)]
+    L.bem_device_free.argtypes = [vp, vp]
+    L.bem_memcpy_h2d.argtypes = [vp, vp, vp, sz]
+    L.bem_memcpy_d2h.argtypes = [vp, vp, vp, sz]
+    L.bem_last_timings.argtypes = [vp, vp]
+    _lib = L
+    return L
+
+
+def omega_tilde(eV, a):
+    """Dimensionless frequency omega a / c (src/HeatTransfer.cpp:10-11)."""
+    return eV * SI_EV / SI_HBAR * a / SI_C
+
+
+def theta(omega, T):
+    """src/HeatTransfer.cpp:3-6 with the dimensionless omega; T = 0 gives 0."""
+    if T == 0.0:
+        return 0.0
+    return SI_HBAR * omega / (np.exp(SI_HBAR * omega / (SI_KB * T)) - 1)
+
+
+def lorentz_eps(eV, eps_inf=6.7, wLO=0.120, wTO=0.0983, gamma=0.00059):
+    """SiC-like Lorentz permittivity in eV used by the benchmark workloads."""
+    w = eV
+    return eps_inf * (w * w - wLO * wLO + 1j * gamma * w) / (w * w - wTO * wTO + 1j * gamma * w)
+
+
+@dataclass
+class Request:
+    eV: float
+    a: float = 1e-6
+    T1: float = 300.0
+    T2: float = 0.0
+    gap: tuple = (0.0, 0.0, 0.0)
+    eps1: complex = 1.0
+    mu1: complex = 1.0
+    eps2: complex = 1.0
+    mu2: complex = 1.0
+    precision: int = BEM_PREC_FP64
+
+    def to_c(self):
+        r = _Req()
+        r.eV, r.a, r.T1, r.T2 = self.eV, self.a, self.T1, self.T2
+        for i in range(3):
+            r.gap[i] = float(self.gap[i])
+        for name in ("eps1", "mu1", "eps2", "mu2"):
+            z = complex(getattr(self, name))
+            arr = getattr(r, name)
+            arr[0], arr[1] = z.real, z.imag
+        r.precision = self.precision
+        r.flags = 0
+        return r
+
+
+def _res_dict(r: _Res):
+    return {k: getattr(r, k) for k, _ in _Res._fields_}
+
+
+class Body:
+    def __init__(self, ctx, handle, mesh):
+        self.ctx = ctx
+        self.handle = handle
+        self.mesh = mesh
+
+    @property
+    def ne(self):
+        n = ctypes.c_int(0)
+        self.ctx._check(self.ctx.lib.bem_body_edges(self.handle, None, ctypes.byref(n)))
+        return n.value
+
+    def edges(self):
+        ne = self.ne
+        out = np.zeros((ne, 4), dtype=np.int32)
+        n = ctypes.c_int(0)
+        self.ctx._check(self.ctx.lib.bem_body_edges(self.handle, out.ctypes.data, ctypes.byref(n)))
+        return out
+
+    def free(self):
+        if self.handle:
+            self.ctx.lib.bem_body_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One HIP device context (bem_ctx)."""
+
+    def __init__(self, device=0, lu_block=0, hbm_budget=0):
+        self.lib = load_library()
+        cfg = _Cfg(device, lu_block, hbm_budget)
+        h = ctypes.c_void_p()
+        rc = self.lib.bem_ctx_create(ctypes.byref(h), ctypes.byref(cfg))
+        if rc != 0:
+            raise BemError(rc, "bem_ctx_create failed (no HIP device?)")
+        self.h = h
+
+    def _check(self, rc):
+        if rc != 0:
+            raise BemError(rc, self.lib.bem_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.bem_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_bytes(self):
+        return self.lib.bem_ctx_device_bytes(self.h)
+
+    def body(self, mesh) -> Body:
+        xyz = np.ascontiguousarray(mesh.xyz, dtype=np.float64)
+        tri = np.ascontiguousarray(mesh.tri, dtype=np.int32)
+        h = ctypes.c_void_p()
+        self._check(self.lib.bem_body_upload(self.h, xyz.ctypes.data, xyz.shape[0], tri.ctypes.data, tri.shape[0],
+                                             ctypes.byref(h)))
+        return Body(self, h, mesh)
+
+    def zts(self, bodies, shifts, eps, mu, omega) -> np.ndarray:
+        """Impedance matrix of the union of bodies (Zts), (2E, 2E) complex128 Fortran order."""
+        nb = len(bodies)
+        arr = (ctypes.c_void_p * nb)(*[b.handle for b in bodies])
+        sh = np.ascontiguousarray(np.asarray(shifts, dtype=np.float64).reshape(nb, 3))
+        E = sum(b.ne for b in bodies)
+        M = 2 * E
+        Z = np.zeros((M, M), dtype=np.complex128, order="F")
+        e = np.array([complex(eps).real, complex(eps).imag])
+        m = np.array([complex(mu).real, complex(mu).imag])
+        self._check(self.lib.bem_assemble(self.h, nb, arr, sh.ctypes.data, e.ctypes.data, m.ctypes.data,
+                                          float(omega), Z.ctypes.data, M, 0))
+        return Z
+
+    def pair_histogram(self, bodies, shifts):
+        nb = len(bodies)
+        arr = (ctypes.c_void_p * nb)(*[b.handle for b in bodies])
+        sh = np.ascontiguousarray(np.asarray(shifts, dtype=np.float64).reshape(nb, 3))
+        h = np.zeros(6, dtype=np.int64)
+        self._check(self.lib.bem_pair_histogram(self.h, nb, arr, sh.ctypes.data, h.ctypes.data))
+        return h
+
+    def calc_heat_transfer(self, b1: Body, b2: Body, req: Request) -> dict:
+        """CalcHeatTransfer: spectral power for one (frequency, gap) request."""
+        r = req.to_c()
+        out = _Res()
+        self._check(self.lib.bem_spectral_power(self.h, b1.handle, b2.handle, ctypes.byref(r), ctypes.byref(out)))
+        return _res_dict(out)
+
+    def sweep(self, b1: Body, b2: Body, reqs) -> list:
+        n = len(reqs)
+        arr = (_Req * n)(*[q.to_c() for q in reqs])
+        out = (_Res * n)()
+        rc = self.lib.bem_sweep(self.h, b1.handle, b2.handle, arr, n, out)
+        res = [_res_dict(out[i]) for i in range(n)]
+        if rc != 0:
+            raise BemError(rc, self.lib.bem_last_error(self.h).decode())
+        return res
+
+    def last_timings(self):
+        out = np.zeros(9)
+        self._check(self.lib.bem_last_timings(self.h, out.ctypes.data))
+        return dict(assemble_ms=out[0], factor_ms=out[4], products_ms=out[5], gemm_ms=out[6],
+                    gemm_launches=int(out[7]), gemm_flops=out[8])
+
+    # ---- dense kernels on host arrays (device round trip), for tests
+    def zgemm_host(self, A, B, C=None, alpha=1.0, beta=0.0):
+        A = np.asfortranarray(A, dtype=np.complex128)
+        B = np.asfortranarray(B, dtype=np.complex128)
+        m, k = A.shape
+        k2, n = B.shape
+        assert k == k2
+        C = np.zeros((m, n), dtype=np.complex128, order="F") if C is None else np.asfortranarray(C, dtype=np.complex128)
+        dA, dB, dC = self._dev(A), self._dev(B), self._dev(C)
+        try:
+            al = np.array([complex(alpha).real, complex(alpha).imag])
+            be = np.array([complex(beta).real, complex(beta).imag])
+            self._check(self.lib.bem_zgemm(self.h, m, n, k, al.ctypes.data, dA, m, dB, k, be.ctypes.data, dC, m))
+            out = np.zeros_like(C)
+            self._check(self.lib.bem_memcpy_d2h(self.h, out.ctypes.data, dC, out.nbytes))
+            return out
+        finally:
+            for p in (dA, dB, dC):
+                self.lib.bem_device_free(self.h, p)
+
+    def zgetrf_host(self, A):
+        """LU factors and interchanges of a square matrix (device round trip)."""
+        A = np.asfortranarray(A, dtype=np.complex128)
+        n = A.shape[0]
+        dA = self._dev(A)
+        dp = self._dev(np.zeros(n + 1, dtype=np.int32))
+        try:
+            rc = self.lib.bem_zgetrf(self.h, dA, n, n, dp)
+            LU = np.zeros_like(A)
+            piv = np.zeros(n + 1, dtype=np.int32)
+            self._check(self.lib.bem_memcpy_d2h(self.h, LU.ctypes.data, dA, LU.nbytes))
+            self._check(self.lib.bem_memcpy_d2h(self.h, piv.ctypes.data, dp, piv.nbytes))
+            self._check(rc)
+            return LU, piv[:n]
+        finally:
+            self.lib.bem_device_free(self.h, dA)
+            self.lib.bem_device_free(self.h, dp)
+
+    def zgesv_host(self, A, B):
+        A = np.asfortranarray(A, dtype=np.complex128)
+        B = np.asfortranarray(B, dtype=np.complex128)
+        n = A.shape[0]
+        nrhs = B.shape[1]
+        dA, dB = self._dev(A), self._dev(B)
+        dp = self._dev(np.zeros(n + 1, dtype=np.int32))
+        try:
+            self._check(self.lib.bem_zgetrf(self.h, dA, n, n, dp))
+            self._check(self.lib.bem_zgetrs(self.h, dA, n, n, dp, dB, n, nrhs))
+            X = np.zeros_like(B)
+            self._check(self.lib.bem_memcpy_d2h(self.h, X.ctypes.data, dB, X.nbytes))
+            return X
+        finally:
+            for p in (dA, dB, dp):
+                self.lib.bem_device_free(self.h, p)
+
+    def _dev(self, arr):
+        p = ctypes.c_void_p()
+        self._check(self.lib.bem_device_alloc(self.h, arr.nbytes, ctypes.byref(p)))
+        self._check(self.lib.bem_memcpy_h2d(self.h, p, arr.ctypes.data, arr.nbytes))
+        return p
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,178 @@
+"""Benchmark of the frequency-sweep hot path on one MI355X.
+
+Workload (default, "c2"): two icospheres of radius 1 (1280 triangles, 1920
+RWG edges each, 7680 complex128 unknowns), centres 2.2 apart, scale
+a = 1e-6 m, SiC-like Lorentz permittivity, T1 = 300 K, T2 = 0 K, frequencies
+on the 64-point grid of [0.08, 0.14] eV.  One step = one spectral-power solve
+(assembly of Z_ext and both Z_int, block LU solve, products and trace) at one
+frequency of the grid.  Inputs (the meshes) are resident on the device side
+before the timed region; every call returns after its stream has drained.
+
+Prints one JSON line (metric = spectral-power solves/s).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def workload(name):
+    from bem_amd import mesh
+    if name == "c2":
+        s = mesh.icosphere(3)
+        return dict(name="c2: icosphere pair 2x1280 tri, M=7680, centre gap 2.2, a=1e-6 m",
+                    m1=s, m2=s, gap=(2.2, 0.0, 0.0), freqs=np.linspace(0.08, 0.14, 64))
+    if name == "c1":
+        c = mesh.cube(2)
+        return dict(name="c1: cube pair 2x48 tri, M=288, gap 0.1, a=1e-6 m",
+                    m1=c, m2=c, gap=(1.1, 0.0, 0.0), freqs=np.linspace(0.09, 0.13, 16))
+    raise SystemExit(f"unknown workload {name}")
+
+
+def pick_freqs(freqs, n, offset=0):
+    idx = (offset + np.arange(n) * max(1, len(freqs) // max(n, 1))) % len(freqs)
+    return [float(freqs[i]) for i in idx]
+
+
+def cpu_baseline(wl, eV, budget_rows=None):
+    """Oracle (C restatement, all host threads via OpenMP) timed on a bounded
+    sample: assembly of a stripe of test-triangle rows of the three impedance
+    matrices (scaled by rows), plus the reference's dense algorithm
+    (LAPACK inverse + products + trace, numpy on all host threads) at full size."""
+    from oracle import oracle as O
+    m1 = wl["m1"].translated(wl["gap"])
+    m2 = wl["m2"]
+    a = 1e-6
+    om = O.omega_tilde(eV, a)
+    eps = O.lorentz_eps(eV)
+    top, t1, t2 = O.Tree([m1, m2]), O.Tree([m1]), O.Tree([m2])
+    T = top.tri.shape[0]
+    rows = budget_rows or max(1, T // 16)
+    t0 = time.perf_counter()
+    top.zts(1.0, 1.0, om, ta_range=(0, rows))
+    t_top = (time.perf_counter() - t0) * T / rows
+    T1 = t1.tri.shape[0]
+    r1 = max(1, rows // 2)
+    t0 = time.perf_counter()
+    t1.zts(eps, 1.0, om, ta_range=(0, r1))
+    t_int = 2 * (time.perf_counter() - t0) * T1 / r1
+    # dense part at full size on synthetic matrices of the right shape
+    M = 2 * top.ne
+    mA = 2 * t1.ne
+    rng = np.random.default_rng(0)
+    W = (rng.normal(size=(M, M)) + 1j * rng.normal(size=(M, M))) + M * np.eye(M)
+    G1 = rng.normal(size=(mA, mA)) + 1j * rng.normal(size=(mA, mA))
+    G2 = rng.normal(size=(M - mA, M - mA)) + 1j * rng.normal(size=(M - mA, M - mA))
+    t0 = time.perf_counter()
+    O.heat_transfer_from_blocks(W, G1, G2)
+    t_solve = time.perf_counter() - t0
+    total = t_top + t_int + t_solve
+    threads = int(O.lib().oracle_num_threads())
+    return dict(value=1.0 / total, unit="solves/s", cores=threads, kind="port",
+                sample=(f"one solve at {eV:.4f} eV: assembly timed on {rows}/{T} exterior and {r1}/{T1} interior "
+                        f"test-triangle rows (scaled linearly), dense inverse+products+trace at full M={M}"),
+                seconds=dict(assembly_ext=t_top, assembly_int=t_int, dense=t_solve))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c2")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--check", action="store_true", help="compare against stored oracle values")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); "
+              "independent (frequency, gap) solves would shard across GPUs by request, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import bem_amd
+    from bem_amd import Request
+
+    wl = workload(args.workload)
+    ctx = bem_amd.Context(0)
+    b1, b2 = ctx.body(wl["m1"]), ctx.body(wl["m2"])
+    freqs_w = pick_freqs(wl["freqs"], args.warmup, offset=1)
+    freqs = pick_freqs(wl["freqs"], args.steps)
+    reqs = [Request(eV=f, gap=wl["gap"], eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f)) for f in freqs]
+    for f in freqs_w:
+        ctx.calc_heat_transfer(b1, b2, Request(eV=f, gap=wl["gap"], eps1=bem_amd.lorentz_eps(f),
+                                               eps2=bem_amd.lorentz_eps(f)))
+    lat, results = [], []
+    gemm_ms = gemm_flops = 0.0
+    gemm_n = 0
+    phase = dict(assemble=0.0, factor=0.0, products=0.0)
+    t_start = time.perf_counter()
+    for q in reqs:
+        t0 = time.perf_counter()
+        r = ctx.calc_heat_transfer(b1, b2, q)
+        lat.append((time.perf_counter() - t0) * 1e3)
+        results.append(r)
+        tm = ctx.last_timings()
+        gemm_ms += tm["gemm_ms"]
+        gemm_flops += tm["gemm_flops"]
+        gemm_n += tm["gemm_launches"]
+        phase["assemble"] += r["ms_assemble"]
+        phase["factor"] += r["ms_factor"]
+        phase["products"] += r["ms_products"]
+    elapsed = time.perf_counter() - t_start
+    K = len(reqs)
+    value = K / elapsed
+    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    out = {
+        "metric": "spectral-power solves/s",
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / K,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": wl["name"], "precision": "fp64 (complex128 throughout)",
+                   "freqs_eV": [round(f, 6) for f in freqs], "T1": 300.0, "T2": 0.0,
+                   "permittivity": "6.7(w^2-0.120^2+i0.00059w)/(w^2-0.0983^2+i0.00059w)"},
+        "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99))},
+        "phases_ms_per_solve": {k: v / K for k, v in phase.items()},
+        "roofline": {"bound": "mfma", "kernel": "zgemm_kernel (FP64 MFMA complex GEMM)",
+                     "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "launches": gemm_n, "flops": gemm_flops, "kernel_ms": gemm_ms},
+        "results": [{"eV": q.eV, "P": r["P"], "ReS": r["ReS"]} for q, r in zip(reqs, results)],
+    }
+    if args.check:
+        ref_path = os.path.join(ROOT, "tests", "golden", f"{args.workload}_oracle.json")
+        if os.path.exists(ref_path):
+            ref = json.load(open(ref_path))
+            errs = []
+            for q, r in zip(reqs, results):
+                key = f"{q.eV:.6f}"
+                if key in ref:
+                    errs.append(abs(r["ReS"] - ref[key]["ReS"]) / abs(ref[key]["ReS"]))
+            out["rel_err_vs_fp64_oracle"] = max(errs) if errs else None
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, freqs[0])
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

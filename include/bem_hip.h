@@ -1,0 +1,140 @@
+/*
+ * bem_hip.h -- C ABI of libbem_hip.so, the MI355X (gfx950) implementation of
+ * the frequency-sweep hot path of the boundary-element heat-transfer code:
+ * RWG/PMCHWT impedance assembly, the dense complex solve and the trace that
+ * yields spectral power.
+ *
+ * Plain C types only (no HIP or torch types): host pointers unless stated,
+ * complex numbers as (re, im) double pairs, matrices column-major complex128
+ * interleaved with an explicit leading dimension.  Every function returns 0
+ * or a negative BEM_E_* code; bem_last_error() gives the message.  A context
+ * owns its device buffers; calls on one context are serialised internally.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   bem_assemble        Zts(RWGTree*, RWGTree*, GreensFunction*, Options&)
+ *                       src/Impedance.h:21, src/Impedance.cpp:388-413, and
+ *                       Zdense src/Impedance.h:20 (external + shared medium)
+ *   bem_spectral_power  Real CalcHeatTransfer(Real a, Real eV, Real T1, Real T2,
+ *                       RWGTree *top, RWGTree *s, RWGTree *t, Options&)
+ *                       src/HeatTransfer.h:33, src/HeatTransfer.cpp:8-59
+ *   bem_sweep           the per-frequency loop a driver runs over
+ *                       FrequencySweep::f (src/FrequencySweep.h:9-13) /
+ *                       the job farm of src/transfer-mpi.cpp:114-179
+ *   bem_body_upload     mesh -> RWG construction of ShapeMesh + RWG
+ *                       (src/ShapeMesh.cpp:821-947, src/RWG.cpp:5-39), with
+ *                       the reference's edge order and orientation
+ *   bem_zgetrf / bem_zgetrs / bem_zgemm
+ *                       the LAPACK/BLAS FFI the reference binds
+ *                       (zgetrf_/zgetri_ src/util/DenseMatrix.cpp:38-77,
+ *                       zgemm_ src/util/MatrixOperations.cpp:70-74), on
+ *                       device pointers
+ */
+#ifndef BEM_HIP_H
+#define BEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BEM_OK 0
+#define BEM_E_INVALID (-1)     /* bad argument */
+#define BEM_E_DEVICE (-2)      /* HIP runtime error */
+#define BEM_E_NOMEM (-3)       /* device allocation failed / over budget */
+#define BEM_E_SINGULAR (-4)    /* exactly singular pivot met */
+#define BEM_E_MESH (-5)        /* mesh is not a closed 2-manifold */
+#define BEM_E_UNSUPPORTED (-6) /* size or mode not supported */
+
+#define BEM_PREC_FP64 0        /* complex128 factorisation */
+#define BEM_PREC_MIXED 1       /* complex64 factorisation + FP64 refinement */
+
+typedef struct bem_ctx bem_ctx;
+typedef struct bem_body bem_body;
+
+typedef struct {
+  int device;        /* HIP device ordinal */
+  int lu_block;      /* LU panel width (0 = default 64, max 64) */
+  size_t hbm_budget; /* bytes a context may hold, 0 = no limit */
+} bem_cfg;
+
+typedef struct {
+  double eV;         /* photon energy (eV) */
+  double a;          /* metres per geometry unit (x3d scale) */
+  double T1, T2;     /* body temperatures (K) */
+  double gap[3];     /* translation applied to body 1 (geometry units) */
+  double eps1[2], mu1[2];  /* body 1 medium (re, im) at eV */
+  double eps2[2], mu2[2];  /* body 2 medium (re, im) at eV */
+  int precision;     /* BEM_PREC_* */
+  int flags;         /* reserved, 0 */
+} bem_request;
+
+typedef struct {
+  double P;          /* Re(S) (Theta(omega,T1) - Theta(omega,T2)) */
+  double ReS, ImS;   /* S = 1/4 tr(B2^H B1) */
+  double theta1, theta2;
+  double omega;      /* dimensionless omega a / c */
+  double rel_err_est;/* refinement residual estimate (mixed), 0 for FP64 */
+  int iters;         /* refinement iterations (mixed), 0 for FP64 */
+  int status;        /* 0 or BEM_E_* for this request */
+  double ms_assemble, ms_factor, ms_products, ms_total;  /* device-event times */
+} bem_result;
+
+/* ---- context ---------------------------------------------------------- */
+int bem_ctx_create(bem_ctx **ctx, const bem_cfg *cfg);
+void bem_ctx_destroy(bem_ctx *ctx);
+const char *bem_last_error(bem_ctx *ctx);
+/* Bytes of device memory currently held by the context. */
+size_t bem_ctx_device_bytes(bem_ctx *ctx);
+
+/* ---- geometry --------------------------------------------------------- */
+/* Upload a closed triangle mesh (xyz: nv x 3, tri: nt x 3 0-based).  Builds
+ * the RWG table in the reference's order: edges in creation order while
+ * scanning triangles, positive half = creating triangle. */
+int bem_body_upload(bem_ctx *ctx, const double *xyz, int nv, const int32_t *tri, int nt, bem_body **out);
+/* Number of RWGs (edges) of a body; copies the edge table {p1, p2, t1, t2}
+ * (ne x 4) when edges != NULL. */
+int bem_body_edges(bem_body *b, int32_t *edges, int *ne);
+void bem_body_free(bem_body *b);
+
+/* ---- assembly (Zts) --------------------------------------------------- */
+/* Impedance matrix of the union of nb (1 or 2) bodies, body k translated by
+ * shifts[3k..3k+2], in medium (eps, mu) at dimensionless omega: 2E x 2E,
+ * E = total RWGs, rows/cols [J_0, M_0, J_1, M_1, ...], body 0 first.
+ * Z is a host pointer (ldz >= 2E) unless flags & BEM_ASM_DEVICE_PTR. */
+#define BEM_ASM_DEVICE_PTR 1
+int bem_assemble(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *shifts, const double *eps,
+                 const double *mu, double omega, void *Z, int64_t ldz, int flags);
+/* Pair-class histogram of the same union: {close, g6, g10, g15, g21, g36}. */
+int bem_pair_histogram(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *shifts, int64_t *hist6);
+
+/* ---- spectral power (CalcHeatTransfer) -------------------------------- */
+int bem_spectral_power(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *req, bem_result *res);
+/* n independent requests on the same body pair, results in request order. */
+int bem_sweep(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *reqs, int n, bem_result *res);
+
+/* ---- dense complex128 kernels on device pointers ---------------------- */
+/* A (n x n, lda) <- P L U in place; ipiv (device, n ints, 0-based
+ * interchanges row i <-> ipiv[i]).  Returns BEM_E_SINGULAR on a zero pivot. */
+int bem_zgetrf(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv);
+/* B (n x nrhs, ldb) <- A^-1 B from the factors of bem_zgetrf. */
+int bem_zgetrs(bem_ctx *ctx, const void *A, int64_t lda, int n, const int32_t *ipiv, void *B, int64_t ldb, int nrhs);
+/* C <- alpha A B + beta C (no transposes), alpha/beta as (re, im). */
+int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void *A, int64_t lda, const void *B,
+              int64_t ldb, const double *beta, void *C, int64_t ldc);
+/* Device memory helpers for callers without their own allocator. */
+int bem_device_alloc(bem_ctx *ctx, size_t bytes, void **ptr);
+int bem_device_free(bem_ctx *ctx, void *ptr);
+int bem_memcpy_h2d(bem_ctx *ctx, void *dst, const void *src, size_t bytes);
+int bem_memcpy_d2h(bem_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Elapsed device time (ms) of the kernels launched by the last call, by kind:
+ * out[0] assembly, out[1] far-g6 kernel, out[2] close kernel, out[3] gather,
+ * out[4] factor+solves, out[5] products+trace, out[6] zgemm total,
+ * out[7] number of zgemm launches, out[8] zgemm flops (real). */
+int bem_last_timings(bem_ctx *ctx, double *out9);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

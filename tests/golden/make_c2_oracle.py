@@ -1,0 +1,37 @@
+"""Spectral power of the 1280-triangle icosphere pair (the benchmark workload)
+computed by the oracle (C restatement of the reference assembly, pinned to the
+reference's own Zts by the golden fixtures, plus LAPACK for the dense part) at
+four frequencies of its 64-point grid.  Stored so the GPU benchmark can report
+its error against the FP64 oracle without a minutes-long CPU run.
+Usage: python tests/golden/make_c2_oracle.py  (about a minute per frequency)
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from bem_amd import mesh  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+
+
+def main():
+    s = mesh.icosphere(3)
+    m1 = s.translated((2.2, 0.0, 0.0))
+    grid = np.linspace(0.08, 0.14, 64)
+    out = {}
+    for i in (0, 16, 32, 48):
+        eV = float(grid[i])
+        eps = O.lorentz_eps(eV)
+        r = O.spectral_power(m1, s, eV, 1e-6, 300.0, 0.0, eps, eps, return_parts=True)
+        out[f"{eV:.6f}"] = {"ReS": float(r["ReS"]), "ImS": float(r["S"].imag), "P": float(r["P"]),
+                            "theta1": float(r["theta1"]), "omega": float(r["omega"])}
+        print(eV, out[f"{eV:.6f}"], flush=True)
+    with open(os.path.join(ROOT, "tests", "golden", "c2_oracle.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,157 @@
+"""The oracle (CPU restatement) pinned against the reference's own Zts
+(golden fixtures made by tests/golden/make_golden.py from oracle/_ref) and
+against mathematics: closed forms, brute-force quadrature, invariants."""
+import numpy as np
+import pytest
+
+from conftest import golden_meshes, load_golden
+from oracle import oracle as O
+
+CASES = ["c1_cubes_0p11", "c1_cubes_0p09", "sphere1_cube1_gap0p05"]
+
+
+def nerr(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_edges_match_reference(name):
+    g = load_golden(name)
+    for b, m in enumerate(golden_meshes(g)):
+        assert np.array_equal(O.mesh_edges(m.tri, m.nv), g[f"edges_b{b}"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_zts_matches_reference(name):
+    g = load_golden(name)
+    ms = golden_meshes(g)
+    om = float(g["omega"])
+    W = O.Tree(ms).zts(1.0, 1.0, om)
+    assert nerr(W, g["W"]) < 1e-12
+    for b, m in enumerate(ms):
+        G = O.Tree([m]).zts(complex(g[f"eps_b{b}"]), complex(g[f"mu_b{b}"]), om)
+        assert nerr(G, g[f"G{b + 1}"]) < 1e-12
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_heat_transfer_composition(name):
+    g = load_golden(name)
+    S = O.heat_transfer_from_blocks(g["W"], g["G1"], g["G2"])
+    assert abs(S.real - float(g["ReS"])) <= 1e-12 * abs(float(g["ReS"]))
+
+
+def test_pair_histogram_matches_fixture():
+    g = load_golden("sphere1_cube1_gap0p05")
+    h = O.Tree(golden_meshes(g)).pair_histogram()
+    assert np.array_equal(h, g["hist"])
+    # the 10/15/21-point far tiers are unreachable (decreasing thresholds)
+    assert h[2] == h[3] == h[4] == 0
+
+
+def test_green_closed_form():
+    rng = np.random.default_rng(1)
+    eps, mu, om = -6.3 + 0.35j, 1.0, 0.55
+    k = np.sqrt(eps * mu * om * om)
+    out = np.zeros(8)
+    for _ in range(20):
+        rt, rs = rng.normal(size=3), rng.normal(size=3)
+        O.lib().oracle_green(eps.real, eps.imag, 1.0, 0.0, om, rt, rs, 0, out)
+        d = rt - rs
+        R = np.linalg.norm(d)
+        gref = np.exp(1j * k * R) / R
+        gg = gref * (1 - 1j * k * R) / R**2 * d   # grad wrt rt is -(1-ikR)... (sign convention of the reference)
+        got = out[0::2] + 1j * out[1::2]
+        assert abs(got[0] - gref) < 1e-13 * abs(gref)
+        assert np.abs(got[1:] - gg).max() < 1e-13 * np.abs(gg).max()
+        # smooth variant: G - 1/R + k^2 R / 2
+        O.lib().oracle_green(eps.real, eps.imag, 1.0, 0.0, om, rt, rs, 1, out)
+        got = out[0::2] + 1j * out[1::2]
+        sref = (np.exp(1j * k * R) - 1) / R + 0.5 * k * k * R
+        assert abs(got[0] - sref) < 1e-12 * abs(sref)
+
+
+def _brute_triangle(f, p0, p1, p2, n=400):
+    """Duffy-collapsed Gauss-Legendre on the triangle, fine enough for smooth-ish f."""
+    x, w = np.polynomial.legendre.leggauss(n)
+    x = 0.5 * (x + 1)
+    w = 0.5 * w
+    U, V = np.meshgrid(x, x, indexing="ij")
+    WW = np.outer(w, w)
+    # (u, v) in [0,1]^2 -> barycentric (u(1-v), uv)
+    a = U * (1 - V)
+    b = U * V
+    jac = U
+    pts = p0[None, None, :] + a[..., None] * (p1 - p0)[None, None, :] + b[..., None] * (p2 - p0)[None, None, :]
+    area2 = np.linalg.norm(np.cross(p1 - p0, p2 - p0))
+    return np.sum((WW * jac)[..., None] * f(pts).reshape(n, n, -1), axis=(0, 1)) * area2
+
+
+@pytest.mark.parametrize("where", ["above", "inplane_out", "inside"])
+def test_eh_nonsmooth_brute_force(where):
+    """ISn1 = int 1/R, IS1 = int R over the source triangle (analytic forms of
+    src/Source.cpp:16-86) against direct quadrature of the integrals."""
+    p = np.array([[0.1, 0.2, 0.0], [1.2, 0.1, 0.05], [0.3, 1.1, -0.02]])
+    c = p.mean(axis=0)
+    n = np.cross(p[1] - p[0], p[2] - p[0])
+    n /= np.linalg.norm(n)
+    if where == "above":
+        rt = c + 0.3 * n + np.array([0.05, -0.02, 0.0])
+    elif where == "inplane_out":
+        rt = p[1] + 0.7 * (p[1] - p[2])
+    else:
+        rt = c + 0.1 * (p[0] - c)
+    out = np.zeros(12)
+    O.lib().oracle_eh_nonsmooth_tri(np.ascontiguousarray(p.reshape(-1)), 1, 2, rt, out)
+    ISn1, IS1 = out[1], out[2]
+    if where == "inside":
+        # singular integrand: split into 3 sub-triangles at rt, each with the
+        # singularity at a vertex (Duffy removes it)
+        def sub(a, b):
+            return _brute_triangle(lambda q: np.stack([1 / np.linalg.norm(q - rt, axis=-1),
+                                                       np.linalg.norm(q - rt, axis=-1)], -1), rt, a, b, 200)
+        tot = sub(p[0], p[1]) + sub(p[1], p[2]) + sub(p[2], p[0])
+    else:
+        tot = _brute_triangle(lambda q: np.stack([1 / np.linalg.norm(q - rt, axis=-1),
+                                                  np.linalg.norm(q - rt, axis=-1)], -1), p[0], p[1], p[2])
+    assert abs(ISn1 - tot[0]) < 1e-9 * abs(tot[0])
+    assert abs(IS1 - tot[1]) < 1e-9 * abs(tot[1])
+
+
+def test_quadrature_weight_sums():
+    import re
+    txt = open(O.HERE + "/../bem_amd/csrc/quadrature_tables.h").read()
+    body = txt.split("#define BEM_QUAD_ABW_INIT")[1].split("#define BEM_LINE15_INIT")[0]
+    tri = np.array([[float(v) for v in grp.split(",")] for grp in re.findall(r"\{([^{}]*)\}", body)])
+    offs = [0, 6, 16, 31, 52, 88, 179, 284]
+    for a, b in zip(offs[:-1], offs[1:]):
+        assert abs(tri[a:b, 2].sum() - 1.0) < 1e-12
+    line = txt.split("#define BEM_LINE15_INIT")[1].split("#ifndef")[0]
+    lv = np.array([[float(v) for v in grp.split(",")] for grp in re.findall(r"\{([^{}]*)\}", line)])
+    assert abs(lv[:, 1].sum() - 2.0) < 1e-14       # the reference's line rule sums to 2
+    # 6-point rule integrates quadratics exactly on the unit triangle
+    a, b, w = tri[0:6].T
+    assert abs(np.sum(w * a * a) - 1 / 6) < 1e-12 and abs(np.sum(w * a * b) - 1 / 12) < 1e-12
+
+
+def test_spectral_power_invariants():
+    """P(T1,T2) = -P(T2,T1); vertex renumbering (same triangles, same vertex
+    cycles) leaves Re S unchanged to rounding; a full renumbering of triangles
+    changes which half-RWG frames the (not fully symmetric) quadrature rules use,
+    so Re S then agrees only to the discretisation error."""
+    from bem_amd import mesh
+    c = mesh.cube(1)
+    b1 = c.translated((1.2, 0, 0))
+    eps = O.lorentz_eps(0.11)
+    p12 = O.spectral_power(b1, c, 0.11, 1e-6, 300, 10, eps, eps)
+    p21 = O.spectral_power(b1, c, 0.11, 1e-6, 10, 300, eps, eps)
+    assert abs(p12 + p21) <= 1e-12 * abs(p12)
+    rng = np.random.default_rng(3)
+    perm = rng.permutation(c.nv)
+    inv = np.argsort(perm)
+    c_v = mesh.Mesh(c.xyz[perm], inv[c.tri], "vperm")
+    c_t = mesh.Mesh(c.xyz[perm], inv[c.tri][rng.permutation(c.nt)], "tperm")
+    r0 = O.spectral_power(b1, c, 0.11, 1e-6, 300, 0, eps, eps, return_parts=True)
+    r1 = O.spectral_power(b1, c_v, 0.11, 1e-6, 300, 0, eps, eps, return_parts=True)
+    r2 = O.spectral_power(b1, c_t, 0.11, 1e-6, 300, 0, eps, eps, return_parts=True)
+    assert abs(r0["ReS"] - r1["ReS"]) <= 1e-12 * abs(r0["ReS"])
+    assert abs(r0["ReS"] - r2["ReS"]) <= 1e-3 * abs(r0["ReS"])
