@@ -141,13 +141,14 @@ __device__ __forceinline__ void inner_point(const Medium &md, const double *r, c
 __device__ __forceinline__ void outer_far(const Medium &md, const double *r, double w, const Inner &m,
                                           const double (*p0t)[3], int nt, const double (*pbs)[3], int ns,
                                           double *Lr, double *Li, double *Kr, double *Ki) {
+#pragma unroll
   for (int i = 0; i < 3; i++) {
-    if (i >= nt) break;
     double phi[3] = {0.5 * (r[0] - p0t[i][0]), 0.5 * (r[1] - p0t[i][1]), 0.5 * (r[2] - p0t[i][2])};
     C2 a = cmk(dot3(phi, m.G1r), dot3(phi, m.G1i));
     C2 c = cmk(dot3(phi, m.D1r), dot3(phi, m.D1i));
+#pragma unroll
     for (int j = 0; j < 3; j++) {
-      if (j >= ns) break;
+      if (i >= nt || j >= ns) continue;
       double b = dot3(phi, pbs[j]);
       double e[3];
       cross3(pbs[j], phi, e);
@@ -165,7 +166,7 @@ __device__ __forceinline__ void outer_far(const Medium &md, const double *r, dou
 }
 
 // EHNonSmoothIntegral (src/Source.cpp:16-86) on the frame half of triangle S.
-__device__ void eh_nonsmooth(const TriGeom &S, const double *rt, double &ISn3h, double &ISn1, double &IS1,
+__device__ __forceinline__ void eh_nonsmooth(const TriGeom &S, const double *rt, double &ISn3h, double &ISn1, double &IS1,
                              double *IL1, double *IL3) {
   double tol = 1e-6 * S.elen;
   double p0[3], p1[3], p2[3];
@@ -217,7 +218,7 @@ __device__ void eh_nonsmooth(const TriGeom &S, const double *rt, double &ISn3h, 
 }
 
 // LKEdgeIntegral (src/Impedance.cpp:13-73) on the frame half of triangle T.
-__device__ void lk_edge(const TriGeom &T, const double *rs, double &ISn1, double *IL1) {
+__device__ __forceinline__ void lk_edge(const TriGeom &T, const double *rs, double &ISn1, double *IL1) {
   double p0[3], p1[3], p2[3];
   for (int c = 0; c < 3; c++) { p0[c] = rs[c] - T.p0[c]; p1[c] = rs[c] - T.p1[c]; p2[c] = rs[c] - T.p2[c]; }
   double R02 = dot3(p0, p0), R0 = sqrt(R02);
@@ -261,7 +262,7 @@ __device__ void lk_edge(const TriGeom &T, const double *rs, double &ISn1, double
 }
 
 // Close-pair outer step (src/Impedance.cpp:194-250) for one outer point v.
-__device__ void outer_close(const Medium &md, const TriGeom &S, const double *v, double w, const Inner &m,
+__device__ __forceinline__ void outer_close(const Medium &md, const TriGeom &S, const double *v, double w, const Inner &m,
                             const double (*p0t)[3], int nt, const double (*p0s)[3], const double (*pbs)[3], int ns,
                             double *Lr, double *Li, double *Kr, double *Ki) {
   double ISn3h, ISn1, IS1, IL1[3], IL3[3];
@@ -288,14 +289,15 @@ __device__ void outer_close(const Medium &md, const TriGeom &S, const double *v,
   double s3 = Ai2 * ISn3h;
   double ISn3hn[3] = {s3 * S.n[0], s3 * S.n[1], s3 * S.n[2]};
   const double *base = coplanar ? v : rho;
+#pragma unroll
   for (int i = 0; i < 3; i++) {
-    if (i >= nt) break;
     double phi[3] = {0.5 * (v[0] - p0t[i][0]), 0.5 * (v[1] - p0t[i][1]), 0.5 * (v[2] - p0t[i][2])};
     C2 a = cmk(dot3(phi, m.G1r), dot3(phi, m.G1i));
     C2 c = cmk(dot3(phi, m.D1r), dot3(phi, m.D1i));
     C2 dIL13 = cmk(dot3(phi, IL13r), dot3(phi, IL13i));
+#pragma unroll
     for (int j = 0; j < 3; j++) {
-      if (j >= ns) break;
+      if (i >= nt || j >= ns) continue;
       double b = dot3(phi, pbs[j]);
       C2 gp = cmk(0.5 * (a.re - b * m.G0.re), 0.5 * (a.im - b * m.G0.im));
       double bj[3] = {base[0] - p0s[j][0], base[1] - p0s[j][1], base[2] - p0s[j][2]};
@@ -431,24 +433,46 @@ __global__ void __launch_bounds__(256) far_g6_kernel(const TriGeom *__restrict__
 
 // Block reduction of 36 accumulators; result valid in thread 0.
 template <int NT>
-__device__ __forceinline__ void block_reduce36(double *acc, double *red /* [NT/64][36] */) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+__device__ __forceinline__ void wave_reduce9(double (&v)[9]) {
 #pragma unroll
-  for (int q = 0; q < 36; q++) {
-    double x = acc[q];
+  for (int q = 0; q < 9; q++) {
+    double x = v[q];
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) x += __shfl_down(x, off, 64);
-    acc[q] = x;
+    v[q] = x;
   }
+}
+
+template <int NT>
+__device__ __forceinline__ void block_reduce36(double (&Lr)[9], double (&Li)[9], double (&Kr)[9], double (&Ki)[9],
+                                               double *red /* [NT/64][36] */) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  wave_reduce9<NT>(Lr);
+  wave_reduce9<NT>(Li);
+  wave_reduce9<NT>(Kr);
+  wave_reduce9<NT>(Ki);
   if (NT > 64) {
-    if (lane == 0)
-      for (int q = 0; q < 36; q++) red[wv * 36 + q] = acc[q];
+    if (lane == 0) {
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        red[wv * 36 + q] = Lr[q];
+        red[wv * 36 + 9 + q] = Li[q];
+        red[wv * 36 + 18 + q] = Kr[q];
+        red[wv * 36 + 27 + q] = Ki[q];
+      }
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int q = 0; q < 36; q++) {
-        double s = red[q];
-        for (int w2 = 1; w2 < NT / 64; w2++) s += red[w2 * 36 + q];
-        acc[q] = s;
+#pragma unroll
+      for (int q = 0; q < 9; q++) {
+        double a = red[q], b = red[9 + q], c = red[18 + q], d = red[27 + q];
+        for (int w2 = 1; w2 < NT / 64; w2++) {
+          a += red[w2 * 36 + q];
+          b += red[w2 * 36 + 9 + q];
+          c += red[w2 * 36 + 18 + q];
+          d += red[w2 * 36 + 27 + q];
+        }
+        Lr[q] = a; Li[q] = b; Kr[q] = c; Ki[q] = d;
       }
     }
   }
@@ -489,10 +513,9 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
       s_spb[i][c] = B.hp0[i][c] - B.p0[c];
     }
     __syncthreads();
-    double acc[36];
+    double Lr[9], Li[9], Kr[9], Ki[9];
 #pragma unroll
-    for (int q = 0; q < 36; q++) acc[q] = 0.0;
-    double *Lr = acc, *Li = acc + 9, *Kr = acc + 18, *Ki = acc + 27;
+    for (int q = 0; q < 9; q++) Lr[q] = Li[q] = Kr[q] = Ki[q] = 0.0;
     int nt = A.nh, ns = B.nh;
     for (int o = threadIdx.x; o < NOUT; o += NT) {
       double r[3];
@@ -526,12 +549,13 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
         double h = dot3(vp, A.n);
         double rho[3] = {v[0] - h * A.n[0], v[1] - h * A.n[1], v[2] - h * A.n[2]};
         double cc = 0.125 * A.Ai * B.Ai;
+#pragma unroll
         for (int i = 0; i < 3; i++) {
-          if (i >= nt) break;
           double kv[3];
           for (int c = 0; c < 3; c++) kv[c] = wq * (IL1[c] + ISn1 * (rho[c] - s_tp0[i][c]));
+#pragma unroll
           for (int j = 0; j < 3; j++) {
-            if (j >= ns) break;
+            if (i >= nt || j >= ns) continue;
             double vv[3] = {s_tp0[i][0] - s_sp0[j][0], s_tp0[i][1] - s_sp0[j][1], s_tp0[i][2] - s_sp0[j][2]};
             double vm[3];
             cross3(vv, me, vm);
@@ -540,13 +564,8 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
         }
       }
     }
-    block_reduce36<NT>(acc, s_red);
-    if (threadIdx.x == 0) {
-      // acc layout Lr(9) Li(9) Kr(9) Ki(9) -> planes (L re/im interleaved, then K)
-      double Lrr[9], Lii[9], Krr[9], Kii[9];
-      for (int q = 0; q < 9; q++) { Lrr[q] = acc[q]; Lii[q] = acc[9 + q]; Krr[q] = acc[18 + q]; Kii[q] = acc[27 + q]; }
-      store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)T + (size_t)b, Lrr, Lii, Krr, Kii);
-    }
+    block_reduce36<NT>(Lr, Li, Kr, Ki, s_red);
+    if (threadIdx.x == 0) store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)T + (size_t)b, Lr, Li, Kr, Ki);
   }
 }
 

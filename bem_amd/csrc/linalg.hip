@@ -101,7 +101,8 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
         }
     }
   }
-  // epilogue: D[i][j] of a 16x16 block sits in lane j + 16*(i/4), element i%4
+  // epilogue: for v_mfma_f64_16x16x4f64, D[i][j] of a 16x16 block sits in
+  // lane j + 16*(i%4), element i/4 (measured on gfx950; the f32 variant differs)
   if (!TRACE) {
 #pragma unroll
     for (int x = 0; x < 2; x++)
@@ -109,7 +110,7 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
       for (int y = 0; y < 2; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * 32 + x * 16 + 4 * lk + r;
+          int row = m0 + wr * 32 + x * 16 + 4 * r + lk;
           int col = n0 + wc * 32 + y * 16 + li;
           if (row < m && col < n) {
             double cr = accr[x][y][r], ci = acci[x][y][r];
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
       for (int y = 0; y < 2; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * 32 + x * 16 + 4 * lk + r;
+          int row = m0 + wr * 32 + x * 16 + 4 * r + lk;
           int col = n0 + wc * 32 + y * 16 + li;
           if (row < m && col < n) {
             double cr = accr[x][y][r], ci = acci[x][y][r];
@@ -254,7 +255,7 @@ __device__ __forceinline__ void gepp_step(double2 (&a)[NB], bool valid, bool &ch
 }
 
 template <int NB, int NT>
-__device__ void gepp_select(double2 (&a)[NB], bool valid, int rowid, int w, int *out, double *s_val, int *s_idx,
+__device__ __forceinline__ void gepp_select(double2 (&a)[NB], bool valid, int rowid, int w, int *out, double *s_val, int *s_idx,
                             double2 *s_prow, int *s_pick) {
   bool chosen = false;
   GeppShared sh{s_val, s_idx, s_prow, s_pick};

@@ -81,33 +81,6 @@ def test_zts_vs_oracle_icospheres(gpu_ctx):
     assert nerr(G, Go) < Z_TOL
 
 
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (65, 130, 17), (200, 64, 300), (257, 31, 64)])
-def test_zgemm(gpu_ctx, mnk):
-    m, n, k = mnk
-    rng = np.random.default_rng(m * 7 + n)
-    A = rng.normal(size=(m, k)) + 1j * rng.normal(size=(m, k))
-    B = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
-    C = rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n))
-    al, be = 0.7 - 0.2j, -1.0 + 0.5j
-    got = gpu_ctx.zgemm_host(A, B, C, al, be)
-    ref = al * (A @ B) + be * C
-    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max() * max(1, k) ** 0.5
-
-
-@pytest.mark.parametrize("n", [5, 64, 200, 333, 700])
-def test_lu_solve(gpu_ctx, n):
-    rng = np.random.default_rng(n)
-    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
-    # rows scaled so that partial pivoting is exercised
-    A *= np.exp(rng.normal(size=(n, 1)))
-    X = rng.normal(size=(n, 7)) + 1j * rng.normal(size=(n, 7))
-    B = A @ X
-    got = gpu_ctx.zgesv_host(A, B)
-    resid = np.abs(A @ got - B).max() / (np.abs(A).max() * np.abs(got).max() * n)
-    assert resid < 1e-14
-    assert np.abs(got - X).max() < 1e-8 * np.abs(X).max()
-
-
 def test_heat_transfer_vs_oracle_icospheres(gpu_ctx):
     from bem_amd import mesh, Request
     from oracle import oracle as O
