@@ -42,13 +42,11 @@ int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_
 size_t bem_trace_partials(int m, int n);
 int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0, int *cand, int *cand2, int *ipiv,
                     int *info, hipStream_t st);
-int bem_getrf_nopiv_small(double2 *A, int64_t lda, int w, int *info, int k0, hipStream_t st);
-int bem_trsm_right_upper(const double2 *U, int64_t ldu, int w, double2 *X, int64_t ldx, int nrows, hipStream_t st);
 int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
               hipStream_t st);
-int bem_trsm_small(int upper, const double2 *T, int64_t ldt, int nb, double2 *X, int64_t ldx, int ncols,
-                   hipStream_t st);
 int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st);
+int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int factor, double2 *inv, int *info,
+                   hipStream_t st);
 int bem_set_identity(double2 *X, int64_t ld, int n, hipStream_t st);
 }
 
@@ -75,7 +73,6 @@ struct EventRec {
 
 struct bem_ctx {
   int device = 0;
-  int nb = 64;
   size_t budget = 0;
   size_t bytes = 0;
   hipStream_t stream = nullptr;
@@ -188,13 +185,30 @@ static void ev_collect(bem_ctx *c, double *gemm_ms, double *gemm_n, double *gemm
 }
 
 // --------------------------------------------------------------- LU / solves
-static int lu_factor(bem_ctx *c, double2 *A, int64_t lda, int n, int *ipiv, int *info) {
-  const int nb = c->nb;
+// Panel width of the blocked LU; the diagonal-block inverses are 64 x 64.
+static const int LU_NB = 64;
+
+static size_t inv_bytes(int n) { return sizeof(double2) * 2 * 64 * 64 * (size_t)((n + LU_NB - 1) / LU_NB); }
+
+// X(r x c block at dst, ld) <- tmp (r x c, ld r)
+static int copy_back(bem_ctx *c, double2 *dst, int64_t ld, const double2 *tmp, int rows, int cols) {
+  DEV(c, hipMemcpy2DAsync(dst, sizeof(double2) * ld, tmp, sizeof(double2) * rows, sizeof(double2) * rows, cols,
+                          hipMemcpyDeviceToDevice, c->stream));
+  return 0;
+}
+
+// Blocked right-looking LU with tournament-pivoted panels.  inv receives the
+// inverses of the unit-lower and upper diagonal blocks (inv_bytes(n)); the
+// panel's two triangular solves run as GEMMs against them.
+static int lu_factor(bem_ctx *c, double2 *A, int64_t lda, int n, int *ipiv, int *info, double2 *inv) {
+  const int nb = LU_NB;
   int sets = (n + 255) / 256;
   int *cand, *cand2;
+  double2 *tmp;
   CHK(ws_get(c, "calu_cand", sizeof(int) * 64 * (size_t)sets, (void **)&cand));
-  CHK(ws_get(c, "calu_cand2", sizeof(int) * 64 * (size_t)((sets + 1) / 2 + 1), (void **)&cand2));
-  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0);
+  CHK(ws_get(c, "calu_cand2", sizeof(int) * 64 * (size_t)((sets + 3) / 4 + 1), (void **)&cand2));
+  CHK(ws_get(c, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
+  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
     int e = bem_calu_pivots(A, lda, n - k0, jb, k0, k0, cand, cand2, ipiv, info, c->stream);
@@ -202,40 +216,47 @@ static int lu_factor(bem_ctx *c, double2 *A, int64_t lda, int n, int *ipiv, int 
     e = bem_laswp(A, lda, 0, n, 0, 0, k0, k0 + jb, ipiv, c->stream);
     if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
     double2 *Akk = A + (int64_t)k0 * lda + k0;
-    e = bem_getrf_nopiv_small(Akk, lda, jb, info, k0, c->stream);
-    if (e) return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
+    const int blk = k0 / nb;
+    e = bem_diag_block(A, lda, n, blk, 1, 1, inv, info, c->stream);
+    if (e) return fail(c, BEM_E_DEVICE, "diag block: %d", e);
+    const double2 *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
     int rest = n - k0 - jb;
     if (rest > 0) {
-      e = bem_trsm_right_upper(Akk, lda, jb, Akk + jb, lda, rest, c->stream);
-      if (e) return fail(c, BEM_E_DEVICE, "trsm right: %d", e);
-      double2 *A12 = A + (int64_t)(k0 + jb) * lda + k0;
-      e = bem_trsm_small(0, Akk, lda, jb, A12, lda, rest, c->stream);
-      if (e) return fail(c, BEM_E_DEVICE, "trsm lower: %d", e);
-      CHK(gemm(c, rest, rest, jb, m1, Akk + jb, lda, A12, lda, one, A12 + jb, lda));
+      double2 *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
+      CHK(gemm(c, rest, jb, jb, one, A21, lda, Uinv, 64, zero, tmp, rest));   // L21 = A21 U11^-1
+      CHK(copy_back(c, A21, lda, tmp, rest, jb));
+      CHK(gemm(c, jb, rest, jb, one, Linv, 64, A12, lda, zero, tmp, jb));     // U12 = L11^-1 A12
+      CHK(copy_back(c, A12, lda, tmp, jb, rest));
+      CHK(gemm(c, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
     }
   }
   return 0;
 }
 
-// X (n x r) <- A^-1 X given the factors (ipiv applied here).
-static int lu_solve(bem_ctx *c, const double2 *A, int64_t lda, int n, const int *ipiv, double2 *X, int64_t ldx,
-                    int r) {
-  const int nb = c->nb;
-  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0);
+// X (n x r) <- A^-1 X from the packed LU, its interchanges and the diagonal
+// block inverses.
+static int lu_solve(bem_ctx *c, const double2 *A, int64_t lda, int n, const int *ipiv, const double2 *inv, double2 *X,
+                    int64_t ldx, int r) {
+  const int nb = LU_NB;
+  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  double2 *tmp;
+  CHK(ws_get(c, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
   int e = bem_laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, c->stream);
   if (e) return fail(c, BEM_E_DEVICE, "laswp rhs: %d", e);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
-    e = bem_trsm_small(0, A + (int64_t)k0 * lda + k0, lda, jb, X + k0, ldx, r, c->stream);
-    if (e) return fail(c, BEM_E_DEVICE, "trsm lower: %d", e);
+    const double2 *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+    CHK(gemm(c, jb, r, jb, one, Linv, 64, X + k0, ldx, zero, tmp, jb));
+    CHK(copy_back(c, X + k0, ldx, tmp, jb, r));
     int rest = n - k0 - jb;
     if (rest > 0) CHK(gemm(c, rest, r, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, X + k0, ldx, one, X + k0 + jb, ldx));
   }
   int last = ((n - 1) / nb) * nb;
   for (int k0 = last; k0 >= 0; k0 -= nb) {
     int jb = std::min(nb, n - k0);
-    e = bem_trsm_small(1, A + (int64_t)k0 * lda + k0, lda, jb, X + k0, ldx, r, c->stream);
-    if (e) return fail(c, BEM_E_DEVICE, "trsm upper: %d", e);
+    const double2 *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
+    CHK(gemm(c, jb, r, jb, one, Uinv, 64, X + k0, ldx, zero, tmp, jb));
+    CHK(copy_back(c, X + k0, ldx, tmp, jb, r));
     if (k0 > 0) CHK(gemm(c, k0, r, jb, m1, A + (int64_t)k0 * lda, lda, X + k0, ldx, one, X, ldx));
   }
   return 0;
@@ -339,7 +360,6 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
   bem_ctx *c = new bem_ctx();
   if (cfg) {
     c->device = cfg->device;
-    if (cfg->lu_block > 0) c->nb = std::min(cfg->lu_block, 64);
     c->budget = cfg->hbm_budget;
   }
   int ndev = 0;
@@ -533,14 +553,17 @@ static int spectral_power_fp64(bem_ctx *c, bem_body *b1, bem_body *b2, const bem
   // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
   double2 *A = W, *Bm = W + (int64_t)m1 * M, *Cm = W + m1, *D = W + (int64_t)m1 * M + m1;
-  CHK(lu_factor(c, A, M, m1, ipA, info));
-  CHK(lu_solve(c, A, M, m1, ipA, Bm, M, m2));   // B <- Y
+  double2 *invA, *invS;
+  CHK(ws_get(c, "invA", inv_bytes(m1), (void **)&invA));
+  CHK(ws_get(c, "invS", inv_bytes(m2), (void **)&invS));
+  CHK(lu_factor(c, A, M, m1, ipA, info, invA));
+  CHK(lu_solve(c, A, M, m1, ipA, invA, Bm, M, m2));   // B <- Y
   CHK(gemm(c, m2, m2, m1, mone, Cm, M, Bm, M, one, D, M));
-  CHK(lu_factor(c, D, M, m2, ipS, info + 1));
+  CHK(lu_factor(c, D, M, m2, ipS, info + 1, invS));
   CHK(ws_get(c, "Sinv", sizeof(double2) * (size_t)m2 * m2, (void **)&Sinv));
   e = bem_set_identity(Sinv, m2, m2, c->stream);
   if (e) return fail(c, BEM_E_DEVICE, "identity: %d", e);
-  CHK(lu_solve(c, D, M, m2, ipS, Sinv, m2, m2));
+  CHK(lu_solve(c, D, M, m2, ipS, invS, Sinv, m2, m2));
   CHK(ws_get(c, "X", sizeof(double2) * (size_t)m1 * m2, (void **)&X));
   CHK(gemm(c, m1, m2, m2, mone, Bm, M, Sinv, m2, zero, X, m1));
   DEV(c, hipEventRecord(e2, c->stream));
@@ -629,10 +652,12 @@ int bem_zgetrf(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv) {
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   int *info;
+  double2 *inv;
   CHK(ws_get(c, "info_getrf", sizeof(int), (void **)&info));
+  CHK(ws_get(c, "inv_getrf", inv_bytes(n), (void **)&inv));
   DEV(c, hipMemsetAsync(info, 0, sizeof(int), c->stream));
   ev_reset(c);
-  CHK(lu_factor(c, (double2 *)A, lda, n, ipiv, info));
+  CHK(lu_factor(c, (double2 *)A, lda, n, ipiv, info, inv));
   int h = 0;
   DEV(c, hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
   DEV(c, hipStreamSynchronize(c->stream));
@@ -645,7 +670,11 @@ int bem_zgetrs(bem_ctx *c, const void *A, int64_t lda, int n, const int32_t *ipi
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   ev_reset(c);
-  CHK(lu_solve(c, (const double2 *)A, lda, n, ipiv, (double2 *)B, ldb, nrhs));
+  double2 *inv;
+  CHK(ws_get(c, "inv_getrs", inv_bytes(n), (void **)&inv));
+  int e = bem_diag_block((double2 *)A, lda, n, 0, (n + LU_NB - 1) / LU_NB, 0, inv, nullptr, c->stream);
+  if (e) return fail(c, BEM_E_DEVICE, "diag block inverses: %d", e);
+  CHK(lu_solve(c, (const double2 *)A, lda, n, ipiv, inv, (double2 *)B, ldb, nrhs));
   DEV(c, hipStreamSynchronize(c->stream));
   return 0;
 }

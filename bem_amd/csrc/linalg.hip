@@ -10,11 +10,10 @@
 //                   one partial per block (the HeatTransfer trace
 //                   tr(B2^H B1) = sum conj(B2) .* B1, src/HeatTransfer.cpp:53-55).
 //  tslu_*           tournament pivot selection for a panel (CALU)
-//  getrf_nopiv_small / trsm_right_upper   the panel's factors once its
-//                   pivot rows are in place
+//  diag_block       the panel's 64x64 diagonal block once its pivot rows are
+//                   in place: LU without pivoting and the inverses of both
+//                   triangles, which turn every triangular solve into ZGEMMs
 //  laswp_kernel     row interchanges on a column range
-//  trsm_small       X = L^-1 X (unit lower) / U^-1 X (upper) with the nb x nb
-//                   triangle staged in LDS, one thread per right-hand side
 //  symm_kernel      G <- G + G^H (src/util/DenseMatrix.cpp:477-486)
 #include "bem_common.h"
 
@@ -182,202 +181,251 @@ __global__ void sum_partials_kernel(const double2 *__restrict__ partial, int n, 
 
 // ------------------------------------------------ tournament-pivoted panel
 // Communication-avoiding partial pivoting (CALU, Grigori-Demmel-Xiang 2011):
-// the panel's candidate pivot rows are chosen by a reduction tree of
-// partial-pivoting eliminations (izamax criterion |re|+|im|) instead of one
-// global argmax per column, so the panel needs no grid-wide barrier.
+// the panel's pivot rows are chosen by a reduction tree of partial-pivoting
+// eliminations (izamax criterion |re|+|im|) over blocks of rows, instead of
+// one global argmax per column, so a panel needs no grid-wide barrier.
 //
-// gepp_select: rows held one per thread (a[NB]); runs Gaussian elimination with
-// partial pivoting on the valid rows and writes the chosen rows' ids in pivot
-// order to out[0..w).  Rows are eliminated in place (not moved).
-struct GeppShared {
-  double *val;
-  int *idx;
-  double2 *prow;
-  int *pick;
+// Row storage: the 64 columns of a row are split over a lane pair (2r, 2r+1),
+// 32 complex values each (128 VGPRs), so a 512-thread block holds 256 rows
+// without spilling.  A column step costs two block barriers; the LDS used by
+// step J is double-buffered on J's parity so no closing barrier is needed.
+#define GEPP_NT 512
+#define GEPP_ROWS (GEPP_NT / 2)
+
+struct GeppLds {
+  double val[2][GEPP_NT / 64];
+  int idx[2][GEPP_NT / 64];
+  double2 prow[2][64];
 };
 
-// Column step J of the elimination; recursion keeps J a compile-time index
-// so the row stays in registers.
-template <int NB, int NT, int J>
-__device__ __forceinline__ void gepp_step(double2 (&a)[NB], bool valid, bool &chosen, int rowid, int w, int *out,
-                                          const GeppShared &sh) {
-  if constexpr (J < NB) {
+template <int J>
+__device__ __forceinline__ void gepp2_step(double2 (&a)[32], bool valid, bool &chosen, int rowid, int w, int *out,
+                                           GeppLds &L) {
+  if constexpr (J < 64) {
     if (J < w) {
-      const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-      double best = (valid && !chosen) ? fabs(a[J].x) + fabs(a[J].y) : -1.0;
-      int bidx = (valid && !chosen) ? tid : 0x7fffffff;
+      constexpr int H = J / 32, CJ = J % 32, P = J & 1;
+      const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, half = tid & 1;
+      const bool owner = (half == H);
+      double best = (valid && !chosen && owner) ? fabs(a[CJ].x) + fabs(a[CJ].y) : -1.0;
+      int bidx = (valid && !chosen && owner) ? tid : 0x7fffffff;
 #pragma unroll
       for (int off = 32; off >= 1; off >>= 1) {
-        double ov = __shfl_down(best, off, 64);
-        int oi = __shfl_down(bidx, off, 64);
+        double ov = __shfl_xor(best, off, 64);
+        int oi = __shfl_xor(bidx, off, 64);
         if (ov > best || (ov == best && oi < bidx)) { best = ov; bidx = oi; }
       }
-      if (lane == 0) { sh.val[wave] = best; sh.idx[wave] = bidx; }
+      if (lane == 0) { L.val[P][wave] = best; L.idx[P][wave] = bidx; }
       __syncthreads();
-      if (tid == 0) {
-        double bv = sh.val[0];
-        int bi = sh.idx[0];
-        for (int x = 1; x < NT / 64; x++)
-          if (sh.val[x] > bv || (sh.val[x] == bv && sh.idx[x] < bi)) { bv = sh.val[x]; bi = sh.idx[x]; }
-        *sh.pick = (bv >= 0.0) ? bi : -1;   // -1: no valid row left
+      double bv = L.val[P][0];
+      int bi = L.idx[P][0];
+#pragma unroll
+      for (int x = 1; x < GEPP_NT / 64; x++) {
+        double v = L.val[P][x];
+        int ii = L.idx[P][x];
+        if (v > bv || (v == bv && ii < bi)) { bv = v; bi = ii; }
       }
+      const bool none = bv < 0.0;
+      const int prow_t = none ? -1 : (bi & ~1);     // first lane of the pivot row's pair
+      const bool is_piv = (tid & ~1) == prow_t;
+      if (is_piv) {
+        chosen = true;
+#pragma unroll
+        for (int c = 0; c < 32; c++) L.prow[P][32 * half + c] = a[c];
+        if (half == 0) out[J] = rowid;
+      }
+      if (none && tid == 0) out[J] = -1;
       __syncthreads();
-      const int p = *sh.pick;
-      if (p < 0) {
-        if (tid == 0) out[J] = -1;
-      } else {
-        if (tid == p) {
-          chosen = true;
-          out[J] = rowid;
-#pragma unroll
-          for (int c = 0; c < NB; c++) sh.prow[c] = a[c];
-        }
-        __syncthreads();
-        const double2 u = sh.prow[J];
+      if (!none) {
+        const double2 u = L.prow[P][J];
         const double den = u.x * u.x + u.y * u.y;
-        if (valid && !chosen && den > 0.0) {
-          const double2 ui = make_double2(u.x / den, -u.y / den);
-          double2 x = a[J];
-          double2 l = make_double2(x.x * ui.x - x.y * ui.y, x.x * ui.y + x.y * ui.x);
-          a[J] = l;
+        double2 l = make_double2(0.0, 0.0);
+        const bool act = valid && !chosen && den > 0.0;
+        if (act && owner) {
+          double2 x = a[CJ];
+          double ur = u.x / den, ui = -u.y / den;
+          l = make_double2(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
+          a[CJ] = l;
+        }
+        // the partner lane receives the multiplier
+        double lx = __shfl_xor(l.x, 1, 64), ly = __shfl_xor(l.y, 1, 64);
+        if (!owner) l = make_double2(lx, ly);
+        if (act) {
 #pragma unroll
-          for (int c = J + 1; c < NB; c++) {
-            double2 uc = sh.prow[c];
-            a[c].x -= l.x * uc.x - l.y * uc.y;
-            a[c].y -= l.x * uc.y + l.y * uc.x;
+          for (int c = 0; c < 32; c++) {
+            const int g = 32 * half + c;   // global column of this value
+            if (g > J) {
+              double2 uc = L.prow[P][g];
+              a[c].x -= l.x * uc.x - l.y * uc.y;
+              a[c].y -= l.x * uc.y + l.y * uc.x;
+            }
           }
         }
       }
-      __syncthreads();
-      gepp_step<NB, NT, J + 1>(a, valid, chosen, rowid, w, out, sh);
+      gepp2_step<J + 1>(a, valid, chosen, rowid, w, out, L);
     }
   }
 }
 
-template <int NB, int NT>
-__device__ __forceinline__ void gepp_select(double2 (&a)[NB], bool valid, int rowid, int w, int *out, double *s_val, int *s_idx,
-                            double2 *s_prow, int *s_pick) {
+// Loads row `id` (matrix row, or -1 for none) columns [c0, c0+w) into the lane
+// pair's halves.
+__device__ __forceinline__ void gepp2_load(double2 (&a)[32], const double2 *__restrict__ A, int64_t lda, int id, int w,
+                                           int c0) {
+  const int half = threadIdx.x & 1;
+#pragma unroll
+  for (int c = 0; c < 32; c++) {
+    int g = 32 * half + c;
+    a[c] = (id >= 0 && g < w) ? A[(int64_t)(c0 + g) * lda + id] : make_double2(0.0, 0.0);
+  }
+}
+
+// Leaves: block b takes panel rows [b*256, b*256+256) of the m-row panel that
+// starts at matrix row r0 and emits w candidate row ids (64 slots per block).
+__global__ void __launch_bounds__(GEPP_NT) tslu_leaf_kernel(const double2 *__restrict__ A, int64_t lda, int m, int w,
+                                                            int r0, int c0, int *__restrict__ cand) {
+  __shared__ GeppLds L;
+  int r = blockIdx.x * GEPP_ROWS + (threadIdx.x >> 1);
+  bool valid = r < m;
+  int id = valid ? r0 + r : -1;
+  double2 a[32];
+  gepp2_load(a, A, lda, id, w, c0);
   bool chosen = false;
-  GeppShared sh{s_val, s_idx, s_prow, s_pick};
-  gepp_step<NB, NT, 0>(a, valid, chosen, rowid, w, out, sh);
+  gepp2_step<0>(a, valid, chosen, id, w, cand + (size_t)blockIdx.x * 64, L);
 }
 
-// Leaves of the tournament: block b takes panel rows [b*256, b*256+256) of
-// the m-row panel starting at matrix row r0 and emits w candidate row ids.
-template <int NB>
-__global__ void __launch_bounds__(256) tslu_leaf_kernel(const double2 *__restrict__ A, int64_t lda, int m, int w,
-                                                        int r0, int c0, int *__restrict__ cand) {
-  __shared__ double s_val[4];
-  __shared__ int s_idx[4], s_pick;
-  __shared__ double2 s_prow[NB];
-  int row = blockIdx.x * 256 + threadIdx.x;
-  bool valid = row < m;
-  double2 a[NB];
-#pragma unroll
-  for (int c = 0; c < NB; c++)
-    a[c] = (valid && c < w) ? A[(int64_t)(c0 + c) * lda + r0 + row] : make_double2(0.0, 0.0);
-  gepp_select<NB, 256>(a, valid, r0 + row, w, cand + (size_t)blockIdx.x * NB, s_val, s_idx, s_prow, &s_pick);
+// One tournament round: block b merges candidate sets 4b..4b+3 (64 slots
+// each, ids into the matrix, -1 = empty) by GEPP on their original rows.
+__global__ void __launch_bounds__(GEPP_NT) tslu_merge_kernel(const double2 *__restrict__ A, int64_t lda, int w, int c0,
+                                                             const int *__restrict__ cin, int nsets,
+                                                             int *__restrict__ cout) {
+  __shared__ GeppLds L;
+  int r = threadIdx.x >> 1;            // 0..255
+  int set = 4 * blockIdx.x + r / 64, slot = r % 64;
+  int id = (set < nsets && slot < w) ? cin[(size_t)set * 64 + slot] : -1;
+  double2 a[32];
+  gepp2_load(a, A, lda, id, w, c0);
+  bool chosen = false;
+  gepp2_step<0>(a, id >= 0, chosen, id, w, cout + (size_t)blockIdx.x * 64, L);
 }
 
-// One tournament round: block b merges candidate sets 2b and 2b+1 (ids into
-// the matrix, -1 = empty) by GEPP on their original rows.
-template <int NB>
-__global__ void __launch_bounds__(2 * NB) tslu_merge_kernel(const double2 *__restrict__ A, int64_t lda, int w, int c0,
-                                                            const int *__restrict__ cin, int nsets,
-                                                            int *__restrict__ cout) {
-  __shared__ double s_val[(2 * NB + 63) / 64];
-  __shared__ int s_idx[(2 * NB + 63) / 64], s_pick;
-  __shared__ double2 s_prow[NB];
-  int set = 2 * blockIdx.x + threadIdx.x / NB;
-  int slot = threadIdx.x % NB;
-  int id = (set < nsets && slot < w) ? cin[(size_t)set * NB + slot] : -1;
-  bool valid = id >= 0;
-  double2 a[NB];
-#pragma unroll
-  for (int c = 0; c < NB; c++)
-    a[c] = (valid && c < w) ? A[(int64_t)(c0 + c) * lda + id] : make_double2(0.0, 0.0);
-  gepp_select<NB, 2 * NB>(a, valid, id, w, cout + (size_t)blockIdx.x * NB, s_val, s_idx, s_prow, &s_pick);
-}
-
-// Turn the winner list into LAPACK-style interchanges ipiv[k0+i] (row k0+i
-// swapped with row ipiv[k0+i], applied in order).  Single thread: w is small.
-__global__ void winners_to_ipiv_kernel(const int *__restrict__ win, int w, int k0, int *__restrict__ ipiv,
-                                       int *__restrict__ info) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int pos[128];
-  for (int i = 0; i < w; i++) pos[i] = win[i];
+// Winner list -> LAPACK-style interchanges ipiv[k0+i] (row k0+i swapped with
+// row ipiv[k0+i], applied in order).  One wave: lane t tracks where winner t
+// currently sits while the swaps are replayed in order.
+__global__ void __launch_bounds__(64) winners_to_ipiv_kernel(const int *__restrict__ win, int w, int k0,
+                                                             int *__restrict__ ipiv, int *__restrict__ info) {
+  const int t = threadIdx.x;
+  int pos = t < w ? win[t] : -1;
   for (int i = 0; i < w; i++) {
-    int target = k0 + i;
-    int src = pos[i];
-    if (src < 0) {   // rank-deficient panel: keep the row, flag it
-      src = target;
-      if (info && *info == 0) *info = target + 1;
+    int src = __shfl(pos, i, 64);
+    bool bad = src < 0;
+    if (bad) src = k0 + i;   // rank-deficient panel: keep the row, flag it
+    if (t == 0) {
+      ipiv[k0 + i] = src;
+      if (bad && info && *info == 0) *info = k0 + i + 1;
     }
-    ipiv[target] = src;
-    // the row at 'target' moves to 'src'
-    for (int t = i + 1; t < w; t++)
-      if (pos[t] == target) pos[t] = src;
+    // the row that sat at k0+i moves to src
+    if (t > i && pos == k0 + i) pos = src;
   }
 }
 
-// LU without pivoting of the w x w diagonal block (rows already in pivot
-// order), in place: unit L below the diagonal, U on and above.
-__global__ void __launch_bounds__(256) getrf_nopiv_small_kernel(double2 *__restrict__ A, int64_t lda, int w,
-                                                               int *__restrict__ info, int k0) {
-  extern __shared__ double2 sA[];   // w x w column-major
-  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) sA[idx] = A[(int64_t)(idx / w) * lda + idx % w];
+// The w x w diagonal block (rows already in pivot order) in one 256-thread
+// block: LU without pivoting in place in LDS (unit L below, U on and above),
+// written back to A; then both triangles are inverted in place (the unit-lower
+// and the upper inverse share the packed storage) and written to
+// inv + blk*2*64*64 as two 64 x 64 column-major matrices (Linv, Uinv),
+// identity-padded when w < 64.  With factor=0 only the inverses are formed
+// from an existing packed LU (the batched form used before triangular solves).
+__global__ void __launch_bounds__(256) diag_block_kernel(double2 *__restrict__ A, int64_t lda, int n, int blk0,
+                                                         int factor, double2 *__restrict__ inv, int *__restrict__ info) {
+  __shared__ double2 sA[64 * 64];   // column-major, padded to 64 x 64
+  const int blk = blk0 + blockIdx.x, k0 = blk * 64;
+  const int w = min(64, n - k0);
+  const int tid = threadIdx.x;
+  double2 *Ak = A + (int64_t)k0 * lda + k0;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int r = idx & 63, c = idx >> 6;
+    sA[idx] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
   __syncthreads();
-  for (int j = 0; j < w; j++) {
-    double2 u = sA[j * w + j];
-    double den = u.x * u.x + u.y * u.y;
-    if (den == 0.0) {
-      if (threadIdx.x == 0 && info && *info == 0) *info = k0 + j + 1;
+  if (factor) {
+    for (int j = 0; j < w; j++) {
+      double2 u = sA[j * 64 + j];
+      double den = u.x * u.x + u.y * u.y;
+      if (den == 0.0) {   // uniform: den comes from LDS
+        if (tid == 0 && info && *info == 0) *info = k0 + j + 1;
+        continue;
+      }
+      double2 ui = make_double2(u.x / den, -u.y / den);
+      if (tid > j && tid < 64) {
+        double2 x = sA[j * 64 + tid];
+        sA[j * 64 + tid] = make_double2(x.x * ui.x - x.y * ui.y, x.x * ui.y + x.y * ui.x);
+      }
       __syncthreads();
-      continue;
+      int rem = 63 - j;
+      for (int idx = tid; idx < rem * rem; idx += 256) {
+        int i = j + 1 + idx % rem, c = j + 1 + idx / rem;
+        double2 l = sA[j * 64 + i], uc = sA[c * 64 + j];
+        double2 x = sA[c * 64 + i];
+        x.x -= l.x * uc.x - l.y * uc.y;
+        x.y -= l.x * uc.y + l.y * uc.x;
+        sA[c * 64 + i] = x;
+      }
+      __syncthreads();
     }
-    double2 ui = make_double2(u.x / den, -u.y / den);
-    __syncthreads();
-    for (int i = j + 1 + (int)threadIdx.x; i < w; i += blockDim.x) {
-      double2 x = sA[j * w + i];
-      sA[j * w + i] = make_double2(x.x * ui.x - x.y * ui.y, x.x * ui.y + x.y * ui.x);
-    }
-    __syncthreads();
-    int rem = w - j - 1;
-    for (int idx = threadIdx.x; idx < rem * rem; idx += blockDim.x) {
-      int i = j + 1 + idx % rem, c = j + 1 + idx / rem;
-      double2 l = sA[j * w + i], uc = sA[c * w + j];
-      double2 x = sA[c * w + i];
-      x.x -= l.x * uc.x - l.y * uc.y;
-      x.y -= l.x * uc.y + l.y * uc.x;
-      sA[c * w + i] = x;
+    for (int idx = tid; idx < w * w; idx += 256) {
+      int r = idx % w, c = idx / w;
+      Ak[(int64_t)c * lda + r] = sA[c * 64 + r];
     }
     __syncthreads();
   }
-  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) A[(int64_t)(idx / w) * lda + idx % w] = sA[idx];
-}
-
-// L21 <- L21 U11^-1 for the rows below the diagonal block (one thread per row).
-__global__ void __launch_bounds__(256) trsm_right_upper_kernel(const double2 *__restrict__ U, int64_t ldu, int w,
-                                                               double2 *__restrict__ X, int64_t ldx, int nrows) {
-  extern __shared__ double2 sU[];
-  for (int idx = threadIdx.x; idx < w * w; idx += blockDim.x) sU[idx] = U[(int64_t)(idx / w) * ldu + idx % w];
-  __syncthreads();
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= nrows) return;
-  double2 x[64];
-  for (int j = 0; j < w; j++) x[j] = X[(int64_t)j * ldx + r];
-  for (int j = 0; j < w; j++) {
-    double2 s = x[j];
-    for (int i = 0; i < j; i++) {
-      double2 u = sU[j * w + i];
-      s.x -= x[i].x * u.x - x[i].y * u.y;
-      s.y -= x[i].x * u.y + x[i].y * u.x;
+  // In-place inversion.  L (unit, strictly below the diagonal): columns from
+  // right to left, Linv[i][j] = -(L[i][j] + sum_{j<k<i} Linv[i][k] L[k][j]).
+  // U: columns left to right, Uinv[j][j] = 1/U[j][j],
+  // Uinv[i][j] = -(sum_{i<=k<j} Uinv[i][k] U[k][j]) / U[j][j].
+  // Thread t < 64 owns row t of the current L column and of the U column.
+  for (int s = 0; s < 64; s++) {
+    const int jl = 63 - s;     // L column this step
+    const int ju = s;          // U column this step
+    double2 nl = make_double2(0, 0), nu = make_double2(0, 0);
+    const int i = tid;
+    if (i < 64) {
+      if (i > jl) {
+        double2 acc = sA[jl * 64 + i];
+        for (int k = jl + 1; k < i; k++) {
+          double2 li = sA[k * 64 + i], lk = sA[jl * 64 + k];
+          acc.x += li.x * lk.x - li.y * lk.y;
+          acc.y += li.x * lk.y + li.y * lk.x;
+        }
+        nl = make_double2(-acc.x, -acc.y);
+      }
+      double2 d = sA[ju * 64 + ju];
+      double den = d.x * d.x + d.y * d.y;
+      double2 dinv = make_double2(d.x / den, -d.y / den);
+      if (i < ju) {
+        double2 acc = make_double2(0, 0);
+        for (int k = i; k < ju; k++) {
+          double2 ui = sA[k * 64 + i], uk = sA[ju * 64 + k];
+          acc.x += ui.x * uk.x - ui.y * uk.y;
+          acc.y += ui.x * uk.y + ui.y * uk.x;
+        }
+        nu = make_double2(-(acc.x * dinv.x - acc.y * dinv.y), -(acc.x * dinv.y + acc.y * dinv.x));
+      } else if (i == ju) {
+        nu = dinv;
+      }
     }
-    double2 d = sU[j * w + j];
-    double den = d.x * d.x + d.y * d.y;
-    x[j] = den > 0.0 ? make_double2((s.x * d.x + s.y * d.y) / den, (s.y * d.x - s.x * d.y) / den) : s;
+    __syncthreads();
+    if (i < 64) {
+      if (i > jl) sA[jl * 64 + i] = nl;
+      if (i <= ju) sA[ju * 64 + i] = nu;
+    }
+    __syncthreads();
   }
-  for (int j = 0; j < w; j++) X[(int64_t)j * ldx + r] = x[j];
+  double2 *Li = inv + (size_t)blk * 2 * 64 * 64, *Ui = Li + 64 * 64;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int r = idx & 63, c = idx >> 6;
+    double2 v = sA[idx];
+    Li[idx] = r > c ? v : make_double2(r == c ? 1.0 : 0.0, 0.0);
+    Ui[idx] = r <= c ? v : make_double2(0.0, 0.0);
+  }
 }
 
 // Row interchanges: for p in [k0, k1): swap rows p and ipiv[p] in every column
@@ -393,49 +441,6 @@ __global__ void laswp_kernel(double2 *__restrict__ A, int64_t lda, int c0, int c
       double2 t = col[p];
       col[p] = col[q];
       col[q] = t;
-    }
-  }
-}
-
-// X (nb x ncols) <- L^-1 X with L unit lower (UPPER=false) or U^-1 X with U
-// upper non-unit (UPPER=true); the nb x nb triangle comes from T (ld ldt).
-template <bool UPPER>
-__global__ void __launch_bounds__(256) trsm_small_kernel(const double2 *__restrict__ T, int64_t ldt, int nb,
-                                                         double2 *__restrict__ X, int64_t ldx, int ncols) {
-  extern __shared__ double2 sT[];  // nb x nb, column-major
-  for (int idx = threadIdx.x; idx < nb * nb; idx += blockDim.x) {
-    int r = idx % nb, c = idx / nb;
-    sT[idx] = T[(int64_t)c * ldt + r];
-  }
-  __syncthreads();
-  int col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= ncols) return;
-  double2 *x = X + (int64_t)col * ldx;
-  if (!UPPER) {
-    for (int j = 0; j < nb; j++) {
-      double2 xj = x[j];
-      for (int i = j + 1; i < nb; i++) {
-        double2 l = sT[j * nb + i];
-        double2 xi = x[i];
-        xi.x -= l.x * xj.x - l.y * xj.y;
-        xi.y -= l.x * xj.y + l.y * xj.x;
-        x[i] = xi;
-      }
-    }
-  } else {
-    for (int j = nb - 1; j >= 0; j--) {
-      double2 d = sT[j * nb + j];
-      double den = d.x * d.x + d.y * d.y;
-      double2 xj = x[j];
-      double2 v = make_double2((xj.x * d.x + xj.y * d.y) / den, (xj.y * d.x - xj.x * d.y) / den);
-      x[j] = v;
-      for (int i = 0; i < j; i++) {
-        double2 u = sT[j * nb + i];
-        double2 xi = x[i];
-        xi.x -= u.x * v.x - u.y * v.y;
-        xi.y -= u.x * v.y + u.y * v.x;
-        x[i] = xi;
-      }
     }
   }
 }
@@ -490,18 +495,18 @@ int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_
 size_t bem_trace_partials(int m, int n) { return (size_t)((m + GBM - 1) / GBM) * (size_t)((n + GBN - 1) / GBN); }
 
 // Tournament pivot selection for the panel A(r0:r0+m, c0:c0+w), w <= 64.
-// cand must hold ceil(m/256)*64 ints, cand2 half of that (ping-pong);
+// cand must hold ceil(m/256)*64 ints, cand2 ceil(ceil(m/256)/4)*64 (ping-pong);
 // the winners end up in ipiv[k0..k0+w) as interchanges (k0 = r0).
 int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0, int *cand, int *cand2, int *ipiv,
                     int *info, hipStream_t st) {
   if (w <= 0) return 0;
   if (w > 64) return -1;
-  int nsets = (m + 255) / 256;
-  hipLaunchKernelGGL(tslu_leaf_kernel<64>, dim3(nsets), dim3(256), 0, st, A, lda, m, w, r0, c0, cand);
+  int nsets = (m + GEPP_ROWS - 1) / GEPP_ROWS;
+  hipLaunchKernelGGL(tslu_leaf_kernel, dim3(nsets), dim3(GEPP_NT), 0, st, A, lda, m, w, r0, c0, cand);
   int *cin = cand, *cout = cand2;
   while (nsets > 1) {
-    int nout = (nsets + 1) / 2;
-    hipLaunchKernelGGL(tslu_merge_kernel<64>, dim3(nout), dim3(128), 0, st, A, lda, w, c0, cin, nsets, cout);
+    int nout = (nsets + 3) / 4;
+    hipLaunchKernelGGL(tslu_merge_kernel, dim3(nout), dim3(GEPP_NT), 0, st, A, lda, w, c0, cin, nsets, cout);
     int *t = cin; cin = cout; cout = t;
     nsets = nout;
   }
@@ -509,17 +514,13 @@ int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0,
   return (int)hipGetLastError();
 }
 
-int bem_getrf_nopiv_small(double2 *A, int64_t lda, int w, int *info, int k0, hipStream_t st) {
-  if (w <= 0) return 0;
-  hipLaunchKernelGGL(getrf_nopiv_small_kernel, dim3(1), dim3(256), sizeof(double2) * w * w, st, A, lda, w, info, k0);
-  return (int)hipGetLastError();
-}
-
-int bem_trsm_right_upper(const double2 *U, int64_t ldu, int w, double2 *X, int64_t ldx, int nrows, hipStream_t st) {
-  if (nrows <= 0 || w <= 0) return 0;
-  if (w > 64) return -1;
-  hipLaunchKernelGGL(trsm_right_upper_kernel, dim3((nrows + 255) / 256), dim3(256), sizeof(double2) * w * w, st, U,
-                     ldu, w, X, ldx, nrows);
+// Diagonal blocks [blk0, blk0+nblk) (64 wide) of an n x n matrix: factor=1
+// factors block blk0 without pivoting and inverts its triangles; factor=0 only
+// forms the inverses of an existing packed LU.
+int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int factor, double2 *inv, int *info,
+                   hipStream_t st) {
+  if (nblk <= 0) return 0;
+  hipLaunchKernelGGL(diag_block_kernel, dim3(nblk), dim3(256), 0, st, A, lda, n, blk0, factor, inv, info);
   return (int)hipGetLastError();
 }
 
@@ -528,18 +529,6 @@ int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int
   if (c1 <= c0 || k1 <= k0) return 0;
   hipLaunchKernelGGL(laswp_kernel, dim3((c1 - c0 + 127) / 128), dim3(128), 0, st, A, lda, c0, c1, skip0, skip1, k0, k1,
                      ipiv);
-  return (int)hipGetLastError();
-}
-
-int bem_trsm_small(int upper, const double2 *T, int64_t ldt, int nb, double2 *X, int64_t ldx, int ncols,
-                   hipStream_t st) {
-  if (ncols <= 0 || nb <= 0) return 0;
-  size_t sh = sizeof(double2) * (size_t)nb * (size_t)nb;
-  if (upper)
-    hipLaunchKernelGGL(trsm_small_kernel<true>, dim3((ncols + 255) / 256), dim3(256), sh, st, T, ldt, nb, X, ldx, ncols);
-  else
-    hipLaunchKernelGGL(trsm_small_kernel<false>, dim3((ncols + 255) / 256), dim3(256), sh, st, T, ldt, nb, X, ldx,
-                       ncols);
   return (int)hipGetLastError();
 }
 

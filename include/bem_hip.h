@@ -55,7 +55,7 @@ typedef struct bem_body bem_body;
 
 typedef struct {
   int device;        /* HIP device ordinal */
-  int lu_block;      /* LU panel width (0 = default 64, max 64) */
+  int lu_block;      /* reserved (the LU panel width is fixed at 64), pass 0 */
   size_t hbm_budget; /* bytes a context may hold, 0 = no limit */
 } bem_cfg;
 
