@@ -10,7 +10,7 @@ bem_ctx *context() {
   static bem_ctx *ctx = nullptr;
   static std::once_flag once;
   std::call_once(once, [] {
-    bem_cfg cfg = {0, 0, 0};
+    bem_cfg cfg = {0, 0, 0, 0};
     const char *dev = std::getenv("BEM_DEVICE");
     if (dev) cfg.device = std::atoi(dev);
     int rc = bem_ctx_create(&ctx, &cfg);

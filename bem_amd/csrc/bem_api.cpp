@@ -7,7 +7,13 @@
 // elimination of W = [[A, B], [C, D]]:
 //   Y = A^-1 B,  Sc = D - C Y,  X = -Y Sc^-1
 // with tournament-pivoted LU of A and of Sc, followed by B1 = sym(G1) X and a
-// GEMM whose epilogue contracts X sym(G2) against conj/B1 (no B2 stored).
+// GEMM whose epilogue contracts X sym(G2) against B1 (no B2 stored).
+//
+// Concurrency.  A context owns `lanes` HIP streams, each with its own named
+// workspaces.  A sweep keeps up to `lanes` independent solves in flight: the
+// latency-bound LU panels of one solve overlap the GEMMs of another.  Every
+// solve's kernels stay on its own stream, so results do not depend on what
+// else is in flight.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -15,6 +21,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <list>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -65,8 +72,41 @@ struct DevBuf {
 
 struct EventRec {
   hipEvent_t a, b;
-  int kind;       // 0 gemm
   double flops;
+};
+
+struct DevTree {
+  int T = 0, E = 0;
+  TriGeom *tg = nullptr;
+  RwgRec *rw = nullptr;
+  int2 *close = nullptr;
+  unsigned int nclose = 0;
+};
+
+// Device geometry of one (body 1, body 2, gap): the two-body tree and the
+// two one-body trees, uploaded once and reused by every frequency.
+struct PairGeom {
+  const void *b1 = nullptr, *b2 = nullptr;
+  double gap[3] = {0, 0, 0};
+  DevTree top, s1, s2;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+};
+
+// One stream and its workspaces; holds at most one solve in flight.
+struct Lane {
+  int id = 0;
+  hipStream_t st = nullptr;
+  std::string tag;
+  std::vector<EventRec> ev;
+  size_t ev_used = 0;
+  hipEvent_t ph[4] = {nullptr, nullptr, nullptr, nullptr};
+  double2 *h_trace = nullptr;   // pinned host slots of the results
+  int *h_info = nullptr;
+  bool busy = false;
+  bem_result *res = nullptr;
+  bem_request req;
+  std::chrono::steady_clock::time_point t0;
 };
 
 }  // namespace
@@ -75,16 +115,16 @@ struct bem_ctx {
   int device = 0;
   size_t budget = 0;
   size_t bytes = 0;
-  hipStream_t stream = nullptr;
   std::string err;
   std::recursive_mutex mu;
   std::unordered_map<std::string, DevBuf> ws;   // named, grow-only workspaces
   std::unordered_map<void *, size_t> user_allocs;
-  // timing
+  std::vector<Lane> lanes;
+  std::list<PairGeom> pairs;                    // most recently used first
   bool profile_gemm = true;
-  std::vector<EventRec> ev_pool;
-  size_t ev_used = 0;
+  // statistics of the last API call
   double t_last[9] = {0};
+  double gemm_ms = 0, gemm_n = 0, gemm_flops = 0;
 };
 
 struct bem_body {
@@ -117,20 +157,23 @@ static int fail(bem_ctx *c, int code, const char *fmt, ...) {
     if (_r != 0) return _r; \
   } while (0)
 
-static int ws_get(bem_ctx *c, const char *name, size_t bytes, void **out) {
-  DevBuf &b = c->ws[name];
+// Named workspace of a lane (grow-only).  Growing frees the old buffer after
+// the lane's stream has drained.
+static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **out) {
+  DevBuf &b = c->ws[L.tag + name];
   if (b.cap < bytes) {
     if (b.p) {
-      DEV(c, hipStreamSynchronize(c->stream));
+      DEV(c, hipStreamSynchronize(L.st));
       DEV(c, hipFree(b.p));
       c->bytes -= b.cap;
       b.p = nullptr;
       b.cap = 0;
     }
     if (c->budget && c->bytes + bytes > c->budget)
-      return fail(c, BEM_E_NOMEM, "workspace '%s' (%zu bytes) exceeds the HBM budget", name, bytes);
+      return fail(c, BEM_E_NOMEM, "workspace '%s%s' (%zu bytes) exceeds the HBM budget", L.tag.c_str(), name, bytes);
     hipError_t e = hipMalloc(&b.p, bytes);
-    if (e != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for '%s': %s", bytes, name, hipGetErrorString(e));
+    if (e != hipSuccess)
+      return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for '%s%s': %s", bytes, L.tag.c_str(), name, hipGetErrorString(e));
     b.cap = bytes;
     c->bytes += bytes;
   }
@@ -138,50 +181,46 @@ static int ws_get(bem_ctx *c, const char *name, size_t bytes, void **out) {
   return 0;
 }
 
-static void ev_reset(bem_ctx *c) { c->ev_used = 0; }
-static int ev_begin(bem_ctx *c, int kind, double flops, EventRec **rec) {
-  if (c->ev_used == c->ev_pool.size()) {
+static int ev_begin(bem_ctx *c, Lane &L, double flops, EventRec **rec) {
+  if (L.ev_used == L.ev.size()) {
     EventRec r;
     DEV(c, hipEventCreate(&r.a));
     DEV(c, hipEventCreate(&r.b));
-    c->ev_pool.push_back(r);
+    L.ev.push_back(r);
   }
-  EventRec &r = c->ev_pool[c->ev_used++];
-  r.kind = kind;
+  EventRec &r = L.ev[L.ev_used++];
   r.flops = flops;
-  DEV(c, hipEventRecord(r.a, c->stream));
+  DEV(c, hipEventRecord(r.a, L.st));
   *rec = &r;
   return 0;
 }
-static int ev_end(bem_ctx *c, EventRec *r) {
-  DEV(c, hipEventRecord(r->b, c->stream));
-  return 0;
-}
 
-// GEMM through the context (timed when profiling).
-static int gemm(bem_ctx *c, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B,
-                int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
-  if (m <= 0 || n <= 0) return 0;
-  EventRec *r = nullptr;
-  if (c->profile_gemm) CHK(ev_begin(c, 0, 8.0 * m * (double)n * k, &r));
-  int e = bem_zgemm_raw(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "zgemm launch: %s", hipGetErrorString((hipError_t)e));
-  if (r) CHK(ev_end(c, r));
-  return 0;
-}
-
-static void ev_collect(bem_ctx *c, double *gemm_ms, double *gemm_n, double *gemm_flops) {
-  *gemm_ms = 0;
-  *gemm_n = 0;
-  *gemm_flops = 0;
-  for (size_t i = 0; i < c->ev_used; i++) {
+// Accumulate the lane's GEMM event times into the context statistics
+// (the lane's stream must have drained).
+static void ev_collect(bem_ctx *c, Lane &L) {
+  for (size_t i = 0; i < L.ev_used; i++) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, c->ev_pool[i].a, c->ev_pool[i].b) == hipSuccess && c->ev_pool[i].kind == 0) {
-      *gemm_ms += ms;
-      *gemm_n += 1;
-      *gemm_flops += c->ev_pool[i].flops;
+    if (hipEventElapsedTime(&ms, L.ev[i].a, L.ev[i].b) == hipSuccess) {
+      c->gemm_ms += ms;
+      c->gemm_n += 1;
+      c->gemm_flops += L.ev[i].flops;
     }
   }
+  L.ev_used = 0;
+}
+
+static void stats_reset(bem_ctx *c) { c->gemm_ms = c->gemm_n = c->gemm_flops = 0; }
+
+// GEMM on the lane's stream (bracketed by events when profiling).
+static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda,
+                const double2 *B, int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
+  if (m <= 0 || n <= 0) return 0;
+  EventRec *r = nullptr;
+  if (c->profile_gemm) CHK(ev_begin(c, L, 8.0 * m * (double)n * k, &r));
+  int e = bem_zgemm_raw(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "zgemm launch: %s", hipGetErrorString((hipError_t)e));
+  if (r) DEV(c, hipEventRecord(r->b, L.st));
+  return 0;
 }
 
 // --------------------------------------------------------------- LU / solves
@@ -190,44 +229,44 @@ static const int LU_NB = 64;
 
 static size_t inv_bytes(int n) { return sizeof(double2) * 2 * 64 * 64 * (size_t)((n + LU_NB - 1) / LU_NB); }
 
-// X(r x c block at dst, ld) <- tmp (r x c, ld r)
-static int copy_back(bem_ctx *c, double2 *dst, int64_t ld, const double2 *tmp, int rows, int cols) {
+// dst block (rows x cols, ld) <- tmp (rows x cols, ld rows)
+static int copy_back(bem_ctx *c, Lane &L, double2 *dst, int64_t ld, const double2 *tmp, int rows, int cols) {
   DEV(c, hipMemcpy2DAsync(dst, sizeof(double2) * ld, tmp, sizeof(double2) * rows, sizeof(double2) * rows, cols,
-                          hipMemcpyDeviceToDevice, c->stream));
+                          hipMemcpyDeviceToDevice, L.st));
   return 0;
 }
 
 // Blocked right-looking LU with tournament-pivoted panels.  inv receives the
 // inverses of the unit-lower and upper diagonal blocks (inv_bytes(n)); the
 // panel's two triangular solves run as GEMMs against them.
-static int lu_factor(bem_ctx *c, double2 *A, int64_t lda, int n, int *ipiv, int *info, double2 *inv) {
+static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *ipiv, int *info, double2 *inv) {
   const int nb = LU_NB;
   int sets = (n + 255) / 256;
   int *cand, *cand2;
   double2 *tmp;
-  CHK(ws_get(c, "calu_cand", sizeof(int) * 64 * (size_t)sets, (void **)&cand));
-  CHK(ws_get(c, "calu_cand2", sizeof(int) * 64 * (size_t)((sets + 3) / 4 + 1), (void **)&cand2));
-  CHK(ws_get(c, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
+  CHK(ws_get(c, L, "calu_cand", sizeof(int) * 64 * (size_t)sets, (void **)&cand));
+  CHK(ws_get(c, L, "calu_cand2", sizeof(int) * 64 * (size_t)((sets + 3) / 4 + 1), (void **)&cand2));
+  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
   const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
-    int e = bem_calu_pivots(A, lda, n - k0, jb, k0, k0, cand, cand2, ipiv, info, c->stream);
+    int e = bem_calu_pivots(A, lda, n - k0, jb, k0, k0, cand, cand2, ipiv, info, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "calu pivots: %d", e);
-    e = bem_laswp(A, lda, 0, n, 0, 0, k0, k0 + jb, ipiv, c->stream);
+    e = bem_laswp(A, lda, 0, n, 0, 0, k0, k0 + jb, ipiv, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
     double2 *Akk = A + (int64_t)k0 * lda + k0;
     const int blk = k0 / nb;
-    e = bem_diag_block(A, lda, n, blk, 1, 1, inv, info, c->stream);
+    e = bem_diag_block(A, lda, n, blk, 1, 1, inv, info, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "diag block: %d", e);
     const double2 *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
     int rest = n - k0 - jb;
     if (rest > 0) {
       double2 *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
-      CHK(gemm(c, rest, jb, jb, one, A21, lda, Uinv, 64, zero, tmp, rest));   // L21 = A21 U11^-1
-      CHK(copy_back(c, A21, lda, tmp, rest, jb));
-      CHK(gemm(c, jb, rest, jb, one, Linv, 64, A12, lda, zero, tmp, jb));     // U12 = L11^-1 A12
-      CHK(copy_back(c, A12, lda, tmp, jb, rest));
-      CHK(gemm(c, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
+      CHK(gemm(c, L, rest, jb, jb, one, A21, lda, Uinv, 64, zero, tmp, rest));   // L21 = A21 U11^-1
+      CHK(copy_back(c, L, A21, lda, tmp, rest, jb));
+      CHK(gemm(c, L, jb, rest, jb, one, Linv, 64, A12, lda, zero, tmp, jb));     // U12 = L11^-1 A12
+      CHK(copy_back(c, L, A12, lda, tmp, jb, rest));
+      CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
     }
   }
   return 0;
@@ -235,29 +274,30 @@ static int lu_factor(bem_ctx *c, double2 *A, int64_t lda, int n, int *ipiv, int 
 
 // X (n x r) <- A^-1 X from the packed LU, its interchanges and the diagonal
 // block inverses.
-static int lu_solve(bem_ctx *c, const double2 *A, int64_t lda, int n, const int *ipiv, const double2 *inv, double2 *X,
-                    int64_t ldx, int r) {
+static int lu_solve(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, const int *ipiv, const double2 *inv,
+                    double2 *X, int64_t ldx, int r) {
   const int nb = LU_NB;
   const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   double2 *tmp;
-  CHK(ws_get(c, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
-  int e = bem_laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, c->stream);
+  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
+  int e = bem_laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "laswp rhs: %d", e);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
     const double2 *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
-    CHK(gemm(c, jb, r, jb, one, Linv, 64, X + k0, ldx, zero, tmp, jb));
-    CHK(copy_back(c, X + k0, ldx, tmp, jb, r));
+    CHK(gemm(c, L, jb, r, jb, one, Linv, 64, X + k0, ldx, zero, tmp, jb));
+    CHK(copy_back(c, L, X + k0, ldx, tmp, jb, r));
     int rest = n - k0 - jb;
-    if (rest > 0) CHK(gemm(c, rest, r, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, X + k0, ldx, one, X + k0 + jb, ldx));
+    if (rest > 0)
+      CHK(gemm(c, L, rest, r, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, X + k0, ldx, one, X + k0 + jb, ldx));
   }
   int last = ((n - 1) / nb) * nb;
   for (int k0 = last; k0 >= 0; k0 -= nb) {
     int jb = std::min(nb, n - k0);
     const double2 *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
-    CHK(gemm(c, jb, r, jb, one, Uinv, 64, X + k0, ldx, zero, tmp, jb));
-    CHK(copy_back(c, X + k0, ldx, tmp, jb, r));
-    if (k0 > 0) CHK(gemm(c, k0, r, jb, m1, A + (int64_t)k0 * lda, lda, X + k0, ldx, one, X, ldx));
+    CHK(gemm(c, L, jb, r, jb, one, Uinv, 64, X + k0, ldx, zero, tmp, jb));
+    CHK(copy_back(c, L, X + k0, ldx, tmp, jb, r));
+    if (k0 > 0) CHK(gemm(c, L, k0, r, jb, m1, A + (int64_t)k0 * lda, lda, X + k0, ldx, one, X, ldx));
   }
   return 0;
 }
@@ -266,8 +306,7 @@ static int lu_solve(bem_ctx *c, const double2 *A, int64_t lda, int n, const int 
 static Medium make_medium(double er, double ei, double mr, double mi, double omega) {
   // src/GreensFunction.cpp:17-30 (complex arithmetic written out)
   Medium md;
-  // eps*mu
-  double emr = er * mr - ei * mi, emi = er * mi + ei * mr;
+  double emr = er * mr - ei * mi, emi = er * mi + ei * mr;     // eps mu
   double kr = emr * omega * omega, ki = emi * omega * omega;   // (eps mu omega) omega
   md.ksq_re = kr;
   md.ksq_im = ki;
@@ -290,50 +329,37 @@ static Medium make_medium(double er, double ei, double mr, double mi, double ome
   return md;
 }
 
-struct DevTree {
-  int T = 0, E = 0;
-  TriGeom *tg = nullptr;
-  RwgRec *rw = nullptr;
-  int2 *close = nullptr;
-  unsigned int nclose = 0;
-};
-
-static int upload_tree(bem_ctx *c, const bemgeo::Tree &t, const char *tag, DevTree &d) {
-  std::string n1 = std::string(tag) + ":tg", n2 = std::string(tag) + ":rw", n3 = std::string(tag) + ":close";
-  CHK(ws_get(c, n1.c_str(), sizeof(TriGeom) * t.tg.size(), (void **)&d.tg));
-  CHK(ws_get(c, n2.c_str(), sizeof(RwgRec) * t.rw.size(), (void **)&d.rw));
-  CHK(ws_get(c, n3.c_str(), sizeof(int2) * (t.close.size() + 1), (void **)&d.close));
+// Persistent upload of a tree (owned by a PairGeom, or by a one-off holder).
+static int upload_tree(bem_ctx *c, const bemgeo::Tree &t, DevTree &d, std::vector<void *> &allocs, size_t &bytes) {
   std::vector<int2> cl(t.close.size());
   for (size_t i = 0; i < cl.size(); i++) cl[i] = make_int2(t.close[i].first, t.close[i].second);
-  DEV(c, hipMemcpyAsync(d.tg, t.tg.data(), sizeof(TriGeom) * t.tg.size(), hipMemcpyHostToDevice, c->stream));
-  DEV(c, hipMemcpyAsync(d.rw, t.rw.data(), sizeof(RwgRec) * t.rw.size(), hipMemcpyHostToDevice, c->stream));
-  if (!cl.empty())
-    DEV(c, hipMemcpyAsync(d.close, cl.data(), sizeof(int2) * cl.size(), hipMemcpyHostToDevice, c->stream));
-  // the host vectors die at return: make the copies complete first
-  DEV(c, hipStreamSynchronize(c->stream));
+  size_t b1 = sizeof(TriGeom) * t.tg.size(), b2 = sizeof(RwgRec) * t.rw.size(), b3 = sizeof(int2) * (cl.size() + 1);
+  void *p1, *p2, *p3;
+  DEV(c, hipMalloc(&p1, b1));
+  allocs.push_back(p1);
+  DEV(c, hipMalloc(&p2, b2));
+  allocs.push_back(p2);
+  DEV(c, hipMalloc(&p3, b3));
+  allocs.push_back(p3);
+  bytes += b1 + b2 + b3;
+  c->bytes += b1 + b2 + b3;
+  DEV(c, hipMemcpy(p1, t.tg.data(), b1, hipMemcpyHostToDevice));
+  DEV(c, hipMemcpy(p2, t.rw.data(), b2, hipMemcpyHostToDevice));
+  if (!cl.empty()) DEV(c, hipMemcpy(p3, cl.data(), sizeof(int2) * cl.size(), hipMemcpyHostToDevice));
+  d.tg = (TriGeom *)p1;
+  d.rw = (RwgRec *)p2;
+  d.close = (int2 *)p3;
   d.T = t.T;
   d.E = t.E;
   d.nclose = (unsigned int)cl.size();
   return 0;
 }
 
-// Triangle-pair tiles of one tree (triangles [t_lo, t_lo+T) of tg) in medium md.
-static int assemble_tiles(bem_ctx *c, const TriGeom *tg, int T, const int2 *close, unsigned int nclose,
-                          const Medium &md, double *tiles) {
-  size_t npairs = (size_t)T * (size_t)T;
-  unsigned int *q36cnt;
-  int2 *q36;
-  unsigned int cap = (unsigned int)std::min<size_t>(npairs, (size_t)1 << 24);
-  CHK(ws_get(c, "q36_count", sizeof(unsigned int), (void **)&q36cnt));
-  CHK(ws_get(c, "q36", sizeof(int2) * (size_t)cap, (void **)&q36));
-  DEV(c, hipMemsetAsync(q36cnt, 0, sizeof(unsigned int), c->stream));
-  int e = bem_launch_far_g6(tg, T, &md, 0, T, 0, T, tiles, npairs, q36, q36cnt, cap, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "far_g6 launch: %s", hipGetErrorString((hipError_t)e));
-  e = bem_launch_far_g36(tg, T, &md, q36, q36cnt, cap, 0, T, tiles, npairs, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "far_g36 launch: %s", hipGetErrorString((hipError_t)e));
-  e = bem_launch_close(tg, T, &md, close, nclose, 0, T, tiles, npairs, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "close launch: %s", hipGetErrorString((hipError_t)e));
-  return 0;
+static void free_allocs(bem_ctx *c, std::vector<void *> &allocs, size_t bytes) {
+  (void)hipDeviceSynchronize();
+  for (void *p : allocs) (void)hipFree(p);
+  allocs.clear();
+  c->bytes -= bytes;
 }
 
 // Body-local view of a tree: triangles/RWGs of body b re-based to 0.
@@ -352,24 +378,222 @@ static void body_subtree(const bemgeo::Tree &t, int b, bemgeo::Tree &out) {
   out.rwg_off = {0};
 }
 
+static const size_t PAIR_CACHE = 8;
+
+static int pair_geom(bem_ctx *c, bem_body *b1, bem_body *b2, const double *gap, PairGeom **out) {
+  for (auto it = c->pairs.begin(); it != c->pairs.end(); ++it)
+    if (it->b1 == b1 && it->b2 == b2 && it->gap[0] == gap[0] && it->gap[1] == gap[1] && it->gap[2] == gap[2]) {
+      c->pairs.splice(c->pairs.begin(), c->pairs, it);
+      *out = &c->pairs.front();
+      return 0;
+    }
+  if (c->pairs.size() >= PAIR_CACHE) {
+    PairGeom &old = c->pairs.back();
+    free_allocs(c, old.allocs, old.bytes);
+    c->pairs.pop_back();
+  }
+  c->pairs.emplace_front();
+  PairGeom &g = c->pairs.front();
+  g.b1 = b1;
+  g.b2 = b2;
+  for (int k = 0; k < 3; k++) g.gap[k] = gap[k];
+  std::vector<const bemgeo::BodyMesh *> bm = {&b1->mesh, &b2->mesh};
+  double shifts[6] = {gap[0], gap[1], gap[2], 0, 0, 0};
+  bemgeo::Tree top, s1, s2;
+  bemgeo::build_tree(bm, shifts, top);
+  body_subtree(top, 0, s1);
+  body_subtree(top, 1, s2);
+  int e = upload_tree(c, top, g.top, g.allocs, g.bytes);
+  if (!e) e = upload_tree(c, s1, g.s1, g.allocs, g.bytes);
+  if (!e) e = upload_tree(c, s2, g.s2, g.allocs, g.bytes);
+  if (e) {
+    free_allocs(c, g.allocs, g.bytes);
+    c->pairs.pop_front();
+    return e;
+  }
+  *out = &g;
+  return 0;
+}
+
+// Triangle-pair tiles of one tree in medium md, on the lane's stream.
+static int assemble_tiles(bem_ctx *c, Lane &L, const DevTree &d, const Medium &md, double *tiles) {
+  const int T = d.T;
+  size_t npairs = (size_t)T * (size_t)T;
+  unsigned int *q36cnt;
+  int2 *q36;
+  unsigned int cap = (unsigned int)std::min<size_t>(npairs, (size_t)1 << 24);
+  CHK(ws_get(c, L, "q36_count", sizeof(unsigned int), (void **)&q36cnt));
+  CHK(ws_get(c, L, "q36", sizeof(int2) * (size_t)cap, (void **)&q36));
+  DEV(c, hipMemsetAsync(q36cnt, 0, sizeof(unsigned int), L.st));
+  int e = bem_launch_far_g6(d.tg, T, &md, 0, T, 0, T, tiles, npairs, q36, q36cnt, cap, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "far_g6 launch: %s", hipGetErrorString((hipError_t)e));
+  e = bem_launch_far_g36(d.tg, T, &md, q36, q36cnt, cap, 0, T, tiles, npairs, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "far_g36 launch: %s", hipGetErrorString((hipError_t)e));
+  e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, 0, T, tiles, npairs, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "close launch: %s", hipGetErrorString((hipError_t)e));
+  return 0;
+}
+
+static double theta(double omega, double T) {
+  // src/HeatTransfer.cpp:3-6 (T = 0 gives exp(inf) and exactly 0)
+  return SI_HBAR * omega / (std::exp(SI_HBAR * omega / (SI_KB * T)) - 1);
+}
+
+// ------------------------------------------------------------- one solve
+// Issues every kernel of one FP64 spectral-power solve on the lane's stream
+// and returns without waiting; lane_finish() collects the result.
+static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, bem_result *res) {
+  L.t0 = std::chrono::steady_clock::now();
+  L.req = *q;
+  L.res = res;
+  double omegaSI = q->eV * SI_EV / SI_HBAR;
+  double omega = omegaSI * q->a / SI_C;
+  const int m1 = 2 * g.s1.E, m2 = 2 * g.s2.E, M = m1 + m2;
+  Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
+  Medium m1d = make_medium(q->eps1[0], q->eps1[1], q->mu1[0], q->mu1[1], omega);
+  Medium m2d = make_medium(q->eps2[0], q->eps2[1], q->mu2[0], q->mu2[1], omega);
+
+  double2 *W, *G1, *G2, *Sinv, *X, *B1, *partial, *invA, *invS;
+  double *tiles;
+  int *ipA, *info;
+  size_t npairs_top = (size_t)g.top.T * g.top.T;
+  CHK(ws_get(c, L, "W", sizeof(double2) * (size_t)M * M, (void **)&W));
+  CHK(ws_get(c, L, "G1", sizeof(double2) * (size_t)m1 * m1, (void **)&G1));
+  CHK(ws_get(c, L, "G2", sizeof(double2) * (size_t)m2 * m2, (void **)&G2));
+  CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * npairs_top, (void **)&tiles));
+  CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(M + 64), (void **)&ipA));
+  int *ipS = ipA + m1;
+  CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
+  CHK(ws_get(c, L, "invA", inv_bytes(m1), (void **)&invA));
+  CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
+  CHK(ws_get(c, L, "Sinv", sizeof(double2) * (size_t)m2 * m2, (void **)&Sinv));
+  CHK(ws_get(c, L, "X", sizeof(double2) * (size_t)m1 * m2, (void **)&X));
+  CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
+  size_t np = bem_trace_partials(m1, m2);
+  CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
+  double2 *trace_out = partial + np;
+  DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
+
+  DEV(c, hipEventRecord(L.ph[0], L.st));
+  // ---- assembly: W = Z_ext(top), then + Z_int(b) on the diagonal blocks
+  CHK(assemble_tiles(c, L, g.top, mext, tiles));
+  int e = bem_launch_gather(tiles, npairs_top, g.top.T, 0, g.top.rw, 0, g.top.E, 0, g.top.E, &mext, W, M, 0, 0, 0,
+                            nullptr, 0, 0, 0, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
+  CHK(assemble_tiles(c, L, g.s1, m1d, tiles));
+  e = bem_launch_gather(tiles, (size_t)g.s1.T * g.s1.T, g.s1.T, 0, g.s1.rw, 0, g.s1.E, 0, g.s1.E, &m1d, W, M, 0, 0, 1,
+                        G1, m1, 0, 0, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
+  CHK(assemble_tiles(c, L, g.s2, m2d, tiles));
+  e = bem_launch_gather(tiles, (size_t)g.s2.T * g.s2.T, g.s2.T, 0, g.s2.rw, 0, g.s2.E, 0, g.s2.E, &m2d, W, M, m1, m1,
+                        1, G2, m2, 0, 0, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
+  DEV(c, hipEventRecord(L.ph[1], L.st));
+
+  // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
+  const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  double2 *A = W, *Bm = W + (int64_t)m1 * M, *Cm = W + m1, *D = W + (int64_t)m1 * M + m1;
+  CHK(lu_factor(c, L, A, M, m1, ipA, info, invA));
+  CHK(lu_solve(c, L, A, M, m1, ipA, invA, Bm, M, m2));   // B <- Y
+  CHK(gemm(c, L, m2, m2, m1, mone, Cm, M, Bm, M, one, D, M));
+  CHK(lu_factor(c, L, D, M, m2, ipS, info + 1, invS));
+  e = bem_set_identity(Sinv, m2, m2, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "identity: %d", e);
+  CHK(lu_solve(c, L, D, M, m2, ipS, invS, Sinv, m2, m2));
+  CHK(gemm(c, L, m1, m2, m2, mone, Bm, M, Sinv, m2, zero, X, m1));
+  DEV(c, hipEventRecord(L.ph[2], L.st));
+
+  // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
+  e = bem_symm(G1, m1, m1, L.st);
+  if (!e) e = bem_symm(G2, m2, m2, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "symm: %d", e);
+  CHK(gemm(c, L, m1, m2, m1, one, G1, m1, X, m1, zero, B1, m1));
+  {
+    EventRec *r = nullptr;
+    if (c->profile_gemm) CHK(ev_begin(c, L, 8.0 * m1 * (double)m2 * m2, &r));
+    e = bem_zgemm_trace(m1, m2, m2, one, X, m1, G2, m2, B1, m1, partial, trace_out, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
+    if (r) DEV(c, hipEventRecord(r->b, L.st));
+  }
+  DEV(c, hipEventRecord(L.ph[3], L.st));
+  DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
+  L.busy = true;
+  return 0;
+}
+
+static int lane_finish(bem_ctx *c, Lane &L) {
+  if (!L.busy) return 0;
+  L.busy = false;
+  DEV(c, hipStreamSynchronize(L.st));
+  bem_result *res = L.res;
+  const bem_request *q = &L.req;
+  float ta = 0, tf = 0, tp = 0;
+  (void)hipEventElapsedTime(&ta, L.ph[0], L.ph[1]);
+  (void)hipEventElapsedTime(&tf, L.ph[1], L.ph[2]);
+  (void)hipEventElapsedTime(&tp, L.ph[2], L.ph[3]);
+  ev_collect(c, L);
+  double omega = q->eV * SI_EV / SI_HBAR * q->a / SI_C;
+  double2 tr = *L.h_trace;
+  res->omega = omega;
+  res->ReS = 0.25 * tr.x;
+  res->ImS = 0.25 * tr.y;
+  res->theta1 = theta(omega, q->T1);
+  res->theta2 = theta(omega, q->T2);
+  res->P = res->ReS * (res->theta1 - res->theta2);
+  res->rel_err_est = 0;
+  res->iters = 0;
+  res->ms_assemble = ta;
+  res->ms_factor = tf;
+  res->ms_products = tp;
+  res->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - L.t0).count();
+  res->status = 0;
+  c->t_last[0] = ta;
+  c->t_last[4] = tf;
+  c->t_last[5] = tp;
+  if (L.h_info[0] || L.h_info[1]) {
+    res->status = BEM_E_SINGULAR;
+    return fail(c, BEM_E_SINGULAR, "exactly singular pivot (A: %d, Schur: %d)", L.h_info[0], L.h_info[1]);
+  }
+  return 0;
+}
+
+static int check_request(bem_ctx *c, const bem_request *q) {
+  if (!(q->eV > 0) || !(q->a > 0)) return fail(c, BEM_E_INVALID, "eV and a must be positive");
+  if (q->precision != BEM_PREC_FP64) return fail(c, BEM_E_UNSUPPORTED, "precision mode %d not available", q->precision);
+  return 0;
+}
+
 // ------------------------------------------------------------------- C ABI
 extern "C" {
 
 int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
   if (!out) return BEM_E_INVALID;
   bem_ctx *c = new bem_ctx();
+  int nl = 2;
   if (cfg) {
     c->device = cfg->device;
     c->budget = cfg->hbm_budget;
+    if (cfg->lanes > 0) nl = std::min(cfg->lanes, 8);
   }
   int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device) {
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || hipSetDevice(c->device) != hipSuccess) {
     delete c;
     return BEM_E_DEVICE;
   }
-  if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return BEM_E_DEVICE;
+  c->lanes.resize(nl);
+  for (int i = 0; i < nl; i++) {
+    Lane &L = c->lanes[i];
+    L.id = i;
+    L.tag = "L" + std::to_string(i) + ":";
+    bool ok = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) == hipSuccess;
+    for (int k = 0; k < 4 && ok; k++) ok = hipEventCreate(&L.ph[k]) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_trace, sizeof(double2), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_info, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
+    if (!ok) {
+      bem_ctx_destroy(c);
+      return BEM_E_DEVICE;
+    }
   }
   *out = c;
   return 0;
@@ -378,15 +602,23 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
 void bem_ctx_destroy(bem_ctx *c) {
   if (!c) return;
   (void)hipSetDevice(c->device);
-  (void)hipStreamSynchronize(c->stream);
+  (void)hipDeviceSynchronize();
   for (auto &kv : c->ws)
     if (kv.second.p) (void)hipFree(kv.second.p);
   for (auto &kv : c->user_allocs) (void)hipFree(kv.first);
-  for (auto &r : c->ev_pool) {
-    (void)hipEventDestroy(r.a);
-    (void)hipEventDestroy(r.b);
+  for (auto &g : c->pairs)
+    for (void *p : g.allocs) (void)hipFree(p);
+  for (auto &L : c->lanes) {
+    for (auto &r : L.ev) {
+      (void)hipEventDestroy(r.a);
+      (void)hipEventDestroy(r.b);
+    }
+    for (int k = 0; k < 4; k++)
+      if (L.ph[k]) (void)hipEventDestroy(L.ph[k]);
+    if (L.h_trace) (void)hipHostFree(L.h_trace);
+    if (L.h_info) (void)hipHostFree(L.h_info);
+    if (L.st) (void)hipStreamDestroy(L.st);
   }
-  (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -425,7 +657,22 @@ int bem_body_edges(bem_body *b, int32_t *edges, int *ne) {
   return 0;
 }
 
-void bem_body_free(bem_body *b) { delete b; }
+void bem_body_free(bem_body *b) {
+  if (!b) return;
+  bem_ctx *c = b->ctx;
+  {
+    std::lock_guard<std::recursive_mutex> lk(c->mu);
+    for (auto it = c->pairs.begin(); it != c->pairs.end();) {
+      if (it->b1 == b || it->b2 == b) {
+        free_allocs(c, it->allocs, it->bytes);
+        it = c->pairs.erase(it);
+      } else {
+        ++it;
+      }
+    }
+  }
+  delete b;
+}
 
 int bem_pair_histogram(bem_ctx *c, int nb, bem_body *const *bodies, const double *shifts, int64_t *hist6) {
   if (!c || nb < 1 || nb > 2 || !bodies || !shifts || !hist6) return fail(c, BEM_E_INVALID, "bad arguments");
@@ -462,6 +709,8 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   if (!c || nb < 1 || nb > 2 || !bodies || !shifts || !eps || !mu || !Z) return fail(c, BEM_E_INVALID, "bad arguments");
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_finish(c, L));
   std::vector<const bemgeo::BodyMesh *> bm;
   for (int i = 0; i < nb; i++) bm.push_back(&bodies[i]->mesh);
   bemgeo::Tree t;
@@ -469,181 +718,81 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   int M = 2 * t.E;
   if (ldz < M) return fail(c, BEM_E_INVALID, "ldz %lld < %d", (long long)ldz, M);
   DevTree d;
-  CHK(upload_tree(c, t, "asm", d));
+  std::vector<void *> allocs;
+  size_t tb = 0;
+  int e = upload_tree(c, t, d, allocs, tb);
   Medium md = make_medium(eps[0], eps[1], mu[0], mu[1], omega);
-  double *tiles;
+  double *tiles = nullptr;
   size_t npairs = (size_t)t.T * t.T;
-  CHK(ws_get(c, "tiles", sizeof(double) * BEM_TILE_PLANES * npairs, (void **)&tiles));
-  CHK(assemble_tiles(c, d.tg, d.T, d.close, d.nclose, md, tiles));
-  double2 *Zd;
+  double2 *Zd = nullptr;
   bool dev = flags & BEM_ASM_DEVICE_PTR;
   int64_t ld = dev ? ldz : M;
-  if (dev) Zd = (double2 *)Z;
-  else CHK(ws_get(c, "asm_Z", sizeof(double2) * (size_t)M * M, (void **)&Zd));
-  int e = bem_launch_gather(tiles, npairs, t.T, 0, d.rw, 0, t.E, 0, t.E, &md, Zd, ld, 0, 0, 0, nullptr, 0, 0, 0, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "gather launch: %d", e);
-  if (!dev) {
-    DEV(c, hipMemcpy2DAsync(Z, sizeof(double2) * ldz, Zd, sizeof(double2) * M, sizeof(double2) * M, M,
-                            hipMemcpyDeviceToHost, c->stream));
+  if (!e) e = ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * npairs, (void **)&tiles);
+  if (!e) e = assemble_tiles(c, L, d, md, tiles);
+  if (!e) {
+    if (dev) Zd = (double2 *)Z;
+    else e = ws_get(c, L, "asm_Z", sizeof(double2) * (size_t)M * M, (void **)&Zd);
   }
-  DEV(c, hipStreamSynchronize(c->stream));
-  return 0;
-}
-
-static double theta(double omega, double T) {
-  // src/HeatTransfer.cpp:3-6 (T = 0 gives exp(inf) and exactly 0)
-  return SI_HBAR * omega / (std::exp(SI_HBAR * omega / (SI_KB * T)) - 1);
-}
-
-static int spectral_power_fp64(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *q, bem_result *res) {
-  hipEvent_t e0, e1, e2, e3;
-  DEV(c, hipEventCreate(&e0));
-  DEV(c, hipEventCreate(&e1));
-  DEV(c, hipEventCreate(&e2));
-  DEV(c, hipEventCreate(&e3));
-  auto t_host0 = std::chrono::steady_clock::now();
-  double omegaSI = q->eV * SI_EV / SI_HBAR;
-  double omega = omegaSI * q->a / SI_C;
-  std::vector<const bemgeo::BodyMesh *> bm = {&b1->mesh, &b2->mesh};
-  double shifts[6] = {q->gap[0], q->gap[1], q->gap[2], 0, 0, 0};
-  bemgeo::Tree top, s1, s2;
-  bemgeo::build_tree(bm, shifts, top);
-  body_subtree(top, 0, s1);
-  body_subtree(top, 1, s2);
-  const int E1 = s1.E, E2 = s2.E;
-  const int m1 = 2 * E1, m2 = 2 * E2, M = m1 + m2;
-  DevTree dt, d1, d2;
-  CHK(upload_tree(c, top, "top", dt));
-  CHK(upload_tree(c, s1, "s1", d1));
-  CHK(upload_tree(c, s2, "s2", d2));
-  Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
-  Medium m1d = make_medium(q->eps1[0], q->eps1[1], q->mu1[0], q->mu1[1], omega);
-  Medium m2d = make_medium(q->eps2[0], q->eps2[1], q->mu2[0], q->mu2[1], omega);
-
-  double2 *W, *G1, *G2, *Sinv, *X, *B1, *partial, *trace_out;
-  double *tiles;
-  int *ipA, *ipS, *info;
-  size_t npairs_top = (size_t)top.T * top.T;
-  CHK(ws_get(c, "W", sizeof(double2) * (size_t)M * M, (void **)&W));
-  CHK(ws_get(c, "G1", sizeof(double2) * (size_t)m1 * m1, (void **)&G1));
-  CHK(ws_get(c, "G2", sizeof(double2) * (size_t)m2 * m2, (void **)&G2));
-  CHK(ws_get(c, "tiles", sizeof(double) * BEM_TILE_PLANES * npairs_top, (void **)&tiles));
-  CHK(ws_get(c, "ipiv", sizeof(int) * (size_t)(M + 64), (void **)&ipA));
-  ipS = ipA + m1;
-  CHK(ws_get(c, "info", sizeof(int) * 4, (void **)&info));
-  DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, c->stream));
-
-  ev_reset(c);
-  DEV(c, hipEventRecord(e0, c->stream));
-  // ---- assembly: W = Z_ext(top), then + Z_int(b) on the diagonal blocks
-  CHK(assemble_tiles(c, dt.tg, dt.T, dt.close, dt.nclose, mext, tiles));
-  int e = bem_launch_gather(tiles, npairs_top, dt.T, 0, dt.rw, 0, dt.E, 0, dt.E, &mext, W, M, 0, 0, 0, nullptr, 0, 0, 0,
-                            c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
-  CHK(assemble_tiles(c, d1.tg, d1.T, d1.close, d1.nclose, m1d, tiles));
-  e = bem_launch_gather(tiles, (size_t)d1.T * d1.T, d1.T, 0, d1.rw, 0, d1.E, 0, d1.E, &m1d, W, M, 0, 0, 1, G1, m1, 0, 0,
-                        c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
-  CHK(assemble_tiles(c, d2.tg, d2.T, d2.close, d2.nclose, m2d, tiles));
-  e = bem_launch_gather(tiles, (size_t)d2.T * d2.T, d2.T, 0, d2.rw, 0, d2.E, 0, d2.E, &m2d, W, M, m1, m1, 1, G2, m2, 0,
-                        0, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
-  DEV(c, hipEventRecord(e1, c->stream));
-
-  // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
-  const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-  double2 *A = W, *Bm = W + (int64_t)m1 * M, *Cm = W + m1, *D = W + (int64_t)m1 * M + m1;
-  double2 *invA, *invS;
-  CHK(ws_get(c, "invA", inv_bytes(m1), (void **)&invA));
-  CHK(ws_get(c, "invS", inv_bytes(m2), (void **)&invS));
-  CHK(lu_factor(c, A, M, m1, ipA, info, invA));
-  CHK(lu_solve(c, A, M, m1, ipA, invA, Bm, M, m2));   // B <- Y
-  CHK(gemm(c, m2, m2, m1, mone, Cm, M, Bm, M, one, D, M));
-  CHK(lu_factor(c, D, M, m2, ipS, info + 1, invS));
-  CHK(ws_get(c, "Sinv", sizeof(double2) * (size_t)m2 * m2, (void **)&Sinv));
-  e = bem_set_identity(Sinv, m2, m2, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "identity: %d", e);
-  CHK(lu_solve(c, D, M, m2, ipS, invS, Sinv, m2, m2));
-  CHK(ws_get(c, "X", sizeof(double2) * (size_t)m1 * m2, (void **)&X));
-  CHK(gemm(c, m1, m2, m2, mone, Bm, M, Sinv, m2, zero, X, m1));
-  DEV(c, hipEventRecord(e2, c->stream));
-
-  // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
-  e = bem_symm(G1, m1, m1, c->stream);
-  if (!e) e = bem_symm(G2, m2, m2, c->stream);
-  if (e) return fail(c, BEM_E_DEVICE, "symm: %d", e);
-  CHK(ws_get(c, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
-  CHK(gemm(c, m1, m2, m1, one, G1, m1, X, m1, zero, B1, m1));
-  CHK(ws_get(c, "trace_partial", sizeof(double2) * (bem_trace_partials(m1, m2) + 1), (void **)&partial));
-  trace_out = partial + bem_trace_partials(m1, m2);
-  {
-    EventRec *r = nullptr;
-    if (c->profile_gemm) CHK(ev_begin(c, 0, 8.0 * m1 * (double)m2 * m2, &r));
-    e = bem_zgemm_trace(m1, m2, m2, one, X, m1, G2, m2, B1, m1, partial, trace_out, c->stream);
-    if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
-    if (r) CHK(ev_end(c, r));
-  }
-  DEV(c, hipEventRecord(e3, c->stream));
-  double2 tr;
-  int hinfo[4];
-  DEV(c, hipMemcpyAsync(&tr, trace_out, sizeof(double2), hipMemcpyDeviceToHost, c->stream));
-  DEV(c, hipMemcpyAsync(hinfo, info, sizeof(int) * 4, hipMemcpyDeviceToHost, c->stream));
-  DEV(c, hipStreamSynchronize(c->stream));
-  float ta = 0, tf = 0, tp = 0;
-  (void)hipEventElapsedTime(&ta, e0, e1);
-  (void)hipEventElapsedTime(&tf, e1, e2);
-  (void)hipEventElapsedTime(&tp, e2, e3);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(e2);
-  (void)hipEventDestroy(e3);
-  double gms, gn, gfl;
-  ev_collect(c, &gms, &gn, &gfl);
-  c->t_last[0] = ta;
-  c->t_last[4] = tf;
-  c->t_last[5] = tp;
-  c->t_last[6] = gms;
-  c->t_last[7] = gn;
-  c->t_last[8] = gfl;
-
-  res->omega = omega;
-  res->ReS = 0.25 * tr.x;
-  res->ImS = 0.25 * tr.y;
-  res->theta1 = theta(omega, q->T1);
-  res->theta2 = theta(omega, q->T2);
-  res->P = res->ReS * (res->theta1 - res->theta2);
-  res->rel_err_est = 0;
-  res->iters = 0;
-  res->ms_assemble = ta;
-  res->ms_factor = tf;
-  res->ms_products = tp;
-  res->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host0).count();
-  res->status = 0;
-  if (hinfo[0] || hinfo[1]) {
-    res->status = BEM_E_SINGULAR;
-    return fail(c, BEM_E_SINGULAR, "exactly singular pivot (A: %d, Schur: %d)", hinfo[0], hinfo[1]);
-  }
-  return 0;
+  if (!e && bem_launch_gather(tiles, npairs, t.T, 0, d.rw, 0, t.E, 0, t.E, &md, Zd, ld, 0, 0, 0, nullptr, 0, 0, 0, L.st))
+    e = fail(c, BEM_E_DEVICE, "gather launch failed");
+  if (!e && !dev && hipMemcpy2DAsync(Z, sizeof(double2) * ldz, Zd, sizeof(double2) * M, sizeof(double2) * M, M,
+                                     hipMemcpyDeviceToHost, L.st) != hipSuccess)
+    e = fail(c, BEM_E_DEVICE, "copy of Z failed");
+  if (!e && hipStreamSynchronize(L.st) != hipSuccess) e = fail(c, BEM_E_DEVICE, "assembly failed on the device");
+  free_allocs(c, allocs, tb);
+  return e;
 }
 
 int bem_spectral_power(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *q, bem_result *res) {
   if (!c || !b1 || !b2 || !q || !res) return fail(c, BEM_E_INVALID, "bad arguments");
-  if (!(q->eV > 0) || !(q->a > 0)) return fail(c, BEM_E_INVALID, "eV and a must be positive");
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   std::memset(res, 0, sizeof *res);
-  if (q->precision != BEM_PREC_FP64) return fail(c, BEM_E_UNSUPPORTED, "precision mode %d not available", q->precision);
-  return spectral_power_fp64(c, b1, b2, q, res);
+  CHK(check_request(c, q));
+  stats_reset(c);
+  PairGeom *g;
+  CHK(pair_geom(c, b1, b2, q->gap, &g));
+  Lane &L = c->lanes[0];
+  CHK(lane_finish(c, L));
+  int e = lane_issue(c, L, *g, q, res);
+  int f = lane_finish(c, L);
+  if (e) return e;
+  c->t_last[6] = c->gemm_ms;
+  c->t_last[7] = c->gemm_n;
+  c->t_last[8] = c->gemm_flops;
+  return f;
 }
 
 int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, int n, bem_result *res) {
-  if (!c || !reqs || !res || n < 0) return fail(c, BEM_E_INVALID, "bad arguments");
+  if (!c || !b1 || !b2 || !reqs || !res || n < 0) return fail(c, BEM_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  stats_reset(c);
   int first_err = 0;
+  const int nl = (int)c->lanes.size();
   for (int i = 0; i < n; i++) {
-    int e = bem_spectral_power(c, b1, b2, &reqs[i], &res[i]);
-    res[i].status = e;
-    if (e && !first_err) first_err = e;
+    std::memset(&res[i], 0, sizeof res[i]);
+    Lane &L = c->lanes[i % nl];
+    int f = lane_finish(c, L);
+    if (f && !first_err) first_err = f;
+    int e = check_request(c, &reqs[i]);
+    PairGeom *g = nullptr;
+    if (!e) e = pair_geom(c, b1, b2, reqs[i].gap, &g);
+    if (!e) e = lane_issue(c, L, *g, &reqs[i], &res[i]);
+    if (e) {
+      res[i].status = e;
+      if (!first_err) first_err = e;
+      (void)hipStreamSynchronize(L.st);
+      L.busy = false;
+    }
   }
+  for (auto &L : c->lanes) {
+    int f = lane_finish(c, L);
+    if (f && !first_err) first_err = f;
+  }
+  c->t_last[6] = c->gemm_ms;
+  c->t_last[7] = c->gemm_n;
+  c->t_last[8] = c->gemm_flops;
   return first_err;
 }
 
@@ -651,16 +800,19 @@ int bem_zgetrf(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv) {
   if (!c || !A || !ipiv || n < 0 || lda < n) return fail(c, BEM_E_INVALID, "bad arguments");
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_finish(c, L));
+  stats_reset(c);
   int *info;
   double2 *inv;
-  CHK(ws_get(c, "info_getrf", sizeof(int), (void **)&info));
-  CHK(ws_get(c, "inv_getrf", inv_bytes(n), (void **)&inv));
-  DEV(c, hipMemsetAsync(info, 0, sizeof(int), c->stream));
-  ev_reset(c);
-  CHK(lu_factor(c, (double2 *)A, lda, n, ipiv, info, inv));
+  CHK(ws_get(c, L, "info_getrf", sizeof(int), (void **)&info));
+  CHK(ws_get(c, L, "inv_getrf", inv_bytes(n), (void **)&inv));
+  DEV(c, hipMemsetAsync(info, 0, sizeof(int), L.st));
+  CHK(lu_factor(c, L, (double2 *)A, lda, n, ipiv, info, inv));
   int h = 0;
-  DEV(c, hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-  DEV(c, hipStreamSynchronize(c->stream));
+  DEV(c, hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  ev_collect(c, L);
   if (h) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", h - 1);
   return 0;
 }
@@ -669,13 +821,16 @@ int bem_zgetrs(bem_ctx *c, const void *A, int64_t lda, int n, const int32_t *ipi
   if (!c || !A || !ipiv || !B || n < 0 || lda < n || ldb < n) return fail(c, BEM_E_INVALID, "bad arguments");
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
-  ev_reset(c);
+  Lane &L = c->lanes[0];
+  CHK(lane_finish(c, L));
+  stats_reset(c);
   double2 *inv;
-  CHK(ws_get(c, "inv_getrs", inv_bytes(n), (void **)&inv));
-  int e = bem_diag_block((double2 *)A, lda, n, 0, (n + LU_NB - 1) / LU_NB, 0, inv, nullptr, c->stream);
+  CHK(ws_get(c, L, "inv_getrs", inv_bytes(n), (void **)&inv));
+  int e = bem_diag_block((double2 *)A, lda, n, 0, (n + LU_NB - 1) / LU_NB, 0, inv, nullptr, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "diag block inverses: %d", e);
-  CHK(lu_solve(c, (const double2 *)A, lda, n, ipiv, inv, (double2 *)B, ldb, nrhs));
-  DEV(c, hipStreamSynchronize(c->stream));
+  CHK(lu_solve(c, L, (const double2 *)A, lda, n, ipiv, inv, (double2 *)B, ldb, nrhs));
+  DEV(c, hipStreamSynchronize(L.st));
+  ev_collect(c, L);
   return 0;
 }
 
@@ -684,15 +839,16 @@ int bem_zgemm(bem_ctx *c, int m, int n, int k, const double *alpha, const void *
   if (!c || !alpha || !beta || m < 0 || n < 0 || k < 0) return fail(c, BEM_E_INVALID, "bad arguments");
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
-  ev_reset(c);
-  CHK(gemm(c, m, n, k, make_double2(alpha[0], alpha[1]), (const double2 *)A, lda, (const double2 *)B, ldb,
+  Lane &L = c->lanes[0];
+  CHK(lane_finish(c, L));
+  stats_reset(c);
+  CHK(gemm(c, L, m, n, k, make_double2(alpha[0], alpha[1]), (const double2 *)A, lda, (const double2 *)B, ldb,
            make_double2(beta[0], beta[1]), (double2 *)C, ldc));
-  DEV(c, hipStreamSynchronize(c->stream));
-  double gms, gn, gfl;
-  ev_collect(c, &gms, &gn, &gfl);
-  c->t_last[6] = gms;
-  c->t_last[7] = gn;
-  c->t_last[8] = gfl;
+  DEV(c, hipStreamSynchronize(L.st));
+  ev_collect(c, L);
+  c->t_last[6] = c->gemm_ms;
+  c->t_last[7] = c->gemm_n;
+  c->t_last[8] = c->gemm_flops;
   return 0;
 }
 
@@ -713,7 +869,7 @@ int bem_device_free(bem_ctx *c, void *ptr) {
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   auto it = c->user_allocs.find(ptr);
   if (it == c->user_allocs.end()) return fail(c, BEM_E_INVALID, "pointer not allocated by this context");
-  DEV(c, hipStreamSynchronize(c->stream));
+  DEV(c, hipDeviceSynchronize());
   DEV(c, hipFree(ptr));
   c->bytes -= it->second;
   c->user_allocs.erase(it);
@@ -728,7 +884,7 @@ int bem_memcpy_h2d(bem_ctx *c, void *dst, const void *src, size_t bytes) {
 
 int bem_memcpy_d2h(bem_ctx *c, void *dst, const void *src, size_t bytes) {
   if (!c) return BEM_E_INVALID;
-  DEV(c, hipStreamSynchronize(c->stream));
+  DEV(c, hipDeviceSynchronize());
   DEV(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
   return 0;
 }

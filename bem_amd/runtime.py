@@ -40,7 +40,8 @@ class BemError(RuntimeError):
 
 
 class _Cfg(ctypes.Structure):
-    _fields_ = [("device", ctypes.c_int), ("lu_block", ctypes.c_int), ("hbm_budget", ctypes.c_size_t)]
+    _fields_ = [("device", ctypes.c_int), ("lu_block", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("lanes", ctypes.c_int)]
 
 
 class _Req(ctypes.Structure):
@@ -178,9 +179,11 @@ class Body:
         return out
 
     def free(self):
-        if self.handle:
+        # a body belongs to its context: once the context is gone its memory
+        # went with the process's device state and must not be touched
+        if self.handle and getattr(self.ctx, "h", None):
             self.ctx.lib.bem_body_free(self.handle)
-            self.handle = None
+        self.handle = None
 
     def __del__(self):
         try:
@@ -192,9 +195,9 @@ class Body:
 class Context:
     """One HIP device context (bem_ctx)."""
 
-    def __init__(self, device=0, lu_block=0, hbm_budget=0):
+    def __init__(self, device=0, hbm_budget=0, lanes=0):
         self.lib = load_library()
-        cfg = _Cfg(device, lu_block, hbm_budget)
+        cfg = _Cfg(device, 0, hbm_budget, lanes)
         h = ctypes.c_void_p()
         rc = self.lib.bem_ctx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

@@ -57,6 +57,7 @@ typedef struct {
   int device;        /* HIP device ordinal */
   int lu_block;      /* reserved (the LU panel width is fixed at 64), pass 0 */
   size_t hbm_budget; /* bytes a context may hold, 0 = no limit */
+  int lanes;         /* solves a sweep keeps in flight, one stream each (0 = 2, max 8) */
 } bem_cfg;
 
 typedef struct {
@@ -111,7 +112,9 @@ int bem_pair_histogram(bem_ctx *ctx, int nb, bem_body *const *bodies, const doub
 
 /* ---- spectral power (CalcHeatTransfer) -------------------------------- */
 int bem_spectral_power(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *req, bem_result *res);
-/* n independent requests on the same body pair, results in request order. */
+/* n independent requests on the same body pair, results in request order.
+ * Up to cfg.lanes requests run concurrently on separate streams; each result
+ * is computed by the same kernels in the same order as when run alone. */
 int bem_sweep(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *reqs, int n, bem_result *res);
 
 /* ---- dense complex128 kernels on device pointers ---------------------- */
@@ -128,10 +131,10 @@ int bem_device_alloc(bem_ctx *ctx, size_t bytes, void **ptr);
 int bem_device_free(bem_ctx *ctx, void *ptr);
 int bem_memcpy_h2d(bem_ctx *ctx, void *dst, const void *src, size_t bytes);
 int bem_memcpy_d2h(bem_ctx *ctx, void *dst, const void *src, size_t bytes);
-/* Elapsed device time (ms) of the kernels launched by the last call, by kind:
- * out[0] assembly, out[1] far-g6 kernel, out[2] close kernel, out[3] gather,
- * out[4] factor+solves, out[5] products+trace, out[6] zgemm total,
- * out[7] number of zgemm launches, out[8] zgemm flops (real). */
+/* Device-event timings of the last call: out[0] assembly, out[4] factor and
+ * solves, out[5] products and trace (ms, last solve of the call); out[6] sum
+ * of the durations of every zgemm launch of the call (ms), out[7] their
+ * number, out[8] their real flops (8 m n k each).  out[1..3] are reserved. */
 int bem_last_timings(bem_ctx *ctx, double *out9);
 
 #ifdef __cplusplus

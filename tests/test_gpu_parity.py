@@ -109,3 +109,17 @@ def test_singular_request_reports_error(gpu_ctx):
         assert not np.isfinite(r["ReS"]) or r["status"] != 0 or True
     except BemError:
         pass
+
+
+def test_sweep_matches_single_solves(gpu_ctx):
+    """A sweep keeps several solves in flight on separate streams; each result
+    must be bit-identical to the same request solved alone."""
+    from bem_amd import mesh, Request, lorentz_eps
+    s = mesh.icosphere(2)
+    b1, b2 = gpu_ctx.body(s), gpu_ctx.body(s)
+    reqs = [Request(eV=f, gap=(2.3, 0, 0), eps1=lorentz_eps(f), eps2=lorentz_eps(f)) for f in (0.09, 0.1, 0.112, 0.13)]
+    alone = [gpu_ctx.calc_heat_transfer(b1, b2, q) for q in reqs]
+    mixed = gpu_ctx.sweep(b1, b2, reqs)
+    for a, m in zip(alone, mixed):
+        assert m["status"] == 0
+        assert a["ReS"] == m["ReS"] and a["ImS"] == m["ImS"] and a["P"] == m["P"]
