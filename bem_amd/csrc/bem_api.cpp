@@ -52,9 +52,11 @@ int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0,
 int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
               hipStream_t st);
 int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st);
-int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int factor, double2 *inv, int *info,
+int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int bw, int factor, double2 *inv, int *info,
                    hipStream_t st);
 int bem_set_identity(double2 *X, int64_t ld, int n, hipStream_t st);
+int bem_diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st);
+int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st);
 }
 
 // src/util/SI.cpp:3-9
@@ -103,6 +105,7 @@ struct Lane {
   hipEvent_t ph[4] = {nullptr, nullptr, nullptr, nullptr};
   double2 *h_trace = nullptr;   // pinned host slots of the results
   int *h_info = nullptr;
+  unsigned long long *h_growth = nullptr;   // max|U|, max|A| of LU(A), then of LU(S)
   bool busy = false;
   bem_result *res = nullptr;
   bem_request req;
@@ -236,31 +239,90 @@ static int copy_back(bem_ctx *c, Lane &L, double2 *dst, int64_t ld, const double
   return 0;
 }
 
-// Blocked right-looking LU with tournament-pivoted panels.  inv receives the
-// inverses of the unit-lower and upper diagonal blocks (inv_bytes(n)); the
-// panel's two triangular solves run as GEMMs against them.
+// Blocked right-looking LU with tournament-pivoted panels.  Each 64-column
+// block is factored as two 32-column sub-panels (tournament pivots, diagonal
+// 32 x 32 factor + inverses, L21 by GEMM, then the second sub-panel updated);
+// inv receives the inverses of the 64 x 64 unit-lower and upper diagonal
+// blocks (inv_bytes(n)), and every triangular solve runs as a GEMM.
+static const int CALU_W = 32;
+
 static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *ipiv, int *info, double2 *inv) {
-  const int nb = LU_NB;
-  int sets = (n + 255) / 256;
+  const int nb = LU_NB, sb = CALU_W;
+  int sets = (n + 511) / 512;
   int *cand, *cand2;
-  double2 *tmp;
-  CHK(ws_get(c, L, "calu_cand", sizeof(int) * 64 * (size_t)sets, (void **)&cand));
-  CHK(ws_get(c, L, "calu_cand2", sizeof(int) * 64 * (size_t)((sets + 3) / 4 + 1), (void **)&cand2));
+  double2 *tmp, *inv32;
+  CHK(ws_get(c, L, "calu_cand", sizeof(int) * sb * (size_t)sets, (void **)&cand));
+  CHK(ws_get(c, L, "calu_cand2", sizeof(int) * sb * (size_t)((sets + 15) / 16 + 1), (void **)&cand2));
   CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
+  CHK(ws_get(c, L, "inv32", sizeof(double2) * 2 * 64 * 64 * (size_t)((n + sb - 1) / sb), (void **)&inv32));
   const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
-    int e = bem_calu_pivots(A, lda, n - k0, jb, k0, k0, cand, cand2, ipiv, info, L.st);
-    if (e) return fail(c, BEM_E_DEVICE, "calu pivots: %d", e);
-    e = bem_laswp(A, lda, 0, n, 0, 0, k0, k0 + jb, ipiv, L.st);
-    if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
-    double2 *Akk = A + (int64_t)k0 * lda + k0;
+    for (int s0 = k0; s0 < k0 + jb; s0 += sb) {
+      int sw = std::min(sb, k0 + jb - s0);
+      int e = bem_calu_pivots(A, lda, n - s0, sw, s0, s0, cand, cand2, ipiv, info, L.st);
+      if (e) return fail(c, BEM_E_DEVICE, "calu pivots: %d", e);
+      e = bem_laswp(A, lda, 0, n, 0, 0, s0, s0 + sw, ipiv, L.st);
+      if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
+      const int b32 = s0 / sb;
+      e = bem_diag_block(A, lda, n, b32, 1, sb, 1, inv32, info, L.st);
+      if (e) return fail(c, BEM_E_DEVICE, "diag block: %d", e);
+      const double2 *Li = inv32 + (size_t)b32 * 2 * 64 * 64, *Ui = Li + 64 * 64;
+      int rest = n - s0 - sw;
+      double2 *Ass = A + (int64_t)s0 * lda + s0;
+      if (rest > 0) {
+        CHK(gemm(c, L, rest, sw, sw, one, Ass + sw, lda, Ui, 64, zero, tmp, rest));   // L21 = A21 U11^-1
+        CHK(copy_back(c, L, Ass + sw, lda, tmp, rest, sw));
+      }
+      int cols2 = k0 + jb - (s0 + sw);   // columns of this 64-block right of the sub-panel
+      if (cols2 > 0) {
+        double2 *A12 = A + (int64_t)(s0 + sw) * lda + s0;
+        CHK(gemm(c, L, sw, cols2, sw, one, Li, 64, A12, lda, zero, tmp, sw));      // U12 = L11^-1 A12
+        CHK(copy_back(c, L, A12, lda, tmp, sw, cols2));
+        if (rest > 0) CHK(gemm(c, L, rest, cols2, sw, m1, Ass + sw, lda, A12, lda, one, A12 + sw, lda));
+      }
+    }
     const int blk = k0 / nb;
-    e = bem_diag_block(A, lda, n, blk, 1, 1, inv, info, L.st);
-    if (e) return fail(c, BEM_E_DEVICE, "diag block: %d", e);
-    const double2 *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
+    int e = bem_diag_block(A, lda, n, blk, 1, nb, 0, inv, nullptr, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "diag block inverses: %d", e);
+    const double2 *Linv = inv + (size_t)blk * 2 * 64 * 64;
     int rest = n - k0 - jb;
     if (rest > 0) {
+      double2 *Akk = A + (int64_t)k0 * lda + k0;
+      double2 *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
+      CHK(gemm(c, L, jb, rest, jb, one, Linv, 64, A12, lda, zero, tmp, jb));     // U12 = L11^-1 A12
+      CHK(copy_back(c, L, A12, lda, tmp, jb, rest));
+      CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
+    }
+  }
+  return 0;
+}
+
+// Blocked right-looking LU with partial pivoting inside each 64 x 64
+// diagonal block (one kernel factors the block in LDS and inverts its
+// triangles); the row interchanges are then applied to the other columns and
+// the panel's triangular solves and the trailing update run as GEMMs.
+// growth (optional, device) receives max|A| before in growth[1] and
+// max|U| after in growth[0].
+static int lu_factor_block(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *ipiv, int *info, double2 *inv,
+                           unsigned long long *growth) {
+  const int nb = LU_NB;
+  double2 *tmp;
+  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
+  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  int e;
+  if (growth && (e = bem_absmax(A, lda, n, n, 0, growth + 1, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  for (int k0 = 0; k0 < n; k0 += nb) {
+    int jb = std::min(nb, n - k0);
+    const int blk = k0 / nb;
+    double2 *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
+    e = bem_diag_lu_piv(A, lda, n, k0, Linv, ipiv, info, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
+    e = bem_laswp(A, lda, 0, n, k0, k0 + jb, k0, k0 + jb, ipiv, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
+    int rest = n - k0 - jb;
+    if (rest > 0) {
+      double2 *Akk = A + (int64_t)k0 * lda + k0;
       double2 *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
       CHK(gemm(c, L, rest, jb, jb, one, A21, lda, Uinv, 64, zero, tmp, rest));   // L21 = A21 U11^-1
       CHK(copy_back(c, L, A21, lda, tmp, rest, jb));
@@ -269,6 +331,7 @@ static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *i
       CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
     }
   }
+  if (growth && (e = bem_absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
   return 0;
 }
 
@@ -456,6 +519,8 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   double2 *W, *G1, *G2, *Sinv, *X, *B1, *partial, *invA, *invS;
   double *tiles;
   int *ipA, *info;
+  unsigned long long *gro;
+  const bool tournament = (q->flags & BEM_SOLVE_TOURNAMENT) != 0;
   size_t npairs_top = (size_t)g.top.T * g.top.T;
   CHK(ws_get(c, L, "W", sizeof(double2) * (size_t)M * M, (void **)&W));
   CHK(ws_get(c, L, "G1", sizeof(double2) * (size_t)m1 * m1, (void **)&G1));
@@ -464,6 +529,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(M + 64), (void **)&ipA));
   int *ipS = ipA + m1;
   CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
+  CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 4, (void **)&gro));
   CHK(ws_get(c, L, "invA", inv_bytes(m1), (void **)&invA));
   CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
   CHK(ws_get(c, L, "Sinv", sizeof(double2) * (size_t)m2 * m2, (void **)&Sinv));
@@ -473,6 +539,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
   double2 *trace_out = partial + np;
   DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
+  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 4, L.st));
 
   DEV(c, hipEventRecord(L.ph[0], L.st));
   // ---- assembly: W = Z_ext(top), then + Z_int(b) on the diagonal blocks
@@ -493,10 +560,12 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
   double2 *A = W, *Bm = W + (int64_t)m1 * M, *Cm = W + m1, *D = W + (int64_t)m1 * M + m1;
-  CHK(lu_factor(c, L, A, M, m1, ipA, info, invA));
+  if (tournament) CHK(lu_factor(c, L, A, M, m1, ipA, info, invA));
+  else CHK(lu_factor_block(c, L, A, M, m1, ipA, info, invA, gro));
   CHK(lu_solve(c, L, A, M, m1, ipA, invA, Bm, M, m2));   // B <- Y
   CHK(gemm(c, L, m2, m2, m1, mone, Cm, M, Bm, M, one, D, M));
-  CHK(lu_factor(c, L, D, M, m2, ipS, info + 1, invS));
+  if (tournament) CHK(lu_factor(c, L, D, M, m2, ipS, info + 1, invS));
+  else CHK(lu_factor_block(c, L, D, M, m2, ipS, info + 1, invS, gro + 2));
   e = bem_set_identity(Sinv, m2, m2, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "identity: %d", e);
   CHK(lu_solve(c, L, D, M, m2, ipS, invS, Sinv, m2, m2));
@@ -518,6 +587,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   DEV(c, hipEventRecord(L.ph[3], L.st));
   DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2), hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, L.st));
   L.busy = true;
   return 0;
 }
@@ -548,6 +618,15 @@ static int lane_finish(bem_ctx *c, Lane &L) {
   res->ms_products = tp;
   res->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - L.t0).count();
   res->status = 0;
+  res->pivoting = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  res->growth = 0.0;
+  if (!res->pivoting) {
+    double g[4];
+    std::memcpy(g, L.h_growth, sizeof g);
+    double ga = g[1] > 0 ? g[0] / g[1] : 0.0, gs = g[3] > 0 ? g[2] / g[3] : 0.0;
+    res->growth = std::max(ga, gs);
+    if (!std::isfinite(res->growth)) res->growth = INFINITY;
+  }
   c->t_last[0] = ta;
   c->t_last[4] = tf;
   c->t_last[5] = tp;
@@ -590,6 +669,7 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     for (int k = 0; k < 4 && ok; k++) ok = hipEventCreate(&L.ph[k]) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_trace, sizeof(double2), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_info, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_growth, sizeof(unsigned long long) * 4, hipHostMallocDefault) == hipSuccess;
     if (!ok) {
       bem_ctx_destroy(c);
       return BEM_E_DEVICE;
@@ -617,6 +697,7 @@ void bem_ctx_destroy(bem_ctx *c) {
       if (L.ph[k]) (void)hipEventDestroy(L.ph[k]);
     if (L.h_trace) (void)hipHostFree(L.h_trace);
     if (L.h_info) (void)hipHostFree(L.h_info);
+    if (L.h_growth) (void)hipHostFree(L.h_growth);
     if (L.st) (void)hipStreamDestroy(L.st);
   }
   delete c;
@@ -757,6 +838,13 @@ int bem_spectral_power(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request
   int e = lane_issue(c, L, *g, q, res);
   int f = lane_finish(c, L);
   if (e) return e;
+  if (!f && !(res->growth <= BEM_GROWTH_LIMIT) && !(q->flags & BEM_SOLVE_TOURNAMENT)) {
+    bem_request q2 = *q;
+    q2.flags |= BEM_SOLVE_TOURNAMENT;
+    e = lane_issue(c, L, *g, &q2, res);
+    f = lane_finish(c, L);
+    if (e) return e;
+  }
   c->t_last[6] = c->gemm_ms;
   c->t_last[7] = c->gemm_n;
   c->t_last[8] = c->gemm_flops;
@@ -789,6 +877,22 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
   for (auto &L : c->lanes) {
     int f = lane_finish(c, L);
     if (f && !first_err) first_err = f;
+  }
+  // requests whose block-local pivoting grew too much are solved again with
+  // tournament pivoting
+  for (int i = 0; i < n; i++) {
+    if (res[i].status || res[i].pivoting || res[i].growth <= BEM_GROWTH_LIMIT) continue;
+    bem_request q2 = reqs[i];
+    q2.flags |= BEM_SOLVE_TOURNAMENT;
+    Lane &L = c->lanes[0];
+    PairGeom *g = nullptr;
+    int e = pair_geom(c, b1, b2, q2.gap, &g);
+    if (!e) e = lane_issue(c, L, *g, &q2, &res[i]);
+    int f = e ? e : lane_finish(c, L);
+    if (f) {
+      res[i].status = f;
+      if (!first_err) first_err = f;
+    }
   }
   c->t_last[6] = c->gemm_ms;
   c->t_last[7] = c->gemm_n;
@@ -826,7 +930,7 @@ int bem_zgetrs(bem_ctx *c, const void *A, int64_t lda, int n, const int32_t *ipi
   stats_reset(c);
   double2 *inv;
   CHK(ws_get(c, L, "inv_getrs", inv_bytes(n), (void **)&inv));
-  int e = bem_diag_block((double2 *)A, lda, n, 0, (n + LU_NB - 1) / LU_NB, 0, inv, nullptr, L.st);
+  int e = bem_diag_block((double2 *)A, lda, n, 0, (n + LU_NB - 1) / LU_NB, LU_NB, 0, inv, nullptr, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "diag block inverses: %d", e);
   CHK(lu_solve(c, L, (const double2 *)A, lda, n, ipiv, inv, (double2 *)B, ldb, nrhs));
   DEV(c, hipStreamSynchronize(L.st));

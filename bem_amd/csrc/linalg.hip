@@ -185,25 +185,28 @@ __global__ void sum_partials_kernel(const double2 *__restrict__ partial, int n, 
 // eliminations (izamax criterion |re|+|im|) over blocks of rows, instead of
 // one global argmax per column, so a panel needs no grid-wide barrier.
 //
-// Row storage: the 64 columns of a row are split over a lane pair (2r, 2r+1),
-// 32 complex values each (128 VGPRs), so a 512-thread block holds 256 rows
-// without spilling.  A column step costs two block barriers; the LDS used by
+// Row storage: the NB (<= 32) columns of a row are split over a lane pair
+// (2r, 2r+1), NB/2 complex values each, so a 1024-thread block holds 512 rows
+// in registers.  A column step costs two block barriers; the LDS used by
 // step J is double-buffered on J's parity so no closing barrier is needed.
-#define GEPP_NT 512
+#define GEPP_NB 32
+#define GEPP_NT 1024
 #define GEPP_ROWS (GEPP_NT / 2)
+#define GEPP_HALF (GEPP_NB / 2)
+#define GEPP_MERGE (GEPP_ROWS / GEPP_NB)
 
 struct GeppLds {
   double val[2][GEPP_NT / 64];
   int idx[2][GEPP_NT / 64];
-  double2 prow[2][64];
+  double2 prow[2][GEPP_NB];
 };
 
 template <int J>
-__device__ __forceinline__ void gepp2_step(double2 (&a)[32], bool valid, bool &chosen, int rowid, int w, int *out,
-                                           GeppLds &L) {
-  if constexpr (J < 64) {
+__device__ __forceinline__ void gepp2_step(double2 (&a)[GEPP_HALF], bool valid, bool &chosen, int rowid, int w,
+                                           int *out, GeppLds &L) {
+  if constexpr (J < GEPP_NB) {
     if (J < w) {
-      constexpr int H = J / 32, CJ = J % 32, P = J & 1;
+      constexpr int H = J / GEPP_HALF, CJ = J % GEPP_HALF, P = J & 1;
       const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, half = tid & 1;
       const bool owner = (half == H);
       double best = (valid && !chosen && owner) ? fabs(a[CJ].x) + fabs(a[CJ].y) : -1.0;
@@ -230,7 +233,8 @@ __device__ __forceinline__ void gepp2_step(double2 (&a)[32], bool valid, bool &c
       if (is_piv) {
         chosen = true;
 #pragma unroll
-        for (int c = 0; c < 32; c++) L.prow[P][32 * half + c] = a[c];
+        for (int c = 0; c < GEPP_HALF; c++)
+          if (GEPP_HALF * half + c >= J) L.prow[P][GEPP_HALF * half + c] = a[c];
         if (half == 0) out[J] = rowid;
       }
       if (none && tid == 0) out[J] = -1;
@@ -251,8 +255,8 @@ __device__ __forceinline__ void gepp2_step(double2 (&a)[32], bool valid, bool &c
         if (!owner) l = make_double2(lx, ly);
         if (act) {
 #pragma unroll
-          for (int c = 0; c < 32; c++) {
-            const int g = 32 * half + c;   // global column of this value
+          for (int c = 0; c < GEPP_HALF; c++) {
+            const int g = GEPP_HALF * half + c;   // column of this value
             if (g > J) {
               double2 uc = L.prow[P][g];
               a[c].x -= l.x * uc.x - l.y * uc.y;
@@ -268,43 +272,43 @@ __device__ __forceinline__ void gepp2_step(double2 (&a)[32], bool valid, bool &c
 
 // Loads row `id` (matrix row, or -1 for none) columns [c0, c0+w) into the lane
 // pair's halves.
-__device__ __forceinline__ void gepp2_load(double2 (&a)[32], const double2 *__restrict__ A, int64_t lda, int id, int w,
-                                           int c0) {
+__device__ __forceinline__ void gepp2_load(double2 (&a)[GEPP_HALF], const double2 *__restrict__ A, int64_t lda,
+                                           int id, int w, int c0) {
   const int half = threadIdx.x & 1;
 #pragma unroll
-  for (int c = 0; c < 32; c++) {
-    int g = 32 * half + c;
+  for (int c = 0; c < GEPP_HALF; c++) {
+    int g = GEPP_HALF * half + c;
     a[c] = (id >= 0 && g < w) ? A[(int64_t)(c0 + g) * lda + id] : make_double2(0.0, 0.0);
   }
 }
 
-// Leaves: block b takes panel rows [b*256, b*256+256) of the m-row panel that
-// starts at matrix row r0 and emits w candidate row ids (64 slots per block).
+// Leaves: block b takes panel rows [b*512, b*512+512) of the m-row panel that
+// starts at matrix row r0 and emits w candidate row ids (GEPP_NB slots per block).
 __global__ void __launch_bounds__(GEPP_NT) tslu_leaf_kernel(const double2 *__restrict__ A, int64_t lda, int m, int w,
                                                             int r0, int c0, int *__restrict__ cand) {
   __shared__ GeppLds L;
   int r = blockIdx.x * GEPP_ROWS + (threadIdx.x >> 1);
   bool valid = r < m;
   int id = valid ? r0 + r : -1;
-  double2 a[32];
+  double2 a[GEPP_HALF];
   gepp2_load(a, A, lda, id, w, c0);
   bool chosen = false;
-  gepp2_step<0>(a, valid, chosen, id, w, cand + (size_t)blockIdx.x * 64, L);
+  gepp2_step<0>(a, valid, chosen, id, w, cand + (size_t)blockIdx.x * GEPP_NB, L);
 }
 
-// One tournament round: block b merges candidate sets 4b..4b+3 (64 slots
-// each, ids into the matrix, -1 = empty) by GEPP on their original rows.
+// One tournament round: block b merges candidate sets 16b..16b+15 (GEPP_NB
+// slots each, ids into the matrix, -1 = empty) by GEPP on their original rows.
 __global__ void __launch_bounds__(GEPP_NT) tslu_merge_kernel(const double2 *__restrict__ A, int64_t lda, int w, int c0,
                                                              const int *__restrict__ cin, int nsets,
                                                              int *__restrict__ cout) {
   __shared__ GeppLds L;
-  int r = threadIdx.x >> 1;            // 0..255
-  int set = 4 * blockIdx.x + r / 64, slot = r % 64;
-  int id = (set < nsets && slot < w) ? cin[(size_t)set * 64 + slot] : -1;
-  double2 a[32];
+  int r = threadIdx.x >> 1;            // 0..511
+  int set = GEPP_MERGE * blockIdx.x + r / GEPP_NB, slot = r % GEPP_NB;
+  int id = (set < nsets && slot < w) ? cin[(size_t)set * GEPP_NB + slot] : -1;
+  double2 a[GEPP_HALF];
   gepp2_load(a, A, lda, id, w, c0);
   bool chosen = false;
-  gepp2_step<0>(a, id >= 0, chosen, id, w, cout + (size_t)blockIdx.x * 64, L);
+  gepp2_step<0>(a, id >= 0, chosen, id, w, cout + (size_t)blockIdx.x * GEPP_NB, L);
 }
 
 // Winner list -> LAPACK-style interchanges ipiv[k0+i] (row k0+i swapped with
@@ -329,17 +333,20 @@ __global__ void __launch_bounds__(64) winners_to_ipiv_kernel(const int *__restri
 
 // The w x w diagonal block (rows already in pivot order) in one 256-thread
 // block: LU without pivoting in place in LDS (unit L below, U on and above),
-// written back to A; then both triangles are inverted in place (the unit-lower
-// and the upper inverse share the packed storage) and written to
+// written back to A; then the inverses of both triangles, each by a
+// right-looking sweep of rank-1 updates in a second LDS buffer, written to
 // inv + blk*2*64*64 as two 64 x 64 column-major matrices (Linv, Uinv),
 // identity-padded when w < 64.  With factor=0 only the inverses are formed
 // from an existing packed LU (the batched form used before triangular solves).
+// Thread t works on row t & 63 and on every fourth column from t >> 6.
 __global__ void __launch_bounds__(256) diag_block_kernel(double2 *__restrict__ A, int64_t lda, int n, int blk0,
-                                                         int factor, double2 *__restrict__ inv, int *__restrict__ info) {
+                                                         int bw, int factor, double2 *__restrict__ inv,
+                                                         int *__restrict__ info) {
   __shared__ double2 sA[64 * 64];   // column-major, padded to 64 x 64
-  const int blk = blk0 + blockIdx.x, k0 = blk * 64;
-  const int w = min(64, n - k0);
-  const int tid = threadIdx.x;
+  __shared__ double2 sX[64 * 64];   // inverse being formed, column-major
+  const int blk = blk0 + blockIdx.x, k0 = blk * bw;
+  const int w = min(bw, n - k0);
+  const int tid = threadIdx.x, ri = tid & 63, cg = tid >> 6;
   double2 *Ak = A + (int64_t)k0 * lda + k0;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
@@ -354,20 +361,21 @@ __global__ void __launch_bounds__(256) diag_block_kernel(double2 *__restrict__ A
         if (tid == 0 && info && *info == 0) *info = k0 + j + 1;
         continue;
       }
-      double2 ui = make_double2(u.x / den, -u.y / den);
-      if (tid > j && tid < 64) {
-        double2 x = sA[j * 64 + tid];
-        sA[j * 64 + tid] = make_double2(x.x * ui.x - x.y * ui.y, x.x * ui.y + x.y * ui.x);
+      if (cg == 0 && ri > j) {
+        double2 ui = make_double2(u.x / den, -u.y / den);
+        double2 x = sA[j * 64 + ri];
+        sA[j * 64 + ri] = make_double2(x.x * ui.x - x.y * ui.y, x.x * ui.y + x.y * ui.x);
       }
       __syncthreads();
-      int rem = 63 - j;
-      for (int idx = tid; idx < rem * rem; idx += 256) {
-        int i = j + 1 + idx % rem, c = j + 1 + idx / rem;
-        double2 l = sA[j * 64 + i], uc = sA[c * 64 + j];
-        double2 x = sA[c * 64 + i];
-        x.x -= l.x * uc.x - l.y * uc.y;
-        x.y -= l.x * uc.y + l.y * uc.x;
-        sA[c * 64 + i] = x;
+      if (ri > j) {
+        const double2 l = sA[j * 64 + ri];
+        for (int c = j + 1 + cg; c < 64; c += 4) {
+          double2 uc = sA[c * 64 + j];
+          double2 x = sA[c * 64 + ri];
+          x.x -= l.x * uc.x - l.y * uc.y;
+          x.y -= l.x * uc.y + l.y * uc.x;
+          sA[c * 64 + ri] = x;
+        }
       }
       __syncthreads();
     }
@@ -375,57 +383,250 @@ __global__ void __launch_bounds__(256) diag_block_kernel(double2 *__restrict__ A
       int r = idx % w, c = idx / w;
       Ak[(int64_t)c * lda + r] = sA[c * 64 + r];
     }
-    __syncthreads();
   }
-  // In-place inversion.  L (unit, strictly below the diagonal): columns from
-  // right to left, Linv[i][j] = -(L[i][j] + sum_{j<k<i} Linv[i][k] L[k][j]).
-  // U: columns left to right, Uinv[j][j] = 1/U[j][j],
-  // Uinv[i][j] = -(sum_{i<=k<j} Uinv[i][k] U[k][j]) / U[j][j].
-  // Thread t < 64 owns row t of the current L column and of the U column.
-  for (int s = 0; s < 64; s++) {
-    const int jl = 63 - s;     // L column this step
-    const int ju = s;          // U column this step
-    double2 nl = make_double2(0, 0), nu = make_double2(0, 0);
-    const int i = tid;
-    if (i < 64) {
-      if (i > jl) {
-        double2 acc = sA[jl * 64 + i];
-        for (int k = jl + 1; k < i; k++) {
-          double2 li = sA[k * 64 + i], lk = sA[jl * 64 + k];
-          acc.x += li.x * lk.x - li.y * lk.y;
-          acc.y += li.x * lk.y + li.y * lk.x;
-        }
-        nl = make_double2(-acc.x, -acc.y);
+  // Linv: X = I; for j: X[i][c] -= L[i][j] X[j][c] for rows i > j, columns c <= j.
+  for (int idx = tid; idx < 64 * 64; idx += 256) sX[idx] = make_double2((idx & 63) == (idx >> 6) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int j = 0; j < 63; j++) {
+    if (ri > j) {
+      const double2 l = sA[j * 64 + ri];
+      for (int c = cg; c <= j; c += 4) {
+        double2 xj = sX[c * 64 + j];
+        double2 x = sX[c * 64 + ri];
+        x.x -= l.x * xj.x - l.y * xj.y;
+        x.y -= l.x * xj.y + l.y * xj.x;
+        sX[c * 64 + ri] = x;
       }
-      double2 d = sA[ju * 64 + ju];
-      double den = d.x * d.x + d.y * d.y;
-      double2 dinv = make_double2(d.x / den, -d.y / den);
-      if (i < ju) {
-        double2 acc = make_double2(0, 0);
-        for (int k = i; k < ju; k++) {
-          double2 ui = sA[k * 64 + i], uk = sA[ju * 64 + k];
-          acc.x += ui.x * uk.x - ui.y * uk.y;
-          acc.y += ui.x * uk.y + ui.y * uk.x;
-        }
-        nu = make_double2(-(acc.x * dinv.x - acc.y * dinv.y), -(acc.x * dinv.y + acc.y * dinv.x));
-      } else if (i == ju) {
-        nu = dinv;
-      }
-    }
-    __syncthreads();
-    if (i < 64) {
-      if (i > jl) sA[jl * 64 + i] = nl;
-      if (i <= ju) sA[ju * 64 + i] = nu;
     }
     __syncthreads();
   }
   double2 *Li = inv + (size_t)blk * 2 * 64 * 64, *Ui = Li + 64 * 64;
+  for (int idx = tid; idx < 64 * 64; idx += 256) Li[idx] = sX[idx];
+  __syncthreads();
+  // Uinv: X = I; for k = 63..0: X[k][:] /= U[k][k]; X[i][c] -= U[i][k] X[k][c]
+  // for rows i < k, columns c >= k.
+  for (int idx = tid; idx < 64 * 64; idx += 256) sX[idx] = make_double2((idx & 63) == (idx >> 6) ? 1.0 : 0.0, 0.0);
+  __syncthreads();
+  for (int k = 63; k >= 0; k--) {
+    if (tid >= k && tid < 64) {   // row k, column tid >= k
+      double2 d = sA[k * 64 + k];
+      double den = d.x * d.x + d.y * d.y;
+      double2 x = sX[tid * 64 + k];
+      sX[tid * 64 + k] = make_double2((x.x * d.x + x.y * d.y) / den, (x.y * d.x - x.x * d.y) / den);
+    }
+    __syncthreads();
+    if (ri < k) {
+      const double2 u = sA[k * 64 + ri];
+      for (int c = k + cg; c < 64; c += 4) {
+        double2 xk = sX[c * 64 + k];
+        double2 x = sX[c * 64 + ri];
+        x.x -= u.x * xk.x - u.y * xk.y;
+        x.y -= u.x * xk.y + u.y * xk.x;
+        sX[c * 64 + ri] = x;
+      }
+    }
+    __syncthreads();
+  }
+  for (int idx = tid; idx < 64 * 64; idx += 256) Ui[idx] = sX[idx];
+}
+
+// Block-local partial pivoting (the default LU).  The w x w diagonal block at
+// (k0, k0) is factored with partial pivoting over its own rows (izamax
+// criterion |re|+|im|), the reference's block LU over tree blocks pivots the
+// same way (src/util/HMatrix.h:277-316).  One 256-thread block:
+//  1. LU with pivoting in LDS: wave 0 finds the pivot, rows are swapped in LDS,
+//     the multipliers go to a column buffer, all threads update the trailing
+//     block (three barriers per column); interchanges to ipiv[k0..k0+w) as
+//     absolute row indices; the packed factors are written back to A.
+//  2. Inverses of both triangles by recursive doubling over 1, 2, 4, .. 32
+//     blocks: Linv21 = -Linv22 L21 Linv11, Uinv12 = -Uinv11 U12 Uinv22, each
+//     level two small products, three barriers per level.
+//  Output: inv (64 x 64 Linv then 64 x 64 Uinv, column-major, identity
+//  padded when w < 64).
+__global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ A, int64_t lda, int n, int k0,
+                                                          double2 *__restrict__ inv, int *__restrict__ ipiv,
+                                                          int *__restrict__ info) {
+  __shared__ double2 sA[64 * 64];   // column-major: packed LU, later workspace
+  __shared__ double2 sX[64 * 64];   // packed inverses (Linv strictly below, Uinv on/above)
+  __shared__ double2 lcol[64];
+  __shared__ int s_piv;
+  const int w = min(64, n - k0);
+  const int tid = threadIdx.x, ri = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
+  double2 *Ak = A + (int64_t)k0 * lda + k0;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
-    double2 v = sA[idx];
+    sA[idx] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+  __syncthreads();
+  for (int j = 0; j < w; j++) {
+    if (wave == 0) {
+      double v = (ri >= j && ri < w) ? fabs(sA[j * 64 + ri].x) + fabs(sA[j * 64 + ri].y) : -1.0;
+      int id = ri;
+      for (int off = 32; off >= 1; off >>= 1) {
+        double ov = __shfl_xor(v, off, 64);
+        int oi = __shfl_xor(id, off, 64);
+        if (ov > v || (ov == v && oi < id)) { v = ov; id = oi; }
+      }
+      if (lane == 0) {
+        if (!(v > 0.0)) {   // zero column: keep the row, flag it
+          id = j;
+          if (info && *info == 0) *info = k0 + j + 1;
+        }
+        s_piv = id;
+        ipiv[k0 + j] = k0 + id;
+      }
+    }
+    __syncthreads();
+    const int p = s_piv;
+    if (p != j && tid < 64) {
+      double2 t = sA[tid * 64 + j];
+      sA[tid * 64 + j] = sA[tid * 64 + p];
+      sA[tid * 64 + p] = t;
+    }
+    __syncthreads();
+    const double2 u = sA[j * 64 + j];
+    const double den = u.x * u.x + u.y * u.y;
+    if (cg == 0 && ri > j) {
+      double2 l = make_double2(0.0, 0.0);
+      if (den > 0.0) {
+        double2 x = sA[j * 64 + ri];
+        double ur = u.x / den, ui = -u.y / den;
+        l = make_double2(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
+      }
+      sA[j * 64 + ri] = l;
+      lcol[ri] = l;
+    }
+    __syncthreads();
+    if (ri > j) {
+      const double2 l = lcol[ri];
+      for (int c = j + 1 + cg; c < 64; c += 4) {
+        double2 uc = sA[c * 64 + j];
+        double2 x = sA[c * 64 + ri];
+        x.x -= l.x * uc.x - l.y * uc.y;
+        x.y -= l.x * uc.y + l.y * uc.x;
+        sA[c * 64 + ri] = x;
+      }
+    }
+  }
+  __syncthreads();
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int r = idx & 63, c = idx >> 6;
+    if (r < w && c < w) Ak[(int64_t)c * lda + r] = sA[idx];
+  }
+  // level 0: diagonal of the inverses (Linv unit, Uinv 1/u)
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int r = idx & 63, c = idx >> 6;
+    double2 v = make_double2(0.0, 0.0);
+    if (r == c) {
+      double2 d = sA[idx];
+      double den = d.x * d.x + d.y * d.y;
+      v = make_double2(d.x / den, -d.y / den);
+    }
+    sX[idx] = v;
+  }
+  __syncthreads();
+  // levels b = 1, 2, 4, 8, 16, 32: pairs of b-blocks at offset o (multiple of 2b).
+  //  L: T = L21 Linv11 (rows o+b.., cols o..), Linv21 = -Linv22 T
+  //  U: T = U12 Uinv22 (rows o.., cols o+b..), Uinv12 = -Uinv11 T
+  // The L factor part of sA (strictly lower) and its U part are read-only
+  // until phase 1 of the level overwrites the L21/U12 blocks with T.
+  for (int b = 1; b < 64; b <<= 1) {
+    const int per = 32 * b;   // outputs per matrix at this level: (32/b) pairs x b*b
+    double2 t[8];
+    // phase 1: T into registers (slot s holds output q = tid + 256 s)
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const int q = tid + 256 * s;
+      if (q >= 2 * per) break;
+      const bool upper = q >= per;
+      const int qq = upper ? q - per : q;
+      const int pair = qq / (b * b), e = qq % (b * b);
+      const int o = pair * 2 * b, i = e % b, jj = e / b;
+      double2 acc = make_double2(0.0, 0.0);
+      if (!upper) {
+        // T[i][jj] = sum_k L21[i][k] Linv11[k][jj], L21 = sA[o+b+i][o+k], Linv11 (k >= jj)
+        for (int k = jj; k < b; k++) {
+          double2 l = sA[(o + k) * 64 + o + b + i];
+          double2 x = (k == jj) ? make_double2(1.0, 0.0) : sX[(o + jj) * 64 + o + k];
+          acc.x += l.x * x.x - l.y * x.y;
+          acc.y += l.x * x.y + l.y * x.x;
+        }
+      } else {
+        // T[i][jj] = sum_k U12[i][k] Uinv22[k][jj], U12 = sA[o+i][o+b+k], Uinv22 (k <= jj)
+        for (int k = 0; k <= jj; k++) {
+          double2 u = sA[(o + b + k) * 64 + o + i];
+          double2 x = sX[(o + b + jj) * 64 + o + b + k];
+          acc.x += u.x * x.x - u.y * x.y;
+          acc.y += u.x * x.y + u.y * x.x;
+        }
+      }
+      t[s] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 8; s++) {
+      const int q = tid + 256 * s;
+      if (q >= 2 * per) break;
+      const bool upper = q >= per;
+      const int qq = upper ? q - per : q;
+      const int pair = qq / (b * b), e = qq % (b * b);
+      const int o = pair * 2 * b, i = e % b, jj = e / b;
+      if (!upper) sA[(o + jj) * 64 + o + b + i] = t[s];
+      else sA[(o + b + jj) * 64 + o + i] = t[s];
+    }
+    __syncthreads();
+    // phase 2: Linv21 = -Linv22 T, Uinv12 = -Uinv11 T
+    for (int q = tid; q < 2 * per; q += 256) {
+      const bool upper = q >= per;
+      const int qq = upper ? q - per : q;
+      const int pair = qq / (b * b), e = qq % (b * b);
+      const int o = pair * 2 * b, i = e % b, jj = e / b;
+      double2 acc = make_double2(0.0, 0.0);
+      if (!upper) {
+        // sum_k Linv22[i][k] T[k][jj], Linv22 lower with unit diagonal (k <= i)
+        for (int k = 0; k <= i; k++) {
+          double2 x = (k == i) ? make_double2(1.0, 0.0) : sX[(o + b + k) * 64 + o + b + i];
+          double2 tt = sA[(o + jj) * 64 + o + b + k];
+          acc.x += x.x * tt.x - x.y * tt.y;
+          acc.y += x.x * tt.y + x.y * tt.x;
+        }
+        sX[(o + jj) * 64 + o + b + i] = make_double2(-acc.x, -acc.y);
+      } else {
+        // sum_k Uinv11[i][k] T[k][jj], Uinv11 upper (k >= i)
+        for (int k = i; k < b; k++) {
+          double2 x = sX[(o + k) * 64 + o + i];
+          double2 tt = sA[(o + b + jj) * 64 + o + k];
+          acc.x += x.x * tt.x - x.y * tt.y;
+          acc.y += x.x * tt.y + x.y * tt.x;
+        }
+        sX[(o + b + jj) * 64 + o + i] = make_double2(-acc.x, -acc.y);
+      }
+    }
+    __syncthreads();
+  }
+  double2 *Li = inv, *Ui = inv + 64 * 64;
+  for (int idx = tid; idx < 64 * 64; idx += 256) {
+    int r = idx & 63, c = idx >> 6;
+    double2 v = sX[idx];
     Li[idx] = r > c ? v : make_double2(r == c ? 1.0 : 0.0, 0.0);
     Ui[idx] = r <= c ? v : make_double2(0.0, 0.0);
   }
+}
+
+// max |re|+|im| over an m x n block (upper=1: only entries with row <= col),
+// folded into *out as the bit pattern of a non-negative double (atomic max).
+__global__ void absmax_kernel(const double2 *__restrict__ A, int64_t lda, int m, int n, int upper,
+                              unsigned long long *__restrict__ out) {
+  double v = 0.0;
+  for (int c = blockIdx.y; c < n; c += gridDim.y) {
+    int rmax = upper ? min(m, c + 1) : m;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rmax; r += gridDim.x * blockDim.x) {
+      double2 x = A[(int64_t)c * lda + r];
+      v = fmax(v, fabs(x.x) + fabs(x.y));
+    }
+  }
+  for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 
 // Row interchanges: for p in [k0, k1): swap rows p and ipiv[p] in every column
@@ -494,18 +695,18 @@ int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_
 
 size_t bem_trace_partials(int m, int n) { return (size_t)((m + GBM - 1) / GBM) * (size_t)((n + GBN - 1) / GBN); }
 
-// Tournament pivot selection for the panel A(r0:r0+m, c0:c0+w), w <= 64.
-// cand must hold ceil(m/256)*64 ints, cand2 ceil(ceil(m/256)/4)*64 (ping-pong);
+// Tournament pivot selection for the panel A(r0:r0+m, c0:c0+w), w <= 32.
+// cand must hold ceil(m/512)*32 ints, cand2 ceil(ceil(m/512)/16)*32 (ping-pong);
 // the winners end up in ipiv[k0..k0+w) as interchanges (k0 = r0).
 int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0, int *cand, int *cand2, int *ipiv,
                     int *info, hipStream_t st) {
   if (w <= 0) return 0;
-  if (w > 64) return -1;
+  if (w > GEPP_NB) return -1;
   int nsets = (m + GEPP_ROWS - 1) / GEPP_ROWS;
   hipLaunchKernelGGL(tslu_leaf_kernel, dim3(nsets), dim3(GEPP_NT), 0, st, A, lda, m, w, r0, c0, cand);
   int *cin = cand, *cout = cand2;
   while (nsets > 1) {
-    int nout = (nsets + 3) / 4;
+    int nout = (nsets + GEPP_MERGE - 1) / GEPP_MERGE;
     hipLaunchKernelGGL(tslu_merge_kernel, dim3(nout), dim3(GEPP_NT), 0, st, A, lda, w, c0, cin, nsets, cout);
     int *t = cin; cin = cout; cout = t;
     nsets = nout;
@@ -514,13 +715,29 @@ int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0,
   return (int)hipGetLastError();
 }
 
-// Diagonal blocks [blk0, blk0+nblk) (64 wide) of an n x n matrix: factor=1
-// factors block blk0 without pivoting and inverts its triangles; factor=0 only
-// forms the inverses of an existing packed LU.
-int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int factor, double2 *inv, int *info,
+// Diagonal blocks [blk0, blk0+nblk) (bw <= 64 wide) of an n x n matrix:
+// factor=1 factors each block without pivoting and inverts its triangles;
+// factor=0 only forms the inverses of an existing packed LU.  Inverses go to
+// inv + blk*2*64*64 (64 x 64, identity-padded).
+int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int bw, int factor, double2 *inv, int *info,
                    hipStream_t st) {
   if (nblk <= 0) return 0;
-  hipLaunchKernelGGL(diag_block_kernel, dim3(nblk), dim3(256), 0, st, A, lda, n, blk0, factor, inv, info);
+  if (bw <= 0 || bw > 64) return -1;
+  hipLaunchKernelGGL(diag_block_kernel, dim3(nblk), dim3(256), 0, st, A, lda, n, blk0, bw, factor, inv, info);
+  return (int)hipGetLastError();
+}
+
+// Block-local pivoted LU of the diagonal block at (k0, k0) plus the inverses
+// of its triangles (to inv, 2 x 64 x 64).
+int bem_diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st) {
+  hipLaunchKernelGGL(diag_lu_piv_kernel, dim3(1), dim3(256), 0, st, A, lda, n, k0, inv, ipiv, info);
+  return (int)hipGetLastError();
+}
+
+int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  dim3 grid((m + 255) / 256 < 8 ? (m + 255) / 256 : 8, n < 512 ? n : 512);
+  hipLaunchKernelGGL(absmax_kernel, grid, dim3(256), 0, st, A, lda, m, n, upper, out);
   return (int)hipGetLastError();
 }
 

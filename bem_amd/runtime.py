@@ -25,6 +25,7 @@ BEM_E_UNSUPPORTED = -6
 BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_ASM_DEVICE_PTR = 1
+BEM_SOLVE_TOURNAMENT = 1
 
 # src/util/SI.cpp:3-9
 SI_HBAR = 1.05457148e-34
@@ -56,7 +57,8 @@ class _Res(ctypes.Structure):
                 ("theta1", ctypes.c_double), ("theta2", ctypes.c_double), ("omega", ctypes.c_double),
                 ("rel_err_est", ctypes.c_double), ("iters", ctypes.c_int), ("status", ctypes.c_int),
                 ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double),
-                ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double)]
+                ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double), ("growth", ctypes.c_double),
+                ("pivoting", ctypes.c_int)]
 
 
 def lib_path():
@@ -140,6 +142,7 @@ class Request:
     eps2: complex = 1.0
     mu2: complex = 1.0
     precision: int = BEM_PREC_FP64
+    flags: int = 0
 
     def to_c(self):
         r = _Req()
@@ -151,7 +154,7 @@ class Request:
             arr = getattr(r, name)
             arr[0], arr[1] = z.real, z.imag
         r.precision = self.precision
-        r.flags = 0
+        r.flags = self.flags
         return r
 
 

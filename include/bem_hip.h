@@ -68,8 +68,17 @@ typedef struct {
   double eps1[2], mu1[2];  /* body 1 medium (re, im) at eV */
   double eps2[2], mu2[2];  /* body 2 medium (re, im) at eV */
   int precision;     /* BEM_PREC_* */
-  int flags;         /* reserved, 0 */
+  int flags;         /* BEM_SOLVE_* bits */
 } bem_request;
+
+/* Pivoting of the dense solve.  Default: partial pivoting inside each 64 x 64
+ * diagonal block (as the reference's block LU over tree blocks,
+ * src/util/HMatrix.h:277-316), with the growth factor max|U|/max|A| measured
+ * and reported; a solve whose growth exceeds BEM_GROWTH_LIMIT is repeated
+ * with tournament (communication-avoiding) partial pivoting over whole
+ * columns.  BEM_SOLVE_TOURNAMENT requests the latter from the start. */
+#define BEM_SOLVE_TOURNAMENT 1
+#define BEM_GROWTH_LIMIT 1e4
 
 typedef struct {
   double P;          /* Re(S) (Theta(omega,T1) - Theta(omega,T2)) */
@@ -80,6 +89,8 @@ typedef struct {
   int iters;         /* refinement iterations (mixed), 0 for FP64 */
   int status;        /* 0 or BEM_E_* for this request */
   double ms_assemble, ms_factor, ms_products, ms_total;  /* device-event times */
+  double growth;     /* max |U| / max |A| over the two LU factorisations */
+  int pivoting;      /* 0 block-local, 1 tournament */
 } bem_result;
 
 /* ---- context ---------------------------------------------------------- */
@@ -118,8 +129,9 @@ int bem_spectral_power(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_reque
 int bem_sweep(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *reqs, int n, bem_result *res);
 
 /* ---- dense complex128 kernels on device pointers ---------------------- */
-/* A (n x n, lda) <- P L U in place; ipiv (device, n ints, 0-based
- * interchanges row i <-> ipiv[i]).  Returns BEM_E_SINGULAR on a zero pivot. */
+/* A (n x n, lda) <- P L U in place with tournament partial pivoting; ipiv
+ * (device, n ints, 0-based interchanges row i <-> ipiv[i], applied in
+ * order).  Returns BEM_E_SINGULAR on a zero pivot. */
 int bem_zgetrf(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv);
 /* B (n x nrhs, ldb) <- A^-1 B from the factors of bem_zgetrf. */
 int bem_zgetrs(bem_ctx *ctx, const void *A, int64_t lda, int n, const int32_t *ipiv, void *B, int64_t ldb, int nrhs);
