@@ -54,9 +54,9 @@ int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int
 int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st);
 int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int bw, int factor, double2 *inv, int *info,
                    hipStream_t st);
-int bem_set_identity(double2 *X, int64_t ld, int n, hipStream_t st);
 int bem_diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st);
 int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st);
+int bem_col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st);
 }
 
 // src/util/SI.cpp:3-9
@@ -365,6 +365,40 @@ static int lu_solve(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, c
   return 0;
 }
 
+// X (r x n) <- X A^-1 from the packed LU of A (A = P L U): X U^-1 over column
+// blocks left to right, then L^-1 right to left, then the column
+// interchanges X P^T; the diagonal blocks go through their inverses, the
+// coupling terms through GEMMs.
+static int lu_solve_right(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, const int *ipiv,
+                          const double2 *inv, double2 *X, int64_t ldx, int r) {
+  const int nb = LU_NB;
+  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  double2 *tmp;
+  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
+  for (int k0 = 0; k0 < n; k0 += nb) {
+    int jb = std::min(nb, n - k0);
+    const double2 *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
+    double2 *Xk = X + (int64_t)k0 * ldx;
+    CHK(gemm(c, L, r, jb, jb, one, Xk, ldx, Uinv, 64, zero, tmp, r));
+    CHK(copy_back(c, L, Xk, ldx, tmp, r, jb));
+    int rest = n - k0 - jb;
+    if (rest > 0)
+      CHK(gemm(c, L, r, rest, jb, m1, Xk, ldx, A + (int64_t)(k0 + jb) * lda + k0, lda, one, Xk + (int64_t)jb * ldx, ldx));
+  }
+  int last = ((n - 1) / nb) * nb;
+  for (int k0 = last; k0 >= 0; k0 -= nb) {
+    int jb = std::min(nb, n - k0);
+    const double2 *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+    double2 *Xk = X + (int64_t)k0 * ldx;
+    CHK(gemm(c, L, r, jb, jb, one, Xk, ldx, Linv, 64, zero, tmp, r));
+    CHK(copy_back(c, L, Xk, ldx, tmp, r, jb));
+    if (k0 > 0) CHK(gemm(c, L, r, k0, jb, m1, Xk, ldx, A + k0, lda, one, X, ldx));
+  }
+  int e = bem_col_swap(X, ldx, r, 0, n, ipiv, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "column interchanges: %d", e);
+  return 0;
+}
+
 // ------------------------------------------------------------------ assembly
 static Medium make_medium(double er, double ei, double mr, double mi, double omega) {
   // src/GreensFunction.cpp:17-30 (complex arithmetic written out)
@@ -516,7 +550,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   Medium m1d = make_medium(q->eps1[0], q->eps1[1], q->mu1[0], q->mu1[1], omega);
   Medium m2d = make_medium(q->eps2[0], q->eps2[1], q->mu2[0], q->mu2[1], omega);
 
-  double2 *W, *G1, *G2, *Sinv, *X, *B1, *partial, *invA, *invS;
+  double2 *W, *G1, *G2, *B1, *partial, *invA, *invS;
   double *tiles;
   int *ipA, *info;
   unsigned long long *gro;
@@ -532,8 +566,6 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 4, (void **)&gro));
   CHK(ws_get(c, L, "invA", inv_bytes(m1), (void **)&invA));
   CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
-  CHK(ws_get(c, L, "Sinv", sizeof(double2) * (size_t)m2 * m2, (void **)&Sinv));
-  CHK(ws_get(c, L, "X", sizeof(double2) * (size_t)m1 * m2, (void **)&X));
   CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
   size_t np = bem_trace_partials(m1, m2);
   CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
@@ -559,6 +591,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
 
   // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  (void)zero;
   double2 *A = W, *Bm = W + (int64_t)m1 * M, *Cm = W + m1, *D = W + (int64_t)m1 * M + m1;
   if (tournament) CHK(lu_factor(c, L, A, M, m1, ipA, info, invA));
   else CHK(lu_factor_block(c, L, A, M, m1, ipA, info, invA, gro));
@@ -566,21 +599,20 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, be
   CHK(gemm(c, L, m2, m2, m1, mone, Cm, M, Bm, M, one, D, M));
   if (tournament) CHK(lu_factor(c, L, D, M, m2, ipS, info + 1, invS));
   else CHK(lu_factor_block(c, L, D, M, m2, ipS, info + 1, invS, gro + 2));
-  e = bem_set_identity(Sinv, m2, m2, L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "identity: %d", e);
-  CHK(lu_solve(c, L, D, M, m2, ipS, invS, Sinv, m2, m2));
-  CHK(gemm(c, L, m1, m2, m2, mone, Bm, M, Sinv, m2, zero, X, m1));
+  // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
+  CHK(lu_solve_right(c, L, D, M, m2, ipS, invS, Bm, M, m1));
+  double2 *X = Bm;
   DEV(c, hipEventRecord(L.ph[2], L.st));
 
   // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
   e = bem_symm(G1, m1, m1, L.st);
   if (!e) e = bem_symm(G2, m2, m2, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "symm: %d", e);
-  CHK(gemm(c, L, m1, m2, m1, one, G1, m1, X, m1, zero, B1, m1));
+  CHK(gemm(c, L, m1, m2, m1, one, G1, m1, X, M, zero, B1, m1));
   {
     EventRec *r = nullptr;
     if (c->profile_gemm) CHK(ev_begin(c, L, 8.0 * m1 * (double)m2 * m2, &r));
-    e = bem_zgemm_trace(m1, m2, m2, one, X, m1, G2, m2, B1, m1, partial, trace_out, L.st);
+    e = bem_zgemm_trace(m1, m2, m2, one, X, M, G2, m2, B1, m1, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
     if (r) DEV(c, hipEventRecord(r->b, L.st));
   }

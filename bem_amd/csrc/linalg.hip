@@ -1,7 +1,8 @@
 // Dense complex128 linear algebra for the PMCHWT solve, hand-written for gfx950.
 //
-//  zgemm_kernel     C = alpha A B + beta C (column-major, complex128), 64x64
-//                   C tile per 256-thread block, 2x2 waves of 32x32, FP64 MFMA
+//  zgemm_kernel     C = alpha A B + beta C (column-major, complex128), 128x128
+//                   (or 64x64 for skinny shapes) C tile per 256-thread block,
+//                   2x2 waves of 64x64 (32x32), FP64 MFMA
 //                   v_mfma_f64_16x16x4 with the complex product split into four
 //                   real MFMAs (re*re, -im*im, re*im, im*re) into two
 //                   accumulators; A/B tiles staged through LDS as re/im planes.
@@ -22,44 +23,54 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 namespace {
 
 // ------------------------------------------------------------------- ZGEMM
-constexpr int GBM = 64, GBN = 64, GBK = 16, GPAD = 1;
+// Block tile BM x BN, 4 waves as WM x WN, each wave FM x FN blocks of 16 x 16;
+// k staged through LDS in steps of GBK as separate re/im planes, the next
+// step's tiles prefetched into registers while the MFMAs run.
+constexpr int GBK = 16, GPAD = 1;
 
-template <bool TRACE>
+template <int BM, int BN, int WM, int WN, bool TRACE>
 __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2 alpha, const double2 *__restrict__ A,
                                                     int64_t lda, const double2 *__restrict__ B, int64_t ldb,
                                                     double2 beta, double2 *__restrict__ C, int64_t ldc,
                                                     const double2 *__restrict__ E, int64_t lde,
                                                     double2 *__restrict__ partial) {
-  __shared__ double sAr[GBK][GBM + GPAD], sAi[GBK][GBM + GPAD];
-  __shared__ double sBr[GBK][GBN + GPAD], sBi[GBK][GBN + GPAD];
+  static_assert(WM * WN == 4, "four waves");
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int AL = BM * GBK / 256;   // A elements loaded per thread per k step
+  constexpr int BL = BN * GBK / 256;
+  __shared__ double sAr[GBK][BM + GPAD], sAi[GBK][BM + GPAD];
+  __shared__ double sBr[GBK][BN + GPAD], sBi[GBK][BN + GPAD];
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wr = wave >> 1, wc = wave & 1;
+  const int wr = wave / WN, wc = wave % WN;
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.x * GBM, n0 = blockIdx.y * GBN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 
-  dbl4 accr[2][2], acci[2][2];
+  dbl4 accr[FM][FN], acci[FM][FN];
 #pragma unroll
-  for (int a = 0; a < 2; a++)
+  for (int a = 0; a < FM; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) {
+    for (int b = 0; b < FN; b++) {
       accr[a][b] = (dbl4){0, 0, 0, 0};
       acci[a][b] = (dbl4){0, 0, 0, 0};
     }
 
-  // register prefetch of the next tiles
-  double2 ra[4], rb[4];
-  const int arow = tid & 63, acg = tid >> 6;      // A: row, column group of 4
-  const int bkr = tid & 15, bcol = tid >> 4;      // B: k-row, column (+16q)
+  double2 ra[AL], rb[BL];
+  // A tile (BM x GBK): thread t reads row t % BM of columns (t / BM) + (256/BM) q
+  // B tile (GBK x BN): thread t reads k-row t % GBK of columns t / GBK + (256/GBK) q
+  const int arow = tid % BM, acol0 = tid / BM;
+  constexpr int ASTEP = 256 / BM;
+  const int bkr = tid % GBK, bcol0 = tid / GBK;
+  constexpr int BSTEP = 256 / GBK;
   auto load_tiles = [&](int k0) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      int col = k0 + acg * 4 + c, row = m0 + arow;
-      ra[c] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_double2(0.0, 0.0);
+    for (int q = 0; q < AL; q++) {
+      int col = k0 + acol0 + ASTEP * q, row = m0 + arow;
+      ra[q] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_double2(0.0, 0.0);
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      int col = n0 + bcol + 16 * q, row = k0 + bkr;
+    for (int q = 0; q < BL; q++) {
+      int col = n0 + bcol0 + BSTEP * q, row = k0 + bkr;
       rb[q] = (row < k && col < n) ? B[(int64_t)col * ldb + row] : make_double2(0.0, 0.0);
     }
   };
@@ -67,35 +78,43 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
   for (int k0 = 0; k0 < k; k0 += GBK) {
     __syncthreads();
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      sAr[acg * 4 + c][arow] = ra[c].x;
-      sAi[acg * 4 + c][arow] = ra[c].y;
+    for (int q = 0; q < AL; q++) {
+      sAr[acol0 + ASTEP * q][arow] = ra[q].x;
+      sAi[acol0 + ASTEP * q][arow] = ra[q].y;
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      sBr[bkr][bcol + 16 * q] = rb[q].x;
-      sBi[bkr][bcol + 16 * q] = rb[q].y;
+    for (int q = 0; q < BL; q++) {
+      sBr[bkr][bcol0 + BSTEP * q] = rb[q].x;
+      sBi[bkr][bcol0 + BSTEP * q] = rb[q].y;
     }
     __syncthreads();
     if (k0 + GBK < k) load_tiles(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
-      double ar[2], ai[2], nai[2], br[2], bi[2];
+      double ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
 #pragma unroll
-      for (int x = 0; x < 2; x++) {
-        ar[x] = sAr[kk + lk][wr * 32 + x * 16 + li];
-        ai[x] = sAi[kk + lk][wr * 32 + x * 16 + li];
+      for (int x = 0; x < FM; x++) {
+        ar[x] = sAr[kk + lk][wr * (BM / WM) + x * 16 + li];
+        ai[x] = sAi[kk + lk][wr * (BM / WM) + x * 16 + li];
         nai[x] = -ai[x];
-        br[x] = sBr[kk + lk][wc * 32 + x * 16 + li];
-        bi[x] = sBi[kk + lk][wc * 32 + x * 16 + li];
       }
 #pragma unroll
-      for (int x = 0; x < 2; x++)
+      for (int y = 0; y < FN; y++) {
+        br[y] = sBr[kk + lk][wc * (BN / WN) + y * 16 + li];
+        bi[y] = sBi[kk + lk][wc * (BN / WN) + y * 16 + li];
+      }
 #pragma unroll
-        for (int y = 0; y < 2; y++) {
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) {
           accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], accr[x][y], 0, 0, 0);
-          accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], accr[x][y], 0, 0, 0);
           acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acci[x][y], 0, 0, 0);
+        }
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) {
+          accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], accr[x][y], 0, 0, 0);
           acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acci[x][y], 0, 0, 0);
         }
     }
@@ -104,13 +123,13 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
   // lane j + 16*(i%4), element i/4 (measured on gfx950; the f32 variant differs)
   if (!TRACE) {
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < FM; x++)
 #pragma unroll
-      for (int y = 0; y < 2; y++)
+      for (int y = 0; y < FN; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * 32 + x * 16 + 4 * r + lk;
-          int col = n0 + wc * 32 + y * 16 + li;
+          int row = m0 + wr * (BM / WM) + x * 16 + 4 * r + lk;
+          int col = n0 + wc * (BN / WN) + y * 16 + li;
           if (row < m && col < n) {
             double cr = accr[x][y][r], ci = acci[x][y][r];
             double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
@@ -127,13 +146,13 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
     __shared__ double sred[2][4];
     double sr = 0.0, si = 0.0;
 #pragma unroll
-    for (int x = 0; x < 2; x++)
+    for (int x = 0; x < FM; x++)
 #pragma unroll
-      for (int y = 0; y < 2; y++)
+      for (int y = 0; y < FN; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * 32 + x * 16 + 4 * r + lk;
-          int col = n0 + wc * 32 + y * 16 + li;
+          int row = m0 + wr * (BM / WM) + x * 16 + 4 * r + lk;
+          int col = n0 + wc * (BN / WN) + y * 16 + li;
           if (row < m && col < n) {
             double cr = accr[x][y][r], ci = acci[x][y][r];
             double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
@@ -629,6 +648,23 @@ __global__ void absmax_kernel(const double2 *__restrict__ A, int64_t lda, int m,
   if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
 }
 
+// Column interchanges for X <- X P^T with P = P_k0 .. P_{k1-1} the row
+// interchanges of an LU (P_i swaps i and ipiv[i]): columns i and ipiv[i] of X
+// (rows [0, m)) are swapped for i = k1-1 down to k0.  One thread per row.
+__global__ void col_swap_kernel(double2 *__restrict__ X, int64_t ldx, int m, int k0, int k1,
+                                const int *__restrict__ ipiv) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int i = k1 - 1; i >= k0; i--) {
+    int q = ipiv[i];
+    if (q != i) {
+      double2 t = X[(int64_t)i * ldx + r];
+      X[(int64_t)i * ldx + r] = X[(int64_t)q * ldx + r];
+      X[(int64_t)q * ldx + r] = t;
+    }
+  }
+}
+
 // Row interchanges: for p in [k0, k1): swap rows p and ipiv[p] in every column
 // of [c0, c1) except [skip0, skip1).
 __global__ void laswp_kernel(double2 *__restrict__ A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1,
@@ -662,38 +698,55 @@ __global__ void symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
   G[(int64_t)i * ld + j] = make_double2(s.x, -s.y);
 }
 
-__global__ void set_identity_kernel(double2 *__restrict__ X, int64_t ld, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (i >= n) return;
-  X[(int64_t)j * ld + i] = make_double2(i == j ? 1.0 : 0.0, 0.0);
-}
 
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
 extern "C" {
 
+// Tile choice by shape: 32 x 32 tiles (one 16 x 16 block per wave) when
+// either dimension is at most 128, so skinny products still fill the chip;
+// 64 x 64 otherwise (two waves per SIMD; measured faster than 128 x 64 on
+// gfx950 for both k = 64 updates and k = 3840 products).  bem_zgemm_set_tile
+// forces 32, 64 or 128 for measurements (0 restores the shape rule).
+static int g_zgemm_tile = 0;
+int bem_zgemm_set_tile(int t) {
+  g_zgemm_tile = t;
+  return 0;
+}
+
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
-              double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
+                  double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
-  dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
-  hipLaunchKernelGGL(zgemm_kernel<false>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc,
-                     (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
+  int tile = g_zgemm_tile ? g_zgemm_tile : ((m <= 128 || n <= 128) ? 32 : 64);
+  if (tile == 128) {
+    dim3 grid((m + 127) / 128, (n + 63) / 64);
+    hipLaunchKernelGGL((zgemm_kernel<128, 64, 2, 2, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
+  } else if (tile == 32) {
+    dim3 grid((m + 31) / 32, (n + 31) / 32);
+    hipLaunchKernelGGL((zgemm_kernel<32, 32, 2, 2, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
+  } else {
+    dim3 grid((m + 63) / 64, (n + 63) / 64);
+    hipLaunchKernelGGL((zgemm_kernel<64, 64, 2, 2, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
+                       beta, C, ldc, (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
+  }
   return (int)hipGetLastError();
 }
 
-// partial must hold ceil(m/64)*ceil(n/64) entries; out receives sum conj(alpha A B) .* E.
+// Trace tiles are 64 x 64; partial must hold bem_trace_partials(m, n) entries;
+// out receives sum conj(alpha A B) .* E.
+size_t bem_trace_partials(int m, int n) { return (size_t)((m + 63) / 64) * (size_t)((n + 63) / 64); }
+
 int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                     const double2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
-  dim3 grid((m + GBM - 1) / GBM, (n + GBN - 1) / GBN);
-  hipLaunchKernelGGL(zgemm_kernel<true>, grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
+  dim3 grid((m + 63) / 64, (n + 63) / 64);
+  hipLaunchKernelGGL((zgemm_kernel<64, 64, 2, 2, true>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
                      make_double2(0.0, 0.0), (double2 *)nullptr, (int64_t)0, E, lde, partial);
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)(grid.x * grid.y), out);
   return (int)hipGetLastError();
 }
-
-size_t bem_trace_partials(int m, int n) { return (size_t)((m + GBM - 1) / GBM) * (size_t)((n + GBN - 1) / GBN); }
 
 // Tournament pivot selection for the panel A(r0:r0+m, c0:c0+w), w <= 32.
 // cand must hold ceil(m/512)*32 ints, cand2 ceil(ceil(m/512)/16)*32 (ping-pong);
@@ -741,6 +794,12 @@ int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned 
   return (int)hipGetLastError();
 }
 
+int bem_col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st) {
+  if (m <= 0 || k1 <= k0) return 0;
+  hipLaunchKernelGGL(col_swap_kernel, dim3((m + 255) / 256), dim3(256), 0, st, X, ldx, m, k0, k1, ipiv);
+  return (int)hipGetLastError();
+}
+
 int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
               hipStream_t st) {
   if (c1 <= c0 || k1 <= k0) return 0;
@@ -754,9 +813,5 @@ int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
-int bem_set_identity(double2 *X, int64_t ld, int n, hipStream_t st) {
-  hipLaunchKernelGGL(set_identity_kernel, dim3((n + 255) / 256, n), dim3(256), 0, st, X, ld, n);
-  return (int)hipGetLastError();
-}
 
 }  // extern "C"
