@@ -1,19 +1,21 @@
 // Dense complex128 linear algebra for the PMCHWT solve, hand-written for gfx950.
 //
-//  zgemm_kernel     C = alpha A B + beta C (column-major, complex128), 128x128
-//                   (or 64x64 for skinny shapes) C tile per 256-thread block,
-//                   2x2 waves of 64x64 (32x32), FP64 MFMA
-//                   v_mfma_f64_16x16x4 with the complex product split into four
-//                   real MFMAs (re*re, -im*im, re*im, im*re) into two
-//                   accumulators; A/B tiles staged through LDS as re/im planes.
+//  zgemm_kernel     C = alpha A B + beta C (column-major, complex128) on FP64
+//                   MFMA v_mfma_f64_16x16x4; A/B tiles staged through LDS as
+//                   re/im planes.  Default: 32 x 64 C tile per 256-thread
+//                   block and the complex product as three real MFMAs
+//                   (Ar Br, Ai Bi, (Ar + Ai)(Br + Bi)) into three
+//                   accumulators; the four-product form stays available.
 //                   TRACE variant: instead of storing C, the epilogue forms
 //                   sum conj(C_ij) * E_ij against a second matrix E and writes
 //                   one partial per block (the HeatTransfer trace
 //                   tr(B2^H B1) = sum conj(B2) .* B1, src/HeatTransfer.cpp:53-55).
 //  tslu_*           tournament pivot selection for a panel (CALU)
-//  diag_block       the panel's 64x64 diagonal block once its pivot rows are
-//                   in place: LU without pivoting and the inverses of both
-//                   triangles, which turn every triangular solve into ZGEMMs
+//  diag_block       a 64x64 diagonal block once its pivot rows are in place:
+//                   LU without pivoting and the inverses of both triangles,
+//                   which turn every triangular solve into ZGEMMs
+//  diag_lu_piv      a 64x64 diagonal block with partial pivoting inside it
+//                   (the default LU), plus both inverses
 //  laswp_kernel     row interchanges on a column range
 //  symm_kernel      G <- G + G^H (src/util/DenseMatrix.cpp:477-486)
 #include "bem_common.h"
@@ -23,21 +25,29 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 namespace {
 
 // ------------------------------------------------------------------- ZGEMM
-// Block tile BM x BN, 4 waves as WM x WN, each wave FM x FN blocks of 16 x 16;
+// Block tile BM x BN, WM x WN waves, each wave FM x FN blocks of 16 x 16;
 // k staged through LDS in steps of GBK as separate re/im planes, the next
 // step's tiles prefetched into registers while the MFMAs run.
-constexpr int GBK = 16, GPAD = 1;
+// G3M: the complex product takes three real MFMAs per k step instead of
+// four (T1 = Ar Br, T2 = Ai Bi, T3 = (Ar + Ai)(Br + Bi); re = T1 - T2,
+// im = T3 - T1 - T2), with error bounded by u (|Ar| + |Ai|)(|Br| + |Bi|)
+// per term as for the four-product form.
+constexpr int GPAD = 1;
 
-template <int BM, int BN, int WM, int WN, bool TRACE>
-__global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2 alpha, const double2 *__restrict__ A,
-                                                    int64_t lda, const double2 *__restrict__ B, int64_t ldb,
-                                                    double2 beta, double2 *__restrict__ C, int64_t ldc,
-                                                    const double2 *__restrict__ E, int64_t lde,
-                                                    double2 *__restrict__ partial) {
-  static_assert(WM * WN == 4, "four waves");
+template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool TRACE, bool G3M>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
+zgemm_kernel(int m, int n, int k, double2 alpha,
+                                                             const double2 *__restrict__ A, int64_t lda,
+                                                             const double2 *__restrict__ B, int64_t ldb,
+                                                             double2 beta, double2 *__restrict__ C, int64_t ldc,
+                                                             const double2 *__restrict__ E, int64_t lde,
+                                                             double2 *__restrict__ partial) {
+  constexpr int NT = 64 * WM * WN;
   constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
-  constexpr int AL = BM * GBK / 256;   // A elements loaded per thread per k step
-  constexpr int BL = BN * GBK / 256;
+  constexpr int AL = BM * GBK / NT;   // A elements loaded per thread per k step
+  constexpr int BL = BN * GBK / NT;
+  constexpr int NACC = G3M ? 3 : 2;
+  static_assert(AL >= 1 && BL >= 1 && NT % BM == 0 && NT % GBK == 0, "tile shape");
   __shared__ double sAr[GBK][BM + GPAD], sAi[GBK][BM + GPAD];
   __shared__ double sBr[GBK][BN + GPAD], sBi[GBK][BN + GPAD];
   const int tid = threadIdx.x;
@@ -46,22 +56,21 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
   const int li = lane & 15, lk = lane >> 4;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 
-  dbl4 accr[FM][FN], acci[FM][FN];
+  dbl4 acc[NACC][FM][FN];
 #pragma unroll
-  for (int a = 0; a < FM; a++)
+  for (int q = 0; q < NACC; q++)
 #pragma unroll
-    for (int b = 0; b < FN; b++) {
-      accr[a][b] = (dbl4){0, 0, 0, 0};
-      acci[a][b] = (dbl4){0, 0, 0, 0};
-    }
+    for (int a = 0; a < FM; a++)
+#pragma unroll
+      for (int b = 0; b < FN; b++) acc[q][a][b] = (dbl4){0, 0, 0, 0};
 
   double2 ra[AL], rb[BL];
-  // A tile (BM x GBK): thread t reads row t % BM of columns (t / BM) + (256/BM) q
-  // B tile (GBK x BN): thread t reads k-row t % GBK of columns t / GBK + (256/GBK) q
+  // A tile (BM x GBK): thread t reads row t % BM of columns (t / BM) + (NT/BM) q
+  // B tile (GBK x BN): thread t reads k-row t % GBK of columns t / GBK + (NT/GBK) q
   const int arow = tid % BM, acol0 = tid / BM;
-  constexpr int ASTEP = 256 / BM;
+  constexpr int ASTEP = NT / BM;
   const int bkr = tid % GBK, bcol0 = tid / GBK;
-  constexpr int BSTEP = 256 / GBK;
+  constexpr int BSTEP = NT / GBK;
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < AL; q++) {
@@ -91,34 +100,63 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
     if (k0 + GBK < k) load_tiles(k0 + GBK);
 #pragma unroll
     for (int kk = 0; kk < GBK; kk += 4) {
-      double ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+      double ar[FM], ai[FM], br[FN], bi[FN];
 #pragma unroll
       for (int x = 0; x < FM; x++) {
         ar[x] = sAr[kk + lk][wr * (BM / WM) + x * 16 + li];
         ai[x] = sAi[kk + lk][wr * (BM / WM) + x * 16 + li];
-        nai[x] = -ai[x];
       }
 #pragma unroll
       for (int y = 0; y < FN; y++) {
         br[y] = sBr[kk + lk][wc * (BN / WN) + y * 16 + li];
         bi[y] = sBi[kk + lk][wc * (BN / WN) + y * 16 + li];
       }
+      if constexpr (G3M) {
+        double as[FM], bs[FN];
 #pragma unroll
-      for (int x = 0; x < FM; x++)
+        for (int x = 0; x < FM; x++) as[x] = ar[x] + ai[x];
 #pragma unroll
-        for (int y = 0; y < FN; y++) {
-          accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], accr[x][y], 0, 0, 0);
-          acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acci[x][y], 0, 0, 0);
-        }
+        for (int y = 0; y < FN; y++) bs[y] = br[y] + bi[y];
 #pragma unroll
-      for (int x = 0; x < FM; x++)
+        for (int x = 0; x < FM; x++)
 #pragma unroll
-        for (int y = 0; y < FN; y++) {
-          accr[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], accr[x][y], 0, 0, 0);
-          acci[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acci[x][y], 0, 0, 0);
-        }
+          for (int y = 0; y < FN; y++) {
+            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], bi[y], acc[1][x][y], 0, 0, 0);
+            acc[2][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[x], bs[y], acc[2][x][y], 0, 0, 0);
+          }
+      } else {
+        double nai[FM];
+#pragma unroll
+        for (int x = 0; x < FM; x++) nai[x] = -ai[x];
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+          for (int y = 0; y < FN; y++) {
+            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acc[1][x][y], 0, 0, 0);
+          }
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+          for (int y = 0; y < FN; y++) {
+            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acc[1][x][y], 0, 0, 0);
+          }
+      }
     }
   }
+  // the (re, im) of output element r of block (x, y)
+  auto result = [&](int x, int y, int r, double &cr, double &ci) {
+    if constexpr (G3M) {
+      double t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+      cr = t1 - t2;
+      ci = acc[2][x][y][r] - t1 - t2;
+    } else {
+      cr = acc[0][x][y][r];
+      ci = acc[1][x][y][r];
+    }
+  };
   // epilogue: for v_mfma_f64_16x16x4f64, D[i][j] of a 16x16 block sits in
   // lane j + 16*(i%4), element i/4 (measured on gfx950; the f32 variant differs)
   if (!TRACE) {
@@ -131,7 +169,8 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
           int row = m0 + wr * (BM / WM) + x * 16 + 4 * r + lk;
           int col = n0 + wc * (BN / WN) + y * 16 + li;
           if (row < m && col < n) {
-            double cr = accr[x][y][r], ci = acci[x][y][r];
+            double cr, ci;
+            result(x, y, r, cr, ci);
             double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
             double2 *p = C + (int64_t)col * ldc + row;
             if (beta.x != 0.0 || beta.y != 0.0) {
@@ -143,7 +182,8 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
           }
         }
   } else {
-    __shared__ double sred[2][4];
+    constexpr int NW = WM * WN;
+    __shared__ double sred[2][NW];
     double sr = 0.0, si = 0.0;
 #pragma unroll
     for (int x = 0; x < FM; x++)
@@ -154,7 +194,8 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
           int row = m0 + wr * (BM / WM) + x * 16 + 4 * r + lk;
           int col = n0 + wc * (BN / WN) + y * 16 + li;
           if (row < m && col < n) {
-            double cr = accr[x][y][r], ci = acci[x][y][r];
+            double cr, ci;
+            result(x, y, r, cr, ci);
             double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
             double2 e = E[(int64_t)col * lde + row];
             // conj(v) * e
@@ -170,8 +211,9 @@ __global__ void __launch_bounds__(256) zgemm_kernel(int m, int n, int k, double2
     if (lane == 0) { sred[0][wave] = sr; sred[1][wave] = si; }
     __syncthreads();
     if (tid == 0) {
-      double tr = ((sred[0][0] + sred[0][1]) + sred[0][2]) + sred[0][3];
-      double ti = ((sred[1][0] + sred[1][1]) + sred[1][2]) + sred[1][3];
+      double tr = 0.0, ti = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
       partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
     }
   }
@@ -699,16 +741,30 @@ __global__ void symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
 }
 
 
+template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool TRACE, bool G3M>
+void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B,
+                         int64_t ldb, double2 beta, double2 *C, int64_t ldc, const double2 *E, int64_t lde,
+                         double2 *partial, hipStream_t st) {
+  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  hipLaunchKernelGGL((zgemm_kernel<BM, BN, WM, WN, GBK, OCC, TRACE, G3M>), grid, dim3(64 * WM * WN), 0, st, m, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc, E, lde, partial);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
 extern "C" {
 
-// Tile choice by shape: 32 x 32 tiles (one 16 x 16 block per wave) when
-// either dimension is at most 128, so skinny products still fill the chip;
-// 64 x 64 otherwise (two waves per SIMD; measured faster than 128 x 64 on
-// gfx950 for both k = 64 updates and k = 3840 products).  bem_zgemm_set_tile
-// forces 32, 64 or 128 for measurements (0 restores the shape rule).
+// Tile choice by shape (measured on gfx950, complex128, lda = 3840): the
+// three-product form on 32 x 64 tiles (four waves of 16 x 32, four waves per
+// SIMD) runs m = n = k = 3840 at 78 TFLOP/s counted as 8mnk (58.7 executed),
+// against 57.5 for the four-product 64 x 64 tile, and rank-64 updates at
+// 53-58 against 38-40; 32 x 32 tiles when either dimension is at most 128 so
+// skinny products still fill the chip.  bem_zgemm_set_tile forces a variant
+// for measurements (0 restores the shape rule): 32 and 64 are the
+// four-product 32 x 32 and 64 x 64 tiles, 256 the four-product 128 x 128 tile
+// on eight waves; 1032 / 2032 (k step 32) / 1033 (64 x 32) / 1034 (32 x 64) /
+// 1064 / 2064 (k step 32) / 1065 (one wave per SIMD) the three-product ones.
 static int g_zgemm_tile = 0;
 int bem_zgemm_set_tile(int t) {
   g_zgemm_tile = t;
@@ -718,33 +774,39 @@ int bem_zgemm_set_tile(int t) {
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                   double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
-  int tile = g_zgemm_tile ? g_zgemm_tile : ((m <= 128 || n <= 128) ? 32 : 64);
-  if (tile == 128) {
-    dim3 grid((m + 127) / 128, (n + 63) / 64);
-    hipLaunchKernelGGL((zgemm_kernel<128, 64, 2, 2, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
-  } else if (tile == 32) {
-    dim3 grid((m + 31) / 32, (n + 31) / 32);
-    hipLaunchKernelGGL((zgemm_kernel<32, 32, 2, 2, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
-  } else {
-    dim3 grid((m + 63) / 64, (n + 63) / 64);
-    hipLaunchKernelGGL((zgemm_kernel<64, 64, 2, 2, false>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
-                       beta, C, ldc, (const double2 *)nullptr, (int64_t)0, (double2 *)nullptr);
+  int tile = g_zgemm_tile ? g_zgemm_tile : ((m <= 128 || n <= 128) ? 1032 : 1034);
+  const double2 *E = nullptr;
+  double2 *P = nullptr;
+  switch (tile) {
+#define ZG(code, BM, BN, WM, WN, KB, OCC, G3)                                                                   \
+  case code:                                                                                                   \
+    launch_zgemm<BM, BN, WM, WN, KB, OCC, false, G3>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, E, 0, P, st); \
+    break;
+    ZG(32, 32, 32, 2, 2, 16, 1, false)
+    ZG(1032, 32, 32, 2, 2, 16, 1, true)
+    ZG(2032, 32, 32, 2, 2, 32, 1, true)
+    ZG(1033, 64, 32, 2, 2, 16, 2, true)
+    ZG(1034, 32, 64, 2, 2, 16, 2, true)
+    ZG(1064, 64, 64, 2, 2, 16, 2, true)
+    ZG(2064, 64, 64, 2, 2, 32, 2, true)
+    ZG(1065, 64, 64, 2, 2, 16, 1, true)
+    ZG(256, 128, 128, 2, 4, 16, 2, false)
+    ZG(64, 64, 64, 2, 2, 16, 1, false)
+#undef ZG
+    default: return -1;
   }
   return (int)hipGetLastError();
 }
 
-// Trace tiles are 64 x 64; partial must hold bem_trace_partials(m, n) entries;
-// out receives sum conj(alpha A B) .* E.
-size_t bem_trace_partials(int m, int n) { return (size_t)((m + 63) / 64) * (size_t)((n + 63) / 64); }
+// Trace tiles are 32 x 64 (three-product form); partial must hold
+// bem_trace_partials(m, n) entries; out receives sum conj(alpha A B) .* E.
+size_t bem_trace_partials(int m, int n) { return (size_t)((m + 31) / 32) * (size_t)((n + 63) / 64); }
 
 int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                     const double2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
-  dim3 grid((m + 63) / 64, (n + 63) / 64);
-  hipLaunchKernelGGL((zgemm_kernel<64, 64, 2, 2, true>), grid, dim3(256), 0, st, m, n, k, alpha, A, lda, B, ldb,
-                     make_double2(0.0, 0.0), (double2 *)nullptr, (int64_t)0, E, lde, partial);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)(grid.x * grid.y), out);
+  launch_zgemm<32, 64, 2, 2, 16, 2, true, true>(m, n, k, alpha, A, lda, B, ldb, make_double2(0.0, 0.0), nullptr, 0, E,
+                                                lde, partial, st);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)bem_trace_partials(m, n), out);
   return (int)hipGetLastError();
 }
 

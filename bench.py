@@ -93,7 +93,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--lanes", type=int, default=2, help="solves kept in flight (one HIP stream each)")
+    ap.add_argument("--lanes", type=int, default=3, help="solves kept in flight (one HIP stream each)")
     ap.add_argument("--check", action="store_true", help="compare against stored oracle values")
     args = ap.parse_args()
     if args.gpus != 1:
