@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(256) far_g6_kernel(const TriGeom *__restrict__
     for (int q = 0; q < 6; q++) inner_point<false>(md, r, s_in[tx][q], ob, s_in[tx][q][3], m);
     outer_far(md, r, s_out[ty][o][3], m, s_tp0[ty], nt, s_spb[tx], ns, Lr, Li, Kr, Ki);
   }
-  store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)T + (size_t)b, Lr, Li, Kr, Ki);
+  store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)(col_hi - col_lo) + (size_t)(b - col_lo), Lr, Li, Kr, Ki);
 }
 
 // Block reduction of 36 accumulators; result valid in thread 0.
@@ -485,7 +485,8 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
                                                         const int2 *__restrict__ pairs,
                                                         const unsigned int *__restrict__ count_dev,
                                                         unsigned int count_host, int row_lo, int row_hi,
-                                                        double *__restrict__ tiles, size_t npairs) {
+                                                        int col_lo, int col_hi, double *__restrict__ tiles,
+                                                        size_t npairs) {
   __shared__ double s_in[NIN][4];
   __shared__ double s_red[(NT / 64) * 36];
   __shared__ double s_tp0[3][3], s_sp0[3][3], s_spb[3][3];
@@ -496,7 +497,7 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
   for (unsigned int p = blockIdx.x; p < count; p += gridDim.x) {
     int2 pr = pairs[p];
     int a = pr.x, b = pr.y;
-    if (a < row_lo || a >= row_hi) continue;   // uniform per block
+    if (a < row_lo || a >= row_hi || b < col_lo || b >= col_hi) continue;   // uniform per block
     const TriGeom &A = tg[a];
     const TriGeom &B = tg[b];
     __syncthreads();
@@ -565,7 +566,9 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
       }
     }
     block_reduce36<NT>(Lr, Li, Kr, Ki, s_red);
-    if (threadIdx.x == 0) store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)T + (size_t)b, Lr, Li, Kr, Ki);
+    if (threadIdx.x == 0)
+      store_tile(tiles, npairs, (size_t)(a - row_lo) * (size_t)(col_hi - col_lo) + (size_t)(b - col_lo), Lr, Li, Kr,
+                 Ki);
   }
 }
 
@@ -576,8 +579,8 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
 // With ADD the values are added to Z (W = Z_ext + Z_int on the diagonal blocks);
 // Zcopy (optional) receives the same values (the separate G_b matrices).
 // Rows are RWGs [r_lo, r_hi) of the tree; the tile buffer holds test rows
-// [row_lo, row_hi) of triangles.
-__global__ void gather_kernel(const double *__restrict__ tiles, size_t npairs, int T, int row_lo,
+// [row_lo, ..) x source columns [col_lo, col_lo + tld) of triangles.
+__global__ void gather_kernel(const double *__restrict__ tiles, size_t npairs, int tld, int row_lo, int col_lo,
                               const RwgRec *__restrict__ rw, int r_lo, int r_hi, int c_lo, int c_hi, Medium md,
                               double2 *__restrict__ Z, int64_t ldz, int zr0, int zc0, int add,
                               double2 *__restrict__ Zcopy, int64_t ldc, int cr0, int cc0) {
@@ -591,7 +594,7 @@ __global__ void gather_kernel(const double *__restrict__ tiles, size_t npairs, i
   for (int hi = 0; hi < 2; hi++)
     for (int hj = 0; hj < 2; hj++) {
       double sg = (hi == hj) ? 1.0 : -1.0;
-      size_t idx = (size_t)(ta[hi] - row_lo) * (size_t)T + (size_t)tb[hj];
+      size_t idx = (size_t)(ta[hi] - row_lo) * (size_t)tld + (size_t)(tb[hj] - col_lo);
       int q = 3 * sa[hi] + sb[hj];
       Lr += sg * tiles[(size_t)(2 * q) * npairs + idx];
       Li += sg * tiles[(size_t)(2 * q + 1) * npairs + idx];
@@ -637,30 +640,41 @@ int bem_launch_far_g6(const TriGeom *tg, int T, const Medium *md, int row_lo, in
 }
 
 int bem_launch_far_g36(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, const unsigned int *count_dev,
-                       unsigned int max_count, int row_lo, int row_hi, double *tiles, size_t npairs, hipStream_t st) {
+                       unsigned int max_count, int row_lo, int row_hi, int col_lo, int col_hi, double *tiles,
+                       size_t npairs, hipStream_t st) {
   unsigned int blocks = max_count < 4096u ? max_count : 4096u;
   if (blocks == 0) return 0;
   hipLaunchKernelGGL((pair_block_kernel<36, 36, false, 64>), dim3(blocks), dim3(64), 0, st, tg, T, *md, pairs,
-                     count_dev, max_count, row_lo, row_hi, tiles, npairs);
+                     count_dev, max_count, row_lo, row_hi, col_lo, col_hi, tiles, npairs);
   return (int)hipGetLastError();
 }
 
 int bem_launch_close(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, unsigned int count, int row_lo,
-                     int row_hi, double *tiles, size_t npairs, hipStream_t st) {
+                     int row_hi, int col_lo, int col_hi, double *tiles, size_t npairs, hipStream_t st) {
   unsigned int blocks = count < 65535u ? count : 65535u;
   if (blocks == 0) return 0;
   hipLaunchKernelGGL((pair_block_kernel<105, 91, true, 128>), dim3(blocks), dim3(128), 0, st, tg, T, *md, pairs,
-                     (const unsigned int *)nullptr, count, row_lo, row_hi, tiles, npairs);
+                     (const unsigned int *)nullptr, count, row_lo, row_hi, col_lo, col_hi, tiles, npairs);
   return (int)hipGetLastError();
 }
 
-int bem_launch_gather(const double *tiles, size_t npairs, int T, int row_lo, const RwgRec *rw, int r_lo, int r_hi,
-                      int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0, int add,
-                      double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st) {
+// flag <- 1 when the 36-point pair queue overflowed its capacity.
+__global__ void q36_check_kernel(const unsigned int *count, unsigned int cap, int *flag) {
+  if (threadIdx.x == 0 && *count > cap) *flag = 1;
+}
+
+int bem_launch_q36_check(const unsigned int *count, unsigned int cap, int *flag, hipStream_t st) {
+  hipLaunchKernelGGL(q36_check_kernel, dim3(1), dim3(64), 0, st, count, cap, flag);
+  return (int)hipGetLastError();
+}
+
+int bem_launch_gather(const double *tiles, size_t npairs, int tld, int row_lo, int col_lo, const RwgRec *rw, int r_lo,
+                      int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0,
+                      int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st) {
   if (r_hi <= r_lo || c_hi <= c_lo) return 0;
   dim3 grid((r_hi - r_lo + 127) / 128, c_hi - c_lo);
-  hipLaunchKernelGGL(gather_kernel, grid, dim3(128), 0, st, tiles, npairs, T, row_lo, rw, r_lo, r_hi, c_lo, c_hi,
-                     *md, Z, ldz, zr0, zc0, add, Zcopy, ldc, cr0, cc0);
+  hipLaunchKernelGGL(gather_kernel, grid, dim3(128), 0, st, tiles, npairs, tld, row_lo, col_lo, rw, r_lo, r_hi, c_lo,
+                     c_hi, *md, Z, ldz, zr0, zc0, add, Zcopy, ldc, cr0, cc0);
   return (int)hipGetLastError();
 }
 

@@ -25,6 +25,7 @@
 #include <mutex>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "../../include/bem_hip.h"
@@ -36,12 +37,14 @@ int bem_launch_far_g6(const TriGeom *tg, int T, const Medium *md, int row_lo, in
                       double *tiles, size_t npairs, int2 *q36, unsigned int *q36_count, unsigned int q36_cap,
                       hipStream_t st);
 int bem_launch_far_g36(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, const unsigned int *count_dev,
-                       unsigned int max_count, int row_lo, int row_hi, double *tiles, size_t npairs, hipStream_t st);
+                       unsigned int max_count, int row_lo, int row_hi, int col_lo, int col_hi, double *tiles,
+                       size_t npairs, hipStream_t st);
 int bem_launch_close(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, unsigned int count, int row_lo,
-                     int row_hi, double *tiles, size_t npairs, hipStream_t st);
-int bem_launch_gather(const double *tiles, size_t npairs, int T, int row_lo, const RwgRec *rw, int r_lo, int r_hi,
-                      int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0, int add,
-                      double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st);
+                     int row_hi, int col_lo, int col_hi, double *tiles, size_t npairs, hipStream_t st);
+int bem_launch_gather(const double *tiles, size_t npairs, int tld, int row_lo, int col_lo, const RwgRec *rw, int r_lo,
+                      int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0,
+                      int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st);
+int bem_launch_q36_check(const unsigned int *count, unsigned int cap, int *flag, hipStream_t st);
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                   double2 beta, double2 *C, int64_t ldc, hipStream_t st);
 int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
@@ -85,14 +88,43 @@ struct DevTree {
   unsigned int nclose = 0;
 };
 
-// Device geometry of one (body 1, body 2, gap): the two-body tree and the
-// two one-body trees, uploaded once and reused by every frequency.
+// Device geometry of one (body 1, body 2, gap): the two-body tree, whose
+// cross blocks every frequency of that gap assembles.
 struct PairGeom {
   const void *b1 = nullptr, *b2 = nullptr;
   double gap[3] = {0, 0, 0};
-  DevTree top, s1, s2;
+  DevTree top;
+  int T1 = 0, E1 = 0;   // triangles / RWGs of body 1 (the first ones of top)
   std::vector<void *> allocs;
   size_t bytes = 0;
+};
+
+// Host-visible status of a self entry, written by its producing stream.
+struct SelfStatus {
+  int info[4];                    // [0] zero pivot, [1] 36-point queue overflow
+  unsigned long long growth[2];   // max|U|, max|A| of the LU (role 0)
+};
+
+// Gap-independent blocks of one body at one frequency (src/HeatTransfer.cpp:
+// 26-29 assembles them inside Z_ext and Z_int on every call): the self block
+// Z_ext(b, b) + Z_int(b) -- packed LU with interchanges and diagonal-block
+// inverses when the body sits in the first role (A), raw in the second (D) --
+// and sym(Z_int(b)).  Shared by every request on any lane; `ready` is
+// recorded on the producing stream, consumers wait on it.
+struct SelfEntry {
+  const void *body = nullptr;
+  double key[5] = {0, 0, 0, 0, 0};   // eps re/im, mu re/im, omega
+  int role = 0, pivot = 0;
+  int n = 0;
+  double2 *Z = nullptr, *Gs = nullptr, *inv = nullptr;
+  int *ipiv = nullptr, *dinfo = nullptr;
+  unsigned long long *dgrowth = nullptr;
+  SelfStatus *h = nullptr;   // pinned
+  hipEvent_t ready = nullptr;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
+  int pins = 0;
+  uint64_t stamp = 0;
 };
 
 // One stream and its workspaces; holds at most one solve in flight.
@@ -104,11 +136,12 @@ struct Lane {
   size_t ev_used = 0;
   hipEvent_t ph[4] = {nullptr, nullptr, nullptr, nullptr};
   double2 *h_trace = nullptr;   // pinned host slots of the results
-  int *h_info = nullptr;
-  unsigned long long *h_growth = nullptr;   // max|U|, max|A| of LU(A), then of LU(S)
+  int *h_info = nullptr;        // [0] zero pivot in the Schur LU, [1] queue overflow
+  unsigned long long *h_growth = nullptr;   // max|U|, max|A| of LU(S)
   bool busy = false;
   bem_result *res = nullptr;
   bem_request req;
+  SelfEntry *e1 = nullptr, *e2 = nullptr;   // pinned while in flight
   std::chrono::steady_clock::time_point t0;
 };
 
@@ -124,15 +157,24 @@ struct bem_ctx {
   std::unordered_map<void *, size_t> user_allocs;
   std::vector<Lane> lanes;
   std::list<PairGeom> pairs;                    // most recently used first
+  std::list<SelfEntry> selfs;
+  size_t self_bytes = 0, self_cap = 0;
+  uint64_t clock = 0;
   bool profile_gemm = true;
   // statistics of the last API call
   double t_last[9] = {0};
   double gemm_ms = 0, gemm_n = 0, gemm_flops = 0;
+  int self_hits = 0, self_misses = 0;
+  std::unordered_set<bem_body *> bodies;
 };
 
 struct bem_body {
   bem_ctx *ctx;
   bemgeo::BodyMesh mesh;
+  DevTree dt;                   // the body alone, untranslated (self blocks)
+  bool up = false;
+  std::vector<void *> allocs;
+  size_t bytes = 0;
 };
 
 static int fail(bem_ctx *c, int code, const char *fmt, ...) {
@@ -459,22 +501,6 @@ static void free_allocs(bem_ctx *c, std::vector<void *> &allocs, size_t bytes) {
   c->bytes -= bytes;
 }
 
-// Body-local view of a tree: triangles/RWGs of body b re-based to 0.
-static void body_subtree(const bemgeo::Tree &t, int b, bemgeo::Tree &out) {
-  int t0 = t.tri_off[b], t1 = (b + 1 < (int)t.tri_off.size()) ? t.tri_off[b + 1] : t.T;
-  int e0 = t.rwg_off[b], e1 = (b + 1 < (int)t.rwg_off.size()) ? t.rwg_off[b + 1] : t.E;
-  out.T = t1 - t0;
-  out.E = e1 - e0;
-  out.tg.assign(t.tg.begin() + t0, t.tg.begin() + t1);
-  out.rw.assign(t.rw.begin() + e0, t.rw.begin() + e1);
-  for (auto &r : out.rw) { r.tp -= t0; r.tn -= t0; }
-  out.close.clear();
-  for (auto &p : t.close)
-    if (p.first >= t0 && p.first < t1) out.close.push_back({p.first - t0, p.second - t0});
-  out.tri_off = {0};
-  out.rwg_off = {0};
-}
-
 static const size_t PAIR_CACHE = 8;
 
 static int pair_geom(bem_ctx *c, bem_body *b1, bem_body *b2, const double *gap, PairGeom **out) {
@@ -496,13 +522,12 @@ static int pair_geom(bem_ctx *c, bem_body *b1, bem_body *b2, const double *gap, 
   for (int k = 0; k < 3; k++) g.gap[k] = gap[k];
   std::vector<const bemgeo::BodyMesh *> bm = {&b1->mesh, &b2->mesh};
   double shifts[6] = {gap[0], gap[1], gap[2], 0, 0, 0};
-  bemgeo::Tree top, s1, s2;
+  bemgeo::Tree top;
   bemgeo::build_tree(bm, shifts, top);
-  body_subtree(top, 0, s1);
-  body_subtree(top, 1, s2);
+  g.T1 = top.tri_off.size() > 1 ? top.tri_off[1] : top.T;
+  g.E1 = top.rwg_off.size() > 1 ? top.rwg_off[1] : top.E;
+  top.close.clear();   // cross blocks never hold close pairs
   int e = upload_tree(c, top, g.top, g.allocs, g.bytes);
-  if (!e) e = upload_tree(c, s1, g.s1, g.allocs, g.bytes);
-  if (!e) e = upload_tree(c, s2, g.s2, g.allocs, g.bytes);
   if (e) {
     free_allocs(c, g.allocs, g.bytes);
     c->pairs.pop_front();
@@ -512,22 +537,41 @@ static int pair_geom(bem_ctx *c, bem_body *b1, bem_body *b2, const double *gap, 
   return 0;
 }
 
-// Triangle-pair tiles of one tree in medium md, on the lane's stream.
-static int assemble_tiles(bem_ctx *c, Lane &L, const DevTree &d, const Medium &md, double *tiles) {
+// The one-body tree of b (untranslated), uploaded on first use.
+static int body_tree(bem_ctx *c, bem_body *b, const DevTree **out) {
+  if (!b->up) {
+    std::vector<const bemgeo::BodyMesh *> bm = {&b->mesh};
+    double shift[3] = {0, 0, 0};
+    bemgeo::Tree t;
+    bemgeo::build_tree(bm, shift, t);
+    CHK(upload_tree(c, t, b->dt, b->allocs, b->bytes));
+    b->up = true;
+  }
+  *out = &b->dt;
+  return 0;
+}
+
+// Triangle-pair tiles of test rows [r0, r1) x source columns [c0, c1) of a
+// tree in medium md, on the lane's stream (tile leading dimension c1 - c0).
+// A 36-point queue overflow sets *flag (device int).
+static int assemble_tiles(bem_ctx *c, Lane &L, const DevTree &d, const Medium &md, double *tiles, int r0, int r1,
+                          int c0, int c1, int *flag) {
   const int T = d.T;
-  size_t npairs = (size_t)T * (size_t)T;
+  size_t npairs = (size_t)(r1 - r0) * (size_t)(c1 - c0);
   unsigned int *q36cnt;
   int2 *q36;
-  unsigned int cap = (unsigned int)std::min<size_t>(npairs, (size_t)1 << 24);
+  unsigned int cap = (unsigned int)std::min<size_t>(std::max<size_t>(npairs, 1), (size_t)1 << 24);
   CHK(ws_get(c, L, "q36_count", sizeof(unsigned int), (void **)&q36cnt));
   CHK(ws_get(c, L, "q36", sizeof(int2) * (size_t)cap, (void **)&q36));
   DEV(c, hipMemsetAsync(q36cnt, 0, sizeof(unsigned int), L.st));
-  int e = bem_launch_far_g6(d.tg, T, &md, 0, T, 0, T, tiles, npairs, q36, q36cnt, cap, L.st);
+  int e = bem_launch_far_g6(d.tg, T, &md, r0, r1, c0, c1, tiles, npairs, q36, q36cnt, cap, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "far_g6 launch: %s", hipGetErrorString((hipError_t)e));
-  e = bem_launch_far_g36(d.tg, T, &md, q36, q36cnt, cap, 0, T, tiles, npairs, L.st);
+  e = bem_launch_far_g36(d.tg, T, &md, q36, q36cnt, cap, r0, r1, c0, c1, tiles, npairs, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "far_g36 launch: %s", hipGetErrorString((hipError_t)e));
-  e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, 0, T, tiles, npairs, L.st);
+  e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, r0, r1, c0, c1, tiles, npairs, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "close launch: %s", hipGetErrorString((hipError_t)e));
+  if (flag && (e = bem_launch_q36_check(q36cnt, cap, flag, L.st)))
+    return fail(c, BEM_E_DEVICE, "queue check launch: %s", hipGetErrorString((hipError_t)e));
   return 0;
 }
 
@@ -536,100 +580,249 @@ static double theta(double omega, double T) {
   return SI_HBAR * omega / (std::exp(SI_HBAR * omega / (SI_KB * T)) - 1);
 }
 
+// ------------------------------------------------------------- self blocks
+static void self_free(bem_ctx *c, SelfEntry &e) {
+  if (e.ready) (void)hipEventSynchronize(e.ready);
+  for (void *p : e.allocs) (void)hipFree(p);
+  e.allocs.clear();
+  if (e.h) (void)hipHostFree(e.h);
+  e.h = nullptr;
+  if (e.ready) (void)hipEventDestroy(e.ready);
+  e.ready = nullptr;
+  c->bytes -= e.bytes;
+  c->self_bytes -= e.bytes;
+  e.bytes = 0;
+}
+
+// Drop least recently used unpinned entries until `need` more bytes fit.
+static void self_evict(bem_ctx *c, size_t need) {
+  while (c->self_bytes + need > c->self_cap) {
+    auto victim = c->selfs.end();
+    for (auto it = c->selfs.begin(); it != c->selfs.end(); ++it)
+      if (it->pins == 0 && (victim == c->selfs.end() || it->stamp < victim->stamp)) victim = it;
+    if (victim == c->selfs.end()) return;   // everything in use: the cache runs over its cap
+    self_free(c, *victim);
+    c->selfs.erase(victim);
+  }
+}
+
+static int self_alloc(bem_ctx *c, SelfEntry &e, void **p, size_t bytes) {
+  if (c->budget && c->bytes + bytes > c->budget)
+    return fail(c, BEM_E_NOMEM, "self-block cache entry (%zu bytes) exceeds the HBM budget", bytes);
+  hipError_t r = hipMalloc(p, bytes ? bytes : 1);
+  if (r != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for a self block: %s", bytes, hipGetErrorString(r));
+  e.allocs.push_back(*p);
+  e.bytes += bytes;
+  c->bytes += bytes;
+  c->self_bytes += bytes;
+  return 0;
+}
+
+// Self entry of body b in medium (eps, mu) at omega for role 0 (factored) or
+// 1 (raw), pinned for the lane; produced on the lane's stream when missing.
+static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const double *mu, double omega, int role,
+                    int pivot, SelfEntry **out) {
+  const double key[5] = {eps[0], eps[1], mu[0], mu[1], omega};
+  for (auto &e : c->selfs)
+    if (e.body == b && e.role == role && e.pivot == pivot && std::memcmp(e.key, key, sizeof key) == 0) {
+      e.pins++;
+      e.stamp = ++c->clock;
+      DEV(c, hipStreamWaitEvent(L.st, e.ready, 0));
+      c->self_hits++;
+      *out = &e;
+      return 0;
+    }
+  c->self_misses++;
+  const DevTree *d;
+  CHK(body_tree(c, b, &d));
+  const int T = d->T, n = 2 * d->E;
+  size_t nn = sizeof(double2) * (size_t)n * n;
+  size_t need = 2 * nn + (role == 0 ? inv_bytes(n) + sizeof(int) * (size_t)(n + 64) : 0) + 256;
+  self_evict(c, need);
+  c->selfs.emplace_front();
+  SelfEntry &e = c->selfs.front();
+  e.body = b;
+  std::memcpy(e.key, key, sizeof key);
+  e.role = role;
+  e.pivot = pivot;
+  e.n = n;
+  e.pins = 1;
+  e.stamp = ++c->clock;
+  auto undo = [&](int code) {
+    self_free(c, e);
+    c->selfs.pop_front();
+    return code;
+  };
+  int r = self_alloc(c, e, (void **)&e.Z, nn);
+  if (!r) r = self_alloc(c, e, (void **)&e.Gs, nn);
+  if (!r) r = self_alloc(c, e, (void **)&e.dinfo, sizeof(int) * 4 + sizeof(unsigned long long) * 2);
+  if (!r && role == 0) r = self_alloc(c, e, (void **)&e.inv, inv_bytes(n));
+  if (!r && role == 0) r = self_alloc(c, e, (void **)&e.ipiv, sizeof(int) * (size_t)(n + 64));
+  if (!r && hipHostMalloc((void **)&e.h, sizeof(SelfStatus), hipHostMallocDefault) != hipSuccess)
+    r = fail(c, BEM_E_NOMEM, "pinned status slot");
+  if (!r && hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess)
+    r = fail(c, BEM_E_DEVICE, "event creation");
+  if (r) return undo(r);
+  e.dgrowth = (unsigned long long *)(e.dinfo + 4);
+  // Z_ext(b, b) (vacuum) then + Z_int(b), whose copy is G
+  Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
+  Medium mint = make_medium(eps[0], eps[1], mu[0], mu[1], omega);
+  double *tiles;
+  size_t np = (size_t)T * T;
+  r = ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * np, (void **)&tiles);
+  if (!r && hipMemsetAsync(e.dinfo, 0, sizeof(int) * 4 + sizeof(unsigned long long) * 2, L.st) != hipSuccess)
+    r = fail(c, BEM_E_DEVICE, "memset");
+  if (!r) r = assemble_tiles(c, L, *d, mext, tiles, 0, T, 0, T, e.dinfo + 1);
+  if (!r && bem_launch_gather(tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mext, e.Z, n, 0, 0, 0, nullptr, 0, 0, 0,
+                              L.st))
+    r = fail(c, BEM_E_DEVICE, "gather launch");
+  if (!r) r = assemble_tiles(c, L, *d, mint, tiles, 0, T, 0, T, e.dinfo + 1);
+  if (!r && bem_launch_gather(tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mint, e.Z, n, 0, 0, 1, e.Gs, n, 0, 0,
+                              L.st))
+    r = fail(c, BEM_E_DEVICE, "gather launch");
+  if (!r && bem_symm(e.Gs, n, n, L.st)) r = fail(c, BEM_E_DEVICE, "symm launch");
+  if (!r && role == 0) {
+    if (pivot) r = lu_factor(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv);
+    else r = lu_factor_block(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv, e.dgrowth);
+  }
+  if (!r && hipMemcpyAsync(e.h, e.dinfo, sizeof(SelfStatus), hipMemcpyDeviceToHost, L.st) != hipSuccess)
+    r = fail(c, BEM_E_DEVICE, "status copy");
+  if (!r && hipEventRecord(e.ready, L.st) != hipSuccess) r = fail(c, BEM_E_DEVICE, "event record");
+  if (r) {
+    (void)hipStreamSynchronize(L.st);
+    return undo(r);
+  }
+  *out = &e;
+  return 0;
+}
+
+static void self_release(bem_ctx *c, Lane &L) {
+  for (SelfEntry **p : {&L.e1, &L.e2})
+    if (*p) {
+      (*p)->pins--;
+      *p = nullptr;
+    }
+  (void)c;
+}
+
+// Drops every cached entry and tree of body b (after the device drained).
+static void self_drop_body(bem_ctx *c, const void *b) {
+  for (auto it = c->selfs.begin(); it != c->selfs.end();) {
+    if (it->body == b) {
+      self_free(c, *it);
+      it = c->selfs.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
 // ------------------------------------------------------------- one solve
 // Issues every kernel of one FP64 spectral-power solve on the lane's stream
 // and returns without waiting; lane_finish() collects the result.
-static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, const bem_request *q, bem_result *res) {
+//   A = Z_ext(1,1) + Z_int(1) (cached, factored), D = Z_ext(2,2) + Z_int(2)
+//   (cached), B = Z_ext(1,2), C = Z_ext(2,1) (assembled for this gap);
+//   Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1, S = 1/4 tr((X sym G2)^H sym G1 X).
+static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *b2, const bem_request *q,
+                      bem_result *res) {
   L.t0 = std::chrono::steady_clock::now();
   L.req = *q;
   L.res = res;
   double omegaSI = q->eV * SI_EV / SI_HBAR;
   double omega = omegaSI * q->a / SI_C;
-  const int m1 = 2 * g.s1.E, m2 = 2 * g.s2.E, M = m1 + m2;
+  const int pivot = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
   Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
-  Medium m1d = make_medium(q->eps1[0], q->eps1[1], q->mu1[0], q->mu1[1], omega);
-  Medium m2d = make_medium(q->eps2[0], q->eps2[1], q->mu2[0], q->mu2[1], omega);
+  DEV(c, hipEventRecord(L.ph[0], L.st));
+  CHK(self_get(c, L, b1, q->eps1, q->mu1, omega, 0, pivot, &L.e1));
+  CHK(self_get(c, L, b2, q->eps2, q->mu2, omega, 1, 0, &L.e2));
+  const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
+  const int m1 = E1.n, m2 = E2.n;
+  const int T1 = g.T1, T2 = g.top.T - g.T1, Eb1 = g.E1, E = g.top.E;
+  if (m1 != 2 * Eb1 || m2 != 2 * (E - Eb1)) return fail(c, BEM_E_INVALID, "body sizes do not match the pair tree");
 
-  double2 *W, *G1, *G2, *B1, *partial, *invA, *invS;
+  double2 *Bm, *Cm, *Sc, *B1, *partial, *invS;
   double *tiles;
-  int *ipA, *info;
+  int *ipS, *info;
   unsigned long long *gro;
-  const bool tournament = (q->flags & BEM_SOLVE_TOURNAMENT) != 0;
-  size_t npairs_top = (size_t)g.top.T * g.top.T;
-  CHK(ws_get(c, L, "W", sizeof(double2) * (size_t)M * M, (void **)&W));
-  CHK(ws_get(c, L, "G1", sizeof(double2) * (size_t)m1 * m1, (void **)&G1));
-  CHK(ws_get(c, L, "G2", sizeof(double2) * (size_t)m2 * m2, (void **)&G2));
-  CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * npairs_top, (void **)&tiles));
-  CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(M + 64), (void **)&ipA));
-  int *ipS = ipA + m1;
-  CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
-  CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 4, (void **)&gro));
-  CHK(ws_get(c, L, "invA", inv_bytes(m1), (void **)&invA));
-  CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
+  size_t npc = (size_t)T1 * T2;
+  CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
+  CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
+  CHK(ws_get(c, L, "Sc", sizeof(double2) * (size_t)m2 * m2, (void **)&Sc));
   CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
+  CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(npc, 1), (void **)&tiles));
+  CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
+  CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
+  CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 2, (void **)&gro));
+  CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
   size_t np = bem_trace_partials(m1, m2);
   CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
   double2 *trace_out = partial + np;
   DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
-  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 4, L.st));
+  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 2, L.st));
 
-  DEV(c, hipEventRecord(L.ph[0], L.st));
-  // ---- assembly: W = Z_ext(top), then + Z_int(b) on the diagonal blocks
-  CHK(assemble_tiles(c, L, g.top, mext, tiles));
-  int e = bem_launch_gather(tiles, npairs_top, g.top.T, 0, g.top.rw, 0, g.top.E, 0, g.top.E, &mext, W, M, 0, 0, 0,
-                            nullptr, 0, 0, 0, L.st);
+  // ---- cross blocks of Z_ext for this gap
+  CHK(assemble_tiles(c, L, g.top, mext, tiles, 0, T1, T1, T1 + T2, info + 1));
+  int e = bem_launch_gather(tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0,
+                            L.st);
   if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
-  CHK(assemble_tiles(c, L, g.s1, m1d, tiles));
-  e = bem_launch_gather(tiles, (size_t)g.s1.T * g.s1.T, g.s1.T, 0, g.s1.rw, 0, g.s1.E, 0, g.s1.E, &m1d, W, M, 0, 0, 1,
-                        G1, m1, 0, 0, L.st);
+  CHK(assemble_tiles(c, L, g.top, mext, tiles, T1, T1 + T2, 0, T1, info + 1));
+  e = bem_launch_gather(tiles, npc, T1, T1, 0, g.top.rw, Eb1, E, 0, Eb1, &mext, Cm, m2, 0, 0, 0, nullptr, 0, 0, 0,
+                        L.st);
   if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
-  CHK(assemble_tiles(c, L, g.s2, m2d, tiles));
-  e = bem_launch_gather(tiles, (size_t)g.s2.T * g.s2.T, g.s2.T, 0, g.s2.rw, 0, g.s2.E, 0, g.s2.E, &m2d, W, M, m1, m1,
-                        1, G2, m2, 0, 0, L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
+  DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
   // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-  (void)zero;
-  double2 *A = W, *Bm = W + (int64_t)m1 * M, *Cm = W + m1, *D = W + (int64_t)m1 * M + m1;
-  if (tournament) CHK(lu_factor(c, L, A, M, m1, ipA, info, invA));
-  else CHK(lu_factor_block(c, L, A, M, m1, ipA, info, invA, gro));
-  CHK(lu_solve(c, L, A, M, m1, ipA, invA, Bm, M, m2));   // B <- Y
-  CHK(gemm(c, L, m2, m2, m1, mone, Cm, M, Bm, M, one, D, M));
-  if (tournament) CHK(lu_factor(c, L, D, M, m2, ipS, info + 1, invS));
-  else CHK(lu_factor_block(c, L, D, M, m2, ipS, info + 1, invS, gro + 2));
+  CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2));   // B <- Y
+  CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
+  if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
+  else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
   // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
-  CHK(lu_solve_right(c, L, D, M, m2, ipS, invS, Bm, M, m1));
+  CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1));
   double2 *X = Bm;
   DEV(c, hipEventRecord(L.ph[2], L.st));
 
   // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
-  e = bem_symm(G1, m1, m1, L.st);
-  if (!e) e = bem_symm(G2, m2, m2, L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "symm: %d", e);
-  CHK(gemm(c, L, m1, m2, m1, one, G1, m1, X, M, zero, B1, m1));
+  CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
   {
     EventRec *r = nullptr;
     if (c->profile_gemm) CHK(ev_begin(c, L, 8.0 * m1 * (double)m2 * m2, &r));
-    e = bem_zgemm_trace(m1, m2, m2, one, X, M, G2, m2, B1, m1, partial, trace_out, L.st);
+    e = bem_zgemm_trace(m1, m2, m2, one, X, m1, E2.Gs, m2, B1, m1, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
     if (r) DEV(c, hipEventRecord(r->b, L.st));
   }
   DEV(c, hipEventRecord(L.ph[3], L.st));
   DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2), hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
-  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 4, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, L.st));
   L.busy = true;
   return 0;
 }
 
+static double growth_ratio(const unsigned long long *g) {
+  double u, a;
+  std::memcpy(&u, &g[0], sizeof u);
+  std::memcpy(&a, &g[1], sizeof a);
+  double r = a > 0 ? u / a : 0.0;
+  return std::isfinite(r) ? r : INFINITY;
+}
+
 static int lane_finish(bem_ctx *c, Lane &L) {
-  if (!L.busy) return 0;
+  if (!L.busy) {
+    self_release(c, L);
+    return 0;
+  }
   L.busy = false;
-  DEV(c, hipStreamSynchronize(L.st));
+  hipError_t se = hipStreamSynchronize(L.st);
+  const SelfEntry *E1 = L.e1, *E2 = L.e2;
   bem_result *res = L.res;
   const bem_request *q = &L.req;
+  if (se != hipSuccess) {
+    self_release(c, L);
+    res->status = BEM_E_DEVICE;
+    return fail(c, BEM_E_DEVICE, "solve failed on the device: %s", hipGetErrorString(se));
+  }
   float ta = 0, tf = 0, tp = 0;
   (void)hipEventElapsedTime(&ta, L.ph[0], L.ph[1]);
   (void)hipEventElapsedTime(&tf, L.ph[1], L.ph[2]);
@@ -652,19 +845,19 @@ static int lane_finish(bem_ctx *c, Lane &L) {
   res->status = 0;
   res->pivoting = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
   res->growth = 0.0;
-  if (!res->pivoting) {
-    double g[4];
-    std::memcpy(g, L.h_growth, sizeof g);
-    double ga = g[1] > 0 ? g[0] / g[1] : 0.0, gs = g[3] > 0 ? g[2] / g[3] : 0.0;
-    res->growth = std::max(ga, gs);
-    if (!std::isfinite(res->growth)) res->growth = INFINITY;
-  }
+  int sing_a = E1 ? E1->h->info[0] : 0, ovf = L.h_info[1] || (E1 && E1->h->info[1]) || (E2 && E2->h->info[1]);
+  if (!res->pivoting && E1) res->growth = std::max(growth_ratio(E1->h->growth), growth_ratio(L.h_growth));
+  self_release(c, L);
   c->t_last[0] = ta;
   c->t_last[4] = tf;
   c->t_last[5] = tp;
-  if (L.h_info[0] || L.h_info[1]) {
+  if (ovf) {
+    res->status = BEM_E_DEVICE;
+    return fail(c, BEM_E_DEVICE, "36-point pair queue overflowed");
+  }
+  if (sing_a || L.h_info[0]) {
     res->status = BEM_E_SINGULAR;
-    return fail(c, BEM_E_SINGULAR, "exactly singular pivot (A: %d, Schur: %d)", L.h_info[0], L.h_info[1]);
+    return fail(c, BEM_E_SINGULAR, "exactly singular pivot (A: %d, Schur: %d)", sing_a, L.h_info[0]);
   }
   return 0;
 }
@@ -692,6 +885,9 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     delete c;
     return BEM_E_DEVICE;
   }
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) tot = 0;
+  c->self_cap = (cfg && cfg->cache_bytes) ? cfg->cache_bytes : (size_t)(0.4 * (double)tot);
   c->lanes.resize(nl);
   for (int i = 0; i < nl; i++) {
     Lane &L = c->lanes[i];
@@ -720,6 +916,13 @@ void bem_ctx_destroy(bem_ctx *c) {
   for (auto &kv : c->user_allocs) (void)hipFree(kv.first);
   for (auto &g : c->pairs)
     for (void *p : g.allocs) (void)hipFree(p);
+  for (auto &e : c->selfs) self_free(c, e);
+  for (bem_body *b : c->bodies) {
+    for (void *p : b->allocs) (void)hipFree(p);
+    b->allocs.clear();
+    b->up = false;
+    b->ctx = nullptr;   // bem_body_free then only releases the host object
+  }
   for (auto &L : c->lanes) {
     for (auto &r : L.ev) {
       (void)hipEventDestroy(r.a);
@@ -759,6 +962,7 @@ int bem_body_upload(bem_ctx *c, const double *xyz, int nv, const int32_t *tri, i
   }
   M.ne = ne;
   bemgeo::build_close_pairs(M);
+  c->bodies.insert(b);
   *out = b;
   return 0;
 }
@@ -773,8 +977,10 @@ int bem_body_edges(bem_body *b, int32_t *edges, int *ne) {
 void bem_body_free(bem_body *b) {
   if (!b) return;
   bem_ctx *c = b->ctx;
-  {
+  if (c) {
     std::lock_guard<std::recursive_mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    for (auto &L : c->lanes) (void)lane_finish(c, L);
     for (auto it = c->pairs.begin(); it != c->pairs.end();) {
       if (it->b1 == b || it->b2 == b) {
         free_allocs(c, it->allocs, it->bytes);
@@ -783,6 +989,9 @@ void bem_body_free(bem_body *b) {
         ++it;
       }
     }
+    self_drop_body(c, b);
+    if (!b->allocs.empty()) free_allocs(c, b->allocs, b->bytes);
+    c->bodies.erase(b);
   }
   delete b;
 }
@@ -841,17 +1050,25 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   bool dev = flags & BEM_ASM_DEVICE_PTR;
   int64_t ld = dev ? ldz : M;
   if (!e) e = ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * npairs, (void **)&tiles);
-  if (!e) e = assemble_tiles(c, L, d, md, tiles);
+  int *flag = nullptr;
+  if (!e) e = ws_get(c, L, "info", sizeof(int) * 4, (void **)&flag);
+  if (!e && hipMemsetAsync(flag, 0, sizeof(int) * 4, L.st) != hipSuccess) e = fail(c, BEM_E_DEVICE, "memset");
+  if (!e) e = assemble_tiles(c, L, d, md, tiles, 0, t.T, 0, t.T, flag + 1);
   if (!e) {
     if (dev) Zd = (double2 *)Z;
     else e = ws_get(c, L, "asm_Z", sizeof(double2) * (size_t)M * M, (void **)&Zd);
   }
-  if (!e && bem_launch_gather(tiles, npairs, t.T, 0, d.rw, 0, t.E, 0, t.E, &md, Zd, ld, 0, 0, 0, nullptr, 0, 0, 0, L.st))
+  if (!e && bem_launch_gather(tiles, npairs, t.T, 0, 0, d.rw, 0, t.E, 0, t.E, &md, Zd, ld, 0, 0, 0, nullptr, 0, 0, 0,
+                             L.st))
     e = fail(c, BEM_E_DEVICE, "gather launch failed");
   if (!e && !dev && hipMemcpy2DAsync(Z, sizeof(double2) * ldz, Zd, sizeof(double2) * M, sizeof(double2) * M, M,
                                      hipMemcpyDeviceToHost, L.st) != hipSuccess)
     e = fail(c, BEM_E_DEVICE, "copy of Z failed");
+  int hflag[4] = {0, 0, 0, 0};
+  if (!e && hipMemcpyAsync(hflag, flag, sizeof hflag, hipMemcpyDeviceToHost, L.st) != hipSuccess)
+    e = fail(c, BEM_E_DEVICE, "flag copy");
   if (!e && hipStreamSynchronize(L.st) != hipSuccess) e = fail(c, BEM_E_DEVICE, "assembly failed on the device");
+  if (!e && hflag[1]) e = fail(c, BEM_E_DEVICE, "36-point pair queue overflowed");
   free_allocs(c, allocs, tb);
   return e;
 }
@@ -867,13 +1084,13 @@ int bem_spectral_power(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request
   CHK(pair_geom(c, b1, b2, q->gap, &g));
   Lane &L = c->lanes[0];
   CHK(lane_finish(c, L));
-  int e = lane_issue(c, L, *g, q, res);
+  int e = lane_issue(c, L, *g, b1, b2, q, res);
   int f = lane_finish(c, L);
   if (e) return e;
   if (!f && !(res->growth <= BEM_GROWTH_LIMIT) && !(q->flags & BEM_SOLVE_TOURNAMENT)) {
     bem_request q2 = *q;
     q2.flags |= BEM_SOLVE_TOURNAMENT;
-    e = lane_issue(c, L, *g, &q2, res);
+    e = lane_issue(c, L, *g, b1, b2, &q2, res);
     f = lane_finish(c, L);
     if (e) return e;
   }
@@ -898,12 +1115,13 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
     int e = check_request(c, &reqs[i]);
     PairGeom *g = nullptr;
     if (!e) e = pair_geom(c, b1, b2, reqs[i].gap, &g);
-    if (!e) e = lane_issue(c, L, *g, &reqs[i], &res[i]);
+    if (!e) e = lane_issue(c, L, *g, b1, b2, &reqs[i], &res[i]);
     if (e) {
       res[i].status = e;
       if (!first_err) first_err = e;
       (void)hipStreamSynchronize(L.st);
       L.busy = false;
+      self_release(c, L);
     }
   }
   for (auto &L : c->lanes) {
@@ -919,7 +1137,7 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
     Lane &L = c->lanes[0];
     PairGeom *g = nullptr;
     int e = pair_geom(c, b1, b2, q2.gap, &g);
-    if (!e) e = lane_issue(c, L, *g, &q2, &res[i]);
+    if (!e) e = lane_issue(c, L, *g, b1, b2, &q2, &res[i]);
     int f = e ? e : lane_finish(c, L);
     if (f) {
       res[i].status = f;

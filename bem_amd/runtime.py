@@ -42,7 +42,7 @@ class BemError(RuntimeError):
 
 class _Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("lu_block", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
-                ("lanes", ctypes.c_int)]
+                ("lanes", ctypes.c_int), ("cache_bytes", ctypes.c_size_t)]
 
 
 class _Req(ctypes.Structure):
@@ -182,9 +182,9 @@ class Body:
         return out
 
     def free(self):
-        # a body belongs to its context: once the context is gone its memory
-        # went with the process's device state and must not be touched
-        if self.handle and getattr(self.ctx, "h", None):
+        # after bem_ctx_destroy the body holds no device memory and
+        # bem_body_free only releases the host object
+        if self.handle:
             self.ctx.lib.bem_body_free(self.handle)
         self.handle = None
 
@@ -198,9 +198,9 @@ class Body:
 class Context:
     """One HIP device context (bem_ctx)."""
 
-    def __init__(self, device=0, hbm_budget=0, lanes=0):
+    def __init__(self, device=0, hbm_budget=0, lanes=0, cache_bytes=0):
         self.lib = load_library()
-        cfg = _Cfg(device, 0, hbm_budget, lanes)
+        cfg = _Cfg(device, 0, hbm_budget, lanes, cache_bytes)
         h = ctypes.c_void_p()
         rc = self.lib.bem_ctx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

@@ -58,6 +58,7 @@ typedef struct {
   int lu_block;      /* reserved (the LU panel width is fixed at 64), pass 0 */
   size_t hbm_budget; /* bytes a context may hold, 0 = no limit */
   int lanes;         /* solves a sweep keeps in flight, one stream each (0 = 2, max 8) */
+  size_t cache_bytes;/* device bytes for cached self blocks (0 = 40% of the device) */
 } bem_cfg;
 
 typedef struct {
@@ -122,6 +123,10 @@ int bem_assemble(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *sh
 int bem_pair_histogram(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *shifts, int64_t *hist6);
 
 /* ---- spectral power (CalcHeatTransfer) -------------------------------- */
+/* Gap-independent blocks -- Z_ext(b,b) + Z_int(b) (factored for body 1) and
+ * sym(Z_int(b)) -- are cached on the device per (body, medium, omega) and
+ * reused by later requests on any lane; only the cross blocks Z_ext(1,2),
+ * Z_ext(2,1) are assembled per gap.  Cache capacity: cfg.cache_bytes. */
 int bem_spectral_power(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *req, bem_result *res);
 /* n independent requests on the same body pair, results in request order.
  * Up to cfg.lanes requests run concurrently on separate streams; each result

@@ -143,3 +143,48 @@ def test_pivoting_modes_agree(gpu_ctx, flags):
     assert abs(got["ReS"] - ref["ReS"]) <= S_TOL * abs(ref["ReS"])
     if flags == 0:
         assert 0.0 < got["growth"] < 100.0
+
+
+def test_self_block_cache_across_gaps(gpu_ctx):
+    """Gap-independent blocks are cached per (body, medium, omega): a gap
+    series at one frequency reuses them, and every result is bit-identical to
+    the same request on a fresh context whose cache holds nothing, and within
+    tolerance of the oracle."""
+    import bem_amd
+    from bem_amd import mesh, Request, lorentz_eps
+    from oracle import oracle as O
+    s = mesh.icosphere(2)
+    c = mesh.cube(2)
+    eV = 0.105
+    eps = lorentz_eps(eV)
+    b1, b2 = gpu_ctx.body(s), gpu_ctx.body(c)
+    gaps = [(1.6, 0.2, 0.1), (1.75, 0.0, 0.0), (2.4, -0.3, 0.2)]
+    cached = [gpu_ctx.calc_heat_transfer(b1, b2, Request(eV=eV, gap=g, eps1=eps, eps2=2.5 + 0.1j)) for g in gaps]
+    fresh_ctx = bem_amd.Context(0, lanes=1, cache_bytes=1)
+    try:
+        f1, f2 = fresh_ctx.body(s), fresh_ctx.body(c)
+        for g, r in zip(gaps, cached):
+            f = fresh_ctx.calc_heat_transfer(f1, f2, Request(eV=eV, gap=g, eps1=eps, eps2=2.5 + 0.1j))
+            assert f["ReS"] == r["ReS"] and f["ImS"] == r["ImS"]
+    finally:
+        fresh_ctx.close()
+    ref = O.spectral_power(s.translated(gaps[1]), c, eV, 1e-6, 300.0, 0.0, eps, 2.5 + 0.1j, return_parts=True)
+    assert abs(cached[1]["ReS"] - ref["ReS"]) <= S_TOL * abs(ref["ReS"])
+
+
+def test_body_free_releases_device_memory(gpu_ctx):
+    """Geometry and cached self blocks of a body go when the body is freed:
+    two identical upload-solve-free rounds end at the same device footprint
+    (lane workspaces are grow-only and sized by the first round)."""
+    from bem_amd import mesh, Request, lorentz_eps
+    s = mesh.icosphere(1)
+    held = []
+    for _ in range(2):
+        b1, b2 = gpu_ctx.body(s), gpu_ctx.body(s)
+        r = gpu_ctx.calc_heat_transfer(b1, b2, Request(eV=0.1, gap=(2.5, 0, 0), eps1=lorentz_eps(0.1),
+                                                       eps2=lorentz_eps(0.1)))
+        assert r["status"] == 0
+        b1.free()
+        b2.free()
+        held.append(gpu_ctx.device_bytes())
+    assert held[0] == held[1]
