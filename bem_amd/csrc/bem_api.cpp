@@ -45,6 +45,7 @@ int bem_launch_gather(const double *tiles, size_t npairs, int tld, int row_lo, i
                       int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0,
                       int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st);
 int bem_launch_q36_check(const unsigned int *count, unsigned int cap, int *flag, hipStream_t st);
+int bem_zgemm_wide(int m, int n);
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                   double2 beta, double2 *C, int64_t ldc, hipStream_t st);
 int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
@@ -75,9 +76,11 @@ struct DevBuf {
   size_t cap = 0;
 };
 
+// One timed launch: its kernel class (BEM_KS_*) and algorithmic work.
 struct EventRec {
   hipEvent_t a, b;
-  double flops;
+  int kind;
+  double flops, bytes;
 };
 
 struct DevTree {
@@ -160,10 +163,11 @@ struct bem_ctx {
   std::list<SelfEntry> selfs;
   size_t self_bytes = 0, self_cap = 0;
   uint64_t clock = 0;
-  bool profile_gemm = true;
+  bool profile = true;   // bracket timed kernel classes with events
   // statistics of the last API call
   double t_last[9] = {0};
   double gemm_ms = 0, gemm_n = 0, gemm_flops = 0;
+  double kst[BEM_KS_COUNT][4] = {};   // per class: ms, launches, flops, bytes
   int self_hits = 0, self_misses = 0;
   std::unordered_set<bem_body *> bodies;
 };
@@ -226,7 +230,9 @@ static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **ou
   return 0;
 }
 
-static int ev_begin(bem_ctx *c, Lane &L, double flops, EventRec **rec) {
+static int ev_begin(bem_ctx *c, Lane &L, int kind, double flops, double bytes, EventRec **rec) {
+  *rec = nullptr;
+  if (!c->profile) return 0;
   if (L.ev_used == L.ev.size()) {
     EventRec r;
     DEV(c, hipEventCreate(&r.a));
@@ -234,9 +240,16 @@ static int ev_begin(bem_ctx *c, Lane &L, double flops, EventRec **rec) {
     L.ev.push_back(r);
   }
   EventRec &r = L.ev[L.ev_used++];
+  r.kind = kind;
   r.flops = flops;
+  r.bytes = bytes;
   DEV(c, hipEventRecord(r.a, L.st));
   *rec = &r;
+  return 0;
+}
+
+static int ev_end(bem_ctx *c, Lane &L, EventRec *r) {
+  if (r) DEV(c, hipEventRecord(r->b, L.st));
   return 0;
 }
 
@@ -245,27 +258,40 @@ static int ev_begin(bem_ctx *c, Lane &L, double flops, EventRec **rec) {
 static void ev_collect(bem_ctx *c, Lane &L) {
   for (size_t i = 0; i < L.ev_used; i++) {
     float ms = 0;
-    if (hipEventElapsedTime(&ms, L.ev[i].a, L.ev[i].b) == hipSuccess) {
-      c->gemm_ms += ms;
-      c->gemm_n += 1;
-      c->gemm_flops += L.ev[i].flops;
+    const EventRec &r = L.ev[i];
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      double *k = c->kst[r.kind];
+      k[0] += ms;
+      k[1] += 1;
+      k[2] += r.flops;
+      k[3] += r.bytes;
+      if (r.kind == BEM_KS_ZGEMM || r.kind == BEM_KS_ZGEMM_NARROW || r.kind == BEM_KS_ZTRACE) {
+        c->gemm_ms += ms;
+        c->gemm_n += 1;
+        c->gemm_flops += r.flops;
+      }
     }
   }
   L.ev_used = 0;
 }
 
-static void stats_reset(bem_ctx *c) { c->gemm_ms = c->gemm_n = c->gemm_flops = 0; }
+static void stats_reset(bem_ctx *c) {
+  c->gemm_ms = c->gemm_n = c->gemm_flops = 0;
+  std::memset(c->kst, 0, sizeof c->kst);
+}
 
 // GEMM on the lane's stream (bracketed by events when profiling).
 static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda,
                 const double2 *B, int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
   if (m <= 0 || n <= 0) return 0;
-  EventRec *r = nullptr;
-  if (c->profile_gemm) CHK(ev_begin(c, L, 8.0 * m * (double)n * k, &r));
+  EventRec *r;
+  const int kind = bem_zgemm_wide(m, n) ? BEM_KS_ZGEMM : BEM_KS_ZGEMM_NARROW;
+  const bool rd_c = beta.x != 0.0 || beta.y != 0.0;
+  CHK(ev_begin(c, L, kind, 8.0 * m * (double)n * k,
+               16.0 * ((double)m * k + (double)k * n + (rd_c ? 2.0 : 1.0) * (double)m * n), &r));
   int e = bem_zgemm_raw(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "zgemm launch: %s", hipGetErrorString((hipError_t)e));
-  if (r) DEV(c, hipEventRecord(r->b, L.st));
-  return 0;
+  return ev_end(c, L, r);
 }
 
 // --------------------------------------------------------------- LU / solves
@@ -551,28 +577,64 @@ static int body_tree(bem_ctx *c, bem_body *b, const DevTree **out) {
   return 0;
 }
 
+// Algorithmic work per triangle pair (transcendentals counted as one flop):
+// a 6-point far pair is 6 outer x (6 inner x 166 + 342) + 280 = 8308 FP64
+// flops; a close pair 105 outer x (91 inner x 180 + 980) + neighbour edge
+// terms ~ 1.83e6.  Each pair writes one 36-double tile (288 bytes).
+static const double F_FAR_PAIR = 8308.0;
+static const double F_CLOSE_PAIR = 1.83e6;
+static const double TILE_BYTES = 8.0 * BEM_TILE_PLANES;
+
 // Triangle-pair tiles of test rows [r0, r1) x source columns [c0, c1) of a
 // tree in medium md, on the lane's stream (tile leading dimension c1 - c0).
+// The range holds every close pair of the tree (a whole one-body tree) or
+// none (a cross-body range of a two-body tree).
 // A 36-point queue overflow sets *flag (device int).
 static int assemble_tiles(bem_ctx *c, Lane &L, const DevTree &d, const Medium &md, double *tiles, int r0, int r1,
                           int c0, int c1, int *flag) {
   const int T = d.T;
   size_t npairs = (size_t)(r1 - r0) * (size_t)(c1 - c0);
+  const bool full = r0 == 0 && r1 == T && c0 == 0 && c1 == T;
+  const double nclose = full ? (double)d.nclose : 0.0;
   unsigned int *q36cnt;
   int2 *q36;
   unsigned int cap = (unsigned int)std::min<size_t>(std::max<size_t>(npairs, 1), (size_t)1 << 24);
   CHK(ws_get(c, L, "q36_count", sizeof(unsigned int), (void **)&q36cnt));
   CHK(ws_get(c, L, "q36", sizeof(int2) * (size_t)cap, (void **)&q36));
   DEV(c, hipMemsetAsync(q36cnt, 0, sizeof(unsigned int), L.st));
+  EventRec *r;
+  const double nfar = (double)npairs - nclose;
+  CHK(ev_begin(c, L, BEM_KS_FAR, nfar * F_FAR_PAIR, nfar * TILE_BYTES, &r));
   int e = bem_launch_far_g6(d.tg, T, &md, r0, r1, c0, c1, tiles, npairs, q36, q36cnt, cap, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "far_g6 launch: %s", hipGetErrorString((hipError_t)e));
+  CHK(ev_end(c, L, r));
   e = bem_launch_far_g36(d.tg, T, &md, q36, q36cnt, cap, r0, r1, c0, c1, tiles, npairs, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "far_g36 launch: %s", hipGetErrorString((hipError_t)e));
-  e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, r0, r1, c0, c1, tiles, npairs, L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "close launch: %s", hipGetErrorString((hipError_t)e));
+  if (nclose > 0) {
+    CHK(ev_begin(c, L, BEM_KS_CLOSE, nclose * F_CLOSE_PAIR, nclose * TILE_BYTES, &r));
+    e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, r0, r1, c0, c1, tiles, npairs, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "close launch: %s", hipGetErrorString((hipError_t)e));
+    CHK(ev_end(c, L, r));
+  }
   if (flag && (e = bem_launch_q36_check(q36cnt, cap, flag, L.st)))
     return fail(c, BEM_E_DEVICE, "queue check launch: %s", hipGetErrorString((hipError_t)e));
   return 0;
+}
+
+// Gather of the RWG blocks (see bem_launch_gather), timed as BEM_KS_GATHER:
+// every tile of the range is read once and each RWG pair's 2 x 2 block is
+// written (read too when added, and written twice with a copy).
+static int gather(bem_ctx *c, Lane &L, const double *tiles, size_t npairs, int tld, int row_lo, int col_lo,
+                  const RwgRec *rw, int r_lo, int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz,
+                  int zr0, int zc0, int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0) {
+  EventRec *r;
+  double nrwg = (double)(r_hi - r_lo) * (double)(c_hi - c_lo);
+  CHK(ev_begin(c, L, BEM_KS_GATHER, 0.0, npairs * TILE_BYTES + nrwg * 64.0 * (1 + (add ? 1 : 0) + (Zcopy ? 1 : 0)),
+               &r));
+  int e = bem_launch_gather(tiles, npairs, tld, row_lo, col_lo, rw, r_lo, r_hi, c_lo, c_hi, md, Z, ldz, zr0, zc0, add,
+                            Zcopy, ldc, cr0, cc0, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "gather launch: %s", hipGetErrorString((hipError_t)e));
+  return ev_end(c, L, r);
 }
 
 static double theta(double omega, double T) {
@@ -673,13 +735,9 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   if (!r && hipMemsetAsync(e.dinfo, 0, sizeof(int) * 4 + sizeof(unsigned long long) * 2, L.st) != hipSuccess)
     r = fail(c, BEM_E_DEVICE, "memset");
   if (!r) r = assemble_tiles(c, L, *d, mext, tiles, 0, T, 0, T, e.dinfo + 1);
-  if (!r && bem_launch_gather(tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mext, e.Z, n, 0, 0, 0, nullptr, 0, 0, 0,
-                              L.st))
-    r = fail(c, BEM_E_DEVICE, "gather launch");
+  if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mext, e.Z, n, 0, 0, 0, nullptr, 0, 0, 0);
   if (!r) r = assemble_tiles(c, L, *d, mint, tiles, 0, T, 0, T, e.dinfo + 1);
-  if (!r && bem_launch_gather(tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mint, e.Z, n, 0, 0, 1, e.Gs, n, 0, 0,
-                              L.st))
-    r = fail(c, BEM_E_DEVICE, "gather launch");
+  if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mint, e.Z, n, 0, 0, 1, e.Gs, n, 0, 0);
   if (!r && bem_symm(e.Gs, n, n, L.st)) r = fail(c, BEM_E_DEVICE, "symm launch");
   if (!r && role == 0) {
     if (pivot) r = lu_factor(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv);
@@ -762,13 +820,9 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
 
   // ---- cross blocks of Z_ext for this gap
   CHK(assemble_tiles(c, L, g.top, mext, tiles, 0, T1, T1, T1 + T2, info + 1));
-  int e = bem_launch_gather(tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0,
-                            L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
+  CHK(gather(c, L, tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0));
   CHK(assemble_tiles(c, L, g.top, mext, tiles, T1, T1 + T2, 0, T1, info + 1));
-  e = bem_launch_gather(tiles, npc, T1, T1, 0, g.top.rw, Eb1, E, 0, Eb1, &mext, Cm, m2, 0, 0, 0, nullptr, 0, 0, 0,
-                        L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "gather: %d", e);
+  CHK(gather(c, L, tiles, npc, T1, T1, 0, g.top.rw, Eb1, E, 0, Eb1, &mext, Cm, m2, 0, 0, 0, nullptr, 0, 0, 0));
   DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
@@ -786,11 +840,12 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
   CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
   {
-    EventRec *r = nullptr;
-    if (c->profile_gemm) CHK(ev_begin(c, L, 8.0 * m1 * (double)m2 * m2, &r));
-    e = bem_zgemm_trace(m1, m2, m2, one, X, m1, E2.Gs, m2, B1, m1, partial, trace_out, L.st);
+    EventRec *r;
+    CHK(ev_begin(c, L, BEM_KS_ZTRACE, 8.0 * m1 * (double)m2 * m2,
+                 16.0 * ((double)m1 * m2 + (double)m2 * m2 + (double)m1 * m2), &r));
+    int e = bem_zgemm_trace(m1, m2, m2, one, X, m1, E2.Gs, m2, B1, m1, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
-    if (r) DEV(c, hipEventRecord(r->b, L.st));
+    CHK(ev_end(c, L, r));
   }
   DEV(c, hipEventRecord(L.ph[3], L.st));
   DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2), hipMemcpyDeviceToHost, L.st));
@@ -1240,6 +1295,12 @@ int bem_memcpy_d2h(bem_ctx *c, void *dst, const void *src, size_t bytes) {
   if (!c) return BEM_E_INVALID;
   DEV(c, hipDeviceSynchronize());
   DEV(c, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int bem_kernel_stats(bem_ctx *c, int kind, double *out4) {
+  if (!c || !out4 || kind < 0 || kind >= BEM_KS_COUNT) return BEM_E_INVALID;
+  for (int i = 0; i < 4; i++) out4[i] = c->kst[kind][i];
   return 0;
 }
 

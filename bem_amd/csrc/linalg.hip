@@ -771,6 +771,12 @@ int bem_zgemm_set_tile(int t) {
   return 0;
 }
 
+// 1 when bem_zgemm_raw runs an (m x n) product on the wide 32 x 64 tiles.
+int bem_zgemm_wide(int m, int n) {
+  int tile = g_zgemm_tile ? g_zgemm_tile : ((m <= 128 || n <= 128) ? 1032 : 1034);
+  return tile == 1034;
+}
+
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                   double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;

@@ -71,7 +71,7 @@ _lib = None
 EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_device_bytes", "bem_body_upload",
            "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
            "bem_sweep", "bem_zgetrf", "bem_zgetrs", "bem_zgemm", "bem_device_alloc", "bem_device_free",
-           "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings"]
+           "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats"]
 
 
 def load_library():
@@ -108,6 +108,7 @@ def load_library():
     L.bem_memcpy_h2d.argtypes = [vp, vp, vp, sz]
     L.bem_memcpy_d2h.argtypes = [vp, vp, vp, sz]
     L.bem_last_timings.argtypes = [vp, vp]
+    L.bem_kernel_stats.argtypes = [vp, i32, vp]
     _lib = L
     return L
 
@@ -271,6 +272,18 @@ class Context:
         if rc != 0:
             raise BemError(rc, self.lib.bem_last_error(self.h).decode())
         return res
+
+    KERNEL_CLASSES = {"zgemm": 0, "zgemm_narrow": 1, "ztrace": 2, "far": 3, "close": 4, "gather": 5, "cgemm": 6}
+
+    def kernel_stats(self):
+        """Per kernel class of the last solve/sweep call: summed kernel ms,
+        launches, algorithmic flops and bytes (bem_kernel_stats)."""
+        out = {}
+        buf = np.zeros(4)
+        for name, k in self.KERNEL_CLASSES.items():
+            self._check(self.lib.bem_kernel_stats(self.h, k, buf.ctypes.data))
+            out[name] = dict(ms=buf[0], launches=int(buf[1]), flops=buf[2], bytes=buf[3])
+        return out
 
     def last_timings(self):
         out = np.zeros(9)

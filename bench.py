@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
+ZGEMM_KERNEL = "zgemm_kernel<32, 64, 2, 2, 16, 2, false, true>"
 
 
 def workload(name):
@@ -122,14 +123,24 @@ def main():
     t_start = time.perf_counter()
     results = ctx.sweep(b1, b2, reqs)
     elapsed = time.perf_counter() - t_start
-    tm = ctx.last_timings()
-    gemm_ms, gemm_flops, gemm_n = tm["gemm_ms"], tm["gemm_flops"], tm["gemm_launches"]
+    ks = ctx.kernel_stats()
     lat = [r["ms_total"] for r in results]
     phase = dict(assemble=sum(r["ms_assemble"] for r in results), factor=sum(r["ms_factor"] for r in results),
                  products=sum(r["ms_products"] for r in results))
     K = len(reqs)
     value = K / elapsed
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12 if gemm_ms > 0 else 0.0
+    zg, far, close, gat = ks["zgemm"], ks["far"], ks["close"], ks["gather"]
+
+    def rate(k, scale):
+        return k["flops"] / (k["ms"] * 1e-3) / scale if k["ms"] > 0 else 0.0
+
+    achieved = rate(zg, 1e12)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "c2_fp64_hbm_traffic.json")
+    if os.path.exists(tpath) and args.workload == "c2":
+        t = json.load(open(tpath))["kernels"].get(ZGEMM_KERNEL)
+        if t:
+            traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
         "metric": "spectral-power solves/s",
         "value": value,
@@ -149,11 +160,25 @@ def main():
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
                        "definition": "issue to result of each solve inside the sweep"},
         "phases_ms_per_solve": {k: v / K for k, v in phase.items()},
-        "roofline": {"bound": "mfma", "kernel": "zgemm_kernel (FP64 MFMA complex GEMM)",
-                     "how": "8mnk flops per launch / HIP-event duration of the launch on its stream, summed",
+        "roofline": {"bound": "mfma", "kernel": ZGEMM_KERNEL,
+                     "how": "sum of 8mnk flops of its launches / sum of their HIP-event durations on the lane streams",
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "launches": gemm_n, "flops": gemm_flops, "kernel_ms": gemm_ms},
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
+                     "traffic_source": "profiles/c2_fp64_hbm_traffic.json (FETCH_SIZE, WRITE_SIZE per launch, copy-calibrated)",
+                     "algorithmic_bytes_per_launch": zg["bytes"] / max(zg["launches"], 1),
+                     "launches": zg["launches"], "avg_launch_us": zg["ms"] * 1e3 / max(zg["launches"], 1),
+                     "flops": zg["flops"], "kernel_ms": zg["ms"]},
+        "assembly_roofline": {"bound": "fp64-valu", "kernel": "far_g6_kernel",
+                              "how": "8308 flops per 6-point far pair (transcendentals = 1 flop) / HIP-event durations",
+                              "achieved": rate(far, 1e12), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                              "frac": rate(far, 1e12) / PEAK_FP64_TFLOPS, "launches": far["launches"],
+                              "avg_launch_us": far["ms"] * 1e3 / max(far["launches"], 1),
+                              "close_kernel": {"kernel": "pair_block_kernel<105,91>", "flops_per_pair": 1.83e6,
+                                               "achieved": rate(close, 1e12),
+                                               "frac": rate(close, 1e12) / PEAK_FP64_TFLOPS,
+                                               "launches": close["launches"]},
+                              "gather_kernel": {"bound": "hbm", "achieved_GBs": rate(gat, 1e9) if gat["ms"] else 0.0,
+                                                "peak_GBs": PEAK_HBM_GBS, "launches": gat["launches"]}},
         "results": [{"eV": q.eV, "P": r["P"], "ReS": r["ReS"]} for q, r in zip(reqs, results)],
     }
     if args.check:

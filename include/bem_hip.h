@@ -153,6 +153,23 @@ int bem_memcpy_d2h(bem_ctx *ctx, void *dst, const void *src, size_t bytes);
  * of the durations of every zgemm launch of the call (ms), out[7] their
  * number, out[8] their real flops (8 m n k each).  out[1..3] are reserved. */
 int bem_last_timings(bem_ctx *ctx, double *out9);
+/* Kernel classes timed with device events on their own stream during the
+ * last bem_spectral_power / bem_sweep call (a class may hold several
+ * launches of one kernel). */
+#define BEM_KS_ZGEMM 0         /* zgemm, 32 x 64 tiles (the large products) */
+#define BEM_KS_ZGEMM_NARROW 1  /* zgemm, 32 x 32 tiles (panel-sized products) */
+#define BEM_KS_ZTRACE 2        /* zgemm with the trace epilogue */
+#define BEM_KS_FAR 3           /* far-pair assembly (6-point rule) */
+#define BEM_KS_CLOSE 4         /* close-pair assembly (105 x 91 + analytic) */
+#define BEM_KS_GATHER 5        /* tiles -> 2 x 2 RWG blocks of Z */
+#define BEM_KS_CGEMM 6         /* complex64 GEMM (mixed precision) */
+#define BEM_KS_COUNT 8
+/* out4 = {summed kernel ms, launches, algorithmic flops, algorithmic bytes}
+ * of class `kind` over the last call.  Algorithmic counts: 8 m n k real
+ * flops per complex GEMM and 16 bytes per operand entry read or written once;
+ * 8308 flops per 6-point far pair and 1.83e6 per close pair, 288 bytes of
+ * tile per pair. */
+int bem_kernel_stats(bem_ctx *ctx, int kind, double *out4);
 
 #ifdef __cplusplus
 }
