@@ -7,6 +7,12 @@ and the benchmark; the compute path is the HIP library, and it fails loudly
 when the library is missing or no GPU is present.
 """
 from .runtime import (  # noqa: F401
+    BEM_E_INVALID,
+    BEM_E_NOMEM,
+    BEM_E_SINGULAR,
+    BEM_PENDING,
+    BEM_PREC_FP64,
+    BEM_PREC_MIXED,
     BemError,
     Body,
     Context,

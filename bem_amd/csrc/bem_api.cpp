@@ -14,9 +14,19 @@
 // latency-bound LU panels of one solve overlap the GEMMs of another.  Every
 // solve's kernels stay on its own stream, so results do not depend on what
 // else is in flight.
+//
+// Serving.  bem_submit queues a request and returns a ticket; a scheduler
+// thread of the context admits queued requests in arrival order onto free
+// lanes whenever the request's device footprint (lane workspaces plus any
+// self blocks not yet cached) fits the HBM budget, and retires lanes whose
+// stream signalled completion through a host callback.  Requests arriving
+// while others run are therefore admitted as lanes and memory free up, and
+// their kernels overlap on the device with those already in flight.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <condition_variable>
+#include <deque>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -24,6 +34,7 @@
 #include <list>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -146,6 +157,22 @@ struct Lane {
   bem_request req;
   SelfEntry *e1 = nullptr, *e2 = nullptr;   // pinned while in flight
   std::chrono::steady_clock::time_point t0;
+  size_t ws_bytes = 0;            // bytes of this lane's named workspaces
+  // served request in flight (ticket 0: none)
+  bem_ctx *ctx = nullptr;
+  uint64_t ticket = 0;
+  bem_body *sb1 = nullptr, *sb2 = nullptr;
+  bem_result sres;
+  std::chrono::steady_clock::time_point t_submit, t_issue;
+  volatile int done_flag = 0;     // set by the stream's host callback
+};
+
+// A queued served request.
+struct Pending {
+  uint64_t ticket;
+  bem_body *b1, *b2;
+  bem_request req;
+  std::chrono::steady_clock::time_point t_submit;
 };
 
 }  // namespace
@@ -170,6 +197,16 @@ struct bem_ctx {
   double kst[BEM_KS_COUNT][4] = {};   // per class: ms, launches, flops, bytes
   int self_hits = 0, self_misses = 0;
   std::unordered_set<bem_body *> bodies;
+  // serving (queue, finished results and the scheduler; guarded by qmu)
+  std::mutex qmu;
+  std::condition_variable qcv, done_cv;
+  std::deque<Pending> queue;
+  std::unordered_map<uint64_t, bem_result> done;
+  std::unordered_set<uint64_t> live;   // queued or running
+  uint64_t next_ticket = 1;
+  int running = 0;
+  bool stop = false, kick = false;
+  std::thread sched;
 };
 
 struct bem_body {
@@ -215,6 +252,7 @@ static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **ou
       DEV(c, hipStreamSynchronize(L.st));
       DEV(c, hipFree(b.p));
       c->bytes -= b.cap;
+      L.ws_bytes -= b.cap;
       b.p = nullptr;
       b.cap = 0;
     }
@@ -225,9 +263,22 @@ static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **ou
       return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for '%s%s': %s", bytes, L.tag.c_str(), name, hipGetErrorString(e));
     b.cap = bytes;
     c->bytes += bytes;
+    L.ws_bytes += bytes;
   }
   *out = b.p;
   return 0;
+}
+
+// Frees every workspace of an idle lane (its stream has drained).
+static void ws_trim(bem_ctx *c, Lane &L) {
+  for (auto &kv : c->ws)
+    if (kv.second.p && kv.first.compare(0, L.tag.size(), L.tag) == 0) {
+      (void)hipFree(kv.second.p);
+      c->bytes -= kv.second.cap;
+      kv.second.p = nullptr;
+      kv.second.cap = 0;
+    }
+  L.ws_bytes = 0;
 }
 
 static int ev_begin(bem_ctx *c, Lane &L, int kind, double flops, double bytes, EventRec **rec) {
@@ -923,6 +974,202 @@ static int check_request(bem_ctx *c, const bem_request *q) {
   return 0;
 }
 
+
+// ------------------------------------------------------------------ serving
+// Bytes of a self entry of an n x n block in role 0 (factored) or 1 (raw).
+static size_t self_entry_bytes(int n, int role) {
+  size_t nn = sizeof(double2) * (size_t)n * n;
+  return 2 * nn + (role == 0 ? inv_bytes(n) + sizeof(int) * (size_t)(n + 64) : 0) + 256;
+}
+
+static bool self_cached(bem_ctx *c, const void *b, const double *eps, const double *mu, double omega, int role,
+                        int pivot) {
+  const double key[5] = {eps[0], eps[1], mu[0], mu[1], omega};
+  for (auto &e : c->selfs)
+    if (e.body == b && e.role == role && e.pivot == pivot && std::memcmp(e.key, key, sizeof key) == 0) return true;
+  return false;
+}
+
+// Device bytes a request would newly allocate on lane L: the lane's
+// workspaces beyond what it holds, plus the self entries not yet cached.
+static size_t footprint(bem_ctx *c, const Lane &L, const Pending &p) {
+  const bemgeo::BodyMesh &M1 = p.b1->mesh, &M2 = p.b2->mesh;
+  const size_t m1 = 2 * (size_t)M1.ne, m2 = 2 * (size_t)M2.ne, T1 = M1.nt, T2 = M2.nt;
+  const bem_request &q = p.req;
+  const double omega = q.eV * SI_EV / SI_HBAR * q.a / SI_C;
+  const int pivot = (q.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  const bool h1 = self_cached(c, p.b1, q.eps1, q.mu1, omega, 0, pivot);
+  const bool h2 = self_cached(c, p.b2, q.eps2, q.mu2, omega, 1, 0);
+  size_t selfb = (h1 ? 0 : self_entry_bytes((int)m1, 0)) + (h2 ? 0 : self_entry_bytes((int)m2, 1));
+  size_t pairs = std::max(T1 * T2, std::max(h1 ? 0 : T1 * T1, h2 ? 0 : T2 * T2));
+  size_t ws = sizeof(double2) * (3 * m1 * m2 + m2 * m2) + 8 * (size_t)BEM_TILE_PLANES * pairs +
+              sizeof(int2) * std::min<size_t>(pairs, (size_t)1 << 24) + inv_bytes((int)m2) +
+              sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_trace_partials((int)m1, (int)m2) + 1);
+  return (ws > L.ws_bytes ? ws - L.ws_bytes : 0) + selfb;
+}
+
+// Frees cached self entries (least recently used, unpinned) and the
+// workspaces of idle lanes other than `keep` until `need` more bytes fit the
+// budget or nothing more can go.
+static void make_room(bem_ctx *c, size_t need, const Lane *keep) {
+  while (c->budget && c->bytes + need > c->budget) {
+    auto victim = c->selfs.end();
+    for (auto it = c->selfs.begin(); it != c->selfs.end(); ++it)
+      if (it->pins == 0 && (victim == c->selfs.end() || it->stamp < victim->stamp)) victim = it;
+    if (victim != c->selfs.end()) {
+      self_free(c, *victim);
+      c->selfs.erase(victim);
+      continue;
+    }
+    Lane *idle = nullptr;
+    for (auto &l : c->lanes)
+      if (&l != keep && !l.busy && l.ws_bytes > 0) idle = &l;
+    if (!idle) return;
+    ws_trim(c, *idle);
+  }
+}
+
+static double ms_between(std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+  return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// Finishes the lane's solve.  A served request's result goes to the table of
+// finished tickets (or, when block-local pivoting grew too much, back to the
+// head of the queue with tournament pivoting); its errors are reported
+// through the ticket, not as this call's return value.
+static int lane_retire(bem_ctx *c, Lane &L) {
+  const uint64_t t = L.ticket;
+  int r = lane_finish(c, L);
+  if (!t) return r;
+  L.ticket = 0;
+  bem_result res = L.sres;
+  if (r && !res.status) res.status = r;
+  auto now = std::chrono::steady_clock::now();
+  const bool retry = !res.status && !res.pivoting && !(res.growth <= BEM_GROWTH_LIMIT);
+  std::lock_guard<std::mutex> lk(c->qmu);
+  c->running--;
+  if (retry) {
+    Pending p{t, L.sb1, L.sb2, L.req, L.t_submit};
+    p.req.flags |= BEM_SOLVE_TOURNAMENT;
+    c->queue.push_front(p);
+    c->kick = true;
+    c->qcv.notify_one();
+    return 0;
+  }
+  res.ms_queue = ms_between(L.t_submit, L.t_issue);
+  res.ms_total = ms_between(L.t_submit, now);
+  c->done[t] = res;
+  c->live.erase(t);
+  c->done_cv.notify_all();
+  return 0;
+}
+
+// Host callback enqueued behind a served solve: marks the lane done and wakes
+// the scheduler (no HIP calls here).
+static void lane_done_cb(void *arg) {
+  Lane *L = (Lane *)arg;
+  __atomic_store_n(&L->done_flag, 1, __ATOMIC_RELEASE);
+  bem_ctx *c = L->ctx;
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    c->kick = true;
+  }
+  c->qcv.notify_one();
+}
+
+static void finish_ticket(bem_ctx *c, uint64_t t, int status) {
+  std::lock_guard<std::mutex> lk(c->qmu);
+  bem_result r;
+  std::memset(&r, 0, sizeof r);
+  r.status = status;
+  c->done[t] = r;
+  c->live.erase(t);
+  c->done_cv.notify_all();
+}
+
+// Admits queued requests in arrival order while a lane is free and the
+// request's footprint fits the budget (caller holds c->mu).
+static void admit(bem_ctx *c) {
+  for (;;) {
+    Pending p;
+    {
+      std::lock_guard<std::mutex> lk(c->qmu);
+      if (c->queue.empty()) return;
+      p = c->queue.front();
+    }
+    Lane *L = nullptr;
+    for (auto &l : c->lanes)
+      if (!l.busy) {
+        L = &l;
+        break;
+      }
+    if (!L) return;
+    int e = check_request(c, &p.req);
+    if (!e && c->budget) {
+      size_t extra = footprint(c, *L, p);
+      if (c->bytes + extra > c->budget) make_room(c, extra, L);
+      if (c->bytes + extra > c->budget) {
+        for (auto &l : c->lanes)
+          if (l.busy) return;   // wait until a running request frees its lane
+        e = fail(c, BEM_E_NOMEM, "request needs %zu more device bytes; budget %zu, held %zu", extra, c->budget,
+                 c->bytes);
+      }
+    }
+    {
+      std::lock_guard<std::mutex> lk(c->qmu);
+      c->queue.pop_front();
+      if (!e) c->running++;
+    }
+    if (e) {
+      finish_ticket(c, p.ticket, e);
+      continue;
+    }
+    PairGeom *g = nullptr;
+    e = pair_geom(c, p.b1, p.b2, p.req.gap, &g);
+    if (!e) {
+      L->ticket = p.ticket;
+      L->sb1 = p.b1;
+      L->sb2 = p.b2;
+      L->t_submit = p.t_submit;
+      L->t_issue = std::chrono::steady_clock::now();
+      std::memset(&L->sres, 0, sizeof L->sres);
+      __atomic_store_n(&L->done_flag, 0, __ATOMIC_RELEASE);
+      e = lane_issue(c, *L, *g, p.b1, p.b2, &p.req, &L->sres);
+      if (!e && hipLaunchHostFunc(L->st, lane_done_cb, L) != hipSuccess)
+        e = fail(c, BEM_E_DEVICE, "host callback could not be enqueued");
+      if (e) {
+        (void)hipStreamSynchronize(L->st);
+        L->busy = false;
+        L->ticket = 0;
+        self_release(c, *L);
+      }
+    }
+    if (e) {
+      {
+        std::lock_guard<std::mutex> lk(c->qmu);
+        c->running--;
+      }
+      finish_ticket(c, p.ticket, e);
+    }
+  }
+}
+
+static void sched_main(bem_ctx *c) {
+  (void)hipSetDevice(c->device);
+  for (;;) {
+    {
+      std::unique_lock<std::mutex> lk(c->qmu);
+      c->qcv.wait_for(lk, std::chrono::milliseconds(5), [c] { return c->stop || c->kick; });
+      if (c->stop) return;
+      c->kick = false;
+    }
+    std::lock_guard<std::recursive_mutex> g(c->mu);
+    for (auto &L : c->lanes)
+      if (L.ticket && __atomic_load_n(&L.done_flag, __ATOMIC_ACQUIRE)) (void)lane_retire(c, L);
+    admit(c);
+  }
+}
+
 // ------------------------------------------------------------------- C ABI
 extern "C" {
 
@@ -947,6 +1194,7 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
   for (int i = 0; i < nl; i++) {
     Lane &L = c->lanes[i];
     L.id = i;
+    L.ctx = c;
     L.tag = "L" + std::to_string(i) + ":";
     bool ok = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) == hipSuccess;
     for (int k = 0; k < 4 && ok; k++) ok = hipEventCreate(&L.ph[k]) == hipSuccess;
@@ -964,6 +1212,12 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
 
 void bem_ctx_destroy(bem_ctx *c) {
   if (!c) return;
+  {
+    std::lock_guard<std::mutex> lk(c->qmu);
+    c->stop = true;
+  }
+  c->qcv.notify_all();
+  if (c->sched.joinable()) c->sched.join();
   (void)hipSetDevice(c->device);
   (void)hipDeviceSynchronize();
   for (auto &kv : c->ws)
@@ -1035,7 +1289,7 @@ void bem_body_free(bem_body *b) {
   if (c) {
     std::lock_guard<std::recursive_mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
-    for (auto &L : c->lanes) (void)lane_finish(c, L);
+    for (auto &L : c->lanes) (void)lane_retire(c, L);
     for (auto it = c->pairs.begin(); it != c->pairs.end();) {
       if (it->b1 == b || it->b2 == b) {
         free_allocs(c, it->allocs, it->bytes);
@@ -1047,6 +1301,19 @@ void bem_body_free(bem_body *b) {
     self_drop_body(c, b);
     if (!b->allocs.empty()) free_allocs(c, b->allocs, b->bytes);
     c->bodies.erase(b);
+    std::vector<uint64_t> dropped;
+    {
+      std::lock_guard<std::mutex> lq(c->qmu);
+      for (auto it = c->queue.begin(); it != c->queue.end();) {
+        if (it->b1 == b || it->b2 == b) {
+          dropped.push_back(it->ticket);
+          it = c->queue.erase(it);
+        } else {
+          ++it;
+        }
+      }
+    }
+    for (uint64_t t : dropped) finish_ticket(c, t, BEM_E_INVALID);
   }
   delete b;
 }
@@ -1087,7 +1354,7 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   Lane &L = c->lanes[0];
-  CHK(lane_finish(c, L));
+  CHK(lane_retire(c, L));
   std::vector<const bemgeo::BodyMesh *> bm;
   for (int i = 0; i < nb; i++) bm.push_back(&bodies[i]->mesh);
   bemgeo::Tree t;
@@ -1138,15 +1405,15 @@ int bem_spectral_power(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request
   PairGeom *g;
   CHK(pair_geom(c, b1, b2, q->gap, &g));
   Lane &L = c->lanes[0];
-  CHK(lane_finish(c, L));
+  CHK(lane_retire(c, L));
   int e = lane_issue(c, L, *g, b1, b2, q, res);
-  int f = lane_finish(c, L);
+  int f = lane_retire(c, L);
   if (e) return e;
   if (!f && !(res->growth <= BEM_GROWTH_LIMIT) && !(q->flags & BEM_SOLVE_TOURNAMENT)) {
     bem_request q2 = *q;
     q2.flags |= BEM_SOLVE_TOURNAMENT;
     e = lane_issue(c, L, *g, b1, b2, &q2, res);
-    f = lane_finish(c, L);
+    f = lane_retire(c, L);
     if (e) return e;
   }
   c->t_last[6] = c->gemm_ms;
@@ -1165,7 +1432,7 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
   for (int i = 0; i < n; i++) {
     std::memset(&res[i], 0, sizeof res[i]);
     Lane &L = c->lanes[i % nl];
-    int f = lane_finish(c, L);
+    int f = lane_retire(c, L);
     if (f && !first_err) first_err = f;
     int e = check_request(c, &reqs[i]);
     PairGeom *g = nullptr;
@@ -1180,7 +1447,7 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
     }
   }
   for (auto &L : c->lanes) {
-    int f = lane_finish(c, L);
+    int f = lane_retire(c, L);
     if (f && !first_err) first_err = f;
   }
   // requests whose block-local pivoting grew too much are solved again with
@@ -1193,7 +1460,7 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
     PairGeom *g = nullptr;
     int e = pair_geom(c, b1, b2, q2.gap, &g);
     if (!e) e = lane_issue(c, L, *g, b1, b2, &q2, &res[i]);
-    int f = e ? e : lane_finish(c, L);
+    int f = e ? e : lane_retire(c, L);
     if (f) {
       res[i].status = f;
       if (!first_err) first_err = f;
@@ -1205,12 +1472,50 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
   return first_err;
 }
 
+int bem_submit(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *q, uint64_t *ticket) {
+  if (!c || !b1 || !b2 || !q || !ticket) return BEM_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->qmu);
+  if (c->stop) return BEM_E_INVALID;
+  Pending p{c->next_ticket++, b1, b2, *q, std::chrono::steady_clock::now()};
+  c->queue.push_back(p);
+  c->live.insert(p.ticket);
+  *ticket = p.ticket;
+  if (!c->sched.joinable()) c->sched = std::thread(sched_main, c);
+  c->kick = true;
+  c->qcv.notify_one();
+  return 0;
+}
+
+int bem_poll(bem_ctx *c, uint64_t ticket, bem_result *res, int wait_ms) {
+  if (!c || !res) return BEM_E_INVALID;
+  std::unique_lock<std::mutex> lk(c->qmu);
+  auto ready = [&] { return c->done.count(ticket) > 0; };
+  if (!ready()) {
+    if (!c->live.count(ticket)) return BEM_E_INVALID;
+    if (wait_ms == 0) return BEM_PENDING;
+    if (wait_ms < 0) c->done_cv.wait(lk, ready);
+    else if (!c->done_cv.wait_for(lk, std::chrono::milliseconds(wait_ms), ready)) return BEM_PENDING;
+  }
+  auto it = c->done.find(ticket);
+  *res = it->second;
+  c->done.erase(it);
+  return res->status;
+}
+
+int bem_queue_depth(bem_ctx *c, int *queued, int *running) {
+  if (!c) return BEM_E_INVALID;
+  std::lock_guard<std::mutex> lk(c->qmu);
+  if (queued) *queued = (int)c->queue.size();
+  if (running) *running = c->running;
+  return 0;
+}
+
 int bem_zgetrf(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv) {
   if (!c || !A || !ipiv || n < 0 || lda < n) return fail(c, BEM_E_INVALID, "bad arguments");
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   Lane &L = c->lanes[0];
-  CHK(lane_finish(c, L));
+  CHK(lane_retire(c, L));
   stats_reset(c);
   int *info;
   double2 *inv;
@@ -1231,7 +1536,7 @@ int bem_zgetrs(bem_ctx *c, const void *A, int64_t lda, int n, const int32_t *ipi
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   Lane &L = c->lanes[0];
-  CHK(lane_finish(c, L));
+  CHK(lane_retire(c, L));
   stats_reset(c);
   double2 *inv;
   CHK(ws_get(c, L, "inv_getrs", inv_bytes(n), (void **)&inv));
@@ -1249,7 +1554,7 @@ int bem_zgemm(bem_ctx *c, int m, int n, int k, const double *alpha, const void *
   std::lock_guard<std::recursive_mutex> lk(c->mu);
   DEV(c, hipSetDevice(c->device));
   Lane &L = c->lanes[0];
-  CHK(lane_finish(c, L));
+  CHK(lane_retire(c, L));
   stats_reset(c);
   CHK(gemm(c, L, m, n, k, make_double2(alpha[0], alpha[1]), (const double2 *)A, lda, (const double2 *)B, ldb,
            make_double2(beta[0], beta[1]), (double2 *)C, ldc));

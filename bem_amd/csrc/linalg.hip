@@ -21,8 +21,20 @@
 #include "bem_common.h"
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
 
 namespace {
+
+// complex element types: double2 (complex128) and float2 (complex64)
+template <typename T> struct Cplx;
+template <> struct Cplx<double2> {
+  using R = double;
+  __device__ static double2 mk(double a, double b) { return make_double2(a, b); }
+};
+template <> struct Cplx<float2> {
+  using R = float;
+  __device__ static float2 mk(float a, float b) { return make_float2(a, b); }
+};
 
 // ------------------------------------------------------------------- ZGEMM
 // Block tile BM x BN, WM x WN waves, each wave FM x FN blocks of 16 x 16;
@@ -504,27 +516,29 @@ __global__ void __launch_bounds__(256) diag_block_kernel(double2 *__restrict__ A
 //     level two small products, three barriers per level.
 //  Output: inv (64 x 64 Linv then 64 x 64 Uinv, column-major, identity
 //  padded when w < 64).
-__global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ A, int64_t lda, int n, int k0,
-                                                          double2 *__restrict__ inv, int *__restrict__ ipiv,
+template <typename T>
+__global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int64_t lda, int n, int k0,
+                                                          T *__restrict__ inv, int *__restrict__ ipiv,
                                                           int *__restrict__ info) {
-  __shared__ double2 sA[64 * 64];   // column-major: packed LU, later workspace
-  __shared__ double2 sX[64 * 64];   // packed inverses (Linv strictly below, Uinv on/above)
-  __shared__ double2 lcol[64];
+  using R = typename Cplx<T>::R;
+  __shared__ T sA[64 * 64];   // column-major: packed LU, later workspace
+  __shared__ T sX[64 * 64];   // packed inverses (Linv strictly below, Uinv on/above)
+  __shared__ T lcol[64];
   __shared__ int s_piv;
   const int w = min(64, n - k0);
   const int tid = threadIdx.x, ri = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
-  double2 *Ak = A + (int64_t)k0 * lda + k0;
+  T *Ak = A + (int64_t)k0 * lda + k0;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
-    sA[idx] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : make_double2(r == c ? 1.0 : 0.0, 0.0);
+    sA[idx] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : Cplx<T>::mk(r == c ? 1.0 : 0.0, 0.0);
   }
   __syncthreads();
   for (int j = 0; j < w; j++) {
     if (wave == 0) {
-      double v = (ri >= j && ri < w) ? fabs(sA[j * 64 + ri].x) + fabs(sA[j * 64 + ri].y) : -1.0;
+      R v = (ri >= j && ri < w) ? std::fabs(sA[j * 64 + ri].x) + std::fabs(sA[j * 64 + ri].y) : -1.0;
       int id = ri;
       for (int off = 32; off >= 1; off >>= 1) {
-        double ov = __shfl_xor(v, off, 64);
+        R ov = __shfl_xor(v, off, 64);
         int oi = __shfl_xor(id, off, 64);
         if (ov > v || (ov == v && oi < id)) { v = ov; id = oi; }
       }
@@ -540,29 +554,29 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ 
     __syncthreads();
     const int p = s_piv;
     if (p != j && tid < 64) {
-      double2 t = sA[tid * 64 + j];
+      T t = sA[tid * 64 + j];
       sA[tid * 64 + j] = sA[tid * 64 + p];
       sA[tid * 64 + p] = t;
     }
     __syncthreads();
-    const double2 u = sA[j * 64 + j];
-    const double den = u.x * u.x + u.y * u.y;
+    const T u = sA[j * 64 + j];
+    const R den = u.x * u.x + u.y * u.y;
     if (cg == 0 && ri > j) {
-      double2 l = make_double2(0.0, 0.0);
+      T l = Cplx<T>::mk(0.0, 0.0);
       if (den > 0.0) {
-        double2 x = sA[j * 64 + ri];
-        double ur = u.x / den, ui = -u.y / den;
-        l = make_double2(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
+        T x = sA[j * 64 + ri];
+        R ur = u.x / den, ui = -u.y / den;
+        l = Cplx<T>::mk(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
       }
       sA[j * 64 + ri] = l;
       lcol[ri] = l;
     }
     __syncthreads();
     if (ri > j) {
-      const double2 l = lcol[ri];
+      const T l = lcol[ri];
       for (int c = j + 1 + cg; c < 64; c += 4) {
-        double2 uc = sA[c * 64 + j];
-        double2 x = sA[c * 64 + ri];
+        T uc = sA[c * 64 + j];
+        T x = sA[c * 64 + ri];
         x.x -= l.x * uc.x - l.y * uc.y;
         x.y -= l.x * uc.y + l.y * uc.x;
         sA[c * 64 + ri] = x;
@@ -577,11 +591,11 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ 
   // level 0: diagonal of the inverses (Linv unit, Uinv 1/u)
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
-    double2 v = make_double2(0.0, 0.0);
+    T v = Cplx<T>::mk(0.0, 0.0);
     if (r == c) {
-      double2 d = sA[idx];
-      double den = d.x * d.x + d.y * d.y;
-      v = make_double2(d.x / den, -d.y / den);
+      T d = sA[idx];
+      R den = d.x * d.x + d.y * d.y;
+      v = Cplx<T>::mk(d.x / den, -d.y / den);
     }
     sX[idx] = v;
   }
@@ -593,7 +607,7 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ 
   // until phase 1 of the level overwrites the L21/U12 blocks with T.
   for (int b = 1; b < 64; b <<= 1) {
     const int per = 32 * b;   // outputs per matrix at this level: (32/b) pairs x b*b
-    double2 t[8];
+    T t[8];
     // phase 1: T into registers (slot s holds output q = tid + 256 s)
 #pragma unroll
     for (int s = 0; s < 8; s++) {
@@ -603,20 +617,20 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ 
       const int qq = upper ? q - per : q;
       const int pair = qq / (b * b), e = qq % (b * b);
       const int o = pair * 2 * b, i = e % b, jj = e / b;
-      double2 acc = make_double2(0.0, 0.0);
+      T acc = Cplx<T>::mk(0.0, 0.0);
       if (!upper) {
         // T[i][jj] = sum_k L21[i][k] Linv11[k][jj], L21 = sA[o+b+i][o+k], Linv11 (k >= jj)
         for (int k = jj; k < b; k++) {
-          double2 l = sA[(o + k) * 64 + o + b + i];
-          double2 x = (k == jj) ? make_double2(1.0, 0.0) : sX[(o + jj) * 64 + o + k];
+          T l = sA[(o + k) * 64 + o + b + i];
+          T x = (k == jj) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + jj) * 64 + o + k];
           acc.x += l.x * x.x - l.y * x.y;
           acc.y += l.x * x.y + l.y * x.x;
         }
       } else {
         // T[i][jj] = sum_k U12[i][k] Uinv22[k][jj], U12 = sA[o+i][o+b+k], Uinv22 (k <= jj)
         for (int k = 0; k <= jj; k++) {
-          double2 u = sA[(o + b + k) * 64 + o + i];
-          double2 x = sX[(o + b + jj) * 64 + o + b + k];
+          T u = sA[(o + b + k) * 64 + o + i];
+          T x = sX[(o + b + jj) * 64 + o + b + k];
           acc.x += u.x * x.x - u.y * x.y;
           acc.y += u.x * x.y + u.y * x.x;
         }
@@ -642,48 +656,49 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(double2 *__restrict__ 
       const int qq = upper ? q - per : q;
       const int pair = qq / (b * b), e = qq % (b * b);
       const int o = pair * 2 * b, i = e % b, jj = e / b;
-      double2 acc = make_double2(0.0, 0.0);
+      T acc = Cplx<T>::mk(0.0, 0.0);
       if (!upper) {
         // sum_k Linv22[i][k] T[k][jj], Linv22 lower with unit diagonal (k <= i)
         for (int k = 0; k <= i; k++) {
-          double2 x = (k == i) ? make_double2(1.0, 0.0) : sX[(o + b + k) * 64 + o + b + i];
-          double2 tt = sA[(o + jj) * 64 + o + b + k];
+          T x = (k == i) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + b + k) * 64 + o + b + i];
+          T tt = sA[(o + jj) * 64 + o + b + k];
           acc.x += x.x * tt.x - x.y * tt.y;
           acc.y += x.x * tt.y + x.y * tt.x;
         }
-        sX[(o + jj) * 64 + o + b + i] = make_double2(-acc.x, -acc.y);
+        sX[(o + jj) * 64 + o + b + i] = Cplx<T>::mk(-acc.x, -acc.y);
       } else {
         // sum_k Uinv11[i][k] T[k][jj], Uinv11 upper (k >= i)
         for (int k = i; k < b; k++) {
-          double2 x = sX[(o + k) * 64 + o + i];
-          double2 tt = sA[(o + b + jj) * 64 + o + k];
+          T x = sX[(o + k) * 64 + o + i];
+          T tt = sA[(o + b + jj) * 64 + o + k];
           acc.x += x.x * tt.x - x.y * tt.y;
           acc.y += x.x * tt.y + x.y * tt.x;
         }
-        sX[(o + b + jj) * 64 + o + i] = make_double2(-acc.x, -acc.y);
+        sX[(o + b + jj) * 64 + o + i] = Cplx<T>::mk(-acc.x, -acc.y);
       }
     }
     __syncthreads();
   }
-  double2 *Li = inv, *Ui = inv + 64 * 64;
+  T *Li = inv, *Ui = inv + 64 * 64;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
-    double2 v = sX[idx];
-    Li[idx] = r > c ? v : make_double2(r == c ? 1.0 : 0.0, 0.0);
-    Ui[idx] = r <= c ? v : make_double2(0.0, 0.0);
+    T v = sX[idx];
+    Li[idx] = r > c ? v : Cplx<T>::mk(r == c ? 1.0 : 0.0, 0.0);
+    Ui[idx] = r <= c ? v : Cplx<T>::mk(0.0, 0.0);
   }
 }
 
 // max |re|+|im| over an m x n block (upper=1: only entries with row <= col),
 // folded into *out as the bit pattern of a non-negative double (atomic max).
-__global__ void absmax_kernel(const double2 *__restrict__ A, int64_t lda, int m, int n, int upper,
+template <typename T>
+__global__ void absmax_kernel(const T *__restrict__ A, int64_t lda, int m, int n, int upper,
                               unsigned long long *__restrict__ out) {
   double v = 0.0;
   for (int c = blockIdx.y; c < n; c += gridDim.y) {
     int rmax = upper ? min(m, c + 1) : m;
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rmax; r += gridDim.x * blockDim.x) {
-      double2 x = A[(int64_t)c * lda + r];
-      v = fmax(v, fabs(x.x) + fabs(x.y));
+      T x = A[(int64_t)c * lda + r];
+      v = fmax(v, std::fabs((double)x.x) + std::fabs((double)x.y));
     }
   }
   for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
@@ -693,14 +708,15 @@ __global__ void absmax_kernel(const double2 *__restrict__ A, int64_t lda, int m,
 // Column interchanges for X <- X P^T with P = P_k0 .. P_{k1-1} the row
 // interchanges of an LU (P_i swaps i and ipiv[i]): columns i and ipiv[i] of X
 // (rows [0, m)) are swapped for i = k1-1 down to k0.  One thread per row.
-__global__ void col_swap_kernel(double2 *__restrict__ X, int64_t ldx, int m, int k0, int k1,
+template <typename T>
+__global__ void col_swap_kernel(T *__restrict__ X, int64_t ldx, int m, int k0, int k1,
                                 const int *__restrict__ ipiv) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= m) return;
   for (int i = k1 - 1; i >= k0; i--) {
     int q = ipiv[i];
     if (q != i) {
-      double2 t = X[(int64_t)i * ldx + r];
+      T t = X[(int64_t)i * ldx + r];
       X[(int64_t)i * ldx + r] = X[(int64_t)q * ldx + r];
       X[(int64_t)q * ldx + r] = t;
     }
@@ -709,15 +725,16 @@ __global__ void col_swap_kernel(double2 *__restrict__ X, int64_t ldx, int m, int
 
 // Row interchanges: for p in [k0, k1): swap rows p and ipiv[p] in every column
 // of [c0, c1) except [skip0, skip1).
-__global__ void laswp_kernel(double2 *__restrict__ A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1,
+template <typename T>
+__global__ void laswp_kernel(T *__restrict__ A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1,
                              const int *__restrict__ ipiv) {
   int c = c0 + blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= c1 || (c >= skip0 && c < skip1)) return;
-  double2 *col = A + (int64_t)c * lda;
+  T *col = A + (int64_t)c * lda;
   for (int p = k0; p < k1; p++) {
     int q = ipiv[p];
     if (q != p) {
-      double2 t = col[p];
+      T t = col[p];
       col[p] = col[q];
       col[q] = t;
     }
@@ -740,6 +757,211 @@ __global__ void symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
   G[(int64_t)i * ld + j] = make_double2(s.x, -s.y);
 }
 
+
+// ------------------------------------------------------------------- CGEMM
+// complex64 C = alpha A B + beta C on FP32 MFMA v_mfma_f32_16x16x4f32, the
+// same staging as zgemm_kernel: BM x BN block tile, WM x WN waves of
+// (BM/WM) x (BN/WN), k staged through LDS in steps of GBK as re/im planes
+// with the next step prefetched into registers.  Four real products per
+// complex k step into two FP32 accumulators.  TRACE: the epilogue forms
+// sum conj(alpha C_ij) E_ij in FP64 against a complex64 E and writes one
+// FP64 partial per block.
+template <int BM, int BN, int WM, int WN, int GBK, bool TRACE>
+__global__ void __launch_bounds__(64 * WM * WN)
+cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, int64_t lda,
+             const float2 *__restrict__ B, int64_t ldb, float2 beta, float2 *__restrict__ C, int64_t ldc,
+             const float2 *__restrict__ E, int64_t lde, double2 *__restrict__ partial) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int AL = BM * GBK / NT;
+  constexpr int BL = BN * GBK / NT;
+  static_assert(AL >= 1 && BL >= 1 && NT % BM == 0 && NT % GBK == 0, "tile shape");
+  __shared__ float sAr[GBK][BM + 1], sAi[GBK][BM + 1];
+  __shared__ float sBr[GBK][BN + 1], sBi[GBK][BN + 1];
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  flt4 acc[2][FM][FN];
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int a = 0; a < FM; a++)
+#pragma unroll
+      for (int b = 0; b < FN; b++) acc[q][a][b] = (flt4){0.f, 0.f, 0.f, 0.f};
+  float2 ra[AL], rb[BL];
+  const int arow = tid % BM, acol0 = tid / BM;
+  constexpr int ASTEP = NT / BM;
+  const int bkr = tid % GBK, bcol0 = tid / GBK;
+  constexpr int BSTEP = NT / GBK;
+  auto load_tiles = [&](int k0) {
+#pragma unroll
+    for (int q = 0; q < AL; q++) {
+      int col = k0 + acol0 + ASTEP * q, row = m0 + arow;
+      ra[q] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int q = 0; q < BL; q++) {
+      int col = n0 + bcol0 + BSTEP * q, row = k0 + bkr;
+      rb[q] = (row < k && col < n) ? B[(int64_t)col * ldb + row] : make_float2(0.f, 0.f);
+    }
+  };
+  load_tiles(0);
+  for (int k0 = 0; k0 < k; k0 += GBK) {
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < AL; q++) {
+      sAr[acol0 + ASTEP * q][arow] = ra[q].x;
+      sAi[acol0 + ASTEP * q][arow] = ra[q].y;
+    }
+#pragma unroll
+    for (int q = 0; q < BL; q++) {
+      sBr[bkr][bcol0 + BSTEP * q] = rb[q].x;
+      sBi[bkr][bcol0 + BSTEP * q] = rb[q].y;
+    }
+    __syncthreads();
+    if (k0 + GBK < k) load_tiles(k0 + GBK);
+#pragma unroll
+    for (int kk = 0; kk < GBK; kk += 4) {
+      float ar[FM], ai[FM], nai[FM], br[FN], bi[FN];
+#pragma unroll
+      for (int x = 0; x < FM; x++) {
+        ar[x] = sAr[kk + lk][wr * (BM / WM) + x * 16 + li];
+        ai[x] = sAi[kk + lk][wr * (BM / WM) + x * 16 + li];
+        nai[x] = -ai[x];
+      }
+#pragma unroll
+      for (int y = 0; y < FN; y++) {
+        br[y] = sBr[kk + lk][wc * (BN / WN) + y * 16 + li];
+        bi[y] = sBi[kk + lk][wc * (BN / WN) + y * 16 + li];
+      }
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) {
+          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], br[y], acc[0][x][y], 0, 0, 0);
+          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], bi[y], acc[1][x][y], 0, 0, 0);
+        }
+#pragma unroll
+      for (int x = 0; x < FM; x++)
+#pragma unroll
+        for (int y = 0; y < FN; y++) {
+          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[x], bi[y], acc[0][x][y], 0, 0, 0);
+          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[x], br[y], acc[1][x][y], 0, 0, 0);
+        }
+    }
+  }
+  // v_mfma_f32_16x16x4f32 output: D[i][j] of a 16 x 16 block in lane
+  // j + 16 (i / 4), element i % 4
+  if (!TRACE) {
+#pragma unroll
+    for (int x = 0; x < FM; x++)
+#pragma unroll
+      for (int y = 0; y < FN; y++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int row = m0 + wr * (BM / WM) + x * 16 + 4 * lk + r;
+          int col = n0 + wc * (BN / WN) + y * 16 + li;
+          if (row < m && col < n) {
+            float cr = acc[0][x][y][r], ci = acc[1][x][y][r];
+            float vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
+            float2 *p = C + (int64_t)col * ldc + row;
+            if (beta.x != 0.f || beta.y != 0.f) {
+              float2 o = *p;
+              vr += beta.x * o.x - beta.y * o.y;
+              vi += beta.x * o.y + beta.y * o.x;
+            }
+            *p = make_float2(vr, vi);
+          }
+        }
+  } else {
+    constexpr int NW = WM * WN;
+    __shared__ double sred[2][NW];
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int x = 0; x < FM; x++)
+#pragma unroll
+      for (int y = 0; y < FN; y++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int row = m0 + wr * (BM / WM) + x * 16 + 4 * lk + r;
+          int col = n0 + wc * (BN / WN) + y * 16 + li;
+          if (row < m && col < n) {
+            double cr = acc[0][x][y][r], ci = acc[1][x][y][r];
+            double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
+            float2 e = E[(int64_t)col * lde + row];
+            sr += vr * (double)e.x + vi * (double)e.y;
+            si += vr * (double)e.y - vi * (double)e.x;
+          }
+        }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      sr += __shfl_down(sr, off, 64);
+      si += __shfl_down(si, off, 64);
+    }
+    if (lane == 0) { sred[0][wave] = sr; sred[1][wave] = si; }
+    __syncthreads();
+    if (tid == 0) {
+      double tr = 0.0, ti = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
+      partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
+    }
+  }
+}
+
+// complex128 <-> complex64 copies of an m x n block (one thread per row,
+// blockIdx.y per column); z2c rounds to nearest.
+__global__ void z2c_kernel(const double2 *__restrict__ src, int64_t lds, float2 *__restrict__ dst, int64_t ldd, int m,
+                           int n) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int c = blockIdx.y; c < n; c += gridDim.y) {
+    double2 v = src[(int64_t)c * lds + r];
+    dst[(int64_t)c * ldd + r] = make_float2((float)v.x, (float)v.y);
+  }
+}
+
+__global__ void c2z_kernel(const float2 *__restrict__ src, int64_t lds, double2 *__restrict__ dst, int64_t ldd, int m,
+                           int n) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int c = blockIdx.y; c < n; c += gridDim.y) {
+    float2 v = src[(int64_t)c * lds + r];
+    dst[(int64_t)c * ldd + r] = make_double2(v.x, v.y);
+  }
+}
+
+// dst (complex64) <- dst + src (complex128) rounded: the refinement update.
+__global__ void add_z_to_c_kernel(const double2 *__restrict__ src, int64_t lds, float2 *__restrict__ dst, int64_t ldd,
+                                  int m, int n) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int c = blockIdx.y; c < n; c += gridDim.y) {
+    double2 v = src[(int64_t)c * lds + r];
+    float2 o = dst[(int64_t)c * ldd + r];
+    dst[(int64_t)c * ldd + r] = make_float2((float)(o.x + v.x), (float)(o.y + v.y));
+  }
+}
+
+// C (m x n, complex128) <- I - C on the square part (the residual of an
+// identity right-hand side).
+__global__ void identity_minus_kernel(double2 *__restrict__ Cm, int64_t ldc, int n) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  int c = blockIdx.y;
+  if (r >= n) return;
+  double2 v = Cm[(int64_t)c * ldc + r];
+  Cm[(int64_t)c * ldc + r] = make_double2((r == c ? 1.0 : 0.0) - v.x, -v.y);
+}
+
+template <int BM, int BN, int WM, int WN, int GBK, bool TRACE>
+void launch_cgemm(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
+                  float2 beta, float2 *C, int64_t ldc, const float2 *E, int64_t lde, double2 *partial, hipStream_t st) {
+  dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
+  hipLaunchKernelGGL((cgemm_kernel<BM, BN, WM, WN, GBK, TRACE>), grid, dim3(64 * WM * WN), 0, st, m, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc, E, lde, partial);
+}
 
 template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool TRACE, bool G3M>
 void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B,
@@ -851,28 +1073,28 @@ int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int bw, i
 // Block-local pivoted LU of the diagonal block at (k0, k0) plus the inverses
 // of its triangles (to inv, 2 x 64 x 64).
 int bem_diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st) {
-  hipLaunchKernelGGL(diag_lu_piv_kernel, dim3(1), dim3(256), 0, st, A, lda, n, k0, inv, ipiv, info);
+  hipLaunchKernelGGL(diag_lu_piv_kernel<double2>, dim3(1), dim3(256), 0, st, A, lda, n, k0, inv, ipiv, info);
   return (int)hipGetLastError();
 }
 
 int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
   dim3 grid((m + 255) / 256 < 8 ? (m + 255) / 256 : 8, n < 512 ? n : 512);
-  hipLaunchKernelGGL(absmax_kernel, grid, dim3(256), 0, st, A, lda, m, n, upper, out);
+  hipLaunchKernelGGL(absmax_kernel<double2>, grid, dim3(256), 0, st, A, lda, m, n, upper, out);
   return (int)hipGetLastError();
 }
 
 int bem_col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st) {
   if (m <= 0 || k1 <= k0) return 0;
-  hipLaunchKernelGGL(col_swap_kernel, dim3((m + 255) / 256), dim3(256), 0, st, X, ldx, m, k0, k1, ipiv);
+  hipLaunchKernelGGL(col_swap_kernel<double2>, dim3((m + 255) / 256), dim3(256), 0, st, X, ldx, m, k0, k1, ipiv);
   return (int)hipGetLastError();
 }
 
 int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
               hipStream_t st) {
   if (c1 <= c0 || k1 <= k0) return 0;
-  hipLaunchKernelGGL(laswp_kernel, dim3((c1 - c0 + 127) / 128), dim3(128), 0, st, A, lda, c0, c1, skip0, skip1, k0, k1,
-                     ipiv);
+  hipLaunchKernelGGL(laswp_kernel<double2>, dim3((c1 - c0 + 127) / 128), dim3(128), 0, st, A, lda, c0, c1, skip0, skip1,
+                     k0, k1, ipiv);
   return (int)hipGetLastError();
 }
 
@@ -881,5 +1103,84 @@ int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+
+// ------------------------------------------------------- complex64 (mixed)
+// C <- alpha A B + beta C in complex64: 64 x 64 tiles (2 x 2 waves of
+// 32 x 32), or 32 x 32 tiles when either dimension is at most 128.
+int bem_cgemm_wide(int m, int n) { return !(m <= 128 || n <= 128); }
+
+int bem_cgemm_raw(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
+                  float2 beta, float2 *C, int64_t ldc, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  if (bem_cgemm_wide(m, n))
+    launch_cgemm<64, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st);
+  else
+    launch_cgemm<32, 32, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st);
+  return (int)hipGetLastError();
+}
+
+size_t bem_ctrace_partials(int m, int n) { return (size_t)((m + 63) / 64) * (size_t)((n + 63) / 64); }
+
+// out (complex128) <- sum conj(alpha A B) .* E over an m x n product, the
+// partial sums and the final sum in FP64.
+int bem_cgemm_trace(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
+                    const float2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
+  launch_cgemm<64, 64, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, make_float2(0.f, 0.f), nullptr, 0, E, lde,
+                                       partial, st);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)bem_ctrace_partials(m, n), out);
+  return (int)hipGetLastError();
+}
+
+int bem_diag_lu_piv_c(float2 *A, int64_t lda, int n, int k0, float2 *inv, int *ipiv, int *info, hipStream_t st) {
+  hipLaunchKernelGGL(diag_lu_piv_kernel<float2>, dim3(1), dim3(256), 0, st, A, lda, n, k0, inv, ipiv, info);
+  return (int)hipGetLastError();
+}
+
+int bem_absmax_c(const float2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  dim3 grid((m + 255) / 256 < 8 ? (m + 255) / 256 : 8, n < 512 ? n : 512);
+  hipLaunchKernelGGL(absmax_kernel<float2>, grid, dim3(256), 0, st, A, lda, m, n, upper, out);
+  return (int)hipGetLastError();
+}
+
+int bem_col_swap_c(float2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st) {
+  if (m <= 0 || k1 <= k0) return 0;
+  hipLaunchKernelGGL(col_swap_kernel<float2>, dim3((m + 255) / 256), dim3(256), 0, st, X, ldx, m, k0, k1, ipiv);
+  return (int)hipGetLastError();
+}
+
+int bem_laswp_c(float2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
+                hipStream_t st) {
+  if (c1 <= c0 || k1 <= k0) return 0;
+  hipLaunchKernelGGL(laswp_kernel<float2>, dim3((c1 - c0 + 127) / 128), dim3(128), 0, st, A, lda, c0, c1, skip0, skip1,
+                     k0, k1, ipiv);
+  return (int)hipGetLastError();
+}
+
+static dim3 copy_grid(int m, int n) { return dim3((m + 255) / 256, n < 65535 ? n : 65535); }
+
+int bem_z2c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(z2c_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+int bem_c2z(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(c2z_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+int bem_add_z_to_c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(add_z_to_c_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st) {
+  if (n <= 0) return 0;
+  hipLaunchKernelGGL(identity_minus_kernel, dim3((n + 255) / 256, n), dim3(256), 0, st, Cm, ldc, n);
+  return (int)hipGetLastError();
+}
 
 }  // extern "C"

@@ -22,6 +22,7 @@ BEM_E_NOMEM = -3
 BEM_E_SINGULAR = -4
 BEM_E_MESH = -5
 BEM_E_UNSUPPORTED = -6
+BEM_PENDING = 1
 BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_ASM_DEVICE_PTR = 1
@@ -58,7 +59,7 @@ class _Res(ctypes.Structure):
                 ("rel_err_est", ctypes.c_double), ("iters", ctypes.c_int), ("status", ctypes.c_int),
                 ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double),
                 ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double), ("growth", ctypes.c_double),
-                ("pivoting", ctypes.c_int)]
+                ("pivoting", ctypes.c_int), ("ms_queue", ctypes.c_double)]
 
 
 def lib_path():
@@ -71,7 +72,8 @@ _lib = None
 EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_device_bytes", "bem_body_upload",
            "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
            "bem_sweep", "bem_zgetrf", "bem_zgetrs", "bem_zgemm", "bem_device_alloc", "bem_device_free",
-           "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats"]
+           "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
+           "bem_submit", "bem_poll", "bem_queue_depth"]
 
 
 def load_library():
@@ -109,6 +111,9 @@ def load_library():
     L.bem_memcpy_d2h.argtypes = [vp, vp, vp, sz]
     L.bem_last_timings.argtypes = [vp, vp]
     L.bem_kernel_stats.argtypes = [vp, i32, vp]
+    L.bem_submit.argtypes = [vp, vp, vp, P(_Req), P(ctypes.c_uint64)]
+    L.bem_poll.argtypes = [vp, ctypes.c_uint64, P(_Res), i32]
+    L.bem_queue_depth.argtypes = [vp, P(i32), P(i32)]
     _lib = L
     return L
 
@@ -272,6 +277,29 @@ class Context:
         if rc != 0:
             raise BemError(rc, self.lib.bem_last_error(self.h).decode())
         return res
+
+    # ---- serving (continuous batching)
+    def submit(self, b1: Body, b2: Body, req: Request) -> int:
+        """Queue one request; returns its ticket (bem_submit)."""
+        t = ctypes.c_uint64(0)
+        self._check(self.lib.bem_submit(self.h, b1.handle, b2.handle, ctypes.byref(req.to_c()), ctypes.byref(t)))
+        return t.value
+
+    def poll(self, ticket: int, wait_ms: int = 0):
+        """Result dict of a ticket, or None while it is pending (bem_poll).
+        A failed request returns its dict with a nonzero 'status'."""
+        r = _Res()
+        rc = self.lib.bem_poll(self.h, ctypes.c_uint64(ticket), ctypes.byref(r), int(wait_ms))
+        if rc == BEM_PENDING:
+            return None
+        if rc == BEM_E_INVALID and r.status == 0:
+            raise BemError(rc, f"unknown ticket {ticket}")
+        return _res_dict(r)
+
+    def queue_depth(self):
+        q, r = ctypes.c_int(0), ctypes.c_int(0)
+        self._check(self.lib.bem_queue_depth(self.h, ctypes.byref(q), ctypes.byref(r)))
+        return q.value, r.value
 
     KERNEL_CLASSES = {"zgemm": 0, "zgemm_narrow": 1, "ztrace": 2, "far": 3, "close": 4, "gather": 5, "cgemm": 6}
 

@@ -46,6 +46,7 @@ extern "C" {
 #define BEM_E_SINGULAR (-4)    /* exactly singular pivot met */
 #define BEM_E_MESH (-5)        /* mesh is not a closed 2-manifold */
 #define BEM_E_UNSUPPORTED (-6) /* size or mode not supported */
+#define BEM_PENDING 1          /* bem_poll: the request is queued or running */
 
 #define BEM_PREC_FP64 0        /* complex128 factorisation */
 #define BEM_PREC_MIXED 1       /* complex64 factorisation + FP64 refinement */
@@ -92,6 +93,7 @@ typedef struct {
   double ms_assemble, ms_factor, ms_products, ms_total;  /* device-event times */
   double growth;     /* max |U| / max |A| over the two LU factorisations */
   int pivoting;      /* 0 block-local, 1 tournament */
+  double ms_queue;   /* served requests: submit to issue (ms); ms_total is then submit to result */
 } bem_result;
 
 /* ---- context ---------------------------------------------------------- */
@@ -132,6 +134,25 @@ int bem_spectral_power(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_reque
  * Up to cfg.lanes requests run concurrently on separate streams; each result
  * is computed by the same kernels in the same order as when run alone. */
 int bem_sweep(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *reqs, int n, bem_result *res);
+
+/* ---- serving: continuous batching ------------------------------------ */
+/* Queues one request on the body pair and returns at once with a ticket.  A
+ * scheduler thread of the context admits queued requests in arrival order
+ * onto free lanes (cfg.lanes streams) whenever the request's device footprint
+ * -- lane workspaces plus self blocks not yet cached -- fits cfg.hbm_budget
+ * (evicting unused cached blocks and idle workspaces first), so requests that
+ * arrive while others run join them on the device as lanes and memory free
+ * up.  Each result is computed by the same kernels in the same order as when
+ * the request runs alone.  There is no reference counterpart beyond the job
+ * farm of src/transfer-mpi.cpp:114-179, whose servants this replaces. */
+int bem_submit(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *req, uint64_t *ticket);
+/* Result of a ticket: returns BEM_PENDING while it is queued or running
+ * (waiting up to wait_ms; wait_ms < 0 waits until done), else the request's
+ * status (0 or BEM_E_*) with *res filled, after which the ticket is retired.
+ * BEM_E_INVALID for an unknown ticket. */
+int bem_poll(bem_ctx *ctx, uint64_t ticket, bem_result *res, int wait_ms);
+/* Requests waiting in the queue and running on lanes. */
+int bem_queue_depth(bem_ctx *ctx, int *queued, int *running);
 
 /* ---- dense complex128 kernels on device pointers ---------------------- */
 /* A (n x n, lda) <- P L U in place with tournament partial pivoting; ipiv
