@@ -16,8 +16,8 @@
 // else is in flight.
 //
 // Serving.  bem_submit queues a request and returns a ticket; a scheduler
-// thread of the context admits queued requests in arrival order onto free
-// lanes whenever the request's device footprint (lane workspaces plus any
+// thread of the context admits queued requests (shortest estimated run time
+// first, with aging) onto free lanes whenever the request's device footprint (lane workspaces plus any
 // self blocks not yet cached) fits the HBM budget, and retires lanes whose
 // stream signalled completion through a host callback.  Requests arriving
 // while others run are therefore admitted as lanes and memory free up, and
@@ -72,6 +72,22 @@ int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int bw, i
 int bem_diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st);
 int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st);
 int bem_col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st);
+int bem_cgemm_wide(int m, int n);
+int bem_cgemm_raw(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
+                  float2 beta, float2 *C, int64_t ldc, hipStream_t st);
+size_t bem_ctrace_partials(int m, int n);
+int bem_cgemm_trace(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
+                    const float2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st);
+int bem_diag_lu_piv_c(float2 *A, int64_t lda, int n, int k0, float2 *inv, int *ipiv, int *info, hipStream_t st);
+int bem_absmax_c(const float2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st);
+int bem_col_swap_c(float2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st);
+int bem_laswp_c(float2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
+                hipStream_t st);
+int bem_z2c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st);
+int bem_c2z(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
+int bem_add_z_to_c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st);
+int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st);
+int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 }
 
 // src/util/SI.cpp:3-9
@@ -119,7 +135,7 @@ struct SelfStatus {
   unsigned long long growth[2];   // max|U|, max|A| of the LU (role 0)
 };
 
-// Gap-independent blocks of one body at one frequency (src/HeatTransfer.cpp:
+// Gap-independent blocks of one body at one frequency and precision (src/HeatTransfer.cpp:
 // 26-29 assembles them inside Z_ext and Z_int on every call): the self block
 // Z_ext(b, b) + Z_int(b) -- packed LU with interchanges and diagonal-block
 // inverses when the body sits in the first role (A), raw in the second (D) --
@@ -128,9 +144,12 @@ struct SelfStatus {
 struct SelfEntry {
   const void *body = nullptr;
   double key[5] = {0, 0, 0, 0, 0};   // eps re/im, mu re/im, omega
-  int role = 0, pivot = 0;
+  int role = 0, pivot = 0, prec = BEM_PREC_FP64;
   int n = 0;
   double2 *Z = nullptr, *Gs = nullptr, *inv = nullptr;
+  // BEM_PREC_MIXED: Z stays the raw FP64 block (refinement residuals) and
+  // Z32 holds its complex64 copy (packed LU with inv32 in role 0)
+  float2 *Z32 = nullptr, *Gs32 = nullptr, *inv32 = nullptr;
   int *ipiv = nullptr, *dinfo = nullptr;
   unsigned long long *dgrowth = nullptr;
   SelfStatus *h = nullptr;   // pinned
@@ -158,6 +177,8 @@ struct Lane {
   SelfEntry *e1 = nullptr, *e2 = nullptr;   // pinned while in flight
   std::chrono::steady_clock::time_point t0;
   size_t ws_bytes = 0;            // bytes of this lane's named workspaces
+  int prec = BEM_PREC_FP64;       // precision of the solve in flight
+  bool refine = false;
   // served request in flight (ticket 0: none)
   bem_ctx *ctx = nullptr;
   uint64_t ticket = 0;
@@ -349,14 +370,60 @@ static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const d
 // Panel width of the blocked LU; the diagonal-block inverses are 64 x 64.
 static const int LU_NB = 64;
 
-static size_t inv_bytes(int n) { return sizeof(double2) * 2 * 64 * 64 * (size_t)((n + LU_NB - 1) / LU_NB); }
+template <typename T = double2>
+static size_t inv_bytes(int n) { return sizeof(T) * 2 * 64 * 64 * (size_t)((n + LU_NB - 1) / LU_NB); }
 
 // dst block (rows x cols, ld) <- tmp (rows x cols, ld rows)
-static int copy_back(bem_ctx *c, Lane &L, double2 *dst, int64_t ld, const double2 *tmp, int rows, int cols) {
-  DEV(c, hipMemcpy2DAsync(dst, sizeof(double2) * ld, tmp, sizeof(double2) * rows, sizeof(double2) * rows, cols,
+template <typename T>
+static int copy_back(bem_ctx *c, Lane &L, T *dst, int64_t ld, const T *tmp, int rows, int cols) {
+  DEV(c, hipMemcpy2DAsync(dst, sizeof(T) * ld, tmp, sizeof(T) * rows, sizeof(T) * rows, cols,
                           hipMemcpyDeviceToDevice, L.st));
   return 0;
 }
+
+// complex64 GEMM on the lane's stream, timed as BEM_KS_CGEMM.
+static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B,
+                int64_t ldb, float2 beta, float2 *C, int64_t ldc) {
+  if (m <= 0 || n <= 0) return 0;
+  EventRec *r;
+  const bool rd_c = beta.x != 0.f || beta.y != 0.f;
+  CHK(ev_begin(c, L, BEM_KS_CGEMM, 8.0 * m * (double)n * k,
+               8.0 * ((double)m * k + (double)k * n + (rd_c ? 2.0 : 1.0) * (double)m * n), &r));
+  int e = bem_cgemm_raw(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "cgemm launch: %s", hipGetErrorString((hipError_t)e));
+  return ev_end(c, L, r);
+}
+
+// precision-generic launches of the LU building blocks
+static int diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st) {
+  return bem_diag_lu_piv(A, lda, n, k0, inv, ipiv, info, st);
+}
+static int diag_lu_piv(float2 *A, int64_t lda, int n, int k0, float2 *inv, int *ipiv, int *info, hipStream_t st) {
+  return bem_diag_lu_piv_c(A, lda, n, k0, inv, ipiv, info, st);
+}
+static int laswp(double2 *A, int64_t lda, int c0, int c1, int s0, int s1, int k0, int k1, const int *ipiv,
+                 hipStream_t st) {
+  return bem_laswp(A, lda, c0, c1, s0, s1, k0, k1, ipiv, st);
+}
+static int laswp(float2 *A, int64_t lda, int c0, int c1, int s0, int s1, int k0, int k1, const int *ipiv,
+                 hipStream_t st) {
+  return bem_laswp_c(A, lda, c0, c1, s0, s1, k0, k1, ipiv, st);
+}
+static int col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st) {
+  return bem_col_swap(X, ldx, m, k0, k1, ipiv, st);
+}
+static int col_swap(float2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st) {
+  return bem_col_swap_c(X, ldx, m, k0, k1, ipiv, st);
+}
+static int absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st) {
+  return bem_absmax(A, lda, m, n, upper, out, st);
+}
+static int absmax(const float2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st) {
+  return bem_absmax_c(A, lda, m, n, upper, out, st);
+}
+template <typename T> static T cnum(double re, double im);
+template <> double2 cnum<double2>(double re, double im) { return make_double2(re, im); }
+template <> float2 cnum<float2>(double re, double im) { return make_float2((float)re, (float)im); }
 
 // Blocked right-looking LU with tournament-pivoted panels.  Each 64-column
 // block is factored as two 32-column sub-panels (tournament pivots, diagonal
@@ -423,26 +490,27 @@ static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *i
 // the panel's triangular solves and the trailing update run as GEMMs.
 // growth (optional, device) receives max|A| before in growth[1] and
 // max|U| after in growth[0].
-static int lu_factor_block(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *ipiv, int *info, double2 *inv,
+template <typename T>
+static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *ipiv, int *info, T *inv,
                            unsigned long long *growth) {
   const int nb = LU_NB;
-  double2 *tmp;
-  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
-  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  T *tmp;
+  CHK(ws_get(c, L, "lu_tmp", sizeof(T) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0), zero = cnum<T>(0.0, 0.0);
   int e;
-  if (growth && (e = bem_absmax(A, lda, n, n, 0, growth + 1, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  if (growth && (e = absmax(A, lda, n, n, 0, growth + 1, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
     const int blk = k0 / nb;
-    double2 *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
-    e = bem_diag_lu_piv(A, lda, n, k0, Linv, ipiv, info, L.st);
+    T *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
+    e = diag_lu_piv(A, lda, n, k0, Linv, ipiv, info, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
-    e = bem_laswp(A, lda, 0, n, k0, k0 + jb, k0, k0 + jb, ipiv, L.st);
+    e = laswp(A, lda, 0, n, k0, k0 + jb, k0, k0 + jb, ipiv, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
     int rest = n - k0 - jb;
     if (rest > 0) {
-      double2 *Akk = A + (int64_t)k0 * lda + k0;
-      double2 *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
+      T *Akk = A + (int64_t)k0 * lda + k0;
+      T *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
       CHK(gemm(c, L, rest, jb, jb, one, A21, lda, Uinv, 64, zero, tmp, rest));   // L21 = A21 U11^-1
       CHK(copy_back(c, L, A21, lda, tmp, rest, jb));
       CHK(gemm(c, L, jb, rest, jb, one, Linv, 64, A12, lda, zero, tmp, jb));     // U12 = L11^-1 A12
@@ -450,23 +518,24 @@ static int lu_factor_block(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, 
       CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
     }
   }
-  if (growth && (e = bem_absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  if (growth && (e = absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
   return 0;
 }
 
 // X (n x r) <- A^-1 X from the packed LU, its interchanges and the diagonal
 // block inverses.
-static int lu_solve(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, const int *ipiv, const double2 *inv,
-                    double2 *X, int64_t ldx, int r) {
+template <typename T>
+static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv, const T *inv,
+                    T *X, int64_t ldx, int r) {
   const int nb = LU_NB;
-  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  double2 *tmp;
-  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
-  int e = bem_laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0), zero = cnum<T>(0.0, 0.0);
+  T *tmp;
+  CHK(ws_get(c, L, "lu_tmp", sizeof(T) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
+  int e = laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "laswp rhs: %d", e);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
-    const double2 *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+    const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
     CHK(gemm(c, L, jb, r, jb, one, Linv, 64, X + k0, ldx, zero, tmp, jb));
     CHK(copy_back(c, L, X + k0, ldx, tmp, jb, r));
     int rest = n - k0 - jb;
@@ -476,7 +545,7 @@ static int lu_solve(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, c
   int last = ((n - 1) / nb) * nb;
   for (int k0 = last; k0 >= 0; k0 -= nb) {
     int jb = std::min(nb, n - k0);
-    const double2 *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
+    const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
     CHK(gemm(c, L, jb, r, jb, one, Uinv, 64, X + k0, ldx, zero, tmp, jb));
     CHK(copy_back(c, L, X + k0, ldx, tmp, jb, r));
     if (k0 > 0) CHK(gemm(c, L, k0, r, jb, m1, A + (int64_t)k0 * lda, lda, X + k0, ldx, one, X, ldx));
@@ -488,16 +557,17 @@ static int lu_solve(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, c
 // blocks left to right, then L^-1 right to left, then the column
 // interchanges X P^T; the diagonal blocks go through their inverses, the
 // coupling terms through GEMMs.
-static int lu_solve_right(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int n, const int *ipiv,
-                          const double2 *inv, double2 *X, int64_t ldx, int r) {
+template <typename T>
+static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv,
+                          const T *inv, T *X, int64_t ldx, int r) {
   const int nb = LU_NB;
-  const double2 m1 = make_double2(-1.0, 0.0), one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
-  double2 *tmp;
-  CHK(ws_get(c, L, "lu_tmp", sizeof(double2) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0), zero = cnum<T>(0.0, 0.0);
+  T *tmp;
+  CHK(ws_get(c, L, "lu_tmp", sizeof(T) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
-    const double2 *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
-    double2 *Xk = X + (int64_t)k0 * ldx;
+    const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
+    T *Xk = X + (int64_t)k0 * ldx;
     CHK(gemm(c, L, r, jb, jb, one, Xk, ldx, Uinv, 64, zero, tmp, r));
     CHK(copy_back(c, L, Xk, ldx, tmp, r, jb));
     int rest = n - k0 - jb;
@@ -507,13 +577,13 @@ static int lu_solve_right(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, in
   int last = ((n - 1) / nb) * nb;
   for (int k0 = last; k0 >= 0; k0 -= nb) {
     int jb = std::min(nb, n - k0);
-    const double2 *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
-    double2 *Xk = X + (int64_t)k0 * ldx;
+    const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+    T *Xk = X + (int64_t)k0 * ldx;
     CHK(gemm(c, L, r, jb, jb, one, Xk, ldx, Linv, 64, zero, tmp, r));
     CHK(copy_back(c, L, Xk, ldx, tmp, r, jb));
     if (k0 > 0) CHK(gemm(c, L, r, k0, jb, m1, Xk, ldx, A + k0, lda, one, X, ldx));
   }
-  int e = bem_col_swap(X, ldx, r, 0, n, ipiv, L.st);
+  int e = col_swap(X, ldx, r, 0, n, ipiv, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "column interchanges: %d", e);
   return 0;
 }
@@ -694,6 +764,16 @@ static double theta(double omega, double T) {
 }
 
 // ------------------------------------------------------------- self blocks
+// Bytes of a self entry of an n x n block in role 0 (factored) or 1 (raw).
+static size_t self_entry_bytes(int n, int role, int prec) {
+  size_t nn = sizeof(double2) * (size_t)n * n;
+  size_t b = 2 * nn + 256;
+  if (role == 0) b += sizeof(int) * (size_t)(n + 64);
+  if (prec == BEM_PREC_MIXED) b += nn + (role == 0 ? inv_bytes<float2>(n) : 0);
+  else if (role == 0) b += inv_bytes(n);
+  return b;
+}
+
 static void self_free(bem_ctx *c, SelfEntry &e) {
   if (e.ready) (void)hipEventSynchronize(e.ready);
   for (void *p : e.allocs) (void)hipFree(p);
@@ -734,10 +814,11 @@ static int self_alloc(bem_ctx *c, SelfEntry &e, void **p, size_t bytes) {
 // Self entry of body b in medium (eps, mu) at omega for role 0 (factored) or
 // 1 (raw), pinned for the lane; produced on the lane's stream when missing.
 static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const double *mu, double omega, int role,
-                    int pivot, SelfEntry **out) {
+                    int pivot, int prec, SelfEntry **out) {
   const double key[5] = {eps[0], eps[1], mu[0], mu[1], omega};
   for (auto &e : c->selfs)
-    if (e.body == b && e.role == role && e.pivot == pivot && std::memcmp(e.key, key, sizeof key) == 0) {
+    if (e.body == b && e.role == role && e.pivot == pivot && e.prec == prec &&
+        std::memcmp(e.key, key, sizeof key) == 0) {
       e.pins++;
       e.stamp = ++c->clock;
       DEV(c, hipStreamWaitEvent(L.st, e.ready, 0));
@@ -750,7 +831,7 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   CHK(body_tree(c, b, &d));
   const int T = d->T, n = 2 * d->E;
   size_t nn = sizeof(double2) * (size_t)n * n;
-  size_t need = 2 * nn + (role == 0 ? inv_bytes(n) + sizeof(int) * (size_t)(n + 64) : 0) + 256;
+  size_t need = self_entry_bytes(n, role, prec);
   self_evict(c, need);
   c->selfs.emplace_front();
   SelfEntry &e = c->selfs.front();
@@ -758,6 +839,7 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   std::memcpy(e.key, key, sizeof key);
   e.role = role;
   e.pivot = pivot;
+  e.prec = prec;
   e.n = n;
   e.pins = 1;
   e.stamp = ++c->clock;
@@ -769,8 +851,12 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   int r = self_alloc(c, e, (void **)&e.Z, nn);
   if (!r) r = self_alloc(c, e, (void **)&e.Gs, nn);
   if (!r) r = self_alloc(c, e, (void **)&e.dinfo, sizeof(int) * 4 + sizeof(unsigned long long) * 2);
-  if (!r && role == 0) r = self_alloc(c, e, (void **)&e.inv, inv_bytes(n));
+  const bool mixed = prec == BEM_PREC_MIXED;
+  if (!r && role == 0 && !mixed) r = self_alloc(c, e, (void **)&e.inv, inv_bytes(n));
   if (!r && role == 0) r = self_alloc(c, e, (void **)&e.ipiv, sizeof(int) * (size_t)(n + 64));
+  if (!r && mixed) r = self_alloc(c, e, (void **)&e.Z32, nn / 2);
+  if (!r && mixed) r = self_alloc(c, e, (void **)&e.Gs32, nn / 2);
+  if (!r && mixed && role == 0) r = self_alloc(c, e, (void **)&e.inv32, inv_bytes<float2>(n));
   if (!r && hipHostMalloc((void **)&e.h, sizeof(SelfStatus), hipHostMallocDefault) != hipSuccess)
     r = fail(c, BEM_E_NOMEM, "pinned status slot");
   if (!r && hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess)
@@ -790,7 +876,11 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   if (!r) r = assemble_tiles(c, L, *d, mint, tiles, 0, T, 0, T, e.dinfo + 1);
   if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mint, e.Z, n, 0, 0, 1, e.Gs, n, 0, 0);
   if (!r && bem_symm(e.Gs, n, n, L.st)) r = fail(c, BEM_E_DEVICE, "symm launch");
-  if (!r && role == 0) {
+  if (!r && mixed) {
+    if (bem_z2c(e.Z, n, e.Z32, n, n, n, L.st) || bem_z2c(e.Gs, n, e.Gs32, n, n, n, L.st))
+      r = fail(c, BEM_E_DEVICE, "complex64 copy launch");
+    if (!r && role == 0) r = lu_factor_block(c, L, e.Z32, n, n, e.ipiv, e.dinfo, e.inv32, e.dgrowth);
+  } else if (!r && role == 0) {
     if (pivot) r = lu_factor(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv);
     else r = lu_factor_block(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv, e.dgrowth);
   }
@@ -827,11 +917,118 @@ static void self_drop_body(bem_ctx *c, const void *b) {
 }
 
 // ------------------------------------------------------------- one solve
-// Issues every kernel of one FP64 spectral-power solve on the lane's stream
-// and returns without waiting; lane_finish() collects the result.
+// Cross blocks of Z_ext for the pair's gap: B = Z_ext(1,2) (m1 x m2) and
+// C = Z_ext(2,1) (m2 x m1), complex128.
+static int assemble_cross(bem_ctx *c, Lane &L, PairGeom &g, const Medium &mext, double2 *Bm, double2 *Cm, int *flag) {
+  const int T1 = g.T1, T2 = g.top.T - g.T1, Eb1 = g.E1, E = g.top.E;
+  const int m1 = 2 * Eb1, m2 = 2 * (E - Eb1);
+  double *tiles;
+  size_t npc = (size_t)T1 * T2;
+  CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(npc, 1), (void **)&tiles));
+  CHK(assemble_tiles(c, L, g.top, mext, tiles, 0, T1, T1, T1 + T2, flag));
+  CHK(gather(c, L, tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0));
+  CHK(assemble_tiles(c, L, g.top, mext, tiles, T1, T1 + T2, 0, T1, flag));
+  CHK(gather(c, L, tiles, npc, T1, T1, 0, g.top.rw, Eb1, E, 0, Eb1, &mext, Cm, m2, 0, 0, 0, nullptr, 0, 0, 0));
+  return 0;
+}
+
+// Mixed-precision solve (BEM_PREC_MIXED) after the FP64 assembly of B, C:
+// the block elimination Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1 and the
+// products run in complex64 on FP32 MFMA from the cached complex64 LU of A
+// (the trace sums in FP64).  With BEM_SOLVE_REFINE one step of iterative
+// refinement follows: the residual of W [X; Sc^-1] = [0; I] against the FP64
+// blocks A, B, C, D in FP64, the correction through the complex64 factors,
+// and the products and trace of the corrected X in FP64; the relative change
+// of S is reported as the error estimate of the complex64 result.
+static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry &E2, double2 *Bm, double2 *Cm,
+                       int *info, unsigned long long *gro, double2 *trace_out, bool refine) {
+  const int m1 = E1.n, m2 = E2.n;
+  const float2 one = cnum<float2>(1, 0), mone = cnum<float2>(-1, 0), zero = cnum<float2>(0, 0);
+  float2 *B32, *C32, *S32, *B1, *invS, *Y32 = nullptr;
+  int *ipS;
+  double2 *partial;
+  CHK(ws_get(c, L, "B32", sizeof(float2) * (size_t)m1 * m2, (void **)&B32));
+  CHK(ws_get(c, L, "C32", sizeof(float2) * (size_t)m2 * m1, (void **)&C32));
+  CHK(ws_get(c, L, "S32", sizeof(float2) * (size_t)m2 * m2, (void **)&S32));
+  CHK(ws_get(c, L, "B1_32", sizeof(float2) * (size_t)m1 * m2, (void **)&B1));
+  CHK(ws_get(c, L, "invS32", inv_bytes<float2>(m2), (void **)&invS));
+  CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
+  size_t np = std::max(bem_ctrace_partials(m1, m2), bem_trace_partials(m1, m2));
+  CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
+  if (bem_z2c(Bm, m1, B32, m1, m1, m2, L.st) || bem_z2c(Cm, m2, C32, m2, m2, m1, L.st))
+    return fail(c, BEM_E_DEVICE, "complex64 copy launch");
+  DEV(c, hipMemcpyAsync(S32, E2.Z32, sizeof(float2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
+  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, B32, m1, m2));   // B32 <- Y
+  CHK(gemm(c, L, m2, m2, m1, mone, C32, m2, B32, m1, one, S32, m2));     // Sc = D - C Y
+  CHK(lu_factor_block(c, L, S32, m2, m2, ipS, info, invS, gro));
+  if (refine) {
+    CHK(ws_get(c, L, "Y32", sizeof(float2) * (size_t)m1 * m2, (void **)&Y32));
+    DEV(c, hipMemcpyAsync(Y32, B32, sizeof(float2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
+  }
+  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, B32, m1, m1));        // B32 <- Y Sc^-1 = -X
+  // products: S = 1/4 sum conj(X sym G2) .* (sym G1 X), even in X
+  CHK(gemm(c, L, m1, m2, m1, one, E1.Gs32, m1, B32, m1, zero, B1, m1));
+  {
+    EventRec *r;
+    CHK(ev_begin(c, L, BEM_KS_CGEMM, 8.0 * m1 * (double)m2 * m2, 8.0 * (2.0 * m1 * m2 + (double)m2 * m2), &r));
+    int e = bem_cgemm_trace(m1, m2, m2, one, B32, m1, E2.Gs32, m2, B1, m1, partial, trace_out, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "complex64 trace gemm: %d", e);
+    CHK(ev_end(c, L, r));
+  }
+  if (!refine) return 0;
+
+  // ---- one FP64 refinement step.  Xn = -X (m1 x m2), X2 = Sc^-1 (m2 x m2).
+  const double2 d1 = make_double2(1, 0), dm1 = make_double2(-1, 0), d0 = make_double2(0, 0);
+  float2 *X2, *R1c, *R2c;
+  double2 *Xn, *X2d, *R1, *R2, *B1d;
+  CHK(ws_get(c, L, "X2_32", sizeof(float2) * (size_t)m2 * m2, (void **)&X2));
+  CHK(ws_get(c, L, "R1_32", sizeof(float2) * (size_t)m1 * m2, (void **)&R1c));
+  CHK(ws_get(c, L, "R2_32", sizeof(float2) * (size_t)m2 * m2, (void **)&R2c));
+  CHK(ws_get(c, L, "Xn64", sizeof(double2) * (size_t)m1 * m2, (void **)&Xn));
+  CHK(ws_get(c, L, "X2_64", sizeof(double2) * (size_t)m2 * m2, (void **)&X2d));
+  CHK(ws_get(c, L, "R1_64", sizeof(double2) * (size_t)m1 * m2, (void **)&R1));
+  CHK(ws_get(c, L, "R2_64", sizeof(double2) * (size_t)m2 * m2, (void **)&R2));
+  CHK(ws_get(c, L, "B1_64", sizeof(double2) * (size_t)m1 * m2, (void **)&B1d));
+  // X2 = Sc^-1 = I Sc^-1 (I formed as I - 0)
+  DEV(c, hipMemsetAsync(R2, 0, sizeof(double2) * (size_t)m2 * m2, L.st));
+  int e = bem_identity_minus(R2, m2, m2, L.st);
+  if (!e) e = bem_z2c(R2, m2, X2, m2, m2, m2, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
+  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, X2, m2, m2));
+  if (bem_c2z(B32, m1, Xn, m1, m1, m2, L.st) || bem_c2z(X2, m2, X2d, m2, m2, m2, L.st))
+    return fail(c, BEM_E_DEVICE, "complex128 copy launch");
+  // T1 = B X2 - A Xn = -(R1); T2 = D X2 - C Xn, R2 = I - T2
+  CHK(gemm(c, L, m1, m2, m2, d1, Bm, m1, X2d, m2, d0, R1, m1));
+  CHK(gemm(c, L, m1, m2, m1, dm1, E1.Z, m1, Xn, m1, d1, R1, m1));
+  CHK(gemm(c, L, m2, m2, m2, d1, E2.Z, m2, X2d, m2, d0, R2, m2));
+  CHK(gemm(c, L, m2, m2, m1, dm1, Cm, m2, Xn, m1, d1, R2, m2));
+  if ((e = bem_identity_minus(R2, m2, m2, L.st))) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
+  // correction [dX; dX2] = W^-1 [R1; R2] with the complex64 factors, R1 = -T1:
+  //   u = A^-1 T1 (= -A^-1 R1), dX2 = Sc^-1 (R2 + C u), dX = -u - Y dX2
+  if (bem_z2c(R1, m1, R1c, m1, m1, m2, L.st) || bem_z2c(R2, m2, R2c, m2, m2, m2, L.st))
+    return fail(c, BEM_E_DEVICE, "complex64 copy launch");
+  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, R1c, m1, m2));   // R1c <- u
+  CHK(gemm(c, L, m2, m2, m1, one, C32, m2, R1c, m1, one, R2c, m2));      // R2c <- R2 + C u
+  CHK(lu_solve(c, L, S32, m2, m2, ipS, invS, R2c, m2, m2));              // R2c <- dX2
+  CHK(gemm(c, L, m1, m2, m2, one, Y32, m1, R2c, m2, one, R1c, m1));      // R1c <- u + Y dX2 = -dX
+  // corrected -X = Xn - dX = Xn + (u + Y dX2), in FP64
+  if ((e = bem_c2z_add(R1c, m1, Xn, m1, m1, m2, L.st))) return fail(c, BEM_E_DEVICE, "update launch: %d", e);
+  // FP64 products and trace of the corrected X into trace_out[1]
+  CHK(gemm(c, L, m1, m2, m1, d1, E1.Gs, m1, Xn, m1, d0, B1d, m1));
+  EventRec *r;
+  CHK(ev_begin(c, L, BEM_KS_ZTRACE, 8.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + (double)m2 * m2), &r));
+  e = bem_zgemm_trace(m1, m2, m2, d1, Xn, m1, E2.Gs, m2, B1d, m1, partial, trace_out + 1, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
+  return ev_end(c, L, r);
+}
+
+// Issues every kernel of one spectral-power solve on the lane's stream and
+// returns without waiting; lane_finish() collects the result.
 //   A = Z_ext(1,1) + Z_int(1) (cached, factored), D = Z_ext(2,2) + Z_int(2)
 //   (cached), B = Z_ext(1,2), C = Z_ext(2,1) (assembled for this gap);
 //   Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1, S = 1/4 tr((X sym G2)^H sym G1 X).
+// BEM_PREC_MIXED runs the elimination in complex64 (solve_mixed); a mixed
+// request with tournament pivoting runs the FP64 path.
 static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *b2, const bem_request *q,
                       bem_result *res) {
   L.t0 = std::chrono::steady_clock::now();
@@ -840,57 +1037,55 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   double omegaSI = q->eV * SI_EV / SI_HBAR;
   double omega = omegaSI * q->a / SI_C;
   const int pivot = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  const int prec = pivot ? BEM_PREC_FP64 : q->precision;
+  L.prec = prec;
+  L.refine = prec == BEM_PREC_MIXED && (q->flags & BEM_SOLVE_REFINE);
   Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
   DEV(c, hipEventRecord(L.ph[0], L.st));
-  CHK(self_get(c, L, b1, q->eps1, q->mu1, omega, 0, pivot, &L.e1));
-  CHK(self_get(c, L, b2, q->eps2, q->mu2, omega, 1, 0, &L.e2));
+  CHK(self_get(c, L, b1, q->eps1, q->mu1, omega, 0, pivot, prec, &L.e1));
+  CHK(self_get(c, L, b2, q->eps2, q->mu2, omega, 1, 0, prec, &L.e2));
   const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
   const int m1 = E1.n, m2 = E2.n;
-  const int T1 = g.T1, T2 = g.top.T - g.T1, Eb1 = g.E1, E = g.top.E;
-  if (m1 != 2 * Eb1 || m2 != 2 * (E - Eb1)) return fail(c, BEM_E_INVALID, "body sizes do not match the pair tree");
+  if (m1 != 2 * g.E1 || m2 != 2 * (g.top.E - g.E1)) return fail(c, BEM_E_INVALID, "body sizes do not match the pair tree");
 
-  double2 *Bm, *Cm, *Sc, *B1, *partial, *invS;
-  double *tiles;
-  int *ipS, *info;
+  double2 *Bm, *Cm, *trace_out;
+  int *info;
   unsigned long long *gro;
-  size_t npc = (size_t)T1 * T2;
   CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
   CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
-  CHK(ws_get(c, L, "Sc", sizeof(double2) * (size_t)m2 * m2, (void **)&Sc));
-  CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
-  CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(npc, 1), (void **)&tiles));
-  CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
   CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
   CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 2, (void **)&gro));
-  CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
-  size_t np = bem_trace_partials(m1, m2);
-  CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
-  double2 *trace_out = partial + np;
+  CHK(ws_get(c, L, "trace_out", sizeof(double2) * 2, (void **)&trace_out));
   DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
   DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 2, L.st));
-
-  // ---- cross blocks of Z_ext for this gap
-  CHK(assemble_tiles(c, L, g.top, mext, tiles, 0, T1, T1, T1 + T2, info + 1));
-  CHK(gather(c, L, tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0));
-  CHK(assemble_tiles(c, L, g.top, mext, tiles, T1, T1 + T2, 0, T1, info + 1));
-  CHK(gather(c, L, tiles, npc, T1, T1, 0, g.top.rw, Eb1, E, 0, Eb1, &mext, Cm, m2, 0, 0, 0, nullptr, 0, 0, 0));
-  DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
+  CHK(assemble_cross(c, L, g, mext, Bm, Cm, info + 1));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
-  // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
-  const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-  CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2));   // B <- Y
-  CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
-  if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
-  else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
-  // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
-  CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1));
-  double2 *X = Bm;
-  DEV(c, hipEventRecord(L.ph[2], L.st));
-
-  // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
-  CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
-  {
+  if (prec == BEM_PREC_MIXED) {
+    CHK(solve_mixed(c, L, E1, E2, Bm, Cm, info, gro, trace_out, L.refine));
+    DEV(c, hipEventRecord(L.ph[2], L.st));
+  } else {
+    double2 *Sc, *B1, *partial, *invS;
+    int *ipS;
+    CHK(ws_get(c, L, "Sc", sizeof(double2) * (size_t)m2 * m2, (void **)&Sc));
+    CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
+    CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
+    CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
+    size_t np = bem_trace_partials(m1, m2);
+    CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
+    DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
+    // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
+    const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+    CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2));   // B <- Y
+    CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
+    if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
+    else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
+    // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
+    CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1));
+    double2 *X = Bm;
+    DEV(c, hipEventRecord(L.ph[2], L.st));
+    // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
+    CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
     EventRec *r;
     CHK(ev_begin(c, L, BEM_KS_ZTRACE, 8.0 * m1 * (double)m2 * m2,
                  16.0 * ((double)m1 * m2 + (double)m2 * m2 + (double)m1 * m2), &r));
@@ -899,7 +1094,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     CHK(ev_end(c, L, r));
   }
   DEV(c, hipEventRecord(L.ph[3], L.st));
-  DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, L.st));
   L.busy = true;
@@ -935,15 +1130,22 @@ static int lane_finish(bem_ctx *c, Lane &L) {
   (void)hipEventElapsedTime(&tp, L.ph[2], L.ph[3]);
   ev_collect(c, L);
   double omega = q->eV * SI_EV / SI_HBAR * q->a / SI_C;
-  double2 tr = *L.h_trace;
+  double2 tr = L.refine ? L.h_trace[1] : L.h_trace[0];
   res->omega = omega;
   res->ReS = 0.25 * tr.x;
   res->ImS = 0.25 * tr.y;
   res->theta1 = theta(omega, q->T1);
   res->theta2 = theta(omega, q->T2);
   res->P = res->ReS * (res->theta1 - res->theta2);
+  res->precision = L.prec;
   res->rel_err_est = 0;
   res->iters = 0;
+  if (L.prec == BEM_PREC_MIXED) {
+    // refined: the relative change of Re S from the complex64 result;
+    // complex64 alone: not estimated (-1)
+    res->rel_err_est = L.refine ? std::fabs(L.h_trace[1].x - L.h_trace[0].x) / std::fabs(L.h_trace[1].x) : -1.0;
+    res->iters = L.refine ? 1 : 0;
+  }
   res->ms_assemble = ta;
   res->ms_factor = tf;
   res->ms_products = tp;
@@ -970,23 +1172,20 @@ static int lane_finish(bem_ctx *c, Lane &L) {
 
 static int check_request(bem_ctx *c, const bem_request *q) {
   if (!(q->eV > 0) || !(q->a > 0)) return fail(c, BEM_E_INVALID, "eV and a must be positive");
-  if (q->precision != BEM_PREC_FP64) return fail(c, BEM_E_UNSUPPORTED, "precision mode %d not available", q->precision);
+  if (q->precision != BEM_PREC_FP64 && q->precision != BEM_PREC_MIXED)
+    return fail(c, BEM_E_UNSUPPORTED, "precision mode %d not available", q->precision);
   return 0;
 }
 
 
 // ------------------------------------------------------------------ serving
-// Bytes of a self entry of an n x n block in role 0 (factored) or 1 (raw).
-static size_t self_entry_bytes(int n, int role) {
-  size_t nn = sizeof(double2) * (size_t)n * n;
-  return 2 * nn + (role == 0 ? inv_bytes(n) + sizeof(int) * (size_t)(n + 64) : 0) + 256;
-}
-
 static bool self_cached(bem_ctx *c, const void *b, const double *eps, const double *mu, double omega, int role,
-                        int pivot) {
+                        int pivot, int prec) {
   const double key[5] = {eps[0], eps[1], mu[0], mu[1], omega};
   for (auto &e : c->selfs)
-    if (e.body == b && e.role == role && e.pivot == pivot && std::memcmp(e.key, key, sizeof key) == 0) return true;
+    if (e.body == b && e.role == role && e.pivot == pivot && e.prec == prec &&
+        std::memcmp(e.key, key, sizeof key) == 0)
+      return true;
   return false;
 }
 
@@ -998,11 +1197,12 @@ static size_t footprint(bem_ctx *c, const Lane &L, const Pending &p) {
   const bem_request &q = p.req;
   const double omega = q.eV * SI_EV / SI_HBAR * q.a / SI_C;
   const int pivot = (q.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
-  const bool h1 = self_cached(c, p.b1, q.eps1, q.mu1, omega, 0, pivot);
-  const bool h2 = self_cached(c, p.b2, q.eps2, q.mu2, omega, 1, 0);
-  size_t selfb = (h1 ? 0 : self_entry_bytes((int)m1, 0)) + (h2 ? 0 : self_entry_bytes((int)m2, 1));
+  const int prec = pivot ? BEM_PREC_FP64 : q.precision;
+  const bool h1 = self_cached(c, p.b1, q.eps1, q.mu1, omega, 0, pivot, prec);
+  const bool h2 = self_cached(c, p.b2, q.eps2, q.mu2, omega, 1, 0, prec);
+  size_t selfb = (h1 ? 0 : self_entry_bytes((int)m1, 0, prec)) + (h2 ? 0 : self_entry_bytes((int)m2, 1, prec));
   size_t pairs = std::max(T1 * T2, std::max(h1 ? 0 : T1 * T1, h2 ? 0 : T2 * T2));
-  size_t ws = sizeof(double2) * (3 * m1 * m2 + m2 * m2) + 8 * (size_t)BEM_TILE_PLANES * pairs +
+  size_t ws = sizeof(double2) * (3 * m1 * m2 + m2 * m2) * (prec == BEM_PREC_MIXED ? 3 : 1) + 8 * (size_t)BEM_TILE_PLANES * pairs +
               sizeof(int2) * std::min<size_t>(pairs, (size_t)1 << 24) + inv_bytes((int)m2) +
               sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_trace_partials((int)m1, (int)m2) + 1);
   return (ws > L.ws_bytes ? ws - L.ws_bytes : 0) + selfb;
@@ -1087,15 +1287,50 @@ static void finish_ticket(bem_ctx *c, uint64_t t, int status) {
   c->done_cv.notify_all();
 }
 
-// Admits queued requests in arrival order while a lane is free and the
-// request's footprint fits the budget (caller holds c->mu).
+// Estimated device milliseconds of a request: the dense solve's flops at a
+// nominal 50 TFLOP/s (the factorisation of A is skipped when it is cached).
+static double est_ms(bem_ctx *c, const Pending &p) {
+  const double m1 = 2.0 * p.b1->mesh.ne, m2 = 2.0 * p.b2->mesh.ne;
+  const bem_request &q = p.req;
+  const double omega = q.eV * SI_EV / SI_HBAR * q.a / SI_C;
+  const int pivot = (q.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  const int prec = pivot ? BEM_PREC_FP64 : q.precision;
+  double f = 8.0 * (2 * m1 * m1 * m2 + 3 * m1 * m2 * m2) + 8.0 / 3.0 * m2 * m2 * m2;
+  if (!self_cached(c, p.b1, q.eps1, q.mu1, omega, 0, pivot, prec)) f += 8.0 / 3.0 * m1 * m1 * m1;
+  if (prec == BEM_PREC_MIXED) f *= (q.flags & BEM_SOLVE_REFINE) ? 2.0 : 0.5;
+  return f / 50e12 * 1e3;
+}
+
+// Index of the queued request to admit next: the smallest estimated run time
+// minus time already waited (shortest job first with aging, so a long
+// request is admitted once it has waited about its own run time longer than
+// the shorter ones behind it).  Caller holds c->mu and c->qmu.
+static size_t pick_next(bem_ctx *c) {
+  auto now = std::chrono::steady_clock::now();
+  size_t best = 0;
+  double key = 0;
+  for (size_t i = 0; i < c->queue.size(); i++) {
+    const Pending &p = c->queue[i];
+    double k = est_ms(c, p) - ms_between(p.t_submit, now);
+    if (i == 0 || k < key) {
+      best = i;
+      key = k;
+    }
+  }
+  return best;
+}
+
+// Admits queued requests while a lane is free and the chosen request's
+// footprint fits the budget (caller holds c->mu).
 static void admit(bem_ctx *c) {
   for (;;) {
     Pending p;
+    size_t idx;
     {
       std::lock_guard<std::mutex> lk(c->qmu);
       if (c->queue.empty()) return;
-      p = c->queue.front();
+      idx = pick_next(c);
+      p = c->queue[idx];
     }
     Lane *L = nullptr;
     for (auto &l : c->lanes)
@@ -1117,7 +1352,7 @@ static void admit(bem_ctx *c) {
     }
     {
       std::lock_guard<std::mutex> lk(c->qmu);
-      c->queue.pop_front();
+      c->queue.erase(c->queue.begin() + idx);
       if (!e) c->running++;
     }
     if (e) {
@@ -1198,7 +1433,7 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     L.tag = "L" + std::to_string(i) + ":";
     bool ok = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) == hipSuccess;
     for (int k = 0; k < 4 && ok; k++) ok = hipEventCreate(&L.ph[k]) == hipSuccess;
-    ok = ok && hipHostMalloc((void **)&L.h_trace, sizeof(double2), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_trace, sizeof(double2) * 2, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_info, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_growth, sizeof(unsigned long long) * 4, hipHostMallocDefault) == hipSuccess;
     if (!ok) {
@@ -1563,6 +1798,21 @@ int bem_zgemm(bem_ctx *c, int m, int n, int k, const double *alpha, const void *
   c->t_last[6] = c->gemm_ms;
   c->t_last[7] = c->gemm_n;
   c->t_last[8] = c->gemm_flops;
+  return 0;
+}
+
+int bem_cgemm(bem_ctx *c, int m, int n, int k, const float *alpha, const void *A, int64_t lda, const void *B,
+              int64_t ldb, const float *beta, void *C, int64_t ldc) {
+  if (!c || !alpha || !beta || m < 0 || n < 0 || k < 0) return fail(c, BEM_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  stats_reset(c);
+  CHK(gemm(c, L, m, n, k, make_float2(alpha[0], alpha[1]), (const float2 *)A, lda, (const float2 *)B, ldb,
+           make_float2(beta[0], beta[1]), (float2 *)C, ldc));
+  DEV(c, hipStreamSynchronize(L.st));
+  ev_collect(c, L);
   return 0;
 }
 

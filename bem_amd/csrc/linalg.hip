@@ -933,6 +933,18 @@ __global__ void c2z_kernel(const float2 *__restrict__ src, int64_t lds, double2 
   }
 }
 
+// dst (complex128) <- dst + src (complex64)
+__global__ void c2z_add_kernel(const float2 *__restrict__ src, int64_t lds, double2 *__restrict__ dst, int64_t ldd,
+                               int m, int n) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int c = blockIdx.y; c < n; c += gridDim.y) {
+    float2 v = src[(int64_t)c * lds + r];
+    double2 o = dst[(int64_t)c * ldd + r];
+    dst[(int64_t)c * ldd + r] = make_double2(o.x + (double)v.x, o.y + (double)v.y);
+  }
+}
+
 // dst (complex64) <- dst + src (complex128) rounded: the refinement update.
 __global__ void add_z_to_c_kernel(const double2 *__restrict__ src, int64_t lds, float2 *__restrict__ dst, int64_t ldd,
                                   int m, int n) {
@@ -1168,6 +1180,12 @@ int bem_z2c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, in
 int bem_c2z(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
   hipLaunchKernelGGL(c2z_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(c2z_add_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
   return (int)hipGetLastError();
 }
 

@@ -27,6 +27,7 @@ BEM_PREC_FP64 = 0
 BEM_PREC_MIXED = 1
 BEM_ASM_DEVICE_PTR = 1
 BEM_SOLVE_TOURNAMENT = 1
+BEM_SOLVE_REFINE = 2
 
 # src/util/SI.cpp:3-9
 SI_HBAR = 1.05457148e-34
@@ -59,7 +60,7 @@ class _Res(ctypes.Structure):
                 ("rel_err_est", ctypes.c_double), ("iters", ctypes.c_int), ("status", ctypes.c_int),
                 ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double),
                 ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double), ("growth", ctypes.c_double),
-                ("pivoting", ctypes.c_int), ("ms_queue", ctypes.c_double)]
+                ("pivoting", ctypes.c_int), ("ms_queue", ctypes.c_double), ("precision", ctypes.c_int)]
 
 
 def lib_path():
@@ -73,7 +74,7 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
            "bem_sweep", "bem_zgetrf", "bem_zgetrs", "bem_zgemm", "bem_device_alloc", "bem_device_free",
            "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
-           "bem_submit", "bem_poll", "bem_queue_depth"]
+           "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm"]
 
 
 def load_library():
@@ -105,6 +106,7 @@ def load_library():
     L.bem_zgetrf.argtypes = [vp, vp, i64, i32, vp]
     L.bem_zgetrs.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32]
     L.bem_zgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
+    L.bem_cgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
     L.bem_device_alloc.argtypes = [vp, sz, P(vp)]
     L.bem_device_free.argtypes = [vp, vp]
     L.bem_memcpy_h2d.argtypes = [vp, vp, vp, sz]
@@ -332,6 +334,25 @@ class Context:
             al = np.array([complex(alpha).real, complex(alpha).imag])
             be = np.array([complex(beta).real, complex(beta).imag])
             self._check(self.lib.bem_zgemm(self.h, m, n, k, al.ctypes.data, dA, m, dB, k, be.ctypes.data, dC, m))
+            out = np.zeros_like(C)
+            self._check(self.lib.bem_memcpy_d2h(self.h, out.ctypes.data, dC, out.nbytes))
+            return out
+        finally:
+            for p in (dA, dB, dC):
+                self.lib.bem_device_free(self.h, p)
+
+    def cgemm_host(self, A, B, C=None, alpha=1.0, beta=0.0):
+        """complex64 alpha A B + beta C on the device (bem_cgemm)."""
+        A = np.asfortranarray(A, dtype=np.complex64)
+        B = np.asfortranarray(B, dtype=np.complex64)
+        m, k = A.shape
+        n = B.shape[1]
+        C = np.zeros((m, n), dtype=np.complex64, order="F") if C is None else np.asfortranarray(C, dtype=np.complex64)
+        dA, dB, dC = self._dev(A), self._dev(B), self._dev(C)
+        try:
+            al = np.array([complex(alpha).real, complex(alpha).imag], dtype=np.float32)
+            be = np.array([complex(beta).real, complex(beta).imag], dtype=np.float32)
+            self._check(self.lib.bem_cgemm(self.h, m, n, k, al.ctypes.data, dA, m, dB, k, be.ctypes.data, dC, m))
             out = np.zeros_like(C)
             self._check(self.lib.bem_memcpy_d2h(self.h, out.ctypes.data, dC, out.nbytes))
             return out

@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 ZGEMM_KERNEL = "zgemm_kernel<32, 64, 2, 2, 16, 2, false, true>"
+PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32 MFMA
+# --precision -> (bem_request.precision, flags): BEM_PREC_FP64 = 0,
+# BEM_PREC_MIXED = 1, BEM_SOLVE_REFINE = 2
+PRECISIONS = {"fp64": (0, 0), "mixed": (1, 0), "mixed-refine": (1, 2)}
 
 
 def workload(name):
@@ -168,6 +172,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--lanes", type=int, default=3, help="solves kept in flight (one HIP stream each)")
     ap.add_argument("--check", action="store_true", help="compare against stored oracle values")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed-refine"],
+                    help="fp64: complex128 throughout; mixed: complex64 elimination on FP32 MFMA; "
+                         "mixed-refine: plus one FP64 refinement step")
     ap.add_argument("--requests", type=int, default=64, help="c5: number of requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate (requests/s)")
     args = ap.parse_args()
@@ -190,8 +197,11 @@ def main():
     freqs_w = pick_freqs(wl["freqs"], args.warmup, offset=1)
     freqs = pick_freqs(wl["freqs"], args.steps)
 
+    prec, flags = PRECISIONS[args.precision]
+
     def req(f):
-        return Request(eV=f, gap=wl["gap"], eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f))
+        return Request(eV=f, gap=wl["gap"], eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f),
+                       precision=prec, flags=flags)
 
     reqs = [req(f) for f in freqs]
     if freqs_w:
@@ -208,6 +218,7 @@ def main():
     K = len(reqs)
     value = K / elapsed
     zg, far, close, gat = ks["zgemm"], ks["far"], ks["close"], ks["gather"]
+    cg = ks["cgemm"]
 
     def rate(k, scale):
         return k["flops"] / (k["ms"] * 1e-3) / scale if k["ms"] > 0 else 0.0
@@ -230,9 +241,9 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64 assembly + c64 elimination",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "precision": "fp64 (complex128 throughout)", "lanes": args.lanes,
+        "config": {"workload": wl["name"], "precision": args.precision, "lanes": args.lanes,
                    "freqs_eV": [round(f, 6) for f in freqs], "T1": 300.0, "T2": 0.0,
                    "permittivity": "6.7(w^2-0.120^2+i0.00059w)/(w^2-0.0983^2+i0.00059w)"},
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
@@ -246,6 +257,9 @@ def main():
                      "algorithmic_bytes_per_launch": zg["bytes"] / max(zg["launches"], 1),
                      "launches": zg["launches"], "avg_launch_us": zg["ms"] * 1e3 / max(zg["launches"], 1),
                      "flops": zg["flops"], "kernel_ms": zg["ms"]},
+        "cgemm_roofline": {"bound": "mfma", "kernel": "cgemm_kernel (complex64, FP32 MFMA)",
+                           "achieved": rate(cg, 1e12), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                           "frac": rate(cg, 1e12) / PEAK_FP32_TFLOPS, "launches": cg["launches"]},
         "assembly_roofline": {"bound": "fp64-valu", "kernel": "far_g6_kernel",
                               "how": "8308 flops per 6-point far pair (transcendentals = 1 flop) / HIP-event durations",
                               "achieved": rate(far, 1e12), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
@@ -257,7 +271,8 @@ def main():
                                                "launches": close["launches"]},
                               "gather_kernel": {"bound": "hbm", "achieved_GBs": gat["bytes"] / (gat["ms"] * 1e-3) / 1e9 if gat["ms"] else 0.0,
                                                 "peak_GBs": PEAK_HBM_GBS, "launches": gat["launches"]}},
-        "results": [{"eV": q.eV, "P": r["P"], "ReS": r["ReS"]} for q, r in zip(reqs, results)],
+        "results": [{"eV": q.eV, "P": r["P"], "ReS": r["ReS"], "rel_err_est": r["rel_err_est"]}
+                    for q, r in zip(reqs, results)],
     }
     if args.check:
         ref_path = os.path.join(ROOT, "tests", "golden", f"{args.workload}_oracle.json")
@@ -269,6 +284,13 @@ def main():
                 if key in ref:
                     errs.append(abs(r["ReS"] - ref[key]["ReS"]) / abs(ref[key]["ReS"]))
             out["rel_err_vs_fp64_oracle"] = max(errs) if errs else None
+    if args.precision != "fp64":
+        # error of every result against the FP64 path on the same requests
+        # (outside the timed region; the FP64 path meets 1e-8 vs the oracle)
+        ref64 = ctx.sweep(b1, b2, [Request(eV=q.eV, gap=q.gap, eps1=q.eps1, eps2=q.eps2) for q in reqs])
+        errs = [abs(r["ReS"] - f["ReS"]) / abs(f["ReS"]) for r, f in zip(results, ref64)]
+        out["rel_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs)),
+                                  "per_result": [float(e) for e in errs]}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, freqs[0])
     ctx.close()

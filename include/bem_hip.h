@@ -48,8 +48,9 @@ extern "C" {
 #define BEM_E_UNSUPPORTED (-6) /* size or mode not supported */
 #define BEM_PENDING 1          /* bem_poll: the request is queued or running */
 
-#define BEM_PREC_FP64 0        /* complex128 factorisation */
-#define BEM_PREC_MIXED 1       /* complex64 factorisation + FP64 refinement */
+#define BEM_PREC_FP64 0        /* complex128 throughout */
+#define BEM_PREC_MIXED 1       /* FP64 assembly; complex64 elimination and products on FP32 MFMA
+                                  (FP64 trace sum); FP64 refinement with BEM_SOLVE_REFINE */
 
 typedef struct bem_ctx bem_ctx;
 typedef struct bem_body bem_body;
@@ -81,19 +82,27 @@ typedef struct {
  * columns.  BEM_SOLVE_TOURNAMENT requests the latter from the start. */
 #define BEM_SOLVE_TOURNAMENT 1
 #define BEM_GROWTH_LIMIT 1e4
+/* BEM_PREC_MIXED only: one step of iterative refinement -- the residual of
+ * W [X; Sc^-1] = [0; I] against the FP64-assembled blocks in FP64, the
+ * correction through the complex64 factors, and FP64 products and trace of
+ * the corrected X.  rel_err_est then reports the relative change of Re S
+ * from the complex64 result.  A mixed request that needs tournament
+ * pivoting (growth above the limit) is solved on the FP64 path. */
+#define BEM_SOLVE_REFINE 2
 
 typedef struct {
   double P;          /* Re(S) (Theta(omega,T1) - Theta(omega,T2)) */
   double ReS, ImS;   /* S = 1/4 tr(B2^H B1) */
   double theta1, theta2;
   double omega;      /* dimensionless omega a / c */
-  double rel_err_est;/* refinement residual estimate (mixed), 0 for FP64 */
-  int iters;         /* refinement iterations (mixed), 0 for FP64 */
+  double rel_err_est;/* mixed + refine: |Re S - Re S_complex64| / |Re S|; mixed alone: -1; FP64: 0 */
+  int iters;         /* refinement steps (mixed), 0 for FP64 */
   int status;        /* 0 or BEM_E_* for this request */
   double ms_assemble, ms_factor, ms_products, ms_total;  /* device-event times */
   double growth;     /* max |U| / max |A| over the two LU factorisations */
   int pivoting;      /* 0 block-local, 1 tournament */
   double ms_queue;   /* served requests: submit to issue (ms); ms_total is then submit to result */
+  int precision;     /* BEM_PREC_* the result was computed in */
 } bem_result;
 
 /* ---- context ---------------------------------------------------------- */
@@ -137,8 +146,9 @@ int bem_sweep(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *reqs,
 
 /* ---- serving: continuous batching ------------------------------------ */
 /* Queues one request on the body pair and returns at once with a ticket.  A
- * scheduler thread of the context admits queued requests in arrival order
- * onto free lanes (cfg.lanes streams) whenever the request's device footprint
+ * scheduler thread of the context admits queued requests -- shortest
+ * estimated run time first, less the time already waited -- onto free lanes
+ * (cfg.lanes streams) whenever the request's device footprint
  * -- lane workspaces plus self blocks not yet cached -- fits cfg.hbm_budget
  * (evicting unused cached blocks and idle workspaces first), so requests that
  * arrive while others run join them on the device as lanes and memory free
@@ -164,6 +174,11 @@ int bem_zgetrs(bem_ctx *ctx, const void *A, int64_t lda, int n, const int32_t *i
 /* C <- alpha A B + beta C (no transposes), alpha/beta as (re, im). */
 int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void *A, int64_t lda, const void *B,
               int64_t ldb, const double *beta, void *C, int64_t ldc);
+/* complex64 C <- alpha A B + beta C on FP32 MFMA (alpha/beta as (re, im)
+ * floats): the kernel of the mixed-precision elimination.  No reference
+ * counterpart (the reference binds only complex128 BLAS). */
+int bem_cgemm(bem_ctx *ctx, int m, int n, int k, const float *alpha, const void *A, int64_t lda, const void *B,
+              int64_t ldb, const float *beta, void *C, int64_t ldc);
 /* Device memory helpers for callers without their own allocator. */
 int bem_device_alloc(bem_ctx *ctx, size_t bytes, void **ptr);
 int bem_device_free(bem_ctx *ctx, void *ptr);
