@@ -77,6 +77,80 @@ __device__ __forceinline__ void quad_point(const TriGeom &T, int q, double *v) {
   }
 }
 
+// exp(a) (cos b + i sin b) for the Helmholtz kernel exp(ikR), a = -Im(k) R,
+// b = Re(k) R, written from the series: Cody-Waite reduction (Cody & Waite,
+// Software Manual for the Elementary Functions, 1980) of b by pi/2 and of a
+// by ln 2 with two-part constants (32 significant bits in the leading part,
+// so n times it is exact for |n| < 2^20), then Taylor polynomials on the
+// reduced ranges (|r| <= pi/4: sin to r^17, cos to r^16; |t| <= ln2/2: exp to
+// t^13; truncation below 1e-17 relative).  A lossless medium (a == 0 for
+// every lane of the launch) skips the exponential.
+__device__ __forceinline__ void cexp_i(double a, double b, double &er, double &ei) {
+  const double n = rint(b * 0.6366197723675814);
+  const double r = fma(-n, 6.077100506506192e-11, fma(-n, 1.5707963267341256, b));
+  const double r2 = r * r;
+  double q = 2.8114572543455206e-15;
+  q = fma(q, r2, -7.647163731819816e-13);
+  q = fma(q, r2, 1.6059043836821613e-10);
+  q = fma(q, r2, -2.505210838544172e-08);
+  q = fma(q, r2, 2.7557319223985893e-06);
+  q = fma(q, r2, -0.0001984126984126984);
+  q = fma(q, r2, 0.008333333333333333);
+  q = fma(q, r2, -0.16666666666666666);
+  const double sr = fma(r * r2, q, r);
+  double c = 4.779477332387385e-14;
+  c = fma(c, r2, -1.1470745597729725e-11);
+  c = fma(c, r2, 2.08767569878681e-09);
+  c = fma(c, r2, -2.755731922398589e-07);
+  c = fma(c, r2, 2.48015873015873e-05);
+  c = fma(c, r2, -0.001388888888888889);
+  c = fma(c, r2, 0.041666666666666664);
+  c = fma(c, r2, -0.5);
+  const double cr = fma(c, r2, 1.0);
+  const int k = (int)n & 3;
+  double sb = (k & 1) ? cr : sr, cb = (k & 1) ? sr : cr;
+  if (k == 1 || k == 2) cb = -cb;
+  if (k >= 2) sb = -sb;
+  double ex = 1.0;
+  if (a != 0.0) {
+    const double m = rint(a * 1.4426950408889634);
+    const double t = fma(-m, 1.9082149292705877e-10, fma(-m, 0.6931471803691238, a));
+    double p = 1.6059043836821613e-10;
+    p = fma(p, t, 2.08767569878681e-09);
+    p = fma(p, t, 2.505210838544172e-08);
+    p = fma(p, t, 2.755731922398589e-07);
+    p = fma(p, t, 2.7557319223985893e-06);
+    p = fma(p, t, 2.48015873015873e-05);
+    p = fma(p, t, 0.0001984126984126984);
+    p = fma(p, t, 0.001388888888888889);
+    p = fma(p, t, 0.008333333333333333);
+    p = fma(p, t, 0.041666666666666664);
+    p = fma(p, t, 0.16666666666666666);
+    p = fma(p, t, 0.5);
+    p = fma(p, t, 1.0);
+    ex = ldexp(fma(p, t, 1.0), (int)m);
+  }
+  er = ex * cb;
+  ei = ex * sb;
+}
+
+// R = sqrt(x) and 1/R for a normal x > 0: the v_rsq_f64 estimate refined by
+// one coupled Newton step for (sqrt x, 1/(2 sqrt x)) (Markstein, IA-64 and
+// Elementary Functions, 2000), a second step on the reciprocal and a Newton
+// correction of the root.
+__device__ __forceinline__ void root_and_inverse(double x, double &R, double &invR) {
+  const double y = __builtin_amdgcn_rsq(x);
+  double g = x * y, h = 0.5 * y;
+  double r = fma(-g, h, 0.5);
+  g = fma(g, r, g);
+  h = fma(h, r, h);
+  r = fma(-g, h, 0.5);
+  h = fma(h, r, h);
+  const double d = fma(-g, g, x);
+  R = fma(d, h, g);
+  invR = 2.0 * h;
+}
+
 // Inner moments (see file comment).
 struct Inner {
   C2 G0;
@@ -97,13 +171,11 @@ __device__ __forceinline__ void inner_point(const Medium &md, const double *r, c
                                             double w, Inner &m) {
   double d[3] = {r[0] - rp[0], r[1] - rp[1], r[2] - rp[2]};
   double R2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
-  double R = sqrt(R2);
+  double R, invR;
+  root_and_inverse(R2, R, invR);
   double ar = md.ik_re * R, ai = md.ik_im * R;  // ikR
-  double ex = exp(ar);
-  double sn, cs;
-  sincos(ai, &sn, &cs);
-  double er = ex * cs, ei = ex * sn;            // exp(ikR)
-  double invR = 1.0 / R;
+  double er, ei;
+  cexp_i(ar, ai, er, ei);                       // exp(ikR)
   C2 g, q;
   if (!SMOOTH) {
     g = cmk(er * invR, ei * invR);
@@ -580,49 +652,63 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
 // Zcopy (optional) receives the same values (the separate G_b matrices).
 // Rows are RWGs [r_lo, r_hi) of the tree; the tile buffer holds test rows
 // [row_lo, ..) x source columns [col_lo, col_lo + tld) of triangles.
-__global__ void gather_kernel(const double *__restrict__ tiles, size_t npairs, int tld, int row_lo, int col_lo,
-                              const RwgRec *__restrict__ rw, int r_lo, int r_hi, int c_lo, int c_hi, Medium md,
-                              double2 *__restrict__ Z, int64_t ldz, int zr0, int zc0, int add,
-                              double2 *__restrict__ Zcopy, int64_t ldc, int cr0, int cc0) {
-  int i = r_lo + blockIdx.x * blockDim.x + threadIdx.x;
-  int j = c_lo + blockIdx.y;
-  if (i >= r_hi || j >= c_hi) return;
-  RwgRec ri = rw[i], rj = rw[j];
-  double Lr = 0, Li = 0, Kr = 0, Ki = 0;
-  int ta[2] = {ri.tp, ri.tn}, sa[2] = {ri.sp, ri.sn};
-  int tb[2] = {rj.tp, rj.tn}, sb[2] = {rj.sp, rj.sn};
-  for (int hi = 0; hi < 2; hi++)
-    for (int hj = 0; hj < 2; hj++) {
-      double sg = (hi == hj) ? 1.0 : -1.0;
-      size_t idx = (size_t)(ta[hi] - row_lo) * (size_t)tld + (size_t)(tb[hj] - col_lo);
-      int q = 3 * sa[hi] + sb[hj];
-      Lr += sg * tiles[(size_t)(2 * q) * npairs + idx];
-      Li += sg * tiles[(size_t)(2 * q + 1) * npairs + idx];
-      Kr += sg * tiles[(size_t)(18 + 2 * q) * npairs + idx];
-      Ki += sg * tiles[(size_t)(18 + 2 * q + 1) * npairs + idx];
+#define GATHER_GI 32   // test RWGs (rows) per block
+#define GATHER_GJ 16   // source RWGs (columns) per block
+__global__ void __launch_bounds__(256) gather_kernel(const double *__restrict__ tiles, size_t npairs, int tld,
+                                                     int row_lo, int col_lo, const RwgRec *__restrict__ rw, int r_lo,
+                                                     int r_hi, int c_lo, int c_hi, Medium md,
+                                                     double2 *__restrict__ Z, int64_t ldz, int zr0, int zc0, int add,
+                                                     double2 *__restrict__ Zcopy, int64_t ldc, int cr0, int cc0) {
+  // The block's 2GI x 2GJ output goes through LDS: lanes first run along
+  // source RWGs (neighbouring tile columns, so the tile planes are read
+  // nearly contiguously), then along output rows (coalesced column writes).
+  __shared__ double2 sz[2 * GATHER_GJ][2 * GATHER_GI + 1];
+  const int i0 = r_lo + blockIdx.x * GATHER_GI, j0 = c_lo + blockIdx.y * GATHER_GJ;
+  const int t = threadIdx.x, jl = t % GATHER_GJ;
+  const int j = j0 + jl;
+  if (j < c_hi) {
+    const RwgRec rj = rw[j];
+    const int tb[2] = {rj.tp, rj.tn}, sb[2] = {rj.sp, rj.sn};
+    for (int il = t / GATHER_GJ; il < GATHER_GI; il += 256 / GATHER_GJ) {
+      const int i = i0 + il;
+      if (i >= r_hi) break;
+      const RwgRec ri = rw[i];
+      const int ta[2] = {ri.tp, ri.tn}, sa[2] = {ri.sp, ri.sn};
+      double Lr = 0, Li = 0, Kr = 0, Ki = 0;
+      for (int hi = 0; hi < 2; hi++)
+        for (int hj = 0; hj < 2; hj++) {
+          double sg = (hi == hj) ? 1.0 : -1.0;
+          size_t idx = (size_t)(ta[hi] - row_lo) * (size_t)tld + (size_t)(tb[hj] - col_lo);
+          int q = 3 * sa[hi] + sb[hj];
+          Lr += sg * tiles[(size_t)(2 * q) * npairs + idx];
+          Li += sg * tiles[(size_t)(2 * q + 1) * npairs + idx];
+          Kr += sg * tiles[(size_t)(18 + 2 * q) * npairs + idx];
+          Ki += sg * tiles[(size_t)(18 + 2 * q + 1) * npairs + idx];
+        }
+      double c = BEM_ONEOVER4PI * ri.len * rj.len;
+      Lr *= c; Li *= c; Kr *= c; Ki *= c;
+      sz[2 * jl][2 * il] = make_double2(md.Zeps_re * Lr - md.Zeps_im * Li, md.Zeps_re * Li + md.Zeps_im * Lr);
+      sz[2 * jl][2 * il + 1] = make_double2(-Kr, -Ki);
+      sz[2 * jl + 1][2 * il] = make_double2(Kr, Ki);
+      sz[2 * jl + 1][2 * il + 1] = make_double2(md.Zmu_re * Lr - md.Zmu_im * Li, md.Zmu_re * Li + md.Zmu_im * Lr);
     }
-  double c = BEM_ONEOVER4PI * ri.len * rj.len;
-  Lr *= c; Li *= c; Kr *= c; Ki *= c;
-  double2 zee = make_double2(md.Zeps_re * Lr - md.Zeps_im * Li, md.Zeps_re * Li + md.Zeps_im * Lr);
-  double2 zmm = make_double2(md.Zmu_re * Lr - md.Zmu_im * Li, md.Zmu_re * Li + md.Zmu_im * Lr);
-  double2 kp = make_double2(Kr, Ki), km = make_double2(-Kr, -Ki);
-  int64_t zi = zr0 + 2 * (int64_t)(i - r_lo), zj = zc0 + 2 * (int64_t)(j - c_lo);
-  double2 *z0 = Z + zj * ldz + zi;        // column 2j
-  double2 *z1 = Z + (zj + 1) * ldz + zi;  // column 2j+1
-  if (add) {
-    double2 t;
-    t = z0[0]; z0[0] = make_double2(t.x + zee.x, t.y + zee.y);
-    t = z0[1]; z0[1] = make_double2(t.x + km.x, t.y + km.y);
-    t = z1[0]; z1[0] = make_double2(t.x + kp.x, t.y + kp.y);
-    t = z1[1]; z1[1] = make_double2(t.x + zmm.x, t.y + zmm.y);
-  } else {
-    z0[0] = zee; z0[1] = km; z1[0] = kp; z1[1] = zmm;
   }
-  if (Zcopy) {
-    int64_t ci = cr0 + 2 * (int64_t)(i - r_lo), cj = cc0 + 2 * (int64_t)(j - c_lo);
-    double2 *y0 = Zcopy + cj * ldc + ci;
-    double2 *y1 = Zcopy + (cj + 1) * ldc + ci;
-    y0[0] = zee; y0[1] = km; y1[0] = kp; y1[1] = zmm;
+  __syncthreads();
+  const int rows = 2 * min(GATHER_GI, r_hi - i0), cols = 2 * min(GATHER_GJ, c_hi - j0);
+  const int r = t % (2 * GATHER_GI);
+  if (r >= rows) return;
+  const int64_t zi = zr0 + 2 * (int64_t)(i0 - r_lo) + r, zc = zc0 + 2 * (int64_t)(j0 - c_lo);
+  const int64_t yi = cr0 + 2 * (int64_t)(i0 - r_lo) + r, yc = cc0 + 2 * (int64_t)(j0 - c_lo);
+  for (int cc = t / (2 * GATHER_GI); cc < cols; cc += 256 / (2 * GATHER_GI)) {
+    double2 v = sz[cc][r];
+    double2 *p = Z + (zc + cc) * ldz + zi;
+    if (add) {
+      double2 o = *p;
+      *p = make_double2(o.x + v.x, o.y + v.y);
+    } else {
+      *p = v;
+    }
+    if (Zcopy) Zcopy[(yc + cc) * ldc + yi] = v;
   }
 }
 
@@ -672,8 +758,8 @@ int bem_launch_gather(const double *tiles, size_t npairs, int tld, int row_lo, i
                       int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0,
                       int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st) {
   if (r_hi <= r_lo || c_hi <= c_lo) return 0;
-  dim3 grid((r_hi - r_lo + 127) / 128, c_hi - c_lo);
-  hipLaunchKernelGGL(gather_kernel, grid, dim3(128), 0, st, tiles, npairs, tld, row_lo, col_lo, rw, r_lo, r_hi, c_lo,
+  dim3 grid((r_hi - r_lo + GATHER_GI - 1) / GATHER_GI, (c_hi - c_lo + GATHER_GJ - 1) / GATHER_GJ);
+  hipLaunchKernelGGL(gather_kernel, grid, dim3(256), 0, st, tiles, npairs, tld, row_lo, col_lo, rw, r_lo, r_hi, c_lo,
                      c_hi, *md, Z, ldz, zr0, zc0, add, Zcopy, ldc, cr0, cc0);
   return (int)hipGetLastError();
 }
