@@ -133,9 +133,9 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
         for (int x = 0; x < FM; x++)
 #pragma unroll
           for (int y = 0; y < FN; y++) {
-            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc[0][x][y], 0, 0, 0);
-            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], bi[y], acc[1][x][y], 0, 0, 0);
-            acc[2][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(as[x], bs[y], acc[2][x][y], 0, 0, 0);
+            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[y], ar[x], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[y], ai[x], acc[1][x][y], 0, 0, 0);
+            acc[2][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bs[y], as[x], acc[2][x][y], 0, 0, 0);
           }
       } else {
         double nai[FM];
@@ -145,15 +145,15 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
         for (int x = 0; x < FM; x++)
 #pragma unroll
           for (int y = 0; y < FN; y++) {
-            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc[0][x][y], 0, 0, 0);
-            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acc[1][x][y], 0, 0, 0);
+            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[y], ar[x], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[y], ar[x], acc[1][x][y], 0, 0, 0);
           }
 #pragma unroll
         for (int x = 0; x < FM; x++)
 #pragma unroll
           for (int y = 0; y < FN; y++) {
-            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai[x], bi[y], acc[0][x][y], 0, 0, 0);
-            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acc[1][x][y], 0, 0, 0);
+            acc[0][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(bi[y], nai[x], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(br[y], ai[x], acc[1][x][y], 0, 0, 0);
           }
       }
     }
@@ -170,7 +170,11 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
     }
   };
   // epilogue: for v_mfma_f64_16x16x4f64, D[i][j] of a 16x16 block sits in
-  // lane j + 16*(i%4), element i/4 (measured on gfx950; the f32 variant differs)
+  // lane j + 16*(i%4), element i/4 (measured on gfx950; the f32 variant
+  // differs).  The products above run with the B tile as the instruction's
+  // first operand, so D is the transposed block: C[i][j] sits in lane
+  // i + 16*(j%4), element j/4, and lanes run down a column of C (coalesced
+  // column-major loads and stores).
   if (!TRACE) {
 #pragma unroll
     for (int x = 0; x < FM; x++)
@@ -178,8 +182,8 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
       for (int y = 0; y < FN; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + 4 * r + lk;
-          int col = n0 + wc * (BN / WN) + y * 16 + li;
+          int row = m0 + wr * (BM / WM) + x * 16 + li;
+          int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
           if (row < m && col < n) {
             double cr, ci;
             result(x, y, r, cr, ci);
@@ -203,8 +207,8 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
       for (int y = 0; y < FN; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + 4 * r + lk;
-          int col = n0 + wc * (BN / WN) + y * 16 + li;
+          int row = m0 + wr * (BM / WM) + x * 16 + li;
+          int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
           if (row < m && col < n) {
             double cr, ci;
             result(x, y, r, cr, ci);
@@ -840,20 +844,22 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
       for (int x = 0; x < FM; x++)
 #pragma unroll
         for (int y = 0; y < FN; y++) {
-          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], br[y], acc[0][x][y], 0, 0, 0);
-          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[x], bi[y], acc[1][x][y], 0, 0, 0);
+          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ar[x], acc[0][x][y], 0, 0, 0);
+          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], ar[x], acc[1][x][y], 0, 0, 0);
         }
 #pragma unroll
       for (int x = 0; x < FM; x++)
 #pragma unroll
         for (int y = 0; y < FN; y++) {
-          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(nai[x], bi[y], acc[0][x][y], 0, 0, 0);
-          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[x], br[y], acc[1][x][y], 0, 0, 0);
+          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], nai[x], acc[0][x][y], 0, 0, 0);
+          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ai[x], acc[1][x][y], 0, 0, 0);
         }
     }
   }
   // v_mfma_f32_16x16x4f32 output: D[i][j] of a 16 x 16 block in lane
-  // j + 16 (i / 4), element i % 4
+  // j + 16 (i / 4), element i % 4; with the B tile as the first operand D is
+  // the transposed block, so C[i][j] sits in lane i + 16 (j / 4), element
+  // j % 4 and lanes run down a column of C (coalesced)
   if (!TRACE) {
 #pragma unroll
     for (int x = 0; x < FM; x++)
@@ -861,8 +867,8 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
       for (int y = 0; y < FN; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + 4 * lk + r;
-          int col = n0 + wc * (BN / WN) + y * 16 + li;
+          int row = m0 + wr * (BM / WM) + x * 16 + li;
+          int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
           if (row < m && col < n) {
             float cr = acc[0][x][y][r], ci = acc[1][x][y][r];
             float vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
@@ -885,8 +891,8 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
       for (int y = 0; y < FN; y++)
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + 4 * lk + r;
-          int col = n0 + wc * (BN / WN) + y * 16 + li;
+          int row = m0 + wr * (BM / WM) + x * 16 + li;
+          int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
           if (row < m && col < n) {
             double cr = acc[0][x][y][r], ci = acc[1][x][y][r];
             double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
