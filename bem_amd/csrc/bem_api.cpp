@@ -56,7 +56,12 @@ int bem_launch_gather(const double *tiles, size_t npairs, int tld, int row_lo, i
                       int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0,
                       int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st);
 int bem_launch_q36_check(const unsigned int *count, unsigned int cap, int *flag, hipStream_t st);
-int bem_zgemm_wide(int m, int n);
+int bem_zgemm_tiled(int tile, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B,
+                    int64_t ldb, double2 beta, double2 *C, int64_t ldc, hipStream_t st);
+int bem_zgemm_tile_valid(int tile);
+int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B,
+                    int64_t ldb, float2 beta, float2 *C, int64_t ldc, hipStream_t st);
+int bem_cgemm_tile_valid(int tile);
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                   double2 beta, double2 *C, int64_t ldc, hipStream_t st);
 int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
@@ -212,6 +217,8 @@ struct bem_ctx {
   size_t self_bytes = 0, self_cap = 0;
   uint64_t clock = 0;
   bool profile = true;   // bracket timed kernel classes with events
+  // GEMM tile per shape class (gemm_class), 0 = the built-in rule
+  int ztile[3] = {0, 0, 0}, ctile[3] = {0, 0, 0};
   // statistics of the last API call
   double t_last[9] = {0};
   double gemm_ms = 0, gemm_n = 0, gemm_flops = 0;
@@ -352,16 +359,22 @@ static void stats_reset(bem_ctx *c) {
   std::memset(c->kst, 0, sizeof c->kst);
 }
 
+// Shape class of a GEMM for the tile tables: 0 narrow (an output dimension
+// of at most 128: panel products), 1 rank update (k <= 128: LU trailing
+// updates and blocked triangular solves), 2 general.
+static int gemm_class(int m, int n, int k) { return (m <= 128 || n <= 128) ? 0 : (k <= 128 ? 1 : 2); }
+
 // GEMM on the lane's stream (bracketed by events when profiling).
 static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda,
                 const double2 *B, int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
   if (m <= 0 || n <= 0) return 0;
   EventRec *r;
-  const int kind = bem_zgemm_wide(m, n) ? BEM_KS_ZGEMM : BEM_KS_ZGEMM_NARROW;
+  const int cls = gemm_class(m, n, k);
+  const int kind = cls == 0 ? BEM_KS_ZGEMM_NARROW : BEM_KS_ZGEMM;
   const bool rd_c = beta.x != 0.0 || beta.y != 0.0;
   CHK(ev_begin(c, L, kind, 8.0 * m * (double)n * k,
                16.0 * ((double)m * k + (double)k * n + (rd_c ? 2.0 : 1.0) * (double)m * n), &r));
-  int e = bem_zgemm_raw(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
+  int e = bem_zgemm_tiled(c->ztile[cls], m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "zgemm launch: %s", hipGetErrorString((hipError_t)e));
   return ev_end(c, L, r);
 }
@@ -389,7 +402,7 @@ static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, float2 alpha, const fl
   const bool rd_c = beta.x != 0.f || beta.y != 0.f;
   CHK(ev_begin(c, L, BEM_KS_CGEMM, 8.0 * m * (double)n * k,
                8.0 * ((double)m * k + (double)k * n + (rd_c ? 2.0 : 1.0) * (double)m * n), &r));
-  int e = bem_cgemm_raw(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
+  int e = bem_cgemm_tiled(c->ctile[gemm_class(m, n, k)], m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "cgemm launch: %s", hipGetErrorString((hipError_t)e));
   return ev_end(c, L, r);
 }
@@ -1405,6 +1418,84 @@ static void sched_main(bem_ctx *c) {
   }
 }
 
+
+// ---------------------------------------------------------------- autotuner
+// Representative shape (m, n, k) of each GEMM class.
+static const int TUNE_SHAPE[3][3] = {{4096, 64, 64}, {4096, 4096, 64}, {2048, 2048, 2048}};
+static const int ZCANDS[] = {1032, 2032, 1033, 1034, 1064, 2064, 1065, 32, 64, 256};
+static const int CCANDS[] = {1, 2, 3, 4, 5};
+
+// Mean milliseconds of `reps` launches of one tile variant on lane 0.
+static int tune_time(bem_ctx *c, Lane &L, bool c64, int tile, int cls, void *A, void *B, void *Cm, int reps,
+                     double *ms) {
+  const int m = TUNE_SHAPE[cls][0], n = TUNE_SHAPE[cls][1], k = TUNE_SHAPE[cls][2];
+  hipEvent_t e0, e1;
+  DEV(c, hipEventCreate(&e0));
+  DEV(c, hipEventCreate(&e1));
+  int err = 0;
+  for (int rep = 0; rep <= reps && !err; rep++) {
+    if (rep == 1) err = (int)hipEventRecord(e0, L.st);
+    if (!err && c64)
+      err = bem_cgemm_tiled(tile, m, n, k, make_float2(1, 0), (const float2 *)A, m, (const float2 *)B, k,
+                            make_float2(1, 0), (float2 *)Cm, m, L.st);
+    else if (!err)
+      err = bem_zgemm_tiled(tile, m, n, k, make_double2(1, 0), (const double2 *)A, m, (const double2 *)B, k,
+                            make_double2(1, 0), (double2 *)Cm, m, L.st);
+  }
+  float t = 0;
+  if (!err) err = (int)hipEventRecord(e1, L.st);
+  if (!err) err = (int)hipEventSynchronize(e1);
+  if (!err) err = (int)hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err) return fail(c, BEM_E_DEVICE, "autotune launch of tile %d failed (%d)", tile, err);
+  *ms = t / reps;
+  return 0;
+}
+
+static bool tune_load(const char *path, const std::string &arch, int *tiles6) {
+  FILE *f = path ? std::fopen(path, "r") : nullptr;
+  if (!f) return false;
+  char buf[4096];
+  size_t n = std::fread(buf, 1, sizeof buf - 1, f);
+  std::fclose(f);
+  buf[n] = 0;
+  std::string t(buf);
+  size_t pa = t.find("\"arch\": \""), pz = t.find("\"zgemm\": ["), pc = t.find("\"cgemm\": [");
+  if (pa == std::string::npos || pz == std::string::npos || pc == std::string::npos) return false;
+  size_t qa = t.find('"', pa + 9);
+  if (qa == std::string::npos || t.substr(pa + 9, qa - pa - 9) != arch) return false;
+  int v[6];
+  if (std::sscanf(t.c_str() + pz + 10, "%d , %d , %d", &v[0], &v[1], &v[2]) != 3) return false;
+  if (std::sscanf(t.c_str() + pc + 10, "%d , %d , %d", &v[3], &v[4], &v[5]) != 3) return false;
+  for (int i = 0; i < 3; i++)
+    if (!bem_zgemm_tile_valid(v[i]) || !bem_cgemm_tile_valid(v[3 + i])) return false;
+  std::memcpy(tiles6, v, sizeof v);
+  return true;
+}
+
+static void tune_store(const char *path, const std::string &arch, const int *tiles6, const double (*ms)[16],
+                       const int *nz, const int *nc) {
+  FILE *f = path ? std::fopen(path, "w") : nullptr;
+  if (!f) return;
+  std::fprintf(f, "{\"arch\": \"%s\", \"zgemm\": [%d, %d, %d], \"cgemm\": [%d, %d, %d],\n", arch.c_str(), tiles6[0],
+               tiles6[1], tiles6[2], tiles6[3], tiles6[4], tiles6[5]);
+  std::fprintf(f, " \"classes\": [\"narrow 4096x64x64\", \"rank update 4096x4096x64\", \"general 2048^3\"],\n");
+  for (int c64 = 0; c64 < 2; c64++) {
+    std::fprintf(f, " \"%s_ms\": [", c64 ? "cgemm" : "zgemm");
+    for (int cls = 0; cls < 3; cls++) {
+      std::fprintf(f, "%s{", cls ? ", " : "");
+      const int nn = c64 ? *nc : *nz;
+      for (int i = 0; i < nn; i++)
+        std::fprintf(f, "%s\"%d\": %.4f", i ? ", " : "", c64 ? CCANDS[i] : ZCANDS[i], ms[3 * c64 + cls][i]);
+      std::fprintf(f, "}");
+    }
+    std::fprintf(f, "]%s\n", c64 ? "" : ",");
+  }
+  std::fprintf(f, "}\n");
+  std::fclose(f);
+}
+
 // ------------------------------------------------------------------- C ABI
 extern "C" {
 
@@ -1743,6 +1834,73 @@ int bem_queue_depth(bem_ctx *c, int *queued, int *running) {
   if (queued) *queued = (int)c->queue.size();
   if (running) *running = c->running;
   return 0;
+}
+
+int bem_gemm_tiles(bem_ctx *c, const int *set6, int *get6) {
+  if (!c) return BEM_E_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  if (set6) {
+    for (int i = 0; i < 3; i++)
+      if ((set6[i] && !bem_zgemm_tile_valid(set6[i])) || (set6[3 + i] && !bem_cgemm_tile_valid(set6[3 + i])))
+        return fail(c, BEM_E_INVALID, "unknown tile code");
+    for (auto &L : c->lanes) CHK(lane_retire(c, L));
+    for (int i = 0; i < 3; i++) {
+      c->ztile[i] = set6[i];
+      c->ctile[i] = set6[3 + i];
+    }
+  }
+  if (get6)
+    for (int i = 0; i < 3; i++) {
+      get6[i] = c->ztile[i];
+      get6[3 + i] = c->ctile[i];
+    }
+  return 0;
+}
+
+int bem_autotune(bem_ctx *c, const char *cache_path, int flags, int *tiles6) {
+  if (!c) return BEM_E_INVALID;
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  hipDeviceProp_t prop;
+  DEV(c, hipGetDeviceProperties(&prop, c->device));
+  const std::string arch = prop.gcnArchName;
+  int best[6];
+  if (!(flags & BEM_TUNE_FORCE) && tune_load(cache_path, arch, best)) {
+    CHK(bem_gemm_tiles(c, best, tiles6));
+    return 0;
+  }
+  Lane &L = c->lanes[0];
+  for (auto &l : c->lanes) CHK(lane_retire(c, l));
+  const size_t bytes = sizeof(double2) * (size_t)4096 * 4096;
+  void *A = nullptr, *B = nullptr, *Cm = nullptr;
+  int r = 0;
+  if (hipMalloc(&A, bytes) != hipSuccess || hipMalloc(&B, bytes) != hipSuccess || hipMalloc(&Cm, bytes) != hipSuccess)
+    r = fail(c, BEM_E_NOMEM, "autotune scratch (3 x %zu bytes)", bytes);
+  // operands: any finite values; zeros keep every variant's output finite
+  if (!r && (hipMemsetAsync(A, 0, bytes, L.st) || hipMemsetAsync(B, 0, bytes, L.st) ||
+             hipMemsetAsync(Cm, 0, bytes, L.st)))
+    r = fail(c, BEM_E_DEVICE, "autotune scratch fill");
+  double ms[6][16];
+  const int nz = (int)(sizeof ZCANDS / sizeof ZCANDS[0]), nc = (int)(sizeof CCANDS / sizeof CCANDS[0]);
+  for (int cls = 0; cls < 3 && !r; cls++) {
+    best[cls] = 0;
+    best[3 + cls] = 0;
+    double bz = 1e30, bc = 1e30;
+    for (int i = 0; i < nz && !r; i++) {
+      r = tune_time(c, L, false, ZCANDS[i], cls, A, B, Cm, 3, &ms[cls][i]);
+      if (!r && ms[cls][i] < bz) { bz = ms[cls][i]; best[cls] = ZCANDS[i]; }
+    }
+    for (int i = 0; i < nc && !r; i++) {
+      r = tune_time(c, L, true, CCANDS[i], cls, A, B, Cm, 3, &ms[3 + cls][i]);
+      if (!r && ms[3 + cls][i] < bc) { bc = ms[3 + cls][i]; best[3 + cls] = CCANDS[i]; }
+    }
+  }
+  (void)hipStreamSynchronize(L.st);
+  for (void *p : {A, B, Cm})
+    if (p) (void)hipFree(p);
+  if (r) return r;
+  tune_store(cache_path, arch, best, ms, &nz, &nc);
+  return bem_gemm_tiles(c, best, tiles6);
 }
 
 int bem_zgetrf(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv) {

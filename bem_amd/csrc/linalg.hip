@@ -995,32 +995,27 @@ void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t 
 // ------------------------------------------------------------------ launchers
 extern "C" {
 
-// Tile choice by shape (measured on gfx950, complex128, lda = 3840): the
-// three-product form on 32 x 64 tiles (four waves of 16 x 32, four waves per
-// SIMD) runs m = n = k = 3840 at 78 TFLOP/s counted as 8mnk (58.7 executed),
-// against 57.5 for the four-product 64 x 64 tile, and rank-64 updates at
-// 53-58 against 38-40; 32 x 32 tiles when either dimension is at most 128 so
-// skinny products still fill the chip.  bem_zgemm_set_tile forces a variant
-// for measurements (0 restores the shape rule): 32 and 64 are the
-// four-product 32 x 32 and 64 x 64 tiles, 256 the four-product 128 x 128 tile
-// on eight waves; 1032 / 2032 (k step 32) / 1033 (64 x 32) / 1034 (32 x 64) /
-// 1064 / 2064 (k step 32) / 1065 (one wave per SIMD) the three-product ones.
-static int g_zgemm_tile = 0;
-int bem_zgemm_set_tile(int t) {
-  g_zgemm_tile = t;
+// Tile variants (complex128): 32 / 64 are the four-product 32 x 32 and 64 x 64
+// tiles, 256 the four-product 128 x 128 tile on eight waves; the
+// three-product ones are 1032 (32 x 32), 2032 (32 x 32, k step 32), 1033
+// (64 x 32), 1034 (32 x 64), 1064 (64 x 64), 2064 (64 x 64, k step 32), 1065
+// (64 x 64, one wave per SIMD).  Default rule (measured on gfx950 at
+// lda = 3840): 1032 when either output dimension is at most 128, else 1034;
+// the context's autotuner may replace it per shape class.
+static const int ZTILES[] = {32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065};
+
+int bem_zgemm_default_tile(int m, int n) { return (m <= 128 || n <= 128) ? 1032 : 1034; }
+
+int bem_zgemm_tile_valid(int tile) {
+  for (int t : ZTILES)
+    if (t == tile) return 1;
   return 0;
 }
 
-// 1 when bem_zgemm_raw runs an (m x n) product on the wide 32 x 64 tiles.
-int bem_zgemm_wide(int m, int n) {
-  int tile = g_zgemm_tile ? g_zgemm_tile : ((m <= 128 || n <= 128) ? 1032 : 1034);
-  return tile == 1034;
-}
-
-int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
-                  double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
+int bem_zgemm_tiled(int tile, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B,
+                    int64_t ldb, double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
-  int tile = g_zgemm_tile ? g_zgemm_tile : ((m <= 128 || n <= 128) ? 1032 : 1034);
+  if (!tile) tile = bem_zgemm_default_tile(m, n);
   const double2 *E = nullptr;
   double2 *P = nullptr;
   switch (tile) {
@@ -1044,7 +1039,11 @@ int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t 
   return (int)hipGetLastError();
 }
 
-// Trace tiles are 32 x 64 (three-product form); partial must hold
+int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                  double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
+  return bem_zgemm_tiled(0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
+}
+
 // bem_trace_partials(m, n) entries; out receives sum conj(alpha A B) .* E.
 size_t bem_trace_partials(int m, int n) { return (size_t)((m + 31) / 32) * (size_t)((n + 63) / 64); }
 
@@ -1123,18 +1122,31 @@ int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
 
 
 // ------------------------------------------------------- complex64 (mixed)
-// C <- alpha A B + beta C in complex64: 64 x 64 tiles (2 x 2 waves of
-// 32 x 32), or 32 x 32 tiles when either dimension is at most 128.
+// C <- alpha A B + beta C in complex64.  Tile variants: 1 = 64 x 64, 2 = 32 x
+// 32, 3 = 64 x 128, 4 = 128 x 64, 5 = 128 x 128 (2 x 2 waves each).  Default
+// rule: 32 x 32 when either dimension is at most 128, else 64 x 64.
+int bem_cgemm_default_tile(int m, int n) { return (m <= 128 || n <= 128) ? 2 : 1; }
+int bem_cgemm_tile_valid(int tile) { return tile >= 1 && tile <= 5; }
 int bem_cgemm_wide(int m, int n) { return !(m <= 128 || n <= 128); }
+
+int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B,
+                    int64_t ldb, float2 beta, float2 *C, int64_t ldc, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  if (!tile) tile = bem_cgemm_default_tile(m, n);
+  switch (tile) {
+    case 1: launch_cgemm<64, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 2: launch_cgemm<32, 32, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 3: launch_cgemm<64, 128, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 4: launch_cgemm<128, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 5: launch_cgemm<128, 128, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    default: return -1;
+  }
+  return (int)hipGetLastError();
+}
 
 int bem_cgemm_raw(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
                   float2 beta, float2 *C, int64_t ldc, hipStream_t st) {
-  if (m <= 0 || n <= 0) return 0;
-  if (bem_cgemm_wide(m, n))
-    launch_cgemm<64, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st);
-  else
-    launch_cgemm<32, 32, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st);
-  return (int)hipGetLastError();
+  return bem_cgemm_tiled(0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
 }
 
 size_t bem_ctrace_partials(int m, int n) { return (size_t)((m + 63) / 64) * (size_t)((n + 63) / 64); }

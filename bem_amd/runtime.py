@@ -74,7 +74,7 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
            "bem_sweep", "bem_zgetrf", "bem_zgetrs", "bem_zgemm", "bem_device_alloc", "bem_device_free",
            "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
-           "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm"]
+           "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune"]
 
 
 def load_library():
@@ -107,6 +107,8 @@ def load_library():
     L.bem_zgetrs.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32]
     L.bem_zgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
     L.bem_cgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
+    L.bem_gemm_tiles.argtypes = [vp, vp, vp]
+    L.bem_autotune.argtypes = [vp, ctypes.c_char_p, i32, vp]
     L.bem_device_alloc.argtypes = [vp, sz, P(vp)]
     L.bem_device_free.argtypes = [vp, vp]
     L.bem_memcpy_h2d.argtypes = [vp, vp, vp, sz]
@@ -302,6 +304,22 @@ class Context:
         q, r = ctypes.c_int(0), ctypes.c_int(0)
         self._check(self.lib.bem_queue_depth(self.h, ctypes.byref(q), ctypes.byref(r)))
         return q.value, r.value
+
+    # ---- GEMM tiles
+    def gemm_tiles(self, tiles=None):
+        """Set (6 codes: complex128 narrow/rank/general, complex64 same) and
+        return the tile table (bem_gemm_tiles)."""
+        out = np.zeros(6, dtype=np.int32)
+        st = None if tiles is None else np.ascontiguousarray(tiles, dtype=np.int32)
+        self._check(self.lib.bem_gemm_tiles(self.h, None if st is None else st.ctypes.data, out.ctypes.data))
+        return [int(x) for x in out]
+
+    def autotune(self, cache_path=None, force=False):
+        """Run the tile autotuner (bem_autotune); returns the chosen table."""
+        out = np.zeros(6, dtype=np.int32)
+        p = None if cache_path is None else os.fsencode(cache_path)
+        self._check(self.lib.bem_autotune(self.h, p, 1 if force else 0, out.ctypes.data))
+        return [int(x) for x in out]
 
     KERNEL_CLASSES = {"zgemm": 0, "zgemm_narrow": 1, "ztrace": 2, "far": 3, "close": 4, "gather": 5, "cgemm": 6}
 

@@ -26,7 +26,22 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
-ZGEMM_KERNEL = "zgemm_kernel<32, 64, 2, 2, 16, 2, false, true>"
+# complex128 tile code -> kernel instance name (template arguments as rocprof prints them)
+ZGEMM_NAMES = {32: "32, 32, 2, 2, 16, 1, false, false", 64: "64, 64, 2, 2, 16, 1, false, false",
+               256: "128, 128, 2, 4, 16, 2, false, false", 1032: "32, 32, 2, 2, 16, 1, false, true",
+               2032: "32, 32, 2, 2, 32, 1, false, true", 1033: "64, 32, 2, 2, 16, 2, false, true",
+               1034: "32, 64, 2, 2, 16, 2, false, true", 1064: "64, 64, 2, 2, 16, 2, false, true",
+               2064: "64, 64, 2, 2, 32, 2, false, true", 1065: "64, 64, 2, 2, 16, 1, false, true"}
+
+
+def zgemm_kernel_name(tiles):
+    """Instance(s) behind BEM_KS_ZGEMM: the rank-update and general classes."""
+    names = []
+    for code in (tiles[1] or 1034, tiles[2] or 1034):
+        n = f"zgemm_kernel<{ZGEMM_NAMES[code]}>"
+        if n not in names:
+            names.append(n)
+    return " + ".join(names)
 PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32 MFMA
 # --precision -> (bem_request.precision, flags): BEM_PREC_FP64 = 0,
 # BEM_PREC_MIXED = 1, BEM_SOLVE_REFINE = 2
@@ -44,6 +59,120 @@ def workload(name):
         return dict(name="c1: cube pair 2x48 tri, M=288, gap 0.1, a=1e-6 m",
                     m1=c, m2=c, gap=(1.1, 0.0, 0.0), freqs=np.linspace(0.09, 0.13, 16))
     raise SystemExit(f"unknown workload {name}")
+
+
+def c3_meshes():
+    """Level-4 icosphere (2562 vertices, 5120 triangles, radius 1) and a
+    4 x 4 x 0.5 plate meshed to exactly 6000 triangles (30 x 30 x 10 squares),
+    top face at z = 0; the sphere sits at height 1 + gap (surface gap)."""
+    from bem_amd import mesh
+    return mesh.icosphere(4), mesh.cuboid(4.0, 4.0, 0.5, 30, 30, 10, origin=(-2.0, -2.0, -0.5))
+
+
+def run_precision(args):
+    """C3: sphere over plate (M = 33360) at gaps {0.05, 0.1, 0.2, 0.5} and 8
+    frequencies (3 inside the reststrahlen band); the selected precision
+    against the FP64 path per (gap, frequency)."""
+    import bem_amd
+    from bem_amd import Request
+    sph, plate = c3_meshes()
+    gaps = [0.05, 0.1, 0.2, 0.5]
+    freqs = [0.08, 0.09, 0.1, 0.105, 0.112, 0.118, 0.13, 0.14]
+    prec, flags = PRECISIONS[args.precision]
+    ctx = bem_amd.Context(0, lanes=args.lanes)
+    tune(ctx, args)
+    b1, b2 = ctx.body(sph), ctx.body(plate)
+    reqs = [Request(eV=f, gap=(0.0, 0.0, 1.0 + g), eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f),
+                    precision=prec, flags=flags) for f in freqs for g in gaps]
+    n = args.steps if args.steps < len(reqs) else len(reqs)
+    reqs = reqs[:n]
+    t0 = time.perf_counter()
+    res = ctx.sweep(b1, b2, reqs)
+    elapsed = time.perf_counter() - t0
+    out = {"metric": "spectral-power solves/s", "value": n / elapsed, "unit": "solves/s", "n_gpus": 1, "steps": n,
+           "warmup": 0, "ms_per_step": elapsed * 1e3 / n, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f64 assembly + c64 elimination",
+           "data": "synthetic",
+           "config": {"workload": f"c3: icosphere 5120 tri over 4x4x0.5 plate 6000 tri, M={2 * (b1.ne + b2.ne)}, "
+                                  "gaps 0.05/0.1/0.2/0.5, a=1e-6 m", "precision": args.precision,
+                      "lanes": args.lanes},
+           "results": [{"eV": q.eV, "gap": q.gap[2] - 1.0, "ReS": r["ReS"], "P": r["P"],
+                        "rel_err_est": r["rel_err_est"], "growth": r["growth"]} for q, r in zip(reqs, res)]}
+    if args.precision != "fp64":
+        ref = ctx.sweep(b1, b2, [Request(eV=q.eV, gap=q.gap, eps1=q.eps1, eps2=q.eps2) for q in reqs])
+        errs = [abs(r["ReS"] - f["ReS"]) / abs(f["ReS"]) for r, f in zip(res, ref)]
+        for o, e in zip(out["results"], errs):
+            o["rel_err_vs_fp64"] = e
+        out["rel_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs))}
+    ctx.close()
+    return out
+
+
+def batch_instances(n, seed=0):
+    """C4: n instances from a seeded generator: body type in {cube, cuboid,
+    icosphere} with 300..3000 RWG edges per body, surface gap log-uniform in
+    [0.02, 1], frequency uniform in [0.08, 0.14] eV."""
+    from bem_amd import mesh
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        kind = int(rng.integers(0, 3))
+        bodies = []
+        for _b in range(2):
+            if kind == 0:     # cube: 18 n^2 edges
+                k = int(rng.integers(5, 13))
+                bodies.append(mesh.cube(k))
+            elif kind == 1:   # cuboid 1 x 1 x 0.5, nz = nx / 2: 12 nx^2 edges
+                nx = int(rng.integers(6, 16))
+                nz = max(1, nx // 2)
+                bodies.append(mesh.cuboid(1.0, 1.0, 0.5, nx, nx, nz))
+            else:             # icosphere levels 2 (480 edges) and 3 (1920 edges)
+                bodies.append(mesh.icosphere(int(rng.integers(2, 4))))
+        g = float(np.exp(rng.uniform(np.log(0.02), np.log(1.0))))
+        f = float(rng.uniform(0.08, 0.14))
+        lo0 = float(bodies[0].xyz[:, 0].min())
+        hi1 = float(bodies[1].xyz[:, 0].max())
+        # body 1 moved along +x so that its lowest x sits g beyond body 2's highest x
+        out.append((kind, bodies, (hi1 + g - lo0, 0.0, 0.0), f))
+    return out
+
+
+def run_batch(args):
+    """C4: all instances submitted at once and served concurrently on the
+    context's lanes; throughput over the whole batch."""
+    import bem_amd
+    from bem_amd import Request
+    inst = batch_instances(args.requests)
+    prec, flags = PRECISIONS[args.precision]
+    ctx = bem_amd.Context(0, lanes=args.lanes)
+    tune(ctx, args)
+    handles = [(ctx.body(m1), ctx.body(m2)) for _, (m1, m2), _, _ in inst]
+    reqs = [Request(eV=f, gap=gap, eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f), precision=prec,
+                    flags=flags) for _, _, gap, f in inst]
+    t0 = time.perf_counter()
+    tickets = [ctx.submit(b1, b2, q) for (b1, b2), q in zip(handles, reqs)]
+    res = [ctx.poll(t, wait_ms=-1) for t in tickets]
+    elapsed = time.perf_counter() - t0
+    Ms = [2 * (m1.ne + m2.ne) for _, (m1, m2), _, _ in inst]
+    out = {"metric": "spectral-power solves/s (batch)", "value": len(res) / elapsed, "unit": "solves/s",
+           "n_gpus": 1, "steps": len(res), "warmup": 0, "ms_per_step": elapsed * 1e3 / len(res),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if args.precision == "fp64" else "f64 assembly + c64 elimination", "data": "synthetic",
+           "config": {"workload": f"c4: {len(res)} seeded instances (seed 0) of cube/cuboid/icosphere pairs, "
+                                  f"M={min(Ms)}..{max(Ms)}, gap log-U[0.02,1], f U[0.08,0.14] eV",
+                      "precision": args.precision, "lanes": args.lanes},
+           "failed": sum(1 for r in res if r["status"] != 0),
+           "latency_ms": {"p50": float(np.median([r["ms_total"] for r in res])),
+                          "p99": float(np.percentile([r["ms_total"] for r in res], 99)),
+                          "definition": "submit to result, whole batch submitted at t=0"}}
+    if args.precision != "fp64" and args.check:
+        errs = []
+        for (b1, b2), q, r in zip(handles, reqs, res):
+            f = ctx.calc_heat_transfer(b1, b2, Request(eV=q.eV, gap=q.gap, eps1=q.eps1, eps2=q.eps2))
+            errs.append(abs(r["ReS"] - f["ReS"]) / abs(f["ReS"]))
+        out["rel_err_vs_fp64"] = {"max": max(errs), "median": float(np.median(errs))}
+    ctx.close()
+    return out
 
 
 def serving_stream(n, rate, seed=0):
@@ -76,6 +205,7 @@ def run_serving(args):
     from bem_amd import Request
     pairs, stream = serving_stream(args.requests, args.rate)
     ctx = bem_amd.Context(0, lanes=args.lanes)
+    tune(ctx, args)
     bodies = [(ctx.body(m1), ctx.body(m2)) for _, m1, m2, _ in pairs]
 
     def req(k, gap, f):
@@ -116,6 +246,17 @@ def run_serving(args):
                        "definition": "submit to result (queueing included)", "per_pair": per_kind},
         "failed": len(bad),
     }
+
+
+TUNE_CACHE = os.path.join(ROOT, "bem_amd", "tuning", "gemm_tiles.json")
+
+
+def tune(ctx, args):
+    """GEMM tile table from the autotuner (bem_autotune) per --tune."""
+    if args.tune == "off":
+        return ctx.gemm_tiles()
+    os.makedirs(os.path.dirname(TUNE_CACHE), exist_ok=True)
+    return ctx.autotune(TUNE_CACHE, force=args.tune == "force")
 
 
 def pick_freqs(freqs, n, offset=0):
@@ -175,6 +316,9 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed-refine"],
                     help="fp64: complex128 throughout; mixed: complex64 elimination on FP32 MFMA; "
                          "mixed-refine: plus one FP64 refinement step")
+    ap.add_argument("--roofline-solves", type=int, default=2, help="single-lane solves timed per kernel class")
+    ap.add_argument("--tune", default="cache", choices=["cache", "force", "off"],
+                    help="GEMM tiles: from the autotuner cache (measured when missing), re-measured, or built-in rule")
     ap.add_argument("--requests", type=int, default=64, help="c5: number of requests in the stream")
     ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate (requests/s)")
     args = ap.parse_args()
@@ -187,12 +331,19 @@ def main():
     if args.workload == "c5":
         print(json.dumps(run_serving(args)))
         return
+    if args.workload == "c3":
+        print(json.dumps(run_precision(args)))
+        return
+    if args.workload == "c4":
+        print(json.dumps(run_batch(args)))
+        return
 
     import bem_amd
     from bem_amd import Request
 
     wl = workload(args.workload)
     ctx = bem_amd.Context(0, lanes=args.lanes)
+    tiles = tune(ctx, args)
     b1, b2 = ctx.body(wl["m1"]), ctx.body(wl["m2"])
     freqs_w = pick_freqs(wl["freqs"], args.warmup, offset=1)
     freqs = pick_freqs(wl["freqs"], args.steps)
@@ -211,12 +362,22 @@ def main():
     t_start = time.perf_counter()
     results = ctx.sweep(b1, b2, reqs)
     elapsed = time.perf_counter() - t_start
-    ks = ctx.kernel_stats()
+    ks_timed = ctx.kernel_stats()
     lat = [r["ms_total"] for r in results]
     phase = dict(assemble=sum(r["ms_assemble"] for r in results), factor=sum(r["ms_factor"] for r in results),
                  products=sum(r["ms_products"] for r in results))
     K = len(reqs)
     value = K / elapsed
+
+    # Roofline pass (after the timed region): the same solves on one lane, so
+    # each kernel's event-bracketed duration is its own execution time and
+    # not stretched by the other lanes' kernels sharing the GPU.
+    ctx1 = bem_amd.Context(0, lanes=1)
+    ctx1.gemm_tiles(tiles)
+    c1b1, c1b2 = ctx1.body(wl["m1"]), ctx1.body(wl["m2"])
+    ctx1.sweep(c1b1, c1b2, [req(f) for f in pick_freqs(wl["freqs"], args.roofline_solves, offset=2)])
+    ks = ctx1.kernel_stats()
+    ctx1.close()
     zg, far, close, gat = ks["zgemm"], ks["far"], ks["close"], ks["gather"]
     cg = ks["cgemm"]
 
@@ -224,10 +385,11 @@ def main():
         return k["flops"] / (k["ms"] * 1e-3) / scale if k["ms"] > 0 else 0.0
 
     achieved = rate(zg, 1e12)
+    zname = zgemm_kernel_name(tiles)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "c2_fp64_hbm_traffic.json")
     if os.path.exists(tpath) and args.workload == "c2":
-        t = json.load(open(tpath))["kernels"].get(ZGEMM_KERNEL)
+        t = json.load(open(tpath))["kernels"].get(zname)
         if t:
             traffic = t["fetch_bytes"] + t["write_bytes"]
     out = {
@@ -243,14 +405,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64 assembly + c64 elimination",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "precision": args.precision, "lanes": args.lanes,
+        "config": {"workload": wl["name"], "precision": args.precision, "lanes": args.lanes, "gemm_tiles": tiles,
                    "freqs_eV": [round(f, 6) for f in freqs], "T1": 300.0, "T2": 0.0,
                    "permittivity": "6.7(w^2-0.120^2+i0.00059w)/(w^2-0.0983^2+i0.00059w)"},
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
                        "definition": "issue to result of each solve inside the sweep"},
         "phases_ms_per_solve": {k: v / K for k, v in phase.items()},
-        "roofline": {"bound": "mfma", "kernel": ZGEMM_KERNEL,
-                     "how": "sum of 8mnk flops of its launches / sum of their HIP-event durations on the lane streams",
+        "roofline": {"bound": "mfma", "kernel": zname,
+                     "how": ("sum of 8mnk flops of its launches / sum of their HIP-event durations, measured on one "
+                             f"lane over {args.roofline_solves} solves after the timed region"),
+                     "in_timed_region": {"achieved": rate(ks_timed["zgemm"], 1e12),
+                                         "avg_launch_us": ks_timed["zgemm"]["ms"] * 1e3
+                                         / max(ks_timed["zgemm"]["launches"], 1),
+                                         "note": "event brackets on concurrent lanes include time queued behind "
+                                                 "the other lanes' kernels"},
                      "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic,
                      "traffic_source": "profiles/c2_fp64_hbm_traffic.json (FETCH_SIZE, WRITE_SIZE per launch, copy-calibrated)",

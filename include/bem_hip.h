@@ -174,6 +174,20 @@ int bem_zgetrs(bem_ctx *ctx, const void *A, int64_t lda, int n, const int32_t *i
 /* C <- alpha A B + beta C (no transposes), alpha/beta as (re, im). */
 int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void *A, int64_t lda, const void *B,
               int64_t ldb, const double *beta, void *C, int64_t ldc);
+/* ---- GEMM tiles and the autotuner ----------------------------------- */
+/* Tile variant per GEMM shape class {narrow: an output dimension <= 128,
+ * rank update: k <= 128, general} for complex128 (tiles[0..2]; codes 32, 64,
+ * 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065) and complex64 (tiles[3..5];
+ * codes 1..5); 0 selects the built-in shape rule.  set6 / get6 may be NULL. */
+int bem_gemm_tiles(bem_ctx *ctx, const int *set6, int *get6);
+/* Times every tile variant on a representative shape of each class (4096 x
+ * 64 x 64, 4096 x 4096 x 64, 2048^3) and installs the fastest per class.
+ * With cache_path, a cache written for the same GPU architecture is reused
+ * (unless flags & BEM_TUNE_FORCE) and a new measurement is written there as
+ * JSON with every timing.  tiles6 (optional) receives the choice. */
+#define BEM_TUNE_FORCE 1
+int bem_autotune(bem_ctx *ctx, const char *cache_path, int flags, int *tiles6);
+
 /* complex64 C <- alpha A B + beta C on FP32 MFMA (alpha/beta as (re, im)
  * floats): the kernel of the mixed-precision elimination.  No reference
  * counterpart (the reference binds only complex128 BLAS). */

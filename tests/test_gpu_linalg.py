@@ -49,3 +49,57 @@ def test_lu_factors(gpu_ctx):
         assert i <= j < n
         PA[[i, j]] = PA[[j, i]]
     assert np.abs(L @ U - PA).max() < 1e-12 * np.abs(A).max() * n
+
+
+ZTILES = [32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065]
+
+
+@pytest.mark.parametrize("tile", ZTILES)
+def test_zgemm_tile_variants(gpu_ctx, tile):
+    """Every complex128 tile variant the autotuner may pick, in every shape
+    class (narrow, rank update, general)."""
+    saved = gpu_ctx.gemm_tiles()
+    try:
+        gpu_ctx.gemm_tiles([tile, tile, tile] + saved[3:])
+        rng = np.random.default_rng(tile)
+        for m, n, k in [(200, 64, 64), (257, 190, 64), (300, 301, 129)]:
+            A = rng.normal(size=(m, k)) + 1j * rng.normal(size=(m, k))
+            B = rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))
+            C = rng.normal(size=(m, n)) + 1j * rng.normal(size=(m, n))
+            got = gpu_ctx.zgemm_host(A, B, C, 1.0 - 0.5j, 0.25j)
+            ref = (1.0 - 0.5j) * (A @ B) + 0.25j * C
+            assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max() * k ** 0.5, (tile, m, n, k)
+    finally:
+        gpu_ctx.gemm_tiles(saved)
+
+
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+def test_cgemm_tile_variants(gpu_ctx, tile):
+    saved = gpu_ctx.gemm_tiles()
+    try:
+        gpu_ctx.gemm_tiles(saved[:3] + [tile, tile, tile])
+        rng = np.random.default_rng(10 + tile)
+        for m, n, k in [(200, 64, 64), (257, 190, 64), (300, 301, 129)]:
+            A = (rng.normal(size=(m, k)) + 1j * rng.normal(size=(m, k))).astype(np.complex64)
+            B = (rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))).astype(np.complex64)
+            got = gpu_ctx.cgemm_host(A, B).astype(np.complex128)
+            ref = A.astype(np.complex128) @ B.astype(np.complex128)
+            assert np.abs(got - ref).max() <= 4e-6 * np.abs(ref).max() * k ** 0.5, (tile, m, n, k)
+    finally:
+        gpu_ctx.gemm_tiles(saved)
+
+
+def test_autotune_cache_roundtrip(tmp_path):
+    """The autotuner measures every variant once, writes its JSON cache, and a
+    second context loads the same table from it without measuring."""
+    import json
+    import bem_amd
+    path = str(tmp_path / "tiles.json")
+    with bem_amd.Context(0, lanes=1) as ctx:
+        tiles = ctx.autotune(path)
+    d = json.load(open(path))
+    assert d["zgemm"] == tiles[:3] and d["cgemm"] == tiles[3:]
+    assert all(t in ZTILES for t in tiles[:3]) and all(1 <= t <= 5 for t in tiles[3:])
+    with bem_amd.Context(0, lanes=1) as ctx2:
+        assert ctx2.autotune(path) == tiles
+        assert ctx2.gemm_tiles() == tiles

@@ -155,3 +155,18 @@ def test_spectral_power_invariants():
     r2 = O.spectral_power(b1, c_t, 0.11, 1e-6, 300, 0, eps, eps, return_parts=True)
     assert abs(r0["ReS"] - r1["ReS"]) <= 1e-12 * abs(r0["ReS"])
     assert abs(r0["ReS"] - r2["ReS"]) <= 1e-3 * abs(r0["ReS"])
+
+
+def test_heat_transfer_composition_closed_form():
+    """One RWG unknown per body (1 x 1 blocks): X = W^-1[0, 1] = -b / det with
+    W = [[a + g1, b], [c, d + g2]], sym(g) = 2 Re g, so the trace reduces to
+    S = Re(g1) Re(g2) |b|^2 / |det|^2.  Fixes which block of the inverse is
+    used (b, not c), where sym() acts and the factor 1/4."""
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        a, b, c, d, g1, g2 = rng.normal(size=6) + 1j * rng.normal(size=6)
+        W = np.array([[a, b], [c, d]])
+        S = O.heat_transfer_from_blocks(W, np.array([[g1]]), np.array([[g2]]))
+        det = (a + g1) * (d + g2) - b * c
+        ref = g1.real * g2.real * abs(b) ** 2 / abs(det) ** 2
+        assert abs(S - ref) <= 1e-12 * abs(ref)
