@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <list>
 #include <mutex>
@@ -169,6 +170,8 @@ struct SelfEntry {
 struct Lane {
   int id = 0;
   hipStream_t st = nullptr;
+  hipStream_t aux = nullptr;            // look-ahead panel factorisations
+  hipEvent_t la_a = nullptr, la_b = nullptr;
   std::string tag;
   std::vector<EventRec> ev;
   size_t ev_used = 0;
@@ -217,6 +220,7 @@ struct bem_ctx {
   size_t self_bytes = 0, self_cap = 0;
   uint64_t clock = 0;
   bool profile = true;   // bracket timed kernel classes with events
+  bool lookahead = false;   // factor the next LU diagonal block on the lane's aux stream
   // GEMM tile per shape class (gemm_class), 0 = the built-in rule
   int ztile[3] = {0, 0, 0}, ctile[3] = {0, 0, 0};
   // statistics of the last API call
@@ -240,6 +244,7 @@ struct bem_ctx {
 struct bem_body {
   bem_ctx *ctx;
   bemgeo::BodyMesh mesh;
+  uint64_t fp = 0;              // FNV-1a of the vertex and triangle arrays
   DevTree dt;                   // the body alone, untranslated (self blocks)
   bool up = false;
   std::vector<void *> allocs;
@@ -407,6 +412,36 @@ static int gemm(bem_ctx *c, Lane &L, int m, int n, int k, float2 alpha, const fl
   return ev_end(c, L, r);
 }
 
+// In-place product with a 64 x 64 factor M on the lane's stream:
+//   side 'R': X (m x n, n <= 64) <- X M;  side 'L': X (m x n, m <= 64) <- M X.
+// Safe in place because one thread block covers the whole aliased dimension
+// (32 x 64 tiles for 'R', 64 x 32 for 'L' in complex128, 64 x 64 in
+// complex64): a block reads every entry it overwrites during its k loop and
+// stores only in its epilogue.
+static int gemm_inplace(bem_ctx *c, Lane &L, char side, int m, int n, const double2 *M, double2 *X, int64_t ldx) {
+  if (m <= 0 || n <= 0) return 0;
+  const int k = side == 'R' ? n : m;
+  EventRec *r;
+  CHK(ev_begin(c, L, BEM_KS_ZGEMM_NARROW, 8.0 * m * (double)n * k, 16.0 * (64.0 * 64 + 2.0 * m * n), &r));
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+  int e = side == 'R' ? bem_zgemm_tiled(1034, m, n, k, one, X, ldx, M, 64, zero, X, ldx, L.st)
+                      : bem_zgemm_tiled(1033, m, n, k, one, M, 64, X, ldx, zero, X, ldx, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "zgemm launch: %s", hipGetErrorString((hipError_t)e));
+  return ev_end(c, L, r);
+}
+
+static int gemm_inplace(bem_ctx *c, Lane &L, char side, int m, int n, const float2 *M, float2 *X, int64_t ldx) {
+  if (m <= 0 || n <= 0) return 0;
+  const int k = side == 'R' ? n : m;
+  EventRec *r;
+  CHK(ev_begin(c, L, BEM_KS_CGEMM, 8.0 * m * (double)n * k, 8.0 * (64.0 * 64 + 2.0 * m * n), &r));
+  const float2 one = make_float2(1, 0), zero = make_float2(0, 0);
+  int e = side == 'R' ? bem_cgemm_tiled(1, m, n, k, one, X, ldx, M, 64, zero, X, ldx, L.st)
+                      : bem_cgemm_tiled(1, m, n, k, one, M, 64, X, ldx, zero, X, ldx, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "cgemm launch: %s", hipGetErrorString((hipError_t)e));
+  return ev_end(c, L, r);
+}
+
 // precision-generic launches of the LU building blocks
 static int diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st) {
   return bem_diag_lu_piv(A, lda, n, k0, inv, ipiv, info, st);
@@ -501,34 +536,53 @@ static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *i
 // diagonal block (one kernel factors the block in LDS and inverts its
 // triangles); the row interchanges are then applied to the other columns and
 // the panel's triangular solves and the trailing update run as GEMMs.
+// Look-ahead: the trailing update first covers the next 64 columns, then the
+// next diagonal block is factored on the lane's auxiliary stream while the
+// rest of the trailing update (disjoint columns) runs on the main stream, so
+// the latency-bound block factorisation hides behind the large GEMM.
 // growth (optional, device) receives max|A| before in growth[1] and
 // max|U| after in growth[0].
 template <typename T>
 static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *ipiv, int *info, T *inv,
                            unsigned long long *growth) {
   const int nb = LU_NB;
-  T *tmp;
-  CHK(ws_get(c, L, "lu_tmp", sizeof(T) * 64 * (size_t)std::max(n, 1), (void **)&tmp));
-  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0), zero = cnum<T>(0.0, 0.0);
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
   int e;
   if (growth && (e = absmax(A, lda, n, n, 0, growth + 1, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  if (n > 0 && (e = diag_lu_piv(A, lda, n, 0, inv, ipiv, info, L.st)))
+    return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
     const int blk = k0 / nb;
     T *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
-    e = diag_lu_piv(A, lda, n, k0, Linv, ipiv, info, L.st);
-    if (e) return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
     e = laswp(A, lda, 0, n, k0, k0 + jb, k0, k0 + jb, ipiv, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "laswp: %d", e);
     int rest = n - k0 - jb;
     if (rest > 0) {
       T *Akk = A + (int64_t)k0 * lda + k0;
       T *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
-      CHK(gemm(c, L, rest, jb, jb, one, A21, lda, Uinv, 64, zero, tmp, rest));   // L21 = A21 U11^-1
-      CHK(copy_back(c, L, A21, lda, tmp, rest, jb));
-      CHK(gemm(c, L, jb, rest, jb, one, Linv, 64, A12, lda, zero, tmp, jb));     // U12 = L11^-1 A12
-      CHK(copy_back(c, L, A12, lda, tmp, jb, rest));
-      CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
+      CHK(gemm_inplace(c, L, 'R', rest, jb, Uinv, A21, lda));   // L21 = A21 U11^-1
+      CHK(gemm_inplace(c, L, 'L', jb, rest, Linv, A12, lda));   // U12 = L11^-1 A12
+      if (!c->lookahead) {
+        CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
+        T *Linv2 = inv + (size_t)(blk + 1) * 2 * 64 * 64;
+        if ((e = diag_lu_piv(A, lda, n, k0 + jb, Linv2, ipiv, info, L.st)))
+          return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
+        continue;
+      }
+      const int nxt = std::min(nb, rest);
+      // next block column first, then its diagonal block on the aux stream
+      CHK(gemm(c, L, rest, nxt, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
+      DEV(c, hipEventRecord(L.la_a, L.st));
+      DEV(c, hipStreamWaitEvent(L.aux, L.la_a, 0));
+      T *Linv2 = inv + (size_t)(blk + 1) * 2 * 64 * 64;
+      if ((e = diag_lu_piv(A, lda, n, k0 + jb, Linv2, ipiv, info, L.aux)))
+        return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
+      DEV(c, hipEventRecord(L.la_b, L.aux));
+      if (rest > nxt)
+        CHK(gemm(c, L, rest, rest - nxt, jb, m1, A21, lda, A12 + (int64_t)nxt * lda, lda, one,
+                 A12 + (int64_t)nxt * lda + jb, lda));
+      DEV(c, hipStreamWaitEvent(L.st, L.la_b, 0));
     }
   }
   if (growth && (e = absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
@@ -541,16 +595,13 @@ template <typename T>
 static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv, const T *inv,
                     T *X, int64_t ldx, int r) {
   const int nb = LU_NB;
-  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0), zero = cnum<T>(0.0, 0.0);
-  T *tmp;
-  CHK(ws_get(c, L, "lu_tmp", sizeof(T) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
   int e = laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "laswp rhs: %d", e);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
     const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
-    CHK(gemm(c, L, jb, r, jb, one, Linv, 64, X + k0, ldx, zero, tmp, jb));
-    CHK(copy_back(c, L, X + k0, ldx, tmp, jb, r));
+    CHK(gemm_inplace(c, L, 'L', jb, r, Linv, X + k0, ldx));
     int rest = n - k0 - jb;
     if (rest > 0)
       CHK(gemm(c, L, rest, r, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, X + k0, ldx, one, X + k0 + jb, ldx));
@@ -559,8 +610,7 @@ static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const i
   for (int k0 = last; k0 >= 0; k0 -= nb) {
     int jb = std::min(nb, n - k0);
     const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
-    CHK(gemm(c, L, jb, r, jb, one, Uinv, 64, X + k0, ldx, zero, tmp, jb));
-    CHK(copy_back(c, L, X + k0, ldx, tmp, jb, r));
+    CHK(gemm_inplace(c, L, 'L', jb, r, Uinv, X + k0, ldx));
     if (k0 > 0) CHK(gemm(c, L, k0, r, jb, m1, A + (int64_t)k0 * lda, lda, X + k0, ldx, one, X, ldx));
   }
   return 0;
@@ -574,15 +624,12 @@ template <typename T>
 static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv,
                           const T *inv, T *X, int64_t ldx, int r) {
   const int nb = LU_NB;
-  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0), zero = cnum<T>(0.0, 0.0);
-  T *tmp;
-  CHK(ws_get(c, L, "lu_tmp", sizeof(T) * 64 * (size_t)std::max(std::max(n, r), 1), (void **)&tmp));
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
   for (int k0 = 0; k0 < n; k0 += nb) {
     int jb = std::min(nb, n - k0);
     const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
     T *Xk = X + (int64_t)k0 * ldx;
-    CHK(gemm(c, L, r, jb, jb, one, Xk, ldx, Uinv, 64, zero, tmp, r));
-    CHK(copy_back(c, L, Xk, ldx, tmp, r, jb));
+    CHK(gemm_inplace(c, L, 'R', r, jb, Uinv, Xk, ldx));
     int rest = n - k0 - jb;
     if (rest > 0)
       CHK(gemm(c, L, r, rest, jb, m1, Xk, ldx, A + (int64_t)(k0 + jb) * lda + k0, lda, one, Xk + (int64_t)jb * ldx, ldx));
@@ -592,8 +639,7 @@ static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, c
     int jb = std::min(nb, n - k0);
     const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
     T *Xk = X + (int64_t)k0 * ldx;
-    CHK(gemm(c, L, r, jb, jb, one, Xk, ldx, Linv, 64, zero, tmp, r));
-    CHK(copy_back(c, L, Xk, ldx, tmp, r, jb));
+    CHK(gemm_inplace(c, L, 'R', r, jb, Linv, Xk, ldx));
     if (k0 > 0) CHK(gemm(c, L, r, k0, jb, m1, Xk, ldx, A + k0, lda, one, X, ldx));
   }
   int e = col_swap(X, ldx, r, 0, n, ipiv, L.st);
@@ -824,10 +870,19 @@ static int self_alloc(bem_ctx *c, SelfEntry &e, void **p, size_t bytes) {
   return 0;
 }
 
+// Two bodies with identical vertex and triangle arrays have identical self
+// blocks (they are assembled untranslated).
+static bool same_mesh(const bem_body *a, const bem_body *b) {
+  return a->fp == b->fp && a->mesh.xyz == b->mesh.xyz && a->mesh.tri == b->mesh.tri;
+}
+
 // Self entry of body b in medium (eps, mu) at omega for role 0 (factored) or
 // 1 (raw), pinned for the lane; produced on the lane's stream when missing.
+// twin (optional): a raw (role 1) entry pinned by the caller for a body with
+// the same mesh, medium, omega and precision; a missing entry is then copied
+// from it instead of assembled.
 static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const double *mu, double omega, int role,
-                    int pivot, int prec, SelfEntry **out) {
+                    int pivot, int prec, const SelfEntry *twin, SelfEntry **out) {
   const double key[5] = {eps[0], eps[1], mu[0], mu[1], omega};
   for (auto &e : c->selfs)
     if (e.body == b && e.role == role && e.pivot == pivot && e.prec == prec &&
@@ -884,13 +939,23 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   r = ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * np, (void **)&tiles);
   if (!r && hipMemsetAsync(e.dinfo, 0, sizeof(int) * 4 + sizeof(unsigned long long) * 2, L.st) != hipSuccess)
     r = fail(c, BEM_E_DEVICE, "memset");
-  if (!r) r = assemble_tiles(c, L, *d, mext, tiles, 0, T, 0, T, e.dinfo + 1);
-  if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mext, e.Z, n, 0, 0, 0, nullptr, 0, 0, 0);
-  if (!r) r = assemble_tiles(c, L, *d, mint, tiles, 0, T, 0, T, e.dinfo + 1);
-  if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mint, e.Z, n, 0, 0, 1, e.Gs, n, 0, 0);
-  if (!r && bem_symm(e.Gs, n, n, L.st)) r = fail(c, BEM_E_DEVICE, "symm launch");
+  if (twin) {
+    // same geometry, medium and omega: the blocks are bitwise those of twin
+    if (!r && hipStreamWaitEvent(L.st, twin->ready, 0) != hipSuccess) r = fail(c, BEM_E_DEVICE, "event wait");
+    if (!r && (hipMemcpyAsync(e.Z, twin->Z, nn, hipMemcpyDeviceToDevice, L.st) != hipSuccess ||
+               hipMemcpyAsync(e.Gs, twin->Gs, nn, hipMemcpyDeviceToDevice, L.st) != hipSuccess ||
+               (e.Z32 && hipMemcpyAsync(e.Z32, twin->Z32, nn / 2, hipMemcpyDeviceToDevice, L.st) != hipSuccess) ||
+               (e.Gs32 && hipMemcpyAsync(e.Gs32, twin->Gs32, nn / 2, hipMemcpyDeviceToDevice, L.st) != hipSuccess)))
+      r = fail(c, BEM_E_DEVICE, "self block copy");
+  } else {
+    if (!r) r = assemble_tiles(c, L, *d, mext, tiles, 0, T, 0, T, e.dinfo + 1);
+    if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mext, e.Z, n, 0, 0, 0, nullptr, 0, 0, 0);
+    if (!r) r = assemble_tiles(c, L, *d, mint, tiles, 0, T, 0, T, e.dinfo + 1);
+    if (!r) r = gather(c, L, tiles, np, T, 0, 0, d->rw, 0, d->E, 0, d->E, &mint, e.Z, n, 0, 0, 1, e.Gs, n, 0, 0);
+    if (!r && bem_symm(e.Gs, n, n, L.st)) r = fail(c, BEM_E_DEVICE, "symm launch");
+  }
   if (!r && mixed) {
-    if (bem_z2c(e.Z, n, e.Z32, n, n, n, L.st) || bem_z2c(e.Gs, n, e.Gs32, n, n, n, L.st))
+    if (!twin && (bem_z2c(e.Z, n, e.Z32, n, n, n, L.st) || bem_z2c(e.Gs, n, e.Gs32, n, n, n, L.st)))
       r = fail(c, BEM_E_DEVICE, "complex64 copy launch");
     if (!r && role == 0) r = lu_factor_block(c, L, e.Z32, n, n, e.ipiv, e.dinfo, e.inv32, e.dgrowth);
   } else if (!r && role == 0) {
@@ -1055,8 +1120,12 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   L.refine = prec == BEM_PREC_MIXED && (q->flags & BEM_SOLVE_REFINE);
   Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
   DEV(c, hipEventRecord(L.ph[0], L.st));
-  CHK(self_get(c, L, b1, q->eps1, q->mu1, omega, 0, pivot, prec, &L.e1));
-  CHK(self_get(c, L, b2, q->eps2, q->mu2, omega, 1, 0, prec, &L.e2));
+  // body 2 (raw) first: when both bodies share one mesh and medium, body 1's
+  // factored entry is then a copy of it instead of a second assembly
+  CHK(self_get(c, L, b2, q->eps2, q->mu2, omega, 1, 0, prec, nullptr, &L.e2));
+  const bool twins = same_mesh(b1, b2) && std::memcmp(q->eps1, q->eps2, sizeof q->eps1) == 0 &&
+                     std::memcmp(q->mu1, q->mu2, sizeof q->mu1) == 0;
+  CHK(self_get(c, L, b1, q->eps1, q->mu1, omega, 0, pivot, prec, twins ? L.e2 : nullptr, &L.e1));
   const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
   const int m1 = E1.n, m2 = E2.n;
   if (m1 != 2 * g.E1 || m2 != 2 * (g.top.E - g.E1)) return fail(c, BEM_E_INVALID, "body sizes do not match the pair tree");
@@ -1502,12 +1571,18 @@ extern "C" {
 int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
   if (!out) return BEM_E_INVALID;
   bem_ctx *c = new bem_ctx();
-  int nl = 2;
+  // look-ahead needs a second stream per lane; it pays only while every
+  // stream gets a hardware queue of its own (GPU_MAX_HW_QUEUES, default 4)
+  int hwq = 4;
+  if (const char *v = std::getenv("GPU_MAX_HW_QUEUES")) hwq = std::max(1, std::atoi(v));
+  int nl = std::min(4, hwq);   // default: one lane per hardware queue
   if (cfg) {
     c->device = cfg->device;
     c->budget = cfg->hbm_budget;
     if (cfg->lanes > 0) nl = std::min(cfg->lanes, 8);
   }
+  const int la = cfg ? cfg->lookahead : 0;
+  c->lookahead = la > 0 || (la == 0 && 2 * nl <= hwq);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || hipSetDevice(c->device) != hipSuccess) {
     delete c;
@@ -1523,6 +1598,9 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     L.ctx = c;
     L.tag = "L" + std::to_string(i) + ":";
     bool ok = hipStreamCreateWithFlags(&L.st, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&L.aux, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&L.la_a, hipEventDisableTiming) == hipSuccess;
+    ok = ok && hipEventCreateWithFlags(&L.la_b, hipEventDisableTiming) == hipSuccess;
     for (int k = 0; k < 4 && ok; k++) ok = hipEventCreate(&L.ph[k]) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_trace, sizeof(double2) * 2, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_info, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
@@ -1569,6 +1647,9 @@ void bem_ctx_destroy(bem_ctx *c) {
     if (L.h_info) (void)hipHostFree(L.h_info);
     if (L.h_growth) (void)hipHostFree(L.h_growth);
     if (L.st) (void)hipStreamDestroy(L.st);
+    if (L.aux) (void)hipStreamDestroy(L.aux);
+    if (L.la_a) (void)hipEventDestroy(L.la_a);
+    if (L.la_b) (void)hipEventDestroy(L.la_b);
   }
   delete c;
 }
@@ -1597,6 +1678,14 @@ int bem_body_upload(bem_ctx *c, const double *xyz, int nv, const int32_t *tri, i
   }
   M.ne = ne;
   bemgeo::build_close_pairs(M);
+  uint64_t h = 1469598103934665603ull;
+  auto mix = [&h](const void *p, size_t nbytes) {
+    const unsigned char *q = (const unsigned char *)p;
+    for (size_t i = 0; i < nbytes; i++) h = (h ^ q[i]) * 1099511628211ull;
+  };
+  mix(M.xyz.data(), sizeof(double) * M.xyz.size());
+  mix(M.tri.data(), sizeof(int32_t) * M.tri.size());
+  b->fp = h;
   c->bodies.insert(b);
   *out = b;
   return 0;

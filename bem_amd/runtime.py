@@ -44,7 +44,7 @@ class BemError(RuntimeError):
 
 class _Cfg(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int), ("lu_block", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
-                ("lanes", ctypes.c_int), ("cache_bytes", ctypes.c_size_t)]
+                ("lanes", ctypes.c_int), ("cache_bytes", ctypes.c_size_t), ("lookahead", ctypes.c_int)]
 
 
 class _Req(ctypes.Structure):
@@ -208,9 +208,9 @@ class Body:
 class Context:
     """One HIP device context (bem_ctx)."""
 
-    def __init__(self, device=0, hbm_budget=0, lanes=0, cache_bytes=0):
+    def __init__(self, device=0, hbm_budget=0, lanes=0, cache_bytes=0, lookahead=0):
         self.lib = load_library()
-        cfg = _Cfg(device, 0, hbm_budget, lanes, cache_bytes)
+        cfg = _Cfg(device, 0, hbm_budget, lanes, cache_bytes, lookahead)
         h = ctypes.c_void_p()
         rc = self.lib.bem_ctx_create(ctypes.byref(h), ctypes.byref(cfg))
         if rc != 0:

@@ -311,7 +311,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="c2")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--lanes", type=int, default=3, help="solves kept in flight (one HIP stream each)")
+    ap.add_argument("--lanes", type=int, default=4, help="solves kept in flight (one HIP stream each)")
+    ap.add_argument("--lookahead", type=int, default=0, help="LU look-ahead: 1 on, -1 off, 0 automatic")
     ap.add_argument("--check", action="store_true", help="compare against stored oracle values")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed-refine"],
                     help="fp64: complex128 throughout; mixed: complex64 elimination on FP32 MFMA; "
@@ -342,7 +343,7 @@ def main():
     from bem_amd import Request
 
     wl = workload(args.workload)
-    ctx = bem_amd.Context(0, lanes=args.lanes)
+    ctx = bem_amd.Context(0, lanes=args.lanes, lookahead=args.lookahead)
     tiles = tune(ctx, args)
     b1, b2 = ctx.body(wl["m1"]), ctx.body(wl["m2"])
     freqs_w = pick_freqs(wl["freqs"], args.warmup, offset=1)

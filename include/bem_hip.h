@@ -59,8 +59,11 @@ typedef struct {
   int device;        /* HIP device ordinal */
   int lu_block;      /* reserved (the LU panel width is fixed at 64), pass 0 */
   size_t hbm_budget; /* bytes a context may hold, 0 = no limit */
-  int lanes;         /* solves a sweep keeps in flight, one stream each (0 = 2, max 8) */
+  int lanes;         /* solves kept in flight, one stream each (0 = one per hardware queue,
+                        GPU_MAX_HW_QUEUES, at most 4; max 8) */
   size_t cache_bytes;/* device bytes for cached self blocks (0 = 40% of the device) */
+  int lookahead;     /* LU look-ahead on a second stream per lane: 1 on, -1 off, 0 on when the
+                        2 x lanes streams fit the hardware queues (GPU_MAX_HW_QUEUES, default 4) */
 } bem_cfg;
 
 typedef struct {

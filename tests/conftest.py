@@ -33,6 +33,6 @@ def golden_meshes(g):
 @pytest.fixture(scope="session")
 def gpu_ctx():
     from bem_amd import Context
-    ctx = Context(0)
+    ctx = Context(0, lookahead=-1)
     yield ctx
     ctx.close()

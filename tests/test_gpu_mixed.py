@@ -107,11 +107,11 @@ def test_mixed_served_alone_equals_mix():
         for prec, flags in ((BEM_PREC_FP64, 0), (BEM_PREC_MIXED, 0), (BEM_PREC_MIXED, BEM_SOLVE_REFINE)):
             reqs.append(Request(eV=f, gap=(2.3, 0, 0), eps1=lorentz_eps(f), eps2=lorentz_eps(f), precision=prec,
                                 flags=flags))
-    with bem_amd.Context(0, lanes=3) as ctx:
+    with bem_amd.Context(0, lanes=3, lookahead=-1) as ctx:
         b1, b2 = ctx.body(s), ctx.body(s)
         ts = [ctx.submit(b1, b2, q) for q in reqs]
         served = [ctx.poll(t, wait_ms=-1) for t in ts]
-    with bem_amd.Context(0, lanes=1, cache_bytes=1) as ctx:
+    with bem_amd.Context(0, lanes=1, cache_bytes=1, lookahead=-1) as ctx:
         b1, b2 = ctx.body(s), ctx.body(s)
         for q, r in zip(reqs, served):
             a = ctx.calc_heat_transfer(b1, b2, q)

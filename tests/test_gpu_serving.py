@@ -38,7 +38,7 @@ def test_served_results_equal_alone():
     import bem_amd
     pairs, reqs = _mix()
     say(f"serving: {len(reqs)} requests over {len(pairs)} body pairs")
-    with bem_amd.Context(0, lanes=3) as ctx:
+    with bem_amd.Context(0, lanes=3, lookahead=-1) as ctx:
         bodies = [(ctx.body(a), ctx.body(b)) for a, b, _ in pairs]
         tickets = [ctx.submit(*bodies[p], q) for p, q in reqs]
         served = [ctx.poll(t, wait_ms=-1) for t in tickets]
@@ -46,7 +46,7 @@ def test_served_results_equal_alone():
     for r in served:
         assert r["status"] == 0, r
         assert r["ms_total"] >= r["ms_queue"] >= 0.0
-    with bem_amd.Context(0, lanes=1, cache_bytes=1) as alone_ctx:
+    with bem_amd.Context(0, lanes=1, cache_bytes=1, lookahead=-1) as alone_ctx:
         bodies = [(alone_ctx.body(a), alone_ctx.body(b)) for a, b, _ in pairs]
         for (p, q), r in zip(reqs, served):
             a = alone_ctx.calc_heat_transfer(*bodies[p], q)
@@ -78,13 +78,13 @@ def test_budget_admission_serialises_and_rejects():
     from bem_amd import mesh, Request, lorentz_eps, BEM_E_NOMEM
     s = mesh.icosphere(2)   # 320 triangles, m = 960 per body
     reqs = [Request(eV=f, gap=(2.4, 0, 0), eps1=lorentz_eps(f), eps2=lorentz_eps(f)) for f in (0.09, 0.1, 0.11, 0.12)]
-    with bem_amd.Context(0, lanes=1) as ref_ctx:
+    with bem_amd.Context(0, lanes=1, lookahead=-1) as ref_ctx:
         b1, b2 = ref_ctx.body(s), ref_ctx.body(s)
         alone = [ref_ctx.calc_heat_transfer(b1, b2, q) for q in reqs]
         one = ref_ctx.device_bytes()
     say(f"one request footprint ~ {one / 1e6:.1f} MB")
     budget = int(one * 1.5)
-    with bem_amd.Context(0, lanes=3, hbm_budget=budget) as ctx:
+    with bem_amd.Context(0, lanes=3, hbm_budget=budget, lookahead=-1) as ctx:
         b1, b2 = ctx.body(s), ctx.body(s)
         ts = [ctx.submit(b1, b2, q) for q in reqs]
         got = [ctx.poll(t, wait_ms=-1) for t in ts]
