@@ -391,6 +391,13 @@ static const int LU_NB = 64;
 template <typename T = double2>
 static size_t inv_bytes(int n) { return sizeof(T) * 2 * 64 * 64 * (size_t)((n + LU_NB - 1) / LU_NB); }
 
+// Super-block of the two-level LU and triangular solves: inside a super-block
+// of LU_NB2 rows/columns the 64-wide blocks are processed with rank-64
+// updates confined to the super-block; what lies beyond it gets one
+// rank-LU_NB2 update per super-block, a GEMM with four times the arithmetic
+// intensity of a rank-64 update.
+static const int LU_NB2 = 256;
+
 // dst block (rows x cols, ld) <- tmp (rows x cols, ld rows)
 template <typename T>
 static int copy_back(bem_ctx *c, Lane &L, T *dst, int64_t ld, const T *tmp, int rows, int cols) {
@@ -532,6 +539,60 @@ static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *i
   return 0;
 }
 
+// Two-level blocked LU (no look-ahead): super-blocks of LU_NB2 columns are
+// factored 64 columns at a time with the rank-64 updates confined to the
+// super-block; the columns right of it then get their U rows by forward
+// substitution with the super-block's unit-lower triangle and one
+// rank-LU_NB2 trailing update.  Same pivots and factors as the one-level
+// form (block-local partial pivoting); growth as there.
+template <typename T>
+static int lu_factor_block2(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *ipiv, int *info, T *inv,
+                            unsigned long long *growth) {
+  const int nb = LU_NB;
+  const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
+  int e;
+  if (growth && (e = absmax(A, lda, n, n, 0, growth + 1, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  for (int K0 = 0; K0 < n; K0 += LU_NB2) {
+    const int K1 = std::min(n, K0 + LU_NB2);
+    for (int k0 = K0; k0 < K1; k0 += nb) {
+      const int jb = std::min(nb, n - k0);
+      const int blk = k0 / nb;
+      T *Linv = inv + (size_t)blk * 2 * 64 * 64, *Uinv = Linv + 64 * 64;
+      if ((e = diag_lu_piv(A, lda, n, k0, Linv, ipiv, info, L.st)))
+        return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
+      if ((e = laswp(A, lda, 0, n, k0, k0 + jb, k0, k0 + jb, ipiv, L.st)))
+        return fail(c, BEM_E_DEVICE, "laswp: %d", e);
+      const int rest = n - k0 - jb;
+      if (rest <= 0) continue;
+      T *A21 = A + (int64_t)k0 * lda + k0 + jb;
+      CHK(gemm_inplace(c, L, 'R', rest, jb, Uinv, A21, lda));             // L21 = A21 U11^-1
+      const int pc = K1 - k0 - jb;                                         // panel columns right of the block
+      if (pc > 0) {
+        T *A12 = A + (int64_t)(k0 + jb) * lda + k0;
+        CHK(gemm_inplace(c, L, 'L', jb, pc, Linv, A12, lda));             // U12 inside the super-block
+        CHK(gemm(c, L, rest, pc, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
+      }
+    }
+    const int rc = n - K1;
+    if (rc <= 0) continue;
+    // U rows of the super-block for columns [K1, n): forward substitution
+    for (int k0 = K0; k0 < K1; k0 += nb) {
+      const int jb = std::min(nb, n - k0);
+      const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+      T *Ukr = A + (int64_t)K1 * lda + k0;
+      CHK(gemm_inplace(c, L, 'L', jb, rc, Linv, Ukr, lda));
+      const int inner = K1 - k0 - jb;
+      if (inner > 0)
+        CHK(gemm(c, L, inner, rc, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, Ukr, lda, one, Ukr + jb, lda));
+    }
+    // rank-(K1 - K0) trailing update
+    CHK(gemm(c, L, n - K1, rc, K1 - K0, m1, A + (int64_t)K0 * lda + K1, lda, A + (int64_t)K1 * lda + K0, lda, one,
+             A + (int64_t)K1 * lda + K1, lda));
+  }
+  if (growth && (e = absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  return 0;
+}
+
 // Blocked right-looking LU with partial pivoting inside each 64 x 64
 // diagonal block (one kernel factors the block in LDS and inverts its
 // triangles); the row interchanges are then applied to the other columns and
@@ -545,6 +606,7 @@ static int lu_factor(bem_ctx *c, Lane &L, double2 *A, int64_t lda, int n, int *i
 template <typename T>
 static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *ipiv, int *info, T *inv,
                            unsigned long long *growth) {
+  if (!c->lookahead) return lu_factor_block2(c, L, A, lda, n, ipiv, info, inv, growth);
   const int nb = LU_NB;
   const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
   int e;
@@ -563,13 +625,6 @@ static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *i
       T *A21 = Akk + jb, *A12 = A + (int64_t)(k0 + jb) * lda + k0;
       CHK(gemm_inplace(c, L, 'R', rest, jb, Uinv, A21, lda));   // L21 = A21 U11^-1
       CHK(gemm_inplace(c, L, 'L', jb, rest, Linv, A12, lda));   // U12 = L11^-1 A12
-      if (!c->lookahead) {
-        CHK(gemm(c, L, rest, rest, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
-        T *Linv2 = inv + (size_t)(blk + 1) * 2 * 64 * 64;
-        if ((e = diag_lu_piv(A, lda, n, k0 + jb, Linv2, ipiv, info, L.st)))
-          return fail(c, BEM_E_DEVICE, "diag block LU: %d", e);
-        continue;
-      }
       const int nxt = std::min(nb, rest);
       // next block column first, then its diagonal block on the aux stream
       CHK(gemm(c, L, rest, nxt, jb, m1, A21, lda, A12, lda, one, A12 + jb, lda));
@@ -590,7 +645,8 @@ static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *i
 }
 
 // X (n x r) <- A^-1 X from the packed LU, its interchanges and the diagonal
-// block inverses.
+// block inverses: forward substitution with the unit-lower L over super-blocks
+// top to bottom, then backward substitution with U bottom to top.
 template <typename T>
 static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv, const T *inv,
                     T *X, int64_t ldx, int r) {
@@ -598,49 +654,73 @@ static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const i
   const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
   int e = laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "laswp rhs: %d", e);
-  for (int k0 = 0; k0 < n; k0 += nb) {
-    int jb = std::min(nb, n - k0);
-    const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
-    CHK(gemm_inplace(c, L, 'L', jb, r, Linv, X + k0, ldx));
-    int rest = n - k0 - jb;
-    if (rest > 0)
-      CHK(gemm(c, L, rest, r, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, X + k0, ldx, one, X + k0 + jb, ldx));
+  for (int K0 = 0; K0 < n; K0 += LU_NB2) {
+    const int K1 = std::min(n, K0 + LU_NB2);
+    for (int k0 = K0; k0 < K1; k0 += nb) {
+      int jb = std::min(nb, n - k0);
+      const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+      CHK(gemm_inplace(c, L, 'L', jb, r, Linv, X + k0, ldx));
+      int inner = K1 - k0 - jb;
+      if (inner > 0)
+        CHK(gemm(c, L, inner, r, jb, m1, A + (int64_t)k0 * lda + k0 + jb, lda, X + k0, ldx, one, X + k0 + jb, ldx));
+    }
+    if (n > K1)
+      CHK(gemm(c, L, n - K1, r, K1 - K0, m1, A + (int64_t)K0 * lda + K1, lda, X + K0, ldx, one, X + K1, ldx));
   }
-  int last = ((n - 1) / nb) * nb;
-  for (int k0 = last; k0 >= 0; k0 -= nb) {
-    int jb = std::min(nb, n - k0);
-    const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
-    CHK(gemm_inplace(c, L, 'L', jb, r, Uinv, X + k0, ldx));
-    if (k0 > 0) CHK(gemm(c, L, k0, r, jb, m1, A + (int64_t)k0 * lda, lda, X + k0, ldx, one, X, ldx));
+  const int nsb = (n + LU_NB2 - 1) / LU_NB2;
+  for (int sb = nsb - 1; sb >= 0; sb--) {
+    const int K0 = sb * LU_NB2, K1 = std::min(n, K0 + LU_NB2);
+    for (int k0 = ((K1 - 1) / nb) * nb; k0 >= K0; k0 -= nb) {
+      int jb = std::min(nb, n - k0);
+      const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
+      CHK(gemm_inplace(c, L, 'L', jb, r, Uinv, X + k0, ldx));
+      if (k0 > K0)
+        CHK(gemm(c, L, k0 - K0, r, jb, m1, A + (int64_t)k0 * lda + K0, lda, X + k0, ldx, one, X + K0, ldx));
+    }
+    if (K0 > 0) CHK(gemm(c, L, K0, r, K1 - K0, m1, A + (int64_t)K0 * lda, lda, X + K0, ldx, one, X, ldx));
   }
   return 0;
 }
 
 // X (r x n) <- X A^-1 from the packed LU of A (A = P L U): X U^-1 over column
-// blocks left to right, then L^-1 right to left, then the column
+// super-blocks left to right, then L^-1 right to left, then the column
 // interchanges X P^T; the diagonal blocks go through their inverses, the
-// coupling terms through GEMMs.
+// coupling terms through GEMMs (rank-64 inside a super-block, rank-LU_NB2
+// beyond it).
 template <typename T>
 static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv,
                           const T *inv, T *X, int64_t ldx, int r) {
   const int nb = LU_NB;
   const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
-  for (int k0 = 0; k0 < n; k0 += nb) {
-    int jb = std::min(nb, n - k0);
-    const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
-    T *Xk = X + (int64_t)k0 * ldx;
-    CHK(gemm_inplace(c, L, 'R', r, jb, Uinv, Xk, ldx));
-    int rest = n - k0 - jb;
-    if (rest > 0)
-      CHK(gemm(c, L, r, rest, jb, m1, Xk, ldx, A + (int64_t)(k0 + jb) * lda + k0, lda, one, Xk + (int64_t)jb * ldx, ldx));
+  for (int K0 = 0; K0 < n; K0 += LU_NB2) {
+    const int K1 = std::min(n, K0 + LU_NB2);
+    for (int k0 = K0; k0 < K1; k0 += nb) {
+      int jb = std::min(nb, n - k0);
+      const T *Uinv = inv + (size_t)(k0 / nb) * 2 * 64 * 64 + 64 * 64;
+      T *Xk = X + (int64_t)k0 * ldx;
+      CHK(gemm_inplace(c, L, 'R', r, jb, Uinv, Xk, ldx));
+      int inner = K1 - k0 - jb;
+      if (inner > 0)
+        CHK(gemm(c, L, r, inner, jb, m1, Xk, ldx, A + (int64_t)(k0 + jb) * lda + k0, lda, one, Xk + (int64_t)jb * ldx,
+                 ldx));
+    }
+    if (n > K1)
+      CHK(gemm(c, L, r, n - K1, K1 - K0, m1, X + (int64_t)K0 * ldx, ldx, A + (int64_t)K1 * lda + K0, lda, one,
+               X + (int64_t)K1 * ldx, ldx));
   }
-  int last = ((n - 1) / nb) * nb;
-  for (int k0 = last; k0 >= 0; k0 -= nb) {
-    int jb = std::min(nb, n - k0);
-    const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
-    T *Xk = X + (int64_t)k0 * ldx;
-    CHK(gemm_inplace(c, L, 'R', r, jb, Linv, Xk, ldx));
-    if (k0 > 0) CHK(gemm(c, L, r, k0, jb, m1, Xk, ldx, A + k0, lda, one, X, ldx));
+  const int nsb = (n + LU_NB2 - 1) / LU_NB2;
+  for (int sb = nsb - 1; sb >= 0; sb--) {
+    const int K0 = sb * LU_NB2, K1 = std::min(n, K0 + LU_NB2);
+    for (int k0 = ((K1 - 1) / nb) * nb; k0 >= K0; k0 -= nb) {
+      int jb = std::min(nb, n - k0);
+      const T *Linv = inv + (size_t)(k0 / nb) * 2 * 64 * 64;
+      T *Xk = X + (int64_t)k0 * ldx;
+      CHK(gemm_inplace(c, L, 'R', r, jb, Linv, Xk, ldx));
+      if (k0 > K0)
+        CHK(gemm(c, L, r, k0 - K0, jb, m1, Xk, ldx, A + k0 + (int64_t)K0 * lda, lda, one, X + (int64_t)K0 * ldx, ldx));
+    }
+    if (K0 > 0)
+      CHK(gemm(c, L, r, K0, K1 - K0, m1, X + (int64_t)K0 * ldx, ldx, A + K0, lda, one, X, ldx));
   }
   int e = col_swap(X, ldx, r, 0, n, ipiv, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "column interchanges: %d", e);
