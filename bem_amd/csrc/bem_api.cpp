@@ -1572,7 +1572,7 @@ static void sched_main(bem_ctx *c) {
 // Representative shape (m, n, k) of each GEMM class.
 static const int TUNE_SHAPE[3][3] = {{4096, 64, 64}, {4096, 4096, 64}, {2048, 2048, 2048}};
 static const int ZCANDS[] = {1032, 2032, 1033, 1034, 1064, 2064, 1065, 32, 64, 256};
-static const int CCANDS[] = {1, 2, 3, 4, 5};
+static const int CCANDS[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15};
 
 // Mean milliseconds of `reps` launches of one tile variant on lane 0.
 static int tune_time(bem_ctx *c, Lane &L, bool c64, int tile, int cls, void *A, void *B, void *Cm, int reps,

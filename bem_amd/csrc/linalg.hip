@@ -767,10 +767,11 @@ __global__ void symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
 // same staging as zgemm_kernel: BM x BN block tile, WM x WN waves of
 // (BM/WM) x (BN/WN), k staged through LDS in steps of GBK as re/im planes
 // with the next step prefetched into registers.  Four real products per
-// complex k step into two FP32 accumulators.  TRACE: the epilogue forms
+// complex k step into two FP32 accumulators, or with G3M three (Ar Br,
+// Ai Bi, (Ar + Ai)(Br + Bi)) into three, as zgemm_kernel.  TRACE: the epilogue forms
 // sum conj(alpha C_ij) E_ij in FP64 against a complex64 E and writes one
 // FP64 partial per block.
-template <int BM, int BN, int WM, int WN, int GBK, bool TRACE>
+template <int BM, int BN, int WM, int WN, int GBK, bool TRACE, bool G3M>
 __global__ void __launch_bounds__(64 * WM * WN)
 cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, int64_t lda,
              const float2 *__restrict__ B, int64_t ldb, float2 beta, float2 *__restrict__ C, int64_t ldc,
@@ -787,9 +788,10 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
   const int wr = wave / WN, wc = wave % WN;
   const int li = lane & 15, lk = lane >> 4;
   const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
-  flt4 acc[2][FM][FN];
+  constexpr int NACC = G3M ? 3 : 2;
+  flt4 acc[NACC][FM][FN];
 #pragma unroll
-  for (int q = 0; q < 2; q++)
+  for (int q = 0; q < NACC; q++)
 #pragma unroll
     for (int a = 0; a < FM; a++)
 #pragma unroll
@@ -840,20 +842,36 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
         br[y] = sBr[kk + lk][wc * (BN / WN) + y * 16 + li];
         bi[y] = sBi[kk + lk][wc * (BN / WN) + y * 16 + li];
       }
+      if constexpr (G3M) {
+        float as[FM], bs[FN];
 #pragma unroll
-      for (int x = 0; x < FM; x++)
+        for (int x = 0; x < FM; x++) as[x] = ar[x] + ai[x];
 #pragma unroll
-        for (int y = 0; y < FN; y++) {
-          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ar[x], acc[0][x][y], 0, 0, 0);
-          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], ar[x], acc[1][x][y], 0, 0, 0);
-        }
+        for (int y = 0; y < FN; y++) bs[y] = br[y] + bi[y];
 #pragma unroll
-      for (int x = 0; x < FM; x++)
+        for (int x = 0; x < FM; x++)
 #pragma unroll
-        for (int y = 0; y < FN; y++) {
-          acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], nai[x], acc[0][x][y], 0, 0, 0);
-          acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ai[x], acc[1][x][y], 0, 0, 0);
-        }
+          for (int y = 0; y < FN; y++) {
+            acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ar[x], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], ai[x], acc[1][x][y], 0, 0, 0);
+            acc[NACC - 1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bs[y], as[x], acc[NACC - 1][x][y], 0, 0, 0);
+          }
+      } else {
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+          for (int y = 0; y < FN; y++) {
+            acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ar[x], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], ar[x], acc[1][x][y], 0, 0, 0);
+          }
+#pragma unroll
+        for (int x = 0; x < FM; x++)
+#pragma unroll
+          for (int y = 0; y < FN; y++) {
+            acc[0][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(bi[y], nai[x], acc[0][x][y], 0, 0, 0);
+            acc[1][x][y] = __builtin_amdgcn_mfma_f32_16x16x4f32(br[y], ai[x], acc[1][x][y], 0, 0, 0);
+          }
+      }
     }
   }
   // v_mfma_f32_16x16x4f32 output: D[i][j] of a 16 x 16 block in lane
@@ -870,7 +888,15 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
           int row = m0 + wr * (BM / WM) + x * 16 + li;
           int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
           if (row < m && col < n) {
-            float cr = acc[0][x][y][r], ci = acc[1][x][y][r];
+            float cr, ci;
+            if constexpr (G3M) {
+              float t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+              cr = t1 - t2;
+              ci = acc[NACC - 1][x][y][r] - t1 - t2;
+            } else {
+              cr = acc[0][x][y][r];
+              ci = acc[1][x][y][r];
+            }
             float vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
             float2 *p = C + (int64_t)col * ldc + row;
             if (beta.x != 0.f || beta.y != 0.f) {
@@ -894,7 +920,15 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
           int row = m0 + wr * (BM / WM) + x * 16 + li;
           int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
           if (row < m && col < n) {
-            double cr = acc[0][x][y][r], ci = acc[1][x][y][r];
+            double cr, ci;
+            if constexpr (G3M) {
+              double t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+              cr = t1 - t2;
+              ci = (double)acc[NACC - 1][x][y][r] - t1 - t2;
+            } else {
+              cr = acc[0][x][y][r];
+              ci = acc[1][x][y][r];
+            }
             double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
             float2 e = E[(int64_t)col * lde + row];
             sr += vr * (double)e.x + vi * (double)e.y;
@@ -973,12 +1007,12 @@ __global__ void identity_minus_kernel(double2 *__restrict__ Cm, int64_t ldc, int
   Cm[(int64_t)c * ldc + r] = make_double2((r == c ? 1.0 : 0.0) - v.x, -v.y);
 }
 
-template <int BM, int BN, int WM, int WN, int GBK, bool TRACE>
+template <int BM, int BN, int WM, int WN, int GBK, bool TRACE, bool G3M = false>
 void launch_cgemm(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
                   float2 beta, float2 *C, int64_t ldc, const float2 *E, int64_t lde, double2 *partial, hipStream_t st) {
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
-  hipLaunchKernelGGL((cgemm_kernel<BM, BN, WM, WN, GBK, TRACE>), grid, dim3(64 * WM * WN), 0, st, m, n, k, alpha, A,
-                     lda, B, ldb, beta, C, ldc, E, lde, partial);
+  hipLaunchKernelGGL((cgemm_kernel<BM, BN, WM, WN, GBK, TRACE, G3M>), grid, dim3(64 * WM * WN), 0, st, m, n, k,
+                     alpha, A, lda, B, ldb, beta, C, ldc, E, lde, partial);
 }
 
 template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool TRACE, bool G3M>
@@ -1123,10 +1157,11 @@ int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
 
 // ------------------------------------------------------- complex64 (mixed)
 // C <- alpha A B + beta C in complex64.  Tile variants: 1 = 64 x 64, 2 = 32 x
-// 32, 3 = 64 x 128, 4 = 128 x 64, 5 = 128 x 128 (2 x 2 waves each).  Default
-// rule: 32 x 32 when either dimension is at most 128, else 64 x 64.
+// 32, 3 = 64 x 128, 4 = 128 x 64, 5 = 128 x 128 (2 x 2 waves each) with four
+// real products per complex multiply-add; 11..15 the same tiles with three.
+// Default rule: 32 x 32 when either dimension is at most 128, else 64 x 64.
 int bem_cgemm_default_tile(int m, int n) { return (m <= 128 || n <= 128) ? 2 : 1; }
-int bem_cgemm_tile_valid(int tile) { return tile >= 1 && tile <= 5; }
+int bem_cgemm_tile_valid(int tile) { return (tile >= 1 && tile <= 5) || (tile >= 11 && tile <= 15); }
 int bem_cgemm_wide(int m, int n) { return !(m <= 128 || n <= 128); }
 
 int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B,
@@ -1139,6 +1174,11 @@ int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A
     case 3: launch_cgemm<64, 128, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
     case 4: launch_cgemm<128, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
     case 5: launch_cgemm<128, 128, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 11: launch_cgemm<64, 64, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 12: launch_cgemm<32, 32, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 13: launch_cgemm<64, 128, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 14: launch_cgemm<128, 64, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 15: launch_cgemm<128, 128, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
     default: return -1;
   }
   return (int)hipGetLastError();

@@ -181,7 +181,9 @@ int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void
 /* Tile variant per GEMM shape class {narrow: an output dimension <= 128,
  * rank update: k <= 128, general} for complex128 (tiles[0..2]; codes 32, 64,
  * 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065) and complex64 (tiles[3..5];
- * codes 1..5); 0 selects the built-in shape rule.  set6 / get6 may be NULL. */
+ * codes 1..5 four real products per complex multiply-add, 11..15 the same
+ * tiles with three); 0 selects the built-in shape rule.  set6 / get6 may be
+ * NULL. */
 int bem_gemm_tiles(bem_ctx *ctx, const int *set6, int *get6);
 /* Times every tile variant on a representative shape of each class (4096 x
  * 64 x 64, 4096 x 4096 x 64, 2048^3) and installs the fastest per class.

@@ -73,8 +73,10 @@ def test_zgemm_tile_variants(gpu_ctx, tile):
         gpu_ctx.gemm_tiles(saved)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15])
 def test_cgemm_tile_variants(gpu_ctx, tile):
+    """complex64 variants: four real products (1..5) and three (11..15), whose
+    error bound u (|Ar| + |Ai|)(|Br| + |Bi|) per term is of the same order."""
     saved = gpu_ctx.gemm_tiles()
     try:
         gpu_ctx.gemm_tiles(saved[:3] + [tile, tile, tile])
@@ -84,7 +86,7 @@ def test_cgemm_tile_variants(gpu_ctx, tile):
             B = (rng.normal(size=(k, n)) + 1j * rng.normal(size=(k, n))).astype(np.complex64)
             got = gpu_ctx.cgemm_host(A, B).astype(np.complex128)
             ref = A.astype(np.complex128) @ B.astype(np.complex128)
-            assert np.abs(got - ref).max() <= 4e-6 * np.abs(ref).max() * k ** 0.5, (tile, m, n, k)
+            assert np.abs(got - ref).max() <= 8e-6 * np.abs(ref).max() * k ** 0.5, (tile, m, n, k)
     finally:
         gpu_ctx.gemm_tiles(saved)
 
@@ -99,7 +101,7 @@ def test_autotune_cache_roundtrip(tmp_path):
         tiles = ctx.autotune(path)
     d = json.load(open(path))
     assert d["zgemm"] == tiles[:3] and d["cgemm"] == tiles[3:]
-    assert all(t in ZTILES for t in tiles[:3]) and all(1 <= t <= 5 for t in tiles[3:])
+    assert all(t in ZTILES for t in tiles[:3]) and all(t in (1, 2, 3, 4, 5, 11, 12, 13, 14, 15) for t in tiles[3:])
     with bem_amd.Context(0, lanes=1) as ctx2:
         assert ctx2.autotune(path) == tiles
         assert ctx2.gemm_tiles() == tiles
