@@ -65,9 +65,6 @@ int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A
 int bem_cgemm_tile_valid(int tile);
 int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                   double2 beta, double2 *C, int64_t ldc, hipStream_t st);
-int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
-                    const double2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st);
-size_t bem_trace_partials(int m, int n);
 int bem_calu_pivots(const double2 *A, int64_t lda, int m, int w, int r0, int c0, int *cand, int *cand2, int *ipiv,
                     int *info, hipStream_t st);
 int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int k0, int k1, const int *ipiv,
@@ -81,9 +78,12 @@ int bem_col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv
 int bem_cgemm_wide(int m, int n);
 int bem_cgemm_raw(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
                   float2 beta, float2 *C, int64_t ldc, hipStream_t st);
-size_t bem_ctrace_partials(int m, int n);
-int bem_cgemm_trace(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
-                    const float2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st);
+size_t bem_htrace_partials(int n);
+size_t bem_chtrace_partials(int n);
+int bem_zgemm_htrace(int n, int k, const double2 *A, int64_t lda, const double2 *B, int64_t ldb, const double2 *E,
+                     int64_t lde, double2 *partial, double2 *out, hipStream_t st);
+int bem_cgemm_htrace(int n, int k, const float2 *A, int64_t lda, const float2 *B, int64_t ldb, const float2 *E,
+                     int64_t lde, double2 *partial, double2 *out, hipStream_t st);
 int bem_diag_lu_piv_c(float2 *A, int64_t lda, int n, int k0, float2 *inv, int *ipiv, int *info, hipStream_t st);
 int bem_absmax_c(const float2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st);
 int bem_col_swap_c(float2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st);
@@ -93,6 +93,7 @@ int bem_z2c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, in
 int bem_c2z(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 int bem_add_z_to_c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st);
+int bem_signed_transpose(const double2 *B, int64_t ldb, int m, int n, double2 *C, int64_t ldc, hipStream_t st);
 int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 }
 
@@ -133,6 +134,7 @@ struct PairGeom {
   int T1 = 0, E1 = 0;   // triangles / RWGs of body 1 (the first ones of top)
   std::vector<void *> allocs;
   size_t bytes = 0;
+  int pins = 0;         // lanes whose solve in flight reads this tree
 };
 
 // Host-visible status of a self entry, written by its producing stream.
@@ -183,6 +185,7 @@ struct Lane {
   bem_result *res = nullptr;
   bem_request req;
   SelfEntry *e1 = nullptr, *e2 = nullptr;   // pinned while in flight
+  PairGeom *pg = nullptr;                   // pinned while in flight
   std::chrono::steady_clock::time_point t0;
   size_t ws_bytes = 0;            // bytes of this lane's named workspaces
   int prec = BEM_PREC_FP64;       // precision of the solve in flight
@@ -787,7 +790,7 @@ static void free_allocs(bem_ctx *c, std::vector<void *> &allocs, size_t bytes) {
   c->bytes -= bytes;
 }
 
-static const size_t PAIR_CACHE = 8;
+static const size_t PAIR_CACHE = 16;   // more than the lanes (at most 8), so one is always unpinned
 
 static int pair_geom(bem_ctx *c, bem_body *b1, bem_body *b2, const double *gap, PairGeom **out) {
   for (auto it = c->pairs.begin(); it != c->pairs.end(); ++it)
@@ -797,9 +800,15 @@ static int pair_geom(bem_ctx *c, bem_body *b1, bem_body *b2, const double *gap, 
       return 0;
     }
   if (c->pairs.size() >= PAIR_CACHE) {
-    PairGeom &old = c->pairs.back();
-    free_allocs(c, old.allocs, old.bytes);
-    c->pairs.pop_back();
+    // least recently used tree that no solve in flight reads
+    for (auto it = c->pairs.end(); it != c->pairs.begin();) {
+      --it;
+      if (it->pins == 0) {
+        free_allocs(c, it->allocs, it->bytes);
+        c->pairs.erase(it);
+        break;
+      }
+    }
   }
   c->pairs.emplace_front();
   PairGeom &g = c->pairs.front();
@@ -1059,6 +1068,10 @@ static void self_release(bem_ctx *c, Lane &L) {
       (*p)->pins--;
       *p = nullptr;
     }
+  if (L.pg) {
+    L.pg->pins--;
+    L.pg = nullptr;
+  }
   (void)c;
 }
 
@@ -1085,8 +1098,10 @@ static int assemble_cross(bem_ctx *c, Lane &L, PairGeom &g, const Medium &mext, 
   CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(npc, 1), (void **)&tiles));
   CHK(assemble_tiles(c, L, g.top, mext, tiles, 0, T1, T1, T1 + T2, flag));
   CHK(gather(c, L, tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0));
-  CHK(assemble_tiles(c, L, g.top, mext, tiles, T1, T1 + T2, 0, T1, flag));
-  CHK(gather(c, L, tiles, npc, T1, T1, 0, g.top.rw, Eb1, E, 0, Eb1, &mext, Cm, m2, 0, 0, 0, nullptr, 0, 0, 0));
+  // C = Z_ext(2,1) = S B^T S (reciprocity of the far-pair rules, S = +1 on J
+  // and -1 on M unknowns) instead of a second assembly
+  int e = bem_signed_transpose(Bm, m1, m1, m2, Cm, m2, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "transpose launch: %d", e);
   return 0;
 }
 
@@ -1111,7 +1126,7 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   CHK(ws_get(c, L, "B1_32", sizeof(float2) * (size_t)m1 * m2, (void **)&B1));
   CHK(ws_get(c, L, "invS32", inv_bytes<float2>(m2), (void **)&invS));
   CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
-  size_t np = std::max(bem_ctrace_partials(m1, m2), bem_trace_partials(m1, m2));
+  size_t np = std::max(bem_chtrace_partials(m2), bem_htrace_partials(m2));
   CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
   if (bem_z2c(Bm, m1, B32, m1, m1, m2, L.st) || bem_z2c(Cm, m2, C32, m2, m2, m1, L.st))
     return fail(c, BEM_E_DEVICE, "complex64 copy launch");
@@ -1124,12 +1139,13 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
     DEV(c, hipMemcpyAsync(Y32, B32, sizeof(float2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
   }
   CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, B32, m1, m1));        // B32 <- Y Sc^-1 = -X
-  // products: S = 1/4 sum conj(X sym G2) .* (sym G1 X), even in X
+  // products: 4 S = tr(sym G2 X^H sym G1 X), even in X, over the Hermitian
+  // X^H (sym G1 X): half of it is formed
   CHK(gemm(c, L, m1, m2, m1, one, E1.Gs32, m1, B32, m1, zero, B1, m1));
   {
     EventRec *r;
-    CHK(ev_begin(c, L, BEM_KS_CGEMM, 8.0 * m1 * (double)m2 * m2, 8.0 * (2.0 * m1 * m2 + (double)m2 * m2), &r));
-    int e = bem_cgemm_trace(m1, m2, m2, one, B32, m1, E2.Gs32, m2, B1, m1, partial, trace_out, L.st);
+    CHK(ev_begin(c, L, BEM_KS_CGEMM, 4.0 * m1 * (double)m2 * m2, 8.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
+    int e = bem_cgemm_htrace(m2, m1, B32, m1, B1, m1, E2.Gs32, m2, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "complex64 trace gemm: %d", e);
     CHK(ev_end(c, L, r));
   }
@@ -1174,8 +1190,8 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   // FP64 products and trace of the corrected X into trace_out[1]
   CHK(gemm(c, L, m1, m2, m1, d1, E1.Gs, m1, Xn, m1, d0, B1d, m1));
   EventRec *r;
-  CHK(ev_begin(c, L, BEM_KS_ZTRACE, 8.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + (double)m2 * m2), &r));
-  e = bem_zgemm_trace(m1, m2, m2, d1, Xn, m1, E2.Gs, m2, B1d, m1, partial, trace_out + 1, L.st);
+  CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
+  e = bem_zgemm_htrace(m2, m1, Xn, m1, B1d, m1, E2.Gs, m2, partial, trace_out + 1, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
   return ev_end(c, L, r);
 }
@@ -1192,6 +1208,8 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   L.t0 = std::chrono::steady_clock::now();
   L.req = *q;
   L.res = res;
+  L.pg = &g;
+  g.pins++;
   double omegaSI = q->eV * SI_EV / SI_HBAR;
   double omega = omegaSI * q->a / SI_C;
   const int pivot = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
@@ -1233,7 +1251,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
     CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
     CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
-    size_t np = bem_trace_partials(m1, m2);
+    size_t np = bem_htrace_partials(m2);
     CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
     DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
     // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
@@ -1247,11 +1265,12 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     double2 *X = Bm;
     DEV(c, hipEventRecord(L.ph[2], L.st));
     // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
+    // = 1/4 tr(sym(G2) P) with P = X^H sym(G1) X Hermitian (sym(G) is G + G^H),
+    // so only the upper half of P is formed
     CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
     EventRec *r;
-    CHK(ev_begin(c, L, BEM_KS_ZTRACE, 8.0 * m1 * (double)m2 * m2,
-                 16.0 * ((double)m1 * m2 + (double)m2 * m2 + (double)m1 * m2), &r));
-    int e = bem_zgemm_trace(m1, m2, m2, one, X, m1, E2.Gs, m2, B1, m1, partial, trace_out, L.st);
+    CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
+    int e = bem_zgemm_htrace(m2, m1, X, m1, B1, m1, E2.Gs, m2, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
     CHK(ev_end(c, L, r));
   }
@@ -1366,7 +1385,7 @@ static size_t footprint(bem_ctx *c, const Lane &L, const Pending &p) {
   size_t pairs = std::max(T1 * T2, std::max(h1 ? 0 : T1 * T1, h2 ? 0 : T2 * T2));
   size_t ws = sizeof(double2) * (3 * m1 * m2 + m2 * m2) * (prec == BEM_PREC_MIXED ? 3 : 1) + 8 * (size_t)BEM_TILE_PLANES * pairs +
               sizeof(int2) * std::min<size_t>(pairs, (size_t)1 << 24) + inv_bytes((int)m2) +
-              sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_trace_partials((int)m1, (int)m2) + 1);
+              sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_htrace_partials((int)m2) + 1);
   return (ws > L.ws_bytes ? ws - L.ws_bytes : 0) + selfb;
 }
 

@@ -6,10 +6,11 @@
 //                   block and the complex product as three real MFMAs
 //                   (Ar Br, Ai Bi, (Ar + Ai)(Br + Bi)) into three
 //                   accumulators; the four-product form stays available.
-//                   TRACE variant: instead of storing C, the epilogue forms
-//                   sum conj(C_ij) * E_ij against a second matrix E and writes
-//                   one partial per block (the HeatTransfer trace
-//                   tr(B2^H B1) = sum conj(B2) .* B1, src/HeatTransfer.cpp:53-55).
+//  zgemm_htrace     the HeatTransfer trace tr(B2^H B1) = sum conj(B2) .* B1
+//                   (src/HeatTransfer.cpp:53-55) as tr(sym G2 X^H sym G1 X):
+//                   only the upper half of the Hermitian X^H (sym G1 X) is
+//                   formed and contracted against sym G2 in the epilogue, one
+//                   partial per block
 //  tslu_*           tournament pivot selection for a panel (CALU)
 //  diag_block       a 64x64 diagonal block once its pivot rows are in place:
 //                   LU without pivoting and the inverses of both triangles,
@@ -46,14 +47,14 @@ template <> struct Cplx<float2> {
 // per term as for the four-product form.
 constexpr int GPAD = 1;
 
-template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool TRACE, bool G3M>
-__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
-zgemm_kernel(int m, int n, int k, double2 alpha,
-                                                             const double2 *__restrict__ A, int64_t lda,
-                                                             const double2 *__restrict__ B, int64_t ldb,
-                                                             double2 beta, double2 *__restrict__ C, int64_t ldc,
-                                                             const double2 *__restrict__ E, int64_t lde,
-                                                             double2 *__restrict__ partial) {
+// The k loop of the C tile at (m0, n0) of one block: acc receives the real
+// products (two accumulators, three with G3M).  TA: the A operand is the
+// conjugate transpose of the k x m column-major array at A, read with threads
+// along k (as the B tile) and conjugated on its way into LDS.
+template <int BM, int BN, int WM, int WN, int GBK, bool G3M, bool TA>
+__device__ __forceinline__ void zgemm_tile(int m, int n, int k, const double2 *__restrict__ A, int64_t lda,
+                                           const double2 *__restrict__ B, int64_t ldb, int m0, int n0,
+                                           dbl4 (&acc)[G3M ? 3 : 2][BM / WM / 16][BN / WN / 16]) {
   constexpr int NT = 64 * WM * WN;
   constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
   constexpr int AL = BM * GBK / NT;   // A elements loaded per thread per k step
@@ -66,9 +67,7 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave / WN, wc = wave % WN;
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
 
-  dbl4 acc[NACC][FM][FN];
 #pragma unroll
   for (int q = 0; q < NACC; q++)
 #pragma unroll
@@ -78,16 +77,22 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
 
   double2 ra[AL], rb[BL];
   // A tile (BM x GBK): thread t reads row t % BM of columns (t / BM) + (NT/BM) q
+  // (TA: k index t % GBK of rows t / GBK + (NT/GBK) q)
   // B tile (GBK x BN): thread t reads k-row t % GBK of columns t / GBK + (NT/GBK) q
-  const int arow = tid % BM, acol0 = tid / BM;
-  constexpr int ASTEP = NT / BM;
+  const int arow = TA ? tid / GBK : tid % BM, acol0 = TA ? tid % GBK : tid / BM;
+  constexpr int ASTEP = TA ? NT / GBK : NT / BM;
   const int bkr = tid % GBK, bcol0 = tid / GBK;
   constexpr int BSTEP = NT / GBK;
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < AL; q++) {
-      int col = k0 + acol0 + ASTEP * q, row = m0 + arow;
-      ra[q] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_double2(0.0, 0.0);
+      if constexpr (TA) {
+        int row = m0 + arow + ASTEP * q, kk = k0 + acol0;
+        ra[q] = (row < m && kk < k) ? A[(int64_t)row * lda + kk] : make_double2(0.0, 0.0);
+      } else {
+        int col = k0 + acol0 + ASTEP * q, row = m0 + arow;
+        ra[q] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_double2(0.0, 0.0);
+      }
     }
 #pragma unroll
     for (int q = 0; q < BL; q++) {
@@ -100,8 +105,13 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < AL; q++) {
-      sAr[acol0 + ASTEP * q][arow] = ra[q].x;
-      sAi[acol0 + ASTEP * q][arow] = ra[q].y;
+      if constexpr (TA) {
+        sAr[acol0][arow + ASTEP * q] = ra[q].x;
+        sAi[acol0][arow + ASTEP * q] = -ra[q].y;
+      } else {
+        sAr[acol0 + ASTEP * q][arow] = ra[q].x;
+        sAi[acol0 + ASTEP * q][arow] = ra[q].y;
+      }
     }
 #pragma unroll
     for (int q = 0; q < BL; q++) {
@@ -158,6 +168,21 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
       }
     }
   }
+}
+
+template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool G3M>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
+zgemm_kernel(int m, int n, int k, double2 alpha, const double2 *__restrict__ A, int64_t lda,
+             const double2 *__restrict__ B, int64_t ldb, double2 beta, double2 *__restrict__ C, int64_t ldc) {
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int NACC = G3M ? 3 : 2;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  dbl4 acc[NACC][FM][FN];
+  zgemm_tile<BM, BN, WM, WN, GBK, G3M, false>(m, n, k, A, lda, B, ldb, m0, n0, acc);
   // the (re, im) of output element r of block (x, y)
   auto result = [&](int x, int y, int r, double &cr, double &ci) {
     if constexpr (G3M) {
@@ -175,63 +200,97 @@ zgemm_kernel(int m, int n, int k, double2 alpha,
   // first operand, so D is the transposed block: C[i][j] sits in lane
   // i + 16*(j%4), element j/4, and lanes run down a column of C (coalesced
   // column-major loads and stores).
-  if (!TRACE) {
 #pragma unroll
-    for (int x = 0; x < FM; x++)
+  for (int x = 0; x < FM; x++)
 #pragma unroll
-      for (int y = 0; y < FN; y++)
+    for (int y = 0; y < FN; y++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + li;
-          int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
-          if (row < m && col < n) {
-            double cr, ci;
-            result(x, y, r, cr, ci);
-            double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
-            double2 *p = C + (int64_t)col * ldc + row;
-            if (beta.x != 0.0 || beta.y != 0.0) {
-              double2 o = *p;
-              vr += beta.x * o.x - beta.y * o.y;
-              vi += beta.x * o.y + beta.y * o.x;
-            }
-            *p = make_double2(vr, vi);
+      for (int r = 0; r < 4; r++) {
+        int row = m0 + wr * (BM / WM) + x * 16 + li;
+        int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
+        if (row < m && col < n) {
+          double cr, ci;
+          result(x, y, r, cr, ci);
+          double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
+          double2 *p = C + (int64_t)col * ldc + row;
+          if (beta.x != 0.0 || beta.y != 0.0) {
+            double2 o = *p;
+            vr += beta.x * o.x - beta.y * o.y;
+            vi += beta.x * o.y + beta.y * o.x;
+          }
+          *p = make_double2(vr, vi);
+        }
+      }
+}
+
+// Hermitian trace: sum over i, j of conj(P_ij) E_ij for P = A^H B (A: k x n,
+// B: k x n, column-major) when P and E are both Hermitian.  The (i, j) and
+// (j, i) terms are then complex conjugates of each other, so only blocks that
+// meet the upper triangle are computed: an entry above the diagonal adds twice
+// its real part, a diagonal entry adds itself, an entry below adds nothing
+// (the imaginary part of the sum is that of the diagonal terms).  Blocks below
+// the diagonal write a zero partial.  Half the flops of the full product.
+template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool G3M>
+__global__ void __launch_bounds__(64 * WM * WN) __attribute__((amdgpu_waves_per_eu(OCC)))
+zgemm_htrace_kernel(int n, int k, const double2 *__restrict__ A, int64_t lda, const double2 *__restrict__ B,
+                    int64_t ldb, const double2 *__restrict__ E, int64_t lde, double2 *__restrict__ partial) {
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int NACC = G3M ? 3 : 2;
+  constexpr int NW = WM * WN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  if (m0 > n0 + BN - 1) {   // every row of the block below every column
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(0.0, 0.0);
+    return;
+  }
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  const int li = lane & 15, lk = lane >> 4;
+  dbl4 acc[NACC][FM][FN];
+  zgemm_tile<BM, BN, WM, WN, GBK, G3M, true>(n, n, k, A, lda, B, ldb, m0, n0, acc);
+  __shared__ double sred[2][NW];
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int x = 0; x < FM; x++)
+#pragma unroll
+    for (int y = 0; y < FN; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        // element layout as in zgemm_kernel's epilogue
+        int row = m0 + wr * (BM / WM) + x * 16 + li;
+        int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
+        if (row <= col && col < n) {
+          double cr, ci;
+          if constexpr (G3M) {
+            double t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+            cr = t1 - t2;
+            ci = acc[2][x][y][r] - t1 - t2;
+          } else {
+            cr = acc[0][x][y][r];
+            ci = acc[1][x][y][r];
+          }
+          double2 e = E[(int64_t)col * lde + row];
+          double re = cr * e.x + ci * e.y;   // conj(c) * e
+          if (row < col) {
+            sr += 2.0 * re;
+          } else {
+            sr += re;
+            si += cr * e.y - ci * e.x;
           }
         }
-  } else {
-    constexpr int NW = WM * WN;
-    __shared__ double sred[2][NW];
-    double sr = 0.0, si = 0.0;
+      }
 #pragma unroll
-    for (int x = 0; x < FM; x++)
+  for (int off = 32; off >= 1; off >>= 1) {
+    sr += __shfl_down(sr, off, 64);
+    si += __shfl_down(si, off, 64);
+  }
+  if (lane == 0) { sred[0][wave] = sr; sred[1][wave] = si; }
+  __syncthreads();
+  if (tid == 0) {
+    double tr = 0.0, ti = 0.0;
 #pragma unroll
-      for (int y = 0; y < FN; y++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + li;
-          int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
-          if (row < m && col < n) {
-            double cr, ci;
-            result(x, y, r, cr, ci);
-            double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
-            double2 e = E[(int64_t)col * lde + row];
-            // conj(v) * e
-            sr += vr * e.x + vi * e.y;
-            si += vr * e.y - vi * e.x;
-          }
-        }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      sr += __shfl_down(sr, off, 64);
-      si += __shfl_down(si, off, 64);
-    }
-    if (lane == 0) { sred[0][wave] = sr; sred[1][wave] = si; }
-    __syncthreads();
-    if (tid == 0) {
-      double tr = 0.0, ti = 0.0;
-#pragma unroll
-      for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
-      partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
-    }
+    for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
+    partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
   }
 }
 
@@ -768,14 +827,14 @@ __global__ void symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
 // (BM/WM) x (BN/WN), k staged through LDS in steps of GBK as re/im planes
 // with the next step prefetched into registers.  Four real products per
 // complex k step into two FP32 accumulators, or with G3M three (Ar Br,
-// Ai Bi, (Ar + Ai)(Br + Bi)) into three, as zgemm_kernel.  TRACE: the epilogue forms
-// sum conj(alpha C_ij) E_ij in FP64 against a complex64 E and writes one
-// FP64 partial per block.
-template <int BM, int BN, int WM, int WN, int GBK, bool TRACE, bool G3M>
-__global__ void __launch_bounds__(64 * WM * WN)
-cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, int64_t lda,
-             const float2 *__restrict__ B, int64_t ldb, float2 beta, float2 *__restrict__ C, int64_t ldc,
-             const float2 *__restrict__ E, int64_t lde, double2 *__restrict__ partial) {
+// Ai Bi, (Ar + Ai)(Br + Bi)) into three, as zgemm_kernel.
+//
+// The k loop of one block's C tile, as zgemm_tile (TA: A read as the
+// conjugate transpose of a k x m array).
+template <int BM, int BN, int WM, int WN, int GBK, bool G3M, bool TA>
+__device__ __forceinline__ void cgemm_tile(int m, int n, int k, const float2 *__restrict__ A, int64_t lda,
+                                           const float2 *__restrict__ B, int64_t ldb, int m0, int n0,
+                                           flt4 (&acc)[G3M ? 3 : 2][BM / WM / 16][BN / WN / 16]) {
   constexpr int NT = 64 * WM * WN;
   constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
   constexpr int AL = BM * GBK / NT;
@@ -787,9 +846,7 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
   const int lane = tid & 63, wave = tid >> 6;
   const int wr = wave / WN, wc = wave % WN;
   const int li = lane & 15, lk = lane >> 4;
-  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
   constexpr int NACC = G3M ? 3 : 2;
-  flt4 acc[NACC][FM][FN];
 #pragma unroll
   for (int q = 0; q < NACC; q++)
 #pragma unroll
@@ -797,15 +854,20 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
 #pragma unroll
       for (int b = 0; b < FN; b++) acc[q][a][b] = (flt4){0.f, 0.f, 0.f, 0.f};
   float2 ra[AL], rb[BL];
-  const int arow = tid % BM, acol0 = tid / BM;
-  constexpr int ASTEP = NT / BM;
+  const int arow = TA ? tid / GBK : tid % BM, acol0 = TA ? tid % GBK : tid / BM;
+  constexpr int ASTEP = TA ? NT / GBK : NT / BM;
   const int bkr = tid % GBK, bcol0 = tid / GBK;
   constexpr int BSTEP = NT / GBK;
   auto load_tiles = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < AL; q++) {
-      int col = k0 + acol0 + ASTEP * q, row = m0 + arow;
-      ra[q] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_float2(0.f, 0.f);
+      if constexpr (TA) {
+        int row = m0 + arow + ASTEP * q, kk = k0 + acol0;
+        ra[q] = (row < m && kk < k) ? A[(int64_t)row * lda + kk] : make_float2(0.f, 0.f);
+      } else {
+        int col = k0 + acol0 + ASTEP * q, row = m0 + arow;
+        ra[q] = (row < m && col < k) ? A[(int64_t)col * lda + row] : make_float2(0.f, 0.f);
+      }
     }
 #pragma unroll
     for (int q = 0; q < BL; q++) {
@@ -818,8 +880,13 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < AL; q++) {
-      sAr[acol0 + ASTEP * q][arow] = ra[q].x;
-      sAi[acol0 + ASTEP * q][arow] = ra[q].y;
+      if constexpr (TA) {
+        sAr[acol0][arow + ASTEP * q] = ra[q].x;
+        sAi[acol0][arow + ASTEP * q] = -ra[q].y;
+      } else {
+        sAr[acol0 + ASTEP * q][arow] = ra[q].x;
+        sAi[acol0 + ASTEP * q][arow] = ra[q].y;
+      }
     }
 #pragma unroll
     for (int q = 0; q < BL; q++) {
@@ -874,79 +941,142 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
       }
     }
   }
+}
+
+template <int BM, int BN, int WM, int WN, int GBK, bool G3M>
+__global__ void __launch_bounds__(64 * WM * WN)
+cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, int64_t lda,
+             const float2 *__restrict__ B, int64_t ldb, float2 beta, float2 *__restrict__ C, int64_t ldc) {
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int NACC = G3M ? 3 : 2;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  flt4 acc[NACC][FM][FN];
+  cgemm_tile<BM, BN, WM, WN, GBK, G3M, false>(m, n, k, A, lda, B, ldb, m0, n0, acc);
   // v_mfma_f32_16x16x4f32 output: D[i][j] of a 16 x 16 block in lane
   // j + 16 (i / 4), element i % 4; with the B tile as the first operand D is
   // the transposed block, so C[i][j] sits in lane i + 16 (j / 4), element
   // j % 4 and lanes run down a column of C (coalesced)
-  if (!TRACE) {
 #pragma unroll
-    for (int x = 0; x < FM; x++)
+  for (int x = 0; x < FM; x++)
 #pragma unroll
-      for (int y = 0; y < FN; y++)
+    for (int y = 0; y < FN; y++)
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + li;
-          int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
-          if (row < m && col < n) {
-            float cr, ci;
-            if constexpr (G3M) {
-              float t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
-              cr = t1 - t2;
-              ci = acc[NACC - 1][x][y][r] - t1 - t2;
-            } else {
-              cr = acc[0][x][y][r];
-              ci = acc[1][x][y][r];
-            }
-            float vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
-            float2 *p = C + (int64_t)col * ldc + row;
-            if (beta.x != 0.f || beta.y != 0.f) {
-              float2 o = *p;
-              vr += beta.x * o.x - beta.y * o.y;
-              vi += beta.x * o.y + beta.y * o.x;
-            }
-            *p = make_float2(vr, vi);
+      for (int r = 0; r < 4; r++) {
+        int row = m0 + wr * (BM / WM) + x * 16 + li;
+        int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
+        if (row < m && col < n) {
+          float cr, ci;
+          if constexpr (G3M) {
+            float t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+            cr = t1 - t2;
+            ci = acc[NACC - 1][x][y][r] - t1 - t2;
+          } else {
+            cr = acc[0][x][y][r];
+            ci = acc[1][x][y][r];
+          }
+          float vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
+          float2 *p = C + (int64_t)col * ldc + row;
+          if (beta.x != 0.f || beta.y != 0.f) {
+            float2 o = *p;
+            vr += beta.x * o.x - beta.y * o.y;
+            vi += beta.x * o.y + beta.y * o.x;
+          }
+          *p = make_float2(vr, vi);
+        }
+      }
+}
+
+// Hermitian trace in complex64, as zgemm_htrace_kernel (partials in FP64).
+template <int BM, int BN, int WM, int WN, int GBK, bool G3M>
+__global__ void __launch_bounds__(64 * WM * WN)
+cgemm_htrace_kernel(int n, int k, const float2 *__restrict__ A, int64_t lda, const float2 *__restrict__ B,
+                    int64_t ldb, const float2 *__restrict__ E, int64_t lde, double2 *__restrict__ partial) {
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  constexpr int NACC = G3M ? 3 : 2;
+  constexpr int NW = WM * WN;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  if (m0 > n0 + BN - 1) {
+    if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(0.0, 0.0);
+    return;
+  }
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  const int li = lane & 15, lk = lane >> 4;
+  flt4 acc[NACC][FM][FN];
+  cgemm_tile<BM, BN, WM, WN, GBK, G3M, true>(n, n, k, A, lda, B, ldb, m0, n0, acc);
+  __shared__ double sred[2][NW];
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int x = 0; x < FM; x++)
+#pragma unroll
+    for (int y = 0; y < FN; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        // element layout as in cgemm_kernel's epilogue
+        int row = m0 + wr * (BM / WM) + x * 16 + li;
+        int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
+        if (row <= col && col < n) {
+          double cr, ci;
+          if constexpr (G3M) {
+            double t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+            cr = t1 - t2;
+            ci = (double)acc[2][x][y][r] - t1 - t2;
+          } else {
+            cr = acc[0][x][y][r];
+            ci = acc[1][x][y][r];
+          }
+          float2 e = E[(int64_t)col * lde + row];
+          double re = cr * (double)e.x + ci * (double)e.y;
+          if (row < col) {
+            sr += 2.0 * re;
+          } else {
+            sr += re;
+            si += cr * (double)e.y - ci * (double)e.x;
           }
         }
-  } else {
-    constexpr int NW = WM * WN;
-    __shared__ double sred[2][NW];
-    double sr = 0.0, si = 0.0;
+      }
 #pragma unroll
-    for (int x = 0; x < FM; x++)
+  for (int off = 32; off >= 1; off >>= 1) {
+    sr += __shfl_down(sr, off, 64);
+    si += __shfl_down(si, off, 64);
+  }
+  if (lane == 0) { sred[0][wave] = sr; sred[1][wave] = si; }
+  __syncthreads();
+  if (tid == 0) {
+    double tr = 0.0, ti = 0.0;
 #pragma unroll
-      for (int y = 0; y < FN; y++)
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-          int row = m0 + wr * (BM / WM) + x * 16 + li;
-          int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
-          if (row < m && col < n) {
-            double cr, ci;
-            if constexpr (G3M) {
-              double t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
-              cr = t1 - t2;
-              ci = (double)acc[NACC - 1][x][y][r] - t1 - t2;
-            } else {
-              cr = acc[0][x][y][r];
-              ci = acc[1][x][y][r];
-            }
-            double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
-            float2 e = E[(int64_t)col * lde + row];
-            sr += vr * (double)e.x + vi * (double)e.y;
-            si += vr * (double)e.y - vi * (double)e.x;
-          }
-        }
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      sr += __shfl_down(sr, off, 64);
-      si += __shfl_down(si, off, 64);
-    }
-    if (lane == 0) { sred[0][wave] = sr; sred[1][wave] = si; }
-    __syncthreads();
-    if (tid == 0) {
-      double tr = 0.0, ti = 0.0;
-#pragma unroll
-      for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
-      partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
+    for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
+    partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
+  }
+}
+
+// C (n x m) <- S B^T S for B (m x n), S = diag(+1, -1, +1, ...) over the
+// interleaved [J, M] unknowns: the exterior block Z(2,1) from Z(1,2).  Two
+// disjoint bodies couple through far pairs only, whose quadrature is the same
+// rule on both triangles, so Z_ext is reciprocal to rounding there: the J-J
+// and M-M couplings are symmetric and the J-M couplings change sign under
+// transposition.  32 x 32 tiles through LDS, both sides coalesced.
+__global__ void __launch_bounds__(256) signed_transpose_kernel(const double2 *__restrict__ B, int64_t ldb, int m, int n,
+                                                               double2 *__restrict__ C, int64_t ldc) {
+  __shared__ double2 t[32][33];
+  const int b0 = blockIdx.x * 32, c0 = blockIdx.y * 32;   // rows of B, columns of B
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  for (int j = ty; j < 32; j += 8) {
+    int r = b0 + tx, c = c0 + j;
+    if (r < m && c < n) t[j][tx] = B[(int64_t)c * ldb + r];
+  }
+  __syncthreads();
+  for (int j = ty; j < 32; j += 8) {
+    int r = c0 + tx, c = b0 + j;   // C(r, c) = s_r s_c B(c, r)
+    if (r < n && c < m) {
+      double2 v = t[tx][j];
+      if ((r ^ c) & 1) v = make_double2(-v.x, -v.y);
+      C[(int64_t)c * ldc + r] = v;
     }
   }
 }
@@ -1007,21 +1137,20 @@ __global__ void identity_minus_kernel(double2 *__restrict__ Cm, int64_t ldc, int
   Cm[(int64_t)c * ldc + r] = make_double2((r == c ? 1.0 : 0.0) - v.x, -v.y);
 }
 
-template <int BM, int BN, int WM, int WN, int GBK, bool TRACE, bool G3M = false>
+template <int BM, int BN, int WM, int WN, int GBK, bool G3M = false>
 void launch_cgemm(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
-                  float2 beta, float2 *C, int64_t ldc, const float2 *E, int64_t lde, double2 *partial, hipStream_t st) {
+                  float2 beta, float2 *C, int64_t ldc, hipStream_t st) {
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
-  hipLaunchKernelGGL((cgemm_kernel<BM, BN, WM, WN, GBK, TRACE, G3M>), grid, dim3(64 * WM * WN), 0, st, m, n, k,
-                     alpha, A, lda, B, ldb, beta, C, ldc, E, lde, partial);
+  hipLaunchKernelGGL((cgemm_kernel<BM, BN, WM, WN, GBK, G3M>), grid, dim3(64 * WM * WN), 0, st, m, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc);
 }
 
-template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool TRACE, bool G3M>
-void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B,
-                         int64_t ldb, double2 beta, double2 *C, int64_t ldc, const double2 *E, int64_t lde,
-                         double2 *partial, hipStream_t st) {
+template <int BM, int BN, int WM, int WN, int GBK, int OCC, bool G3M>
+void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                  double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
   dim3 grid((m + BM - 1) / BM, (n + BN - 1) / BN);
-  hipLaunchKernelGGL((zgemm_kernel<BM, BN, WM, WN, GBK, OCC, TRACE, G3M>), grid, dim3(64 * WM * WN), 0, st, m, n, k, alpha, A,
-                     lda, B, ldb, beta, C, ldc, E, lde, partial);
+  hipLaunchKernelGGL((zgemm_kernel<BM, BN, WM, WN, GBK, OCC, G3M>), grid, dim3(64 * WM * WN), 0, st, m, n, k, alpha, A,
+                     lda, B, ldb, beta, C, ldc);
 }
 
 }  // namespace
@@ -1050,12 +1179,10 @@ int bem_zgemm_tiled(int tile, int m, int n, int k, double2 alpha, const double2 
                     int64_t ldb, double2 beta, double2 *C, int64_t ldc, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
   if (!tile) tile = bem_zgemm_default_tile(m, n);
-  const double2 *E = nullptr;
-  double2 *P = nullptr;
   switch (tile) {
-#define ZG(code, BM, BN, WM, WN, KB, OCC, G3)                                                                   \
-  case code:                                                                                                   \
-    launch_zgemm<BM, BN, WM, WN, KB, OCC, false, G3>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, E, 0, P, st); \
+#define ZG(code, BM, BN, WM, WN, KB, OCC, G3)                                                 \
+  case code:                                                                                 \
+    launch_zgemm<BM, BN, WM, WN, KB, OCC, G3>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); \
     break;
     ZG(32, 32, 32, 2, 2, 16, 1, false)
     ZG(1032, 32, 32, 2, 2, 16, 1, true)
@@ -1078,14 +1205,16 @@ int bem_zgemm_raw(int m, int n, int k, double2 alpha, const double2 *A, int64_t 
   return bem_zgemm_tiled(0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
 }
 
-// bem_trace_partials(m, n) entries; out receives sum conj(alpha A B) .* E.
-size_t bem_trace_partials(int m, int n) { return (size_t)((m + 31) / 32) * (size_t)((n + 63) / 64); }
+// Hermitian trace (zgemm_htrace_kernel): out <- sum conj(A^H B) .* E for an
+// n x n product with k-long columns, partial holding bem_htrace_partials(n).
+size_t bem_htrace_partials(int n) { return (size_t)((n + 31) / 32) * (size_t)((n + 63) / 64); }
 
-int bem_zgemm_trace(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
-                    const double2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
-  launch_zgemm<32, 64, 2, 2, 16, 2, true, true>(m, n, k, alpha, A, lda, B, ldb, make_double2(0.0, 0.0), nullptr, 0, E,
-                                                lde, partial, st);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)bem_trace_partials(m, n), out);
+int bem_zgemm_htrace(int n, int k, const double2 *A, int64_t lda, const double2 *B, int64_t ldb, const double2 *E,
+                     int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
+  dim3 grid((n + 31) / 32, (n + 63) / 64);
+  hipLaunchKernelGGL((zgemm_htrace_kernel<32, 64, 2, 2, 16, 2, true>), grid, dim3(256), 0, st, n, k, A, lda, B, ldb,
+                     E, lde, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)bem_htrace_partials(n), out);
   return (int)hipGetLastError();
 }
 
@@ -1169,16 +1298,16 @@ int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A
   if (m <= 0 || n <= 0) return 0;
   if (!tile) tile = bem_cgemm_default_tile(m, n);
   switch (tile) {
-    case 1: launch_cgemm<64, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 2: launch_cgemm<32, 32, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 3: launch_cgemm<64, 128, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 4: launch_cgemm<128, 64, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 5: launch_cgemm<128, 128, 2, 2, 16, false>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 11: launch_cgemm<64, 64, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 12: launch_cgemm<32, 32, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 13: launch_cgemm<64, 128, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 14: launch_cgemm<128, 64, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
-    case 15: launch_cgemm<128, 128, 2, 2, 16, false, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 0, nullptr, st); break;
+    case 1: launch_cgemm<64, 64, 2, 2, 16>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 2: launch_cgemm<32, 32, 2, 2, 16>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 3: launch_cgemm<64, 128, 2, 2, 16>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 4: launch_cgemm<128, 64, 2, 2, 16>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 5: launch_cgemm<128, 128, 2, 2, 16>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 11: launch_cgemm<64, 64, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 12: launch_cgemm<32, 32, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 13: launch_cgemm<64, 128, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 14: launch_cgemm<128, 64, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 15: launch_cgemm<128, 128, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
     default: return -1;
   }
   return (int)hipGetLastError();
@@ -1189,15 +1318,16 @@ int bem_cgemm_raw(int m, int n, int k, float2 alpha, const float2 *A, int64_t ld
   return bem_cgemm_tiled(0, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st);
 }
 
-size_t bem_ctrace_partials(int m, int n) { return (size_t)((m + 63) / 64) * (size_t)((n + 63) / 64); }
+size_t bem_chtrace_partials(int n) { return (size_t)((n + 63) / 64) * (size_t)((n + 63) / 64); }
 
-// out (complex128) <- sum conj(alpha A B) .* E over an m x n product, the
-// partial sums and the final sum in FP64.
-int bem_cgemm_trace(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
-                    const float2 *E, int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
-  launch_cgemm<64, 64, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, make_float2(0.f, 0.f), nullptr, 0, E, lde,
-                                       partial, st);
-  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)bem_ctrace_partials(m, n), out);
+// out (complex128) <- sum conj(A^H B) .* E over an n x n Hermitian product in
+// complex64 (cgemm_htrace_kernel), the partial sums and the final sum in FP64.
+int bem_cgemm_htrace(int n, int k, const float2 *A, int64_t lda, const float2 *B, int64_t ldb, const float2 *E,
+                     int64_t lde, double2 *partial, double2 *out, hipStream_t st) {
+  dim3 grid((n + 63) / 64, (n + 63) / 64);
+  hipLaunchKernelGGL((cgemm_htrace_kernel<64, 64, 2, 2, 16, false>), grid, dim3(256), 0, st, n, k, A, lda, B, ldb, E,
+                     lde, partial);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(256), 0, st, partial, (int)bem_chtrace_partials(n), out);
   return (int)hipGetLastError();
 }
 
@@ -1250,6 +1380,14 @@ int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m
 int bem_add_z_to_c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
   hipLaunchKernelGGL(add_z_to_c_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+int bem_signed_transpose(const double2 *B, int64_t ldb, int m, int n, double2 *C, int64_t ldc, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  if ((n + 31) / 32 > 65535) return -1;
+  hipLaunchKernelGGL(signed_transpose_kernel, dim3((m + 31) / 32, (n + 31) / 32), dim3(256), 0, st, B, ldb, m, n, C,
+                     ldc);
   return (int)hipGetLastError();
 }
 

@@ -27,11 +27,11 @@ sys.path.insert(0, ROOT)
 PEAK_FP64_TFLOPS = 78.6     # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 # complex128 tile code -> kernel instance name (template arguments as rocprof prints them)
-ZGEMM_NAMES = {32: "32, 32, 2, 2, 16, 1, false, false", 64: "64, 64, 2, 2, 16, 1, false, false",
-               256: "128, 128, 2, 4, 16, 2, false, false", 1032: "32, 32, 2, 2, 16, 1, false, true",
-               2032: "32, 32, 2, 2, 32, 1, false, true", 1033: "64, 32, 2, 2, 16, 2, false, true",
-               1034: "32, 64, 2, 2, 16, 2, false, true", 1064: "64, 64, 2, 2, 16, 2, false, true",
-               2064: "64, 64, 2, 2, 32, 2, false, true", 1065: "64, 64, 2, 2, 16, 1, false, true"}
+ZGEMM_NAMES = {32: "32, 32, 2, 2, 16, 1, false", 64: "64, 64, 2, 2, 16, 1, false",
+               256: "128, 128, 2, 4, 16, 2, false", 1032: "32, 32, 2, 2, 16, 1, true",
+               2032: "32, 32, 2, 2, 32, 1, true", 1033: "64, 32, 2, 2, 16, 2, true",
+               1034: "32, 64, 2, 2, 16, 2, true", 1064: "64, 64, 2, 2, 16, 2, true",
+               2064: "64, 64, 2, 2, 32, 2, true", 1065: "64, 64, 2, 2, 16, 1, true"}
 
 
 def zgemm_kernel_name(tiles):
