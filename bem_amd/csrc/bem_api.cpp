@@ -94,6 +94,7 @@ int bem_c2z(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, in
 int bem_add_z_to_c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st);
 int bem_signed_transpose(const double2 *B, int64_t ldb, int m, int n, double2 *C, int64_t ldc, hipStream_t st);
+int bem_fill_hash(void *p, size_t bytes, int as_float, unsigned long long seed, hipStream_t st);
 int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 }
 
@@ -399,7 +400,11 @@ static size_t inv_bytes(int n) { return sizeof(T) * 2 * 64 * 64 * (size_t)((n + 
 // updates confined to the super-block; what lies beyond it gets one
 // rank-LU_NB2 update per super-block, a GEMM with four times the arithmetic
 // intensity of a rank-64 update.
-static const int LU_NB2 = 256;
+#ifndef BEM_LU_NB2
+#define BEM_LU_NB2 256
+#endif
+static const int LU_NB2 = BEM_LU_NB2;
+static_assert(LU_NB2 % LU_NB == 0, "super-block of whole 64-wide blocks");
 
 // dst block (rows x cols, ld) <- tmp (rows x cols, ld rows)
 template <typename T>
@@ -1589,8 +1594,8 @@ static void sched_main(bem_ctx *c) {
 
 // ---------------------------------------------------------------- autotuner
 // Representative shape (m, n, k) of each GEMM class.
-static const int TUNE_SHAPE[3][3] = {{4096, 64, 64}, {4096, 4096, 64}, {2048, 2048, 2048}};
-static const int ZCANDS[] = {1032, 2032, 1033, 1034, 1064, 2064, 1065, 32, 64, 256};
+static const int TUNE_SHAPE[3][3] = {{4096, 64, 64}, {4096, 4096, 64}, {4096, 4096, 256}};
+static const int ZCANDS[] = {1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129, 32, 64, 256};
 static const int CCANDS[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15};
 
 // Mean milliseconds of `reps` launches of one tile variant on lane 0.
@@ -1648,7 +1653,7 @@ static void tune_store(const char *path, const std::string &arch, const int *til
   if (!f) return;
   std::fprintf(f, "{\"arch\": \"%s\", \"zgemm\": [%d, %d, %d], \"cgemm\": [%d, %d, %d],\n", arch.c_str(), tiles6[0],
                tiles6[1], tiles6[2], tiles6[3], tiles6[4], tiles6[5]);
-  std::fprintf(f, " \"classes\": [\"narrow 4096x64x64\", \"rank update 4096x4096x64\", \"general 2048^3\"],\n");
+  std::fprintf(f, " \"classes\": [\"narrow 4096x64x64\", \"rank update 4096x4096x64\", \"general 4096x4096x256\"],\n");
   for (int c64 = 0; c64 < 2; c64++) {
     std::fprintf(f, " \"%s_ms\": [", c64 ? "cgemm" : "zgemm");
     for (int cls = 0; cls < 3; cls++) {
@@ -2064,20 +2069,27 @@ int bem_autotune(bem_ctx *c, const char *cache_path, int flags, int *tiles6) {
   int r = 0;
   if (hipMalloc(&A, bytes) != hipSuccess || hipMalloc(&B, bytes) != hipSuccess || hipMalloc(&Cm, bytes) != hipSuccess)
     r = fail(c, BEM_E_NOMEM, "autotune scratch (3 x %zu bytes)", bytes);
-  // operands: any finite values; zeros keep every variant's output finite
-  if (!r && (hipMemsetAsync(A, 0, bytes, L.st) || hipMemsetAsync(B, 0, bytes, L.st) ||
-             hipMemsetAsync(Cm, 0, bytes, L.st)))
-    r = fail(c, BEM_E_DEVICE, "autotune scratch fill");
+  // operands: hashed values in [-1, 1) of the timed precision, so the MFMAs
+  // see realistic bit patterns (all-zero operands draw less power and time
+  // faster than data)
+  auto fill = [&](int as_float) {
+    if (!r && (bem_fill_hash(A, bytes, as_float, 1, L.st) || bem_fill_hash(B, bytes, as_float, 2, L.st) ||
+               bem_fill_hash(Cm, bytes, as_float, 3, L.st)))
+      r = fail(c, BEM_E_DEVICE, "autotune scratch fill");
+  };
   double ms[6][16];
+  static_assert(sizeof ZCANDS / sizeof ZCANDS[0] <= 16 && sizeof CCANDS / sizeof CCANDS[0] <= 16, "timing table");
   const int nz = (int)(sizeof ZCANDS / sizeof ZCANDS[0]), nc = (int)(sizeof CCANDS / sizeof CCANDS[0]);
   for (int cls = 0; cls < 3 && !r; cls++) {
     best[cls] = 0;
     best[3 + cls] = 0;
     double bz = 1e30, bc = 1e30;
+    fill(0);
     for (int i = 0; i < nz && !r; i++) {
       r = tune_time(c, L, false, ZCANDS[i], cls, A, B, Cm, 3, &ms[cls][i]);
       if (!r && ms[cls][i] < bz) { bz = ms[cls][i]; best[cls] = ZCANDS[i]; }
     }
+    fill(1);
     for (int i = 0; i < nc && !r; i++) {
       r = tune_time(c, L, true, CCANDS[i], cls, A, B, Cm, 3, &ms[3 + cls][i]);
       if (!r && ms[3 + cls][i] < bc) { bc = ms[3 + cls][i]; best[3 + cls] = CCANDS[i]; }

@@ -199,28 +199,41 @@ zgemm_kernel(int m, int n, int k, double2 alpha, const double2 *__restrict__ A, 
   // differs).  The products above run with the B tile as the instruction's
   // first operand, so D is the transposed block: C[i][j] sits in lane
   // i + 16*(j%4), element j/4, and lanes run down a column of C (coalesced
-  // column-major loads and stores).
+  // column-major loads and stores).  With beta != 0 the old C values of one
+  // row of 16 x 16 blocks are loaded first, from clamped addresses and
+  // without branches, so those loads are in flight together.
+  const bool rd = beta.x != 0.0 || beta.y != 0.0;
 #pragma unroll
-  for (int x = 0; x < FM; x++)
+  for (int x = 0; x < FM; x++) {
+    const int row = m0 + wr * (BM / WM) + x * 16 + li;
+    double2 old[FN][4];
+    if (rd) {
+#pragma unroll
+      for (int y = 0; y < FN; y++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
+          old[y][r] = C[(row < m && col < n) ? (int64_t)col * ldc + row : 0];
+        }
+    }
 #pragma unroll
     for (int y = 0; y < FN; y++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = m0 + wr * (BM / WM) + x * 16 + li;
         int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
         if (row < m && col < n) {
           double cr, ci;
           result(x, y, r, cr, ci);
           double vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
-          double2 *p = C + (int64_t)col * ldc + row;
-          if (beta.x != 0.0 || beta.y != 0.0) {
-            double2 o = *p;
+          if (rd) {
+            double2 o = old[y][r];
             vr += beta.x * o.x - beta.y * o.y;
             vi += beta.x * o.y + beta.y * o.x;
           }
-          *p = make_double2(vr, vi);
+          C[(int64_t)col * ldc + row] = make_double2(vr, vi);
         }
       }
+  }
 }
 
 // Hermitian trace: sum over i, j of conj(P_ij) E_ij for P = A^H B (A: k x n,
@@ -756,16 +769,25 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
 template <typename T>
 __global__ void absmax_kernel(const T *__restrict__ A, int64_t lda, int m, int n, int upper,
                               unsigned long long *__restrict__ out) {
+  // 256 threads; the block's maximum goes out in one atomic (a non-negative
+  // double orders as its bit pattern)
+  __shared__ double wmax[4];
   double v = 0.0;
   for (int c = blockIdx.y; c < n; c += gridDim.y) {
     int rmax = upper ? min(m, c + 1) : m;
+#pragma unroll 4
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rmax; r += gridDim.x * blockDim.x) {
       T x = A[(int64_t)c * lda + r];
       v = fmax(v, std::fabs((double)x.x) + std::fabs((double)x.y));
     }
   }
   for (int off = 32; off >= 1; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    v = fmax(fmax(wmax[0], wmax[1]), fmax(wmax[2], wmax[3]));
+    atomicMax(out, (unsigned long long)__double_as_longlong(v));
+  }
 }
 
 // Column interchanges for X <- X P^T with P = P_k0 .. P_{k1-1} the row
@@ -804,20 +826,39 @@ __global__ void laswp_kernel(T *__restrict__ A, int64_t lda, int c0, int c1, int
   }
 }
 
-__global__ void symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  int j = blockIdx.y;
-  if (i >= n || i < j) return;
-  if (i == j) {
-    double2 a = G[(int64_t)j * ld + i];
-    G[(int64_t)j * ld + i] = make_double2(a.x + a.x, 0.0);
-    return;
+// G <- G + G^H in place, by 32 x 32 tile pairs (lower tile (bi, bj), bi >= bj,
+// and its mirror) staged through LDS so both sides are read and written
+// coalesced.  Each block owns its pair; diagonal tiles are their own mirror.
+__global__ void __launch_bounds__(256) symm_kernel(double2 *__restrict__ G, int64_t ld, int n) {
+  const int bi = blockIdx.x, bj = blockIdx.y;
+  if (bi < bj) return;
+  __shared__ double2 lo[32][33], up[32][33];   // lo[c][r] = G(r0+r, c0+c), up[c][r] = G(c0+r, r0+c)
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+  const int r0 = bi * 32, c0 = bj * 32;
+  for (int c = ty; c < 32; c += 8) {
+    if (r0 + tx < n && c0 + c < n) lo[c][tx] = G[(int64_t)(c0 + c) * ld + r0 + tx];
+    if (c0 + tx < n && r0 + c < n) up[c][tx] = G[(int64_t)(r0 + c) * ld + c0 + tx];
   }
-  double2 a = G[(int64_t)j * ld + i];   // (i, j), lower
-  double2 b = G[(int64_t)i * ld + j];   // (j, i), upper
-  double2 s = make_double2(a.x + b.x, a.y - b.y);   // a + conj(b)
-  G[(int64_t)j * ld + i] = s;
-  G[(int64_t)i * ld + j] = make_double2(s.x, -s.y);
+  __syncthreads();
+  // lower side and diagonal: G(i, j) = G(i, j) + conj(G(j, i)), i = r0 + tx, j = c0 + c
+  for (int c = ty; c < 32; c += 8) {
+    const int i = r0 + tx, j = c0 + c;
+    if (i >= n || j >= n || i < j) continue;
+    double2 a = lo[c][tx];
+    if (i == j) {
+      G[(int64_t)j * ld + i] = make_double2(a.x + a.x, 0.0);
+    } else {
+      double2 b = up[tx][c];
+      G[(int64_t)j * ld + i] = make_double2(a.x + b.x, a.y - b.y);
+    }
+  }
+  // upper side: G(j, i) = conj of the new G(i, j), j = c0 + tx, i = r0 + c
+  for (int c = ty; c < 32; c += 8) {
+    const int i = r0 + c, j = c0 + tx;
+    if (i >= n || j >= n || i <= j) continue;
+    double2 a = lo[tx][c], b = up[c][tx];
+    G[(int64_t)i * ld + j] = make_double2(a.x + b.x, b.y - a.y);
+  }
 }
 
 
@@ -959,14 +1000,26 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
   // v_mfma_f32_16x16x4f32 output: D[i][j] of a 16 x 16 block in lane
   // j + 16 (i / 4), element i % 4; with the B tile as the first operand D is
   // the transposed block, so C[i][j] sits in lane i + 16 (j / 4), element
-  // j % 4 and lanes run down a column of C (coalesced)
+  // j % 4 and lanes run down a column of C (coalesced).  Old C values are
+  // loaded a row of blocks at a time, as in zgemm_kernel.
+  const bool rd = beta.x != 0.f || beta.y != 0.f;
 #pragma unroll
-  for (int x = 0; x < FM; x++)
+  for (int x = 0; x < FM; x++) {
+    const int row = m0 + wr * (BM / WM) + x * 16 + li;
+    float2 old[FN][4];
+    if (rd) {
+#pragma unroll
+      for (int y = 0; y < FN; y++)
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+          int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
+          old[y][r] = C[(row < m && col < n) ? (int64_t)col * ldc + row : 0];
+        }
+    }
 #pragma unroll
     for (int y = 0; y < FN; y++)
 #pragma unroll
       for (int r = 0; r < 4; r++) {
-        int row = m0 + wr * (BM / WM) + x * 16 + li;
         int col = n0 + wc * (BN / WN) + y * 16 + 4 * lk + r;
         if (row < m && col < n) {
           float cr, ci;
@@ -979,15 +1032,15 @@ cgemm_kernel(int m, int n, int k, float2 alpha, const float2 *__restrict__ A, in
             ci = acc[1][x][y][r];
           }
           float vr = alpha.x * cr - alpha.y * ci, vi = alpha.x * ci + alpha.y * cr;
-          float2 *p = C + (int64_t)col * ldc + row;
-          if (beta.x != 0.f || beta.y != 0.f) {
-            float2 o = *p;
+          if (rd) {
+            float2 o = old[y][r];
             vr += beta.x * o.x - beta.y * o.y;
             vi += beta.x * o.y + beta.y * o.x;
           }
-          *p = make_float2(vr, vi);
+          C[(int64_t)col * ldc + row] = make_float2(vr, vi);
         }
       }
+  }
 }
 
 // Hermitian trace in complex64, as zgemm_htrace_kernel (partials in FP64).
@@ -1052,6 +1105,19 @@ cgemm_htrace_kernel(int n, int k, const float2 *__restrict__ A, int64_t lda, con
 #pragma unroll
     for (int w = 0; w < NW; w++) { tr += sred[0][w]; ti += sred[1][w]; }
     partial[blockIdx.y * gridDim.x + blockIdx.x] = make_double2(tr, ti);
+  }
+}
+
+// p[i] <- a value in [-1, 1) hashed from (seed, i) (splitmix64 finaliser):
+// operands with realistic bit patterns for timing runs (T double or float).
+template <typename T>
+__global__ void fill_hash_kernel(T *__restrict__ p, size_t n, unsigned long long seed) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+    unsigned long long z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    p[i] = (T)((double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0);
   }
 }
 
@@ -1162,10 +1228,11 @@ extern "C" {
 // tiles, 256 the four-product 128 x 128 tile on eight waves; the
 // three-product ones are 1032 (32 x 32), 2032 (32 x 32, k step 32), 1033
 // (64 x 32), 1034 (32 x 64), 1064 (64 x 64), 2064 (64 x 64, k step 32), 1065
-// (64 x 64, one wave per SIMD).  Default rule (measured on gfx950 at
+// (64 x 64, one wave per SIMD), 2034 (32 x 64, k step 32), and on eight waves
+// 1066 (64 x 64), 1128 (64 x 128), 1129 (128 x 64).  Default rule (measured on gfx950 at
 // lda = 3840): 1032 when either output dimension is at most 128, else 1034;
 // the context's autotuner may replace it per shape class.
-static const int ZTILES[] = {32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065};
+static const int ZTILES[] = {32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129};
 
 int bem_zgemm_default_tile(int m, int n) { return (m <= 128 || n <= 128) ? 1032 : 1034; }
 
@@ -1192,6 +1259,10 @@ int bem_zgemm_tiled(int tile, int m, int n, int k, double2 alpha, const double2 
     ZG(1064, 64, 64, 2, 2, 16, 2, true)
     ZG(2064, 64, 64, 2, 2, 32, 2, true)
     ZG(1065, 64, 64, 2, 2, 16, 1, true)
+    ZG(2034, 32, 64, 2, 2, 32, 2, true)
+    ZG(1066, 64, 64, 2, 4, 16, 2, true)
+    ZG(1128, 64, 128, 2, 4, 16, 2, true)
+    ZG(1129, 128, 64, 4, 2, 16, 2, true)
     ZG(256, 128, 128, 2, 4, 16, 2, false)
     ZG(64, 64, 64, 2, 2, 16, 1, false)
 #undef ZG
@@ -1279,7 +1350,10 @@ int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int
 }
 
 int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
-  hipLaunchKernelGGL(symm_kernel, dim3((n + 255) / 256, n), dim3(256), 0, st, G, ld, n);
+  if (n <= 0) return 0;
+  const int nt = (n + 31) / 32;
+  if (nt > 65535) return -1;
+  hipLaunchKernelGGL(symm_kernel, dim3(nt, nt), dim3(256), 0, st, G, ld, n);
   return (int)hipGetLastError();
 }
 
@@ -1380,6 +1454,17 @@ int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m
 int bem_add_z_to_c(const double2 *src, int64_t lds, float2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
   hipLaunchKernelGGL(add_z_to_c_kernel, copy_grid(m, n), dim3(256), 0, st, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+// bytes of p filled with hashed doubles (as_float = 0) or floats (1) in [-1, 1)
+int bem_fill_hash(void *p, size_t bytes, int as_float, unsigned long long seed, hipStream_t st) {
+  if (bytes == 0) return 0;
+  if (as_float)
+    hipLaunchKernelGGL(fill_hash_kernel<float>, dim3(2048), dim3(256), 0, st, (float *)p, bytes / sizeof(float), seed);
+  else
+    hipLaunchKernelGGL(fill_hash_kernel<double>, dim3(2048), dim3(256), 0, st, (double *)p, bytes / sizeof(double),
+                       seed);
   return (int)hipGetLastError();
 }
 

@@ -31,7 +31,9 @@ ZGEMM_NAMES = {32: "32, 32, 2, 2, 16, 1, false", 64: "64, 64, 2, 2, 16, 1, false
                256: "128, 128, 2, 4, 16, 2, false", 1032: "32, 32, 2, 2, 16, 1, true",
                2032: "32, 32, 2, 2, 32, 1, true", 1033: "64, 32, 2, 2, 16, 2, true",
                1034: "32, 64, 2, 2, 16, 2, true", 1064: "64, 64, 2, 2, 16, 2, true",
-               2064: "64, 64, 2, 2, 32, 2, true", 1065: "64, 64, 2, 2, 16, 1, true"}
+               2064: "64, 64, 2, 2, 32, 2, true", 1065: "64, 64, 2, 2, 16, 1, true",
+               2034: "32, 64, 2, 2, 32, 2, true", 1066: "64, 64, 2, 4, 16, 2, true",
+               1128: "64, 128, 2, 4, 16, 2, true", 1129: "128, 64, 4, 2, 16, 2, true"}
 
 
 def zgemm_kernel_name(tiles):
@@ -151,7 +153,7 @@ def run_batch(args):
     context's lanes; throughput over the whole batch."""
     import bem_amd
     from bem_amd import Request
-    inst = batch_instances(args.requests)
+    inst = batch_instances(args.requests or 256)
     prec, flags = PRECISIONS[args.precision]
     ctx = bem_amd.Context(0, lanes=args.lanes)
     tune(ctx, args)
@@ -212,7 +214,7 @@ def serving_stream(n, rate, seed=0):
 def run_serving(args):
     import bem_amd
     from bem_amd import Request
-    pairs, stream = serving_stream(args.requests, args.rate)
+    pairs, stream = serving_stream(args.requests or 64, args.rate)
     ctx = bem_amd.Context(0, lanes=args.lanes)
     tune(ctx, args)
     bodies = [(ctx.body(m1), ctx.body(m2)) for _, m1, m2, _ in pairs]
@@ -331,14 +333,28 @@ def main():
                     help="GEMM tiles: from the autotuner cache (measured when missing), re-measured, or built-in rule")
     ap.add_argument("--no-reference", action="store_true",
                     help="skip the FP64 reference solves that give the error of a mixed run")
-    ap.add_argument("--requests", type=int, default=64, help="c5: number of requests in the stream")
+    ap.add_argument("--requests", type=int, default=None,
+                    help="c4: instances in the batch (default 256); c5: requests in the stream (default 64)")
     ap.add_argument("--rate", type=float, default=200.0, help="c5: mean arrival rate (requests/s)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy three times: the reference kernel that scales "
+                         "the FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU (asked for {args.gpus}); "
-              "independent (frequency, gap) solves would shard across GPUs by request, see DESIGN.md",
+              "independent (frequency, gap) solves would shard across GPUs by request, one context per GPU",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        n = 256 * 1024 * 1024 // 8
+        x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 3}))
+        return
 
     if args.workload == "c5":
         print(json.dumps(run_serving(args)))

@@ -180,13 +180,16 @@ int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void
 /* ---- GEMM tiles and the autotuner ----------------------------------- */
 /* Tile variant per GEMM shape class {narrow: an output dimension <= 128,
  * rank update: k <= 128, general} for complex128 (tiles[0..2]; codes 32, 64,
- * 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065) and complex64 (tiles[3..5];
+ * 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129) and
+ * complex64 (tiles[3..5];
  * codes 1..5 four real products per complex multiply-add, 11..15 the same
  * tiles with three); 0 selects the built-in shape rule.  set6 / get6 may be
  * NULL. */
 int bem_gemm_tiles(bem_ctx *ctx, const int *set6, int *get6);
 /* Times every tile variant on a representative shape of each class (4096 x
- * 64 x 64, 4096 x 4096 x 64, 2048^3) and installs the fastest per class.
+ * 64 x 64, 4096 x 4096 x 64, and 4096 x 4096 x 256, the rank-256 trailing
+ * update that dominates the general class of the two-level LU and solves) and
+ * installs the fastest per class.
  * With cache_path, a cache written for the same GPU architecture is reused
  * (unless flags & BEM_TUNE_FORCE) and a new measurement is written there as
  * JSON with every timing.  tiles6 (optional) receives the choice. */

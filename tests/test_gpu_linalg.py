@@ -51,7 +51,7 @@ def test_lu_factors(gpu_ctx):
     assert np.abs(L @ U - PA).max() < 1e-12 * np.abs(A).max() * n
 
 
-ZTILES = [32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065]
+ZTILES = [32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129]
 
 
 @pytest.mark.parametrize("tile", ZTILES)
