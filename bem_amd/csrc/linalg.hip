@@ -604,9 +604,16 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
   const int w = min(64, n - k0);
   const int tid = threadIdx.x, ri = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
   T *Ak = A + (int64_t)k0 * lda + k0;
-  for (int idx = tid; idx < 64 * 64; idx += 256) {
-    int r = idx & 63, c = idx >> 6;
-    sA[idx] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : Cplx<T>::mk(r == c ? 1.0 : 0.0, 0.0);
+  {
+    // all 16 loads of a thread in flight before the LDS stores
+    T v[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
+      v[q] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : Cplx<T>::mk(r == c ? 1.0 : 0.0, 0.0);
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) sA[tid + 256 * q] = v[q];
   }
   __syncthreads();
   for (int j = 0; j < w; j++) {
