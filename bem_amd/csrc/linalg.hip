@@ -583,89 +583,160 @@ __global__ void __launch_bounds__(256) diag_block_kernel(double2 *__restrict__ A
 // (k0, k0) is factored with partial pivoting over its own rows (izamax
 // criterion |re|+|im|), the reference's block LU over tree blocks pivots the
 // same way (src/util/HMatrix.h:277-316).  One 256-thread block:
-//  1. LU with pivoting in LDS: wave 0 finds the pivot, rows are swapped in LDS,
-//     the multipliers go to a column buffer, all threads update the trailing
-//     block (three barriers per column); interchanges to ipiv[k0..k0+w) as
-//     absolute row indices; the packed factors are written back to A.
+//  1. LU with pivoting in registers: thread (row r, wave g) holds row r of
+//     the columns c = g + 4t, t < 16.  At column j the wave holding it finds
+//     the pivot among the rows not yet chosen (ties to the lowest row
+//     position), forms the multipliers and publishes them with the pivot's
+//     lane through LDS (double-buffered by the parity of j, one barrier per
+//     column); every wave then reads the pivot row's entries of its columns
+//     with readlane and updates its rows.  Rows are not moved: each thread
+//     tracks its row's position under the LAPACK sequence of interchanges,
+//     which goes to ipiv[k0..k0+w) as absolute row indices, and the packed
+//     factors are stored by position (LDS, then A).
 //  2. Inverses of both triangles by recursive doubling over 1, 2, 4, .. 32
 //     blocks: Linv21 = -Linv22 L21 Linv11, Uinv12 = -Uinv11 U12 Uinv22, each
 //     level two small products, three barriers per level.
 //  Output: inv (64 x 64 Linv then 64 x 64 Uinv, column-major, identity
 //  padded when w < 64).
+// Lane l's copy of x (l wave-uniform): readlane on each 32-bit half.
+__device__ __forceinline__ double lane_read(double x, int l) {
+  long long b = __double_as_longlong(x);
+  int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ float lane_read(float x, int l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+// State of one thread of the register-resident block LU: row ri's entries in
+// the columns wave + 4t (a[t]), the row's position under the interchanges so
+// far, and whether it has been chosen as a pivot.  The column loop and the
+// update loop are unrolled by template recursion, so every index into a[] is
+// a constant and the row stays in registers.
+template <typename T>
+struct LuRow {
+  T a[16];
+  int pos;
+  bool done;
+};
+
+// rank-1 update of this thread's column indices [TT, END) except SKIP, those
+// with c > J, by the multiplier l and the pivot row held by lane p of the
+// same wave
+template <int TT, int END, int J, int SKIP, typename T>
+__device__ __forceinline__ void lu_update(LuRow<T> &r, int wave, int p, T l) {
+  if constexpr (TT < END) {
+    if (TT != SKIP && wave + 4 * TT > J) {
+      const T uc = Cplx<T>::mk(lane_read(r.a[TT].x, p), lane_read(r.a[TT].y, p));
+      T x = r.a[TT];
+      x.x -= l.x * uc.x - l.y * uc.y;
+      x.y -= l.x * uc.y + l.y * uc.x;
+      // select, not branch: rows already chosen as pivots keep their U entries
+      r.a[TT].x = r.done ? r.a[TT].x : x.x;
+      r.a[TT].y = r.done ? r.a[TT].y : x.y;
+    }
+    lu_update<TT + 1, END, J, SKIP>(r, wave, p, l);
+  }
+}
+
+// Column J's pivot by the wave holding it (column J already updated): max
+// |re|+|im| over the rows not yet chosen, ties to the lowest position; the
+// multipliers replace the column below the pivot and go to lbuf with the
+// pivot's lane to pbuf (buffers of J's parity); the interchange to ipiv.
+template <int J, typename T>
+__device__ __forceinline__ void lu_pivot(LuRow<T> &r, int ri, int k0, int *__restrict__ ipiv, int *__restrict__ info,
+                                         T (*lbuf)[64], int *pbuf) {
+  using R = typename Cplx<T>::R;
+  constexpr int t = J >> 2, b = J & 1;
+  R v = r.done ? (R)-1.0 : std::fabs(r.a[t].x) + std::fabs(r.a[t].y);
+  int pp = r.pos, who = ri;
+  for (int off = 32; off >= 1; off >>= 1) {
+    R ov = __shfl_xor(v, off, 64);
+    int op = __shfl_xor(pp, off, 64), ow = __shfl_xor(who, off, 64);
+    if (ov > v || (ov == v && op < pp)) { v = ov; pp = op; who = ow; }
+  }
+  if (!(v > 0.0)) {   // zero column: keep the row at position J, flag it
+    who = __builtin_ctzll(__ballot(r.pos == J));
+    pp = J;
+    if (ri == 0 && info && *info == 0) *info = k0 + J + 1;
+  }
+  who = __builtin_amdgcn_readfirstlane(who);
+  const T u = Cplx<T>::mk(lane_read(r.a[t].x, who), lane_read(r.a[t].y, who));
+  const R den = u.x * u.x + u.y * u.y;
+  T l = Cplx<T>::mk(0.0, 0.0);
+  if (!r.done && ri != who && den > 0.0) {
+    T x = r.a[t];
+    R ur = u.x / den, ui = -u.y / den;
+    l = Cplx<T>::mk(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
+  }
+  if (!r.done && ri != who) r.a[t] = l;   // L below the pivot, in place
+  lbuf[b][ri] = l;
+  if (ri == 0) {
+    pbuf[b] = who;
+    ipiv[k0 + J] = k0 + pp;
+  }
+}
+
+// Step J of the block LU (then J + 1, ..): one barrier publishes column J's
+// pivot; every wave updates its columns.  The wave holding column J + 1
+// updates that column first and finds its pivot before its other columns, so
+// the next pivot search overlaps the other waves' updates.
+template <int J, typename T>
+__device__ __forceinline__ void lu_column(LuRow<T> &r, int w, int wave, int ri, int k0, int *__restrict__ ipiv,
+                                          int *__restrict__ info, T (*lbuf)[64], int *pbuf) {
+  if constexpr (J < 64) {
+    constexpr int b = J & 1;
+    if (J < w) {
+      if (J == 0 && wave == 0) lu_pivot<0>(r, ri, k0, ipiv, info, lbuf, pbuf);
+      __syncthreads();
+      const int p = __builtin_amdgcn_readfirstlane(pbuf[b]);
+      const T l = lbuf[b][ri];
+      const int ppos = __builtin_amdgcn_readlane(r.pos, p);
+      if (r.pos == J) r.pos = ppos;   // interchange of positions J and ppos
+      if (ri == p) {
+        r.pos = J;
+        r.done = true;
+      }
+      if constexpr (J + 1 < 64) {
+        if (J + 1 < w && wave == ((J + 1) & 3)) {
+          constexpr int tn = (J + 1) >> 2;
+          lu_update<tn, tn + 1, J, -1>(r, wave, p, l);   // column J + 1 (index tn in this wave)
+          lu_pivot<J + 1>(r, ri, k0, ipiv, info, lbuf, pbuf);
+          lu_update<0, 16, J, tn>(r, wave, p, l);
+        } else {
+          lu_update<0, 16, J, -1>(r, wave, p, l);
+        }
+      }
+    }
+    lu_column<J + 1>(r, w, wave, ri, k0, ipiv, info, lbuf, pbuf);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int64_t lda, int n, int k0,
                                                           T *__restrict__ inv, int *__restrict__ ipiv,
                                                           int *__restrict__ info) {
   using R = typename Cplx<T>::R;
-  __shared__ T sA[64 * 64];   // column-major: packed LU, later workspace
+  __shared__ T sA[64 * 64];   // column-major: packed LU by position, later workspace
   __shared__ T sX[64 * 64];   // packed inverses (Linv strictly below, Uinv on/above)
-  __shared__ T lcol[64];
-  __shared__ int s_piv;
+  __shared__ T lbuf[2][64];   // multipliers of column j, by row (parity of j)
+  __shared__ int pbuf[2];     // pivot row of column j (parity of j)
   const int w = min(64, n - k0);
-  const int tid = threadIdx.x, ri = tid & 63, cg = tid >> 6, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, ri = tid & 63, wave = tid >> 6;
   T *Ak = A + (int64_t)k0 * lda + k0;
-  {
-    // all 16 loads of a thread in flight before the LDS stores
-    T v[16];
+  LuRow<T> row;   // row ri, columns wave + 4t (identity padded beyond w)
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-      const int idx = tid + 256 * q, r = idx & 63, c = idx >> 6;
-      v[q] = (r < w && c < w) ? Ak[(int64_t)c * lda + r] : Cplx<T>::mk(r == c ? 1.0 : 0.0, 0.0);
-    }
+  for (int t = 0; t < 16; t++) {
+    const int c = wave + 4 * t;
+    row.a[t] = (ri < w && c < w) ? Ak[(int64_t)c * lda + ri] : Cplx<T>::mk(ri == c ? 1.0 : 0.0, 0.0);
+  }
+  row.pos = ri;
+  row.done = false;
+  lu_column<0>(row, w, wave, ri, k0, ipiv, info, lbuf, pbuf);
+  // packed factors by position
 #pragma unroll
-    for (int q = 0; q < 16; q++) sA[tid + 256 * q] = v[q];
-  }
-  __syncthreads();
-  for (int j = 0; j < w; j++) {
-    if (wave == 0) {
-      R v = (ri >= j && ri < w) ? std::fabs(sA[j * 64 + ri].x) + std::fabs(sA[j * 64 + ri].y) : -1.0;
-      int id = ri;
-      for (int off = 32; off >= 1; off >>= 1) {
-        R ov = __shfl_xor(v, off, 64);
-        int oi = __shfl_xor(id, off, 64);
-        if (ov > v || (ov == v && oi < id)) { v = ov; id = oi; }
-      }
-      if (lane == 0) {
-        if (!(v > 0.0)) {   // zero column: keep the row, flag it
-          id = j;
-          if (info && *info == 0) *info = k0 + j + 1;
-        }
-        s_piv = id;
-        ipiv[k0 + j] = k0 + id;
-      }
-    }
-    __syncthreads();
-    const int p = s_piv;
-    if (p != j && tid < 64) {
-      T t = sA[tid * 64 + j];
-      sA[tid * 64 + j] = sA[tid * 64 + p];
-      sA[tid * 64 + p] = t;
-    }
-    __syncthreads();
-    const T u = sA[j * 64 + j];
-    const R den = u.x * u.x + u.y * u.y;
-    if (cg == 0 && ri > j) {
-      T l = Cplx<T>::mk(0.0, 0.0);
-      if (den > 0.0) {
-        T x = sA[j * 64 + ri];
-        R ur = u.x / den, ui = -u.y / den;
-        l = Cplx<T>::mk(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
-      }
-      sA[j * 64 + ri] = l;
-      lcol[ri] = l;
-    }
-    __syncthreads();
-    if (ri > j) {
-      const T l = lcol[ri];
-      for (int c = j + 1 + cg; c < 64; c += 4) {
-        T uc = sA[c * 64 + j];
-        T x = sA[c * 64 + ri];
-        x.x -= l.x * uc.x - l.y * uc.y;
-        x.y -= l.x * uc.y + l.y * uc.x;
-        sA[c * 64 + ri] = x;
-      }
-    }
-  }
+  for (int t = 0; t < 16; t++) sA[(wave + 4 * t) * 64 + row.pos] = row.a[t];
   __syncthreads();
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
