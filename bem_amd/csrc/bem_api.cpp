@@ -6,8 +6,12 @@
 // X = W^-1[0:m1, m1:M].  Only X is needed, so the solve here is the block
 // elimination of W = [[A, B], [C, D]]:
 //   Y = A^-1 B,  Sc = D - C Y,  X = -Y Sc^-1
-// with tournament-pivoted LU of A and of Sc, followed by B1 = sym(G1) X and a
-// GEMM whose epilogue contracts X sym(G2) against B1 (no B2 stored).
+// with blocked LU of A and of Sc (partial pivoting inside each 64 x 64
+// diagonal block, tournament pivoting when the growth factor says so),
+// followed by B1 = sym(G1) X and the trace: since sym(G) = G + G^H is
+// Hermitian, 4 S = tr(sym(G2) X^H B1) with X^H B1 Hermitian, so one GEMM forms
+// the upper half of X^H B1 and contracts it against sym(G2) in its epilogue
+// (no B2 stored).
 //
 // Concurrency.  A context owns `lanes` HIP streams, each with its own named
 // workspaces.  A sweep keeps up to `lanes` independent solves in flight: the
@@ -602,8 +606,8 @@ static int lu_factor_block2(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *
 }
 
 // Blocked right-looking LU with partial pivoting inside each 64 x 64
-// diagonal block (one kernel factors the block in LDS and inverts its
-// triangles); the row interchanges are then applied to the other columns and
+// diagonal block (one kernel factors the block, rows in registers, and inverts
+// its triangles); the row interchanges are then applied to the other columns and
 // the panel's triangular solves and the trailing update run as GEMMs.
 // Look-ahead: the trailing update first covers the next 64 columns, then the
 // next diagonal block is factored on the lane's auxiliary stream while the
@@ -1596,7 +1600,7 @@ static void sched_main(bem_ctx *c) {
 // Representative shape (m, n, k) of each GEMM class.
 static const int TUNE_SHAPE[3][3] = {{4096, 64, 64}, {4096, 4096, 64}, {4096, 4096, 256}};
 static const int ZCANDS[] = {1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129, 32, 64, 256};
-static const int CCANDS[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15};
+static const int CCANDS[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17};
 
 // Mean milliseconds of `reps` launches of one tile variant on lane 0.
 static int tune_time(bem_ctx *c, Lane &L, bool c64, int tile, int cls, void *A, void *B, void *Cm, int reps,
@@ -2120,6 +2124,33 @@ int bem_zgetrf(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv) {
   DEV(c, hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, L.st));
   DEV(c, hipStreamSynchronize(L.st));
   ev_collect(c, L);
+  if (h) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", h - 1);
+  return 0;
+}
+
+int bem_zgetrf_local(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv, double *growth) {
+  if (!c || !A || !ipiv || n < 0 || lda < n) return fail(c, BEM_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  stats_reset(c);
+  int *info;
+  unsigned long long *gro;
+  double2 *inv;
+  CHK(ws_get(c, L, "info_getrf", sizeof(int), (void **)&info));
+  CHK(ws_get(c, L, "growth_getrf", sizeof(unsigned long long) * 2, (void **)&gro));
+  CHK(ws_get(c, L, "inv_getrf", inv_bytes(n), (void **)&inv));
+  DEV(c, hipMemsetAsync(info, 0, sizeof(int), L.st));
+  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 2, L.st));
+  CHK(lu_factor_block(c, L, (double2 *)A, lda, n, ipiv, info, inv, gro));
+  int h = 0;
+  unsigned long long g[2] = {0, 0};
+  DEV(c, hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(g, gro, sizeof g, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  ev_collect(c, L);
+  if (growth) *growth = growth_ratio(g);
   if (h) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", h - 1);
   return 0;
 }

@@ -72,8 +72,8 @@ _lib = None
 # every symbol include/bem_hip.h declares
 EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_device_bytes", "bem_body_upload",
            "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
-           "bem_sweep", "bem_zgetrf", "bem_zgetrs", "bem_zgemm", "bem_device_alloc", "bem_device_free",
-           "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
+           "bem_sweep", "bem_zgetrf", "bem_zgetrf_local", "bem_zgetrs", "bem_zgemm", "bem_device_alloc",
+           "bem_device_free", "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
            "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune"]
 
 
@@ -104,6 +104,7 @@ def load_library():
     L.bem_spectral_power.argtypes = [vp, vp, vp, P(_Req), P(_Res)]
     L.bem_sweep.argtypes = [vp, vp, vp, P(_Req), i32, P(_Res)]
     L.bem_zgetrf.argtypes = [vp, vp, i64, i32, vp]
+    L.bem_zgetrf_local.argtypes = [vp, vp, i64, i32, vp, vp]
     L.bem_zgetrs.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32]
     L.bem_zgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
     L.bem_cgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
@@ -392,6 +393,27 @@ class Context:
             self._check(self.lib.bem_memcpy_d2h(self.h, piv.ctypes.data, dp, piv.nbytes))
             self._check(rc)
             return LU, piv[:n]
+        finally:
+            self.lib.bem_device_free(self.h, dA)
+            self.lib.bem_device_free(self.h, dp)
+
+    def zgetrf_local_host(self, A):
+        """LU factors, interchanges and growth factor with pivoting inside each
+        64 x 64 diagonal block (bem_zgetrf_local, device round trip); a
+        singular block raises BemError(BEM_E_SINGULAR)."""
+        A = np.asfortranarray(A, dtype=np.complex128)
+        n = A.shape[0]
+        dA = self._dev(A)
+        dp = self._dev(np.zeros(n + 1, dtype=np.int32))
+        g = ctypes.c_double(0.0)
+        try:
+            rc = self.lib.bem_zgetrf_local(self.h, dA, n, n, dp, ctypes.byref(g))
+            LU = np.zeros_like(A)
+            piv = np.zeros(n + 1, dtype=np.int32)
+            self._check(self.lib.bem_memcpy_d2h(self.h, LU.ctypes.data, dA, LU.nbytes))
+            self._check(self.lib.bem_memcpy_d2h(self.h, piv.ctypes.data, dp, piv.nbytes))
+            self._check(rc)
+            return LU, piv[:n], g.value
         finally:
             self.lib.bem_device_free(self.h, dA)
             self.lib.bem_device_free(self.h, dp)

@@ -23,7 +23,7 @@
  *   bem_body_upload     mesh -> RWG construction of ShapeMesh + RWG
  *                       (src/ShapeMesh.cpp:821-947, src/RWG.cpp:5-39), with
  *                       the reference's edge order and orientation
- *   bem_zgetrf / bem_zgetrs / bem_zgemm
+ *   bem_zgetrf / bem_zgetrf_local / bem_zgetrs / bem_zgemm
  *                       the LAPACK/BLAS FFI the reference binds
  *                       (zgetrf_/zgetri_ src/util/DenseMatrix.cpp:38-77,
  *                       zgemm_ src/util/MatrixOperations.cpp:70-74), on
@@ -172,6 +172,13 @@ int bem_queue_depth(bem_ctx *ctx, int *queued, int *running);
  * (device, n ints, 0-based interchanges row i <-> ipiv[i], applied in
  * order).  Returns BEM_E_SINGULAR on a zero pivot. */
 int bem_zgetrf(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv);
+/* A (n x n, lda) <- P L U in place with partial pivoting inside each 64 x 64
+ * diagonal block: the factorisation the solve uses by default, as the
+ * reference's tree-block LU factors each diagonal block on its own
+ * (src/util/HMatrix.h:276-307).  ipiv as bem_zgetrf, every interchange
+ * within its block; growth (optional, host) receives max|U| / max|A|.
+ * Returns BEM_E_SINGULAR on a zero pivot. */
+int bem_zgetrf_local(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv, double *growth);
 /* B (n x nrhs, ldb) <- A^-1 B from the factors of bem_zgetrf. */
 int bem_zgetrs(bem_ctx *ctx, const void *A, int64_t lda, int n, const int32_t *ipiv, void *B, int64_t ldb, int nrhs);
 /* C <- alpha A B + beta C (no transposes), alpha/beta as (re, im). */
@@ -181,10 +188,10 @@ int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void
 /* Tile variant per GEMM shape class {narrow: an output dimension <= 128,
  * rank update: k <= 128, general} for complex128 (tiles[0..2]; codes 32, 64,
  * 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129) and
- * complex64 (tiles[3..5];
- * codes 1..5 four real products per complex multiply-add, 11..15 the same
- * tiles with three); 0 selects the built-in shape rule.  set6 / get6 may be
- * NULL. */
+ * complex64 (tiles[3..5]; codes 1..5 four real products per complex
+ * multiply-add, 11..15 the same tiles with three, 16 / 17 the 64 x 64 /
+ * 128 x 128 tiles on eight waves); 0 selects the built-in shape rule.
+ * set6 / get6 may be NULL. */
 int bem_gemm_tiles(bem_ctx *ctx, const int *set6, int *get6);
 /* Times every tile variant on a representative shape of each class (4096 x
  * 64 x 64, 4096 x 4096 x 64, and 4096 x 4096 x 256, the rank-256 trailing
@@ -214,9 +221,9 @@ int bem_last_timings(bem_ctx *ctx, double *out9);
 /* Kernel classes timed with device events on their own stream during the
  * last bem_spectral_power / bem_sweep call (a class may hold several
  * launches of one kernel). */
-#define BEM_KS_ZGEMM 0         /* zgemm, 32 x 64 tiles (the large products) */
-#define BEM_KS_ZGEMM_NARROW 1  /* zgemm, 32 x 32 tiles (panel-sized products) */
-#define BEM_KS_ZTRACE 2        /* zgemm with the trace epilogue */
+#define BEM_KS_ZGEMM 0         /* zgemm, rank-update and general classes */
+#define BEM_KS_ZGEMM_NARROW 1  /* zgemm, narrow class (an output side <= 128) */
+#define BEM_KS_ZTRACE 2        /* Hermitian-trace GEMM (upper half of X^H B1) */
 #define BEM_KS_FAR 3           /* far-pair assembly (6-point rule) */
 #define BEM_KS_CLOSE 4         /* close-pair assembly (105 x 91 + analytic) */
 #define BEM_KS_GATHER 5        /* tiles -> 2 x 2 RWG blocks of Z */

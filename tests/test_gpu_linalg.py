@@ -1,5 +1,6 @@
 """Dense complex128 kernels of the HIP library (ZGEMM on FP64 MFMA,
-tournament-pivoted LU + triangular solves) against numpy, through the C ABI."""
+tournament-pivoted LU + triangular solves, the block-local pivoted LU of the
+solve) against numpy, through the C ABI."""
 import numpy as np
 import pytest
 
@@ -51,6 +52,64 @@ def test_lu_factors(gpu_ctx):
     assert np.abs(L @ U - PA).max() < 1e-12 * np.abs(A).max() * n
 
 
+def _block_local_lu(A, nb=64):
+    """Right-looking LU whose pivot for column j is searched only in rows
+    [j, end of j's nb-block) (first maximum of |re|+|im|), with interchanges
+    of whole rows and elimination of every row below: what the blocked GPU
+    factorisation computes in exact arithmetic."""
+    A = np.array(A, dtype=np.complex128)
+    n = A.shape[0]
+    piv = np.zeros(n, dtype=np.int64)
+    for j in range(n):
+        end = min((j // nb + 1) * nb, n)
+        col = np.abs(A[j:end, j].real) + np.abs(A[j:end, j].imag)
+        p = j + int(np.argmax(col))
+        piv[j] = p
+        if p != j:
+            A[[j, p]] = A[[p, j]]
+        A[j + 1:, j] /= A[j, j]
+        A[j + 1:, j + 1:] -= np.outer(A[j + 1:, j], A[j, j + 1:])
+    return A, piv
+
+
+@pytest.mark.parametrize("n", [5, 64, 200, 333])
+def test_block_local_lu(gpu_ctx, n):
+    """bem_zgetrf_local (the solve's default LU, pivoting inside each 64 x 64
+    diagonal block) against the numpy restatement above: the same
+    interchanges, factors within 1e-12, P A = L U, and the growth factor
+    max|U| / max|A| it reports."""
+    rng = np.random.default_rng(100 + n)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A *= np.exp(rng.normal(size=(n, 1)))   # rows scaled so that pivoting matters
+    LU, piv, growth = gpu_ctx.zgetrf_local_host(A)
+    ref, rpiv = _block_local_lu(A)
+    assert np.array_equal(piv, rpiv)
+    assert all((i // 64) * 64 <= piv[i] < min((i // 64 + 1) * 64, n) for i in range(n))
+    scale = np.abs(A).max()
+    assert np.abs(LU - ref).max() <= 1e-12 * scale * n
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    PA = A.copy()
+    for i in range(n):
+        PA[[i, piv[i]]] = PA[[piv[i], i]]
+    assert np.abs(L @ U - PA).max() <= 1e-12 * scale * n
+    l1 = lambda M: (np.abs(M.real) + np.abs(M.imag)).max()
+    assert abs(growth - l1(U) / l1(A)) <= 1e-12 * growth
+
+
+def test_block_local_lu_singular_block(gpu_ctx):
+    """A zero column inside the third diagonal block: BEM_E_SINGULAR."""
+    from bem_amd import BEM_E_SINGULAR
+    from bem_amd.runtime import BemError
+    n = 150
+    rng = np.random.default_rng(7)
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    A[:, 130] = 0.0
+    with pytest.raises(BemError) as e:
+        gpu_ctx.zgetrf_local_host(A)
+    assert e.value.code == BEM_E_SINGULAR
+
+
 ZTILES = [32, 64, 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129]
 
 
@@ -73,7 +132,10 @@ def test_zgemm_tile_variants(gpu_ctx, tile):
         gpu_ctx.gemm_tiles(saved)
 
 
-@pytest.mark.parametrize("tile", [1, 2, 3, 4, 5, 11, 12, 13, 14, 15])
+CTILES = [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17]
+
+
+@pytest.mark.parametrize("tile", CTILES)
 def test_cgemm_tile_variants(gpu_ctx, tile):
     """complex64 variants: four real products (1..5) and three (11..15), whose
     error bound u (|Ar| + |Ai|)(|Br| + |Bi|) per term is of the same order."""
@@ -101,7 +163,7 @@ def test_autotune_cache_roundtrip(tmp_path):
         tiles = ctx.autotune(path)
     d = json.load(open(path))
     assert d["zgemm"] == tiles[:3] and d["cgemm"] == tiles[3:]
-    assert all(t in ZTILES for t in tiles[:3]) and all(t in (1, 2, 3, 4, 5, 11, 12, 13, 14, 15) for t in tiles[3:])
+    assert all(t in ZTILES for t in tiles[:3]) and all(t in CTILES for t in tiles[3:])
     with bem_amd.Context(0, lanes=1) as ctx2:
         assert ctx2.autotune(path) == tiles
         assert ctx2.gemm_tiles() == tiles
