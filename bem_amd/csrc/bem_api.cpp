@@ -2205,6 +2205,13 @@ int bem_cgemm(bem_ctx *c, int m, int n, int k, const float *alpha, const void *A
   return 0;
 }
 
+int bem_synchronize(bem_ctx *c) {
+  if (!c) return BEM_E_INVALID;
+  DEV(c, hipSetDevice(c->device));
+  DEV(c, hipDeviceSynchronize());
+  return 0;
+}
+
 int bem_device_alloc(bem_ctx *c, size_t bytes, void **ptr) {
   if (!c || !ptr) return BEM_E_INVALID;
   std::lock_guard<std::recursive_mutex> lk(c->mu);

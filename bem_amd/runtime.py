@@ -74,7 +74,8 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_body_edges", "bem_body_free", "bem_assemble", "bem_pair_histogram", "bem_spectral_power",
            "bem_sweep", "bem_zgetrf", "bem_zgetrf_local", "bem_zgetrs", "bem_zgemm", "bem_device_alloc",
            "bem_device_free", "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
-           "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune"]
+           "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune",
+           "bem_synchronize"]
 
 
 def load_library():
@@ -90,6 +91,7 @@ def load_library():
     P = ctypes.POINTER
     L.bem_ctx_create.argtypes = [P(vp), P(_Cfg)]
     L.bem_ctx_destroy.argtypes = [vp]
+    L.bem_synchronize.argtypes = [vp]
     L.bem_ctx_destroy.restype = None
     L.bem_last_error.argtypes = [vp]
     L.bem_last_error.restype = ctypes.c_char_p
@@ -232,6 +234,10 @@ class Context:
 
     def __exit__(self, *a):
         self.close()
+
+    def synchronize(self):
+        """Wait until every stream of the device is idle (bem_synchronize)."""
+        self._check(self.lib.bem_synchronize(self.h))
 
     def device_bytes(self):
         return self.lib.bem_ctx_device_bytes(self.h)

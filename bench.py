@@ -387,12 +387,11 @@ def main():
         ctx.sweep(b1, b2, [req(f) for f in freqs_w])
     # timed region: one sweep over K frequencies, up to `lanes` solves in flight;
     # the call returns after every stream has drained, and a device-wide
-    # synchronize (torch's is hipDeviceSynchronize) brackets it on both sides
-    import torch
-    torch.cuda.synchronize()
+    # synchronize brackets it on both sides
+    ctx.synchronize()
     t_start = time.perf_counter()
     results = ctx.sweep(b1, b2, reqs)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     elapsed = time.perf_counter() - t_start
     ks_timed = ctx.kernel_stats()
     lat = [r["ms_total"] for r in results]

@@ -208,6 +208,9 @@ int bem_autotune(bem_ctx *ctx, const char *cache_path, int flags, int *tiles6);
  * counterpart (the reference binds only complex128 BLAS). */
 int bem_cgemm(bem_ctx *ctx, int m, int n, int k, const float *alpha, const void *A, int64_t lda, const void *B,
               int64_t ldb, const float *beta, void *C, int64_t ldc);
+/* Waits until every stream of the context's device is idle (no reference
+ * counterpart: timing harnesses bracket a measured region with it). */
+int bem_synchronize(bem_ctx *ctx);
 /* Device memory helpers for callers without their own allocator. */
 int bem_device_alloc(bem_ctx *ctx, size_t bytes, void **ptr);
 int bem_device_free(bem_ctx *ctx, void *ptr);
