@@ -1600,7 +1600,7 @@ static void sched_main(bem_ctx *c) {
 // Representative shape (m, n, k) of each GEMM class.
 static const int TUNE_SHAPE[3][3] = {{4096, 64, 64}, {4096, 4096, 64}, {4096, 4096, 256}};
 static const int ZCANDS[] = {1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129, 32, 64, 256};
-static const int CCANDS[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17};
+static const int CCANDS[] = {1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 21, 23, 26};
 
 // Mean milliseconds of `reps` launches of one tile variant on lane 0.
 static int tune_time(bem_ctx *c, Lane &L, bool c64, int tile, int cls, void *A, void *B, void *Cm, int reps,

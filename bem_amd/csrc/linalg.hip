@@ -1440,10 +1440,14 @@ int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
 // C <- alpha A B + beta C in complex64.  Tile variants: 1 = 64 x 64, 2 = 32 x
 // 32, 3 = 64 x 128, 4 = 128 x 64, 5 = 128 x 128 (2 x 2 waves each) with four
 // real products per complex multiply-add; 11..15 the same tiles with three;
-// 16 = 64 x 64 and 17 = 128 x 128 on eight waves (2 x 4), three products.
+// 16 = 64 x 64 and 17 = 128 x 128 on eight waves (2 x 4), three products;
+// 21 = 64 x 64, 23 = 64 x 128 and 26 = 64 x 64 on eight waves with k steps of
+// 32 (three products): half the barriers per flop.
 // Default rule: 32 x 32 when either dimension is at most 128, else 64 x 64.
 int bem_cgemm_default_tile(int m, int n) { return (m <= 128 || n <= 128) ? 2 : 1; }
-int bem_cgemm_tile_valid(int tile) { return (tile >= 1 && tile <= 5) || (tile >= 11 && tile <= 17); }
+int bem_cgemm_tile_valid(int tile) {
+  return (tile >= 1 && tile <= 5) || (tile >= 11 && tile <= 17) || tile == 21 || tile == 23 || tile == 26;
+}
 int bem_cgemm_wide(int m, int n) { return !(m <= 128 || n <= 128); }
 
 int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B,
@@ -1463,6 +1467,9 @@ int bem_cgemm_tiled(int tile, int m, int n, int k, float2 alpha, const float2 *A
     case 15: launch_cgemm<128, 128, 2, 2, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
     case 16: launch_cgemm<64, 64, 2, 4, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
     case 17: launch_cgemm<128, 128, 2, 4, 16, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 21: launch_cgemm<64, 64, 2, 2, 32, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 23: launch_cgemm<64, 128, 2, 2, 32, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
+    case 26: launch_cgemm<64, 64, 2, 4, 32, true>(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, st); break;
     default: return -1;
   }
   return (int)hipGetLastError();

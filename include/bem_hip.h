@@ -190,7 +190,8 @@ int bem_zgemm(bem_ctx *ctx, int m, int n, int k, const double *alpha, const void
  * 256, 1032, 2032, 1033, 1034, 1064, 2064, 1065, 2034, 1066, 1128, 1129) and
  * complex64 (tiles[3..5]; codes 1..5 four real products per complex
  * multiply-add, 11..15 the same tiles with three, 16 / 17 the 64 x 64 /
- * 128 x 128 tiles on eight waves); 0 selects the built-in shape rule.
+ * 128 x 128 tiles on eight waves, 21 / 23 / 26 tiles 11 / 13 / 16 with k
+ * steps of 32); 0 selects the built-in shape rule.
  * set6 / get6 may be NULL. */
 int bem_gemm_tiles(bem_ctx *ctx, const int *set6, int *get6);
 /* Times every tile variant on a representative shape of each class (4096 x

@@ -132,7 +132,7 @@ def test_zgemm_tile_variants(gpu_ctx, tile):
         gpu_ctx.gemm_tiles(saved)
 
 
-CTILES = [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17]
+CTILES = [1, 2, 3, 4, 5, 11, 12, 13, 14, 15, 16, 17, 21, 23, 26]
 
 
 @pytest.mark.parametrize("tile", CTILES)
