@@ -1,8 +1,10 @@
 """Spectral power of the 1280-triangle icosphere pair (the benchmark workload)
 computed by the oracle (C restatement of the reference assembly, pinned to the
 reference's own Zts by the golden fixtures, plus LAPACK for the dense part) at
-four frequencies of its 64-point grid.  Stored so the GPU benchmark can report
-its error against the FP64 oracle without a minutes-long CPU run.
+frequencies of its 64-point grid: four spread over the band and the five
+around 0.12 eV, where Re eps of the Lorentz medium crosses zero (omega_LO) and
+the system is worst conditioned.  Stored so the GPU benchmark can report its
+error against the FP64 oracle without a minutes-long CPU run.
 Usage: python tests/golden/make_c2_oracle.py  (about a minute per frequency)
 """
 import json
@@ -22,7 +24,7 @@ def main():
     m1 = s.translated((2.2, 0.0, 0.0))
     grid = np.linspace(0.08, 0.14, 64)
     out = {}
-    for i in (0, 16, 32, 48):
+    for i in (0, 16, 32, 40, 41, 42, 43, 44, 48):
         eV = float(grid[i])
         eps = O.lorentz_eps(eV)
         r = O.spectral_power(m1, s, eV, 1e-6, 300.0, 0.0, eps, eps, return_parts=True)
