@@ -164,6 +164,35 @@ __device__ __forceinline__ void inner_zero(Inner &m) {
   for (int c = 0; c < 3; c++) { m.G1r[c] = m.G1i[c] = m.D0r[c] = m.D0i[c] = m.D1r[c] = m.D1i[c] = 0.0; }
 }
 
+// An Inner in INNER_PLANES planes of stride doubles (structure of arrays, so
+// neighbouring outer points are neighbouring addresses).
+#define INNER_PLANES 20
+__device__ __forceinline__ void inner_store(const Inner &m, double *__restrict__ mom, size_t stride, size_t i) {
+  mom[i] = m.G0.re;
+  mom[stride + i] = m.G0.im;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    mom[(2 + c) * stride + i] = m.G1r[c];
+    mom[(5 + c) * stride + i] = m.G1i[c];
+    mom[(8 + c) * stride + i] = m.D0r[c];
+    mom[(11 + c) * stride + i] = m.D0i[c];
+    mom[(14 + c) * stride + i] = m.D1r[c];
+    mom[(17 + c) * stride + i] = m.D1i[c];
+  }
+}
+__device__ __forceinline__ void inner_load(Inner &m, const double *__restrict__ mom, size_t stride, size_t i) {
+  m.G0 = cmk(mom[i], mom[stride + i]);
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    m.G1r[c] = mom[(2 + c) * stride + i];
+    m.G1i[c] = mom[(5 + c) * stride + i];
+    m.D0r[c] = mom[(8 + c) * stride + i];
+    m.D0i[c] = mom[(11 + c) * stride + i];
+    m.D1r[c] = mom[(14 + c) * stride + i];
+    m.D1i[c] = mom[(17 + c) * stride + i];
+  }
+}
+
 // One inner point: Green's function (regular src/GreensFunction.cpp:41-52 or
 // smooth :54-69) at d = r - r', weighted moment update.
 template <bool SMOOTH>
@@ -550,15 +579,51 @@ __device__ __forceinline__ void block_reduce36(double (&Lr)[9], double (&Li)[9],
   }
 }
 
+// Smooth inner sums of the close pairs, one thread per (pair, outer point)
+// of the listed pairs [0, np): the 91-point inner rule on the source triangle
+// at one 105-point outer point of the test triangle, summed in the order of
+// pair_block_kernel's own loop, stored to mom for the pair's outer part.
+// A flat grid keeps every lane busy, and without the analytic part's state
+// the kernel holds more waves per SIMD.
+__global__ void __launch_bounds__(256) close_inner_kernel(const TriGeom *__restrict__ tg, Medium md,
+                                                          const int2 *__restrict__ pairs, unsigned int np,
+                                                          int row_lo, int row_hi, int col_lo, int col_hi,
+                                                          double *__restrict__ mom, size_t mstride) {
+  const size_t t = (size_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (size_t)np * 105) return;
+  const unsigned int p = (unsigned int)(t / 105);
+  const int o = (int)(t % 105);
+  const int2 pr = pairs[p];
+  const int a = pr.x, b = pr.y;
+  if (a < row_lo || a >= row_hi || b < col_lo || b >= col_hi) return;   // outside the range: not used
+  const TriGeom &A = tg[a];
+  const TriGeom &B = tg[b];
+  double r[3];
+  quad_point(A, BEM_QUAD_OFF_105 + o, r);
+  Inner m;
+  inner_zero(m);
+  for (int q = 0; q < 91; q++) {
+    double v[4];
+    quad_point(B, BEM_QUAD_OFF_91 + q, v);
+    v[3] = c_quad[BEM_QUAD_OFF_91 + q][2];
+    inner_point<true>(md, r, v, B.p0, v[3], m);
+  }
+  inner_store(m, mom, mstride, t);
+}
+
 // One block per listed pair.  CLOSE: Gauss105 outer x Gauss91 smooth inner +
 // analytic terms + neighbour edge correction; otherwise 36 x 36 regular.
-template <int NOUT, int NIN, bool CLOSE, int NT>
+// PRE: the inner sums of every (pair, outer point) were formed beforehand by
+// close_inner_kernel and are read from mom (INNER_PLANES planes of mstride
+// doubles, index pair * NOUT + outer point) instead of being summed here.
+template <int NOUT, int NIN, bool CLOSE, int NT, bool PRE = false>
 __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restrict__ tg, int T, Medium md,
                                                         const int2 *__restrict__ pairs,
                                                         const unsigned int *__restrict__ count_dev,
                                                         unsigned int count_host, int row_lo, int row_hi,
                                                         int col_lo, int col_hi, double *__restrict__ tiles,
-                                                        size_t npairs) {
+                                                        size_t npairs, const double *__restrict__ mom = nullptr,
+                                                        size_t mstride = 0) {
   __shared__ double s_in[NIN][4];
   __shared__ double s_red[(NT / 64) * 36];
   __shared__ double s_tp0[3][3], s_sp0[3][3], s_spb[3][3];
@@ -573,11 +638,13 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
     const TriGeom &A = tg[a];
     const TriGeom &B = tg[b];
     __syncthreads();
-    for (int t = threadIdx.x; t < NIN; t += NT) {
-      double v[3];
-      quad_point(B, offi + t, v);
-      s_in[t][0] = v[0]; s_in[t][1] = v[1]; s_in[t][2] = v[2];
-      s_in[t][3] = c_quad[offi + t][2];
+    if (!PRE) {
+      for (int t = threadIdx.x; t < NIN; t += NT) {
+        double v[3];
+        quad_point(B, offi + t, v);
+        s_in[t][0] = v[0]; s_in[t][1] = v[1]; s_in[t][2] = v[2];
+        s_in[t][3] = c_quad[offi + t][2];
+      }
     }
     if (threadIdx.x < 9) {
       int i = threadIdx.x / 3, c = threadIdx.x % 3;
@@ -595,8 +662,12 @@ __global__ void __launch_bounds__(NT) pair_block_kernel(const TriGeom *__restric
       quad_point(A, offo + o, r);
       double w = c_quad[offo + o][2];
       Inner m;
-      inner_zero(m);
-      for (int q = 0; q < NIN; q++) inner_point<CLOSE>(md, r, s_in[q], B.p0, s_in[q][3], m);
+      if constexpr (PRE) {
+        inner_load(m, mom, mstride, (size_t)p * NOUT + o);
+      } else {
+        inner_zero(m);
+        for (int q = 0; q < NIN; q++) inner_point<CLOSE>(md, r, s_in[q], B.p0, s_in[q][3], m);
+      }
       if (CLOSE)
         outer_close(md, B, r, w, m, s_tp0, nt, s_sp0, s_spb, ns, Lr, Li, Kr, Ki);
       else
@@ -731,17 +802,42 @@ int bem_launch_far_g36(const TriGeom *tg, int T, const Medium *md, const int2 *p
   unsigned int blocks = max_count < 4096u ? max_count : 4096u;
   if (blocks == 0) return 0;
   hipLaunchKernelGGL((pair_block_kernel<36, 36, false, 64>), dim3(blocks), dim3(64), 0, st, tg, T, *md, pairs,
-                     count_dev, max_count, row_lo, row_hi, col_lo, col_hi, tiles, npairs);
+                     count_dev, max_count, row_lo, row_hi, col_lo, col_hi, tiles, npairs,
+                     (const double *)nullptr, (size_t)0);
   return (int)hipGetLastError();
 }
 
+// Close pairs in chunks of at most mom_pairs pairs: the inner sums of a chunk
+// (close_inner_kernel) into mom (INNER_PLANES x 105 x mom_pairs doubles), then
+// the outer part per pair (pair_block_kernel reading them).  mom == nullptr:
+// one pass of pair_block_kernel doing both.
+size_t bem_close_mom_doubles(unsigned int mom_pairs) { return (size_t)INNER_PLANES * 105 * mom_pairs; }
+
 int bem_launch_close(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, unsigned int count, int row_lo,
-                     int row_hi, int col_lo, int col_hi, double *tiles, size_t npairs, hipStream_t st) {
-  unsigned int blocks = count < 65535u ? count : 65535u;
-  if (blocks == 0) return 0;
-  hipLaunchKernelGGL((pair_block_kernel<105, 91, true, 128>), dim3(blocks), dim3(128), 0, st, tg, T, *md, pairs,
-                     (const unsigned int *)nullptr, count, row_lo, row_hi, col_lo, col_hi, tiles, npairs);
-  return (int)hipGetLastError();
+                     int row_hi, int col_lo, int col_hi, double *tiles, size_t npairs, double *mom,
+                     unsigned int mom_pairs, hipStream_t st) {
+  if (count == 0) return 0;
+  if (!mom || mom_pairs == 0) {
+    unsigned int blocks = count < 65535u ? count : 65535u;
+    hipLaunchKernelGGL((pair_block_kernel<105, 91, true, 128>), dim3(blocks), dim3(128), 0, st, tg, T, *md, pairs,
+                       (const unsigned int *)nullptr, count, row_lo, row_hi, col_lo, col_hi, tiles, npairs,
+                       (const double *)nullptr, (size_t)0);
+    return (int)hipGetLastError();
+  }
+  const size_t mstride = (size_t)105 * mom_pairs;
+  for (unsigned int p0 = 0; p0 < count; p0 += mom_pairs) {
+    const unsigned int np = count - p0 < mom_pairs ? count - p0 : mom_pairs;
+    const unsigned int ib = (unsigned int)(((size_t)np * 105 + 255) / 256);
+    hipLaunchKernelGGL(close_inner_kernel, dim3(ib), dim3(256), 0, st, tg, *md, pairs + p0, np, row_lo, row_hi,
+                       col_lo, col_hi, mom, mstride);
+    const unsigned int blocks = np < 65535u ? np : 65535u;
+    hipLaunchKernelGGL((pair_block_kernel<105, 91, true, 128, true>), dim3(blocks), dim3(128), 0, st, tg, T, *md,
+                       pairs + p0, (const unsigned int *)nullptr, np, row_lo, row_hi, col_lo, col_hi, tiles, npairs,
+                       (const double *)mom, mstride);
+    int e = (int)hipGetLastError();
+    if (e) return e;
+  }
+  return 0;
 }
 
 // flag <- 1 when the 36-point pair queue overflowed its capacity.

@@ -56,7 +56,9 @@ int bem_launch_far_g36(const TriGeom *tg, int T, const Medium *md, const int2 *p
                        unsigned int max_count, int row_lo, int row_hi, int col_lo, int col_hi, double *tiles,
                        size_t npairs, hipStream_t st);
 int bem_launch_close(const TriGeom *tg, int T, const Medium *md, const int2 *pairs, unsigned int count, int row_lo,
-                     int row_hi, int col_lo, int col_hi, double *tiles, size_t npairs, hipStream_t st);
+                     int row_hi, int col_lo, int col_hi, double *tiles, size_t npairs, double *mom,
+                     unsigned int mom_pairs, hipStream_t st);
+size_t bem_close_mom_doubles(unsigned int mom_pairs);
 int bem_launch_gather(const double *tiles, size_t npairs, int tld, int row_lo, int col_lo, const RwgRec *rw, int r_lo,
                       int r_hi, int c_lo, int c_hi, const Medium *md, double2 *Z, int64_t ldz, int zr0, int zc0,
                       int add, double2 *Zcopy, int64_t ldc, int cr0, int cc0, hipStream_t st);
@@ -861,6 +863,9 @@ static int body_tree(bem_ctx *c, bem_body *b, const DevTree **out) {
 // terms ~ 1.83e6.  Each pair writes one 36-double tile (288 bytes).
 static const double F_FAR_PAIR = 8308.0;
 static const double F_CLOSE_PAIR = 1.83e6;
+// close pairs whose inner sums share one pass of the moment workspace
+// (20 doubles per (pair, outer point): 69 MB at 4096 pairs)
+static const size_t CLOSE_CHUNK = 4096;
 static const double TILE_BYTES = 8.0 * BEM_TILE_PLANES;
 
 // Triangle-pair tiles of test rows [r0, r1) x source columns [c0, c1) of a
@@ -889,8 +894,12 @@ static int assemble_tiles(bem_ctx *c, Lane &L, const DevTree &d, const Medium &m
   e = bem_launch_far_g36(d.tg, T, &md, q36, q36cnt, cap, r0, r1, c0, c1, tiles, npairs, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "far_g36 launch: %s", hipGetErrorString((hipError_t)e));
   if (nclose > 0) {
+    // inner sums of at most CLOSE_CHUNK pairs at a time in a lane workspace
+    const unsigned int chunk = (unsigned int)std::min<size_t>((size_t)d.nclose, CLOSE_CHUNK);
+    double *mom;
+    CHK(ws_get(c, L, "close_mom", sizeof(double) * bem_close_mom_doubles(chunk), (void **)&mom));
     CHK(ev_begin(c, L, BEM_KS_CLOSE, nclose * F_CLOSE_PAIR, nclose * TILE_BYTES, &r));
-    e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, r0, r1, c0, c1, tiles, npairs, L.st);
+    e = bem_launch_close(d.tg, T, &md, d.close, d.nclose, r0, r1, c0, c1, tiles, npairs, mom, chunk, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "close launch: %s", hipGetErrorString((hipError_t)e));
     CHK(ev_end(c, L, r));
   }
@@ -1394,7 +1403,8 @@ static size_t footprint(bem_ctx *c, const Lane &L, const Pending &p) {
   size_t pairs = std::max(T1 * T2, std::max(h1 ? 0 : T1 * T1, h2 ? 0 : T2 * T2));
   size_t ws = sizeof(double2) * (3 * m1 * m2 + m2 * m2) * (prec == BEM_PREC_MIXED ? 3 : 1) + 8 * (size_t)BEM_TILE_PLANES * pairs +
               sizeof(int2) * std::min<size_t>(pairs, (size_t)1 << 24) + inv_bytes((int)m2) +
-              sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_htrace_partials((int)m2) + 1);
+              sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_htrace_partials((int)m2) + 1) +
+              sizeof(double) * bem_close_mom_doubles((unsigned int)CLOSE_CHUNK);
   return (ws > L.ws_bytes ? ws - L.ws_bytes : 0) + selfb;
 }
 
