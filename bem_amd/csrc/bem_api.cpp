@@ -81,6 +81,10 @@ int bem_diag_block(double2 *A, int64_t lda, int n, int blk0, int nblk, int bw, i
 int bem_diag_lu_piv(double2 *A, int64_t lda, int n, int k0, double2 *inv, int *ipiv, int *info, hipStream_t st);
 int bem_absmax(const double2 *A, int64_t lda, int m, int n, int upper, unsigned long long *out, hipStream_t st);
 int bem_col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st);
+int bem_permute_rows_local(double2 *X, int64_t ldx, int n, int r, const int *ipiv, int *perm, hipStream_t st);
+int bem_permute_cols_local(double2 *X, int64_t ldx, int m, int n, const int *ipiv, int *perm, hipStream_t st);
+int bem_permute_rows_local_c(float2 *X, int64_t ldx, int n, int r, const int *ipiv, int *perm, hipStream_t st);
+int bem_permute_cols_local_c(float2 *X, int64_t ldx, int m, int n, const int *ipiv, int *perm, hipStream_t st);
 int bem_cgemm_wide(int m, int n);
 int bem_cgemm_raw(int m, int n, int k, float2 alpha, const float2 *A, int64_t lda, const float2 *B, int64_t ldb,
                   float2 beta, float2 *C, int64_t ldc, hipStream_t st);
@@ -478,6 +482,18 @@ static int laswp(float2 *A, int64_t lda, int c0, int c1, int s0, int s1, int k0,
                  hipStream_t st) {
   return bem_laswp_c(A, lda, c0, c1, s0, s1, k0, k1, ipiv, st);
 }
+static int permute_rows_local(double2 *X, int64_t ldx, int n, int r, const int *ipiv, int *perm, hipStream_t st) {
+  return bem_permute_rows_local(X, ldx, n, r, ipiv, perm, st);
+}
+static int permute_rows_local(float2 *X, int64_t ldx, int n, int r, const int *ipiv, int *perm, hipStream_t st) {
+  return bem_permute_rows_local_c(X, ldx, n, r, ipiv, perm, st);
+}
+static int permute_cols_local(double2 *X, int64_t ldx, int m, int n, const int *ipiv, int *perm, hipStream_t st) {
+  return bem_permute_cols_local(X, ldx, m, n, ipiv, perm, st);
+}
+static int permute_cols_local(float2 *X, int64_t ldx, int m, int n, const int *ipiv, int *perm, hipStream_t st) {
+  return bem_permute_cols_local_c(X, ldx, m, n, ipiv, perm, st);
+}
 static int col_swap(double2 *X, int64_t ldx, int m, int k0, int k1, const int *ipiv, hipStream_t st) {
   return bem_col_swap(X, ldx, m, k0, k1, ipiv, st);
 }
@@ -663,10 +679,17 @@ static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *i
 // top to bottom, then backward substitution with U bottom to top.
 template <typename T>
 static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv, const T *inv,
-                    T *X, int64_t ldx, int r) {
+                    T *X, int64_t ldx, int r, bool local = false) {
   const int nb = LU_NB;
   const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
-  int e = laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
+  int e;
+  if (local) {   // block-local interchanges: one permutation per 64-row block, applied in parallel
+    int *perm;
+    CHK(ws_get(c, L, "perm", sizeof(int) * (size_t)(n + 64), (void **)&perm));
+    e = permute_rows_local(X, ldx, n, r, ipiv, perm, L.st);
+  } else {
+    e = laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
+  }
   if (e) return fail(c, BEM_E_DEVICE, "laswp rhs: %d", e);
   for (int K0 = 0; K0 < n; K0 += LU_NB2) {
     const int K1 = std::min(n, K0 + LU_NB2);
@@ -703,7 +726,7 @@ static int lu_solve(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const i
 // beyond it).
 template <typename T>
 static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, const int *ipiv,
-                          const T *inv, T *X, int64_t ldx, int r) {
+                          const T *inv, T *X, int64_t ldx, int r, bool local = false) {
   const int nb = LU_NB;
   const T m1 = cnum<T>(-1.0, 0.0), one = cnum<T>(1.0, 0.0);
   for (int K0 = 0; K0 < n; K0 += LU_NB2) {
@@ -736,7 +759,14 @@ static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, c
     if (K0 > 0)
       CHK(gemm(c, L, r, K0, K1 - K0, m1, X + (int64_t)K0 * ldx, ldx, A + K0, lda, one, X, ldx));
   }
-  int e = col_swap(X, ldx, r, 0, n, ipiv, L.st);
+  int e;
+  if (local) {   // block-local interchanges: one permutation per 64-column block
+    int *perm;
+    CHK(ws_get(c, L, "perm", sizeof(int) * (size_t)(n + 64), (void **)&perm));
+    e = permute_cols_local(X, ldx, r, n, ipiv, perm, L.st);
+  } else {
+    e = col_swap(X, ldx, r, 0, n, ipiv, L.st);
+  }
   if (e) return fail(c, BEM_E_DEVICE, "column interchanges: %d", e);
   return 0;
 }
@@ -1149,14 +1179,14 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   if (bem_z2c(Bm, m1, B32, m1, m1, m2, L.st) || bem_z2c(Cm, m2, C32, m2, m2, m1, L.st))
     return fail(c, BEM_E_DEVICE, "complex64 copy launch");
   DEV(c, hipMemcpyAsync(S32, E2.Z32, sizeof(float2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
-  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, B32, m1, m2));   // B32 <- Y
+  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, B32, m1, m2, true));   // B32 <- Y
   CHK(gemm(c, L, m2, m2, m1, mone, C32, m2, B32, m1, one, S32, m2));     // Sc = D - C Y
   CHK(lu_factor_block(c, L, S32, m2, m2, ipS, info, invS, gro));
   if (refine) {
     CHK(ws_get(c, L, "Y32", sizeof(float2) * (size_t)m1 * m2, (void **)&Y32));
     DEV(c, hipMemcpyAsync(Y32, B32, sizeof(float2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
   }
-  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, B32, m1, m1));        // B32 <- Y Sc^-1 = -X
+  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, B32, m1, m1, true));  // B32 <- Y Sc^-1 = -X
   // products: 4 S = tr(sym G2 X^H sym G1 X), even in X, over the Hermitian
   // X^H (sym G1 X): half of it is formed
   CHK(gemm(c, L, m1, m2, m1, one, E1.Gs32, m1, B32, m1, zero, B1, m1));
@@ -1186,7 +1216,7 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   int e = bem_identity_minus(R2, m2, m2, L.st);
   if (!e) e = bem_z2c(R2, m2, X2, m2, m2, m2, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
-  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, X2, m2, m2));
+  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, X2, m2, m2, true));
   if (bem_c2z(B32, m1, Xn, m1, m1, m2, L.st) || bem_c2z(X2, m2, X2d, m2, m2, m2, L.st))
     return fail(c, BEM_E_DEVICE, "complex128 copy launch");
   // T1 = B X2 - A Xn = -(R1); T2 = D X2 - C Xn, R2 = I - T2
@@ -1199,9 +1229,9 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   //   u = A^-1 T1 (= -A^-1 R1), dX2 = Sc^-1 (R2 + C u), dX = -u - Y dX2
   if (bem_z2c(R1, m1, R1c, m1, m1, m2, L.st) || bem_z2c(R2, m2, R2c, m2, m2, m2, L.st))
     return fail(c, BEM_E_DEVICE, "complex64 copy launch");
-  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, R1c, m1, m2));   // R1c <- u
+  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, R1c, m1, m2, true));   // R1c <- u
   CHK(gemm(c, L, m2, m2, m1, one, C32, m2, R1c, m1, one, R2c, m2));      // R2c <- R2 + C u
-  CHK(lu_solve(c, L, S32, m2, m2, ipS, invS, R2c, m2, m2));              // R2c <- dX2
+  CHK(lu_solve(c, L, S32, m2, m2, ipS, invS, R2c, m2, m2, true));        // R2c <- dX2
   CHK(gemm(c, L, m1, m2, m2, one, Y32, m1, R2c, m2, one, R1c, m1));      // R1c <- u + Y dX2 = -dX
   // corrected -X = Xn - dX = Xn + (u + Y dX2), in FP64
   if ((e = bem_c2z_add(R1c, m1, Xn, m1, m1, m2, L.st))) return fail(c, BEM_E_DEVICE, "update launch: %d", e);
@@ -1274,12 +1304,12 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
     // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
     const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-    CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2));   // B <- Y
+    CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2, !E1.pivot));   // B <- Y
     CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
     if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
     else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
     // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
-    CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1));
+    CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1, !pivot));
     double2 *X = Bm;
     DEV(c, hipEventRecord(L.ph[2], L.st));
     // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)

@@ -886,6 +886,66 @@ __global__ void col_swap_kernel(T *__restrict__ X, int64_t ldx, int m, int k0, i
   }
 }
 
+// Interchanges confined to 64-blocks (block-local pivoting: every ipiv[p]
+// lies in p's block) as one permutation per block: perm[k0 + i] = the index
+// whose row (or column) ends at position k0 + i after the block's
+// interchanges, applied in increasing order (rows) or decreasing order
+// (REV: the column interchanges X P^T).  One 64-thread block per 64-block.
+template <bool REV>
+__global__ void __launch_bounds__(64) block_perm_kernel(const int *__restrict__ ipiv, int n, int *__restrict__ perm) {
+  __shared__ int idx[64];
+  const int k0 = blockIdx.x * 64, w = min(64, n - k0), t = threadIdx.x;
+  idx[t] = k0 + t;
+  __syncthreads();
+  if (t == 0) {
+    for (int s = 0; s < w; s++) {
+      const int p = REV ? w - 1 - s : s, q = ipiv[k0 + p] - k0;
+      if (q < 0 || q >= w) continue;   // not block-local: callers never pass such ipiv
+      const int a = idx[p];
+      idx[p] = idx[q];
+      idx[q] = a;
+    }
+  }
+  __syncthreads();
+  if (t < w) perm[k0 + t] = idx[t];
+}
+
+// X (n x r) rows permuted within each 64-block: row k0 + i <- row perm[k0 + i].
+// A block stages 64 rows x 4 columns in LDS (coalesced both ways).
+template <typename T>
+__global__ void __launch_bounds__(256) row_permute_kernel(T *__restrict__ X, int64_t ldx, int n, int r,
+                                                          const int *__restrict__ perm) {
+  __shared__ T s[4][64];
+  const int k0 = blockIdx.x * 64, i = threadIdx.x & 63, cc = threadIdx.x >> 6;
+  const int c = blockIdx.y * 4 + cc, w = min(64, n - k0);
+  const bool in = i < w && c < r;
+  if (in) s[cc][i] = X[(int64_t)c * ldx + k0 + i];
+  __syncthreads();
+  if (in) X[(int64_t)c * ldx + k0 + i] = s[cc][perm[k0 + i] - k0];
+}
+
+// X (m x n) columns permuted within each 64-block: column k0 + j <- column
+// perm[k0 + j].  A block stages 32 rows x 64 columns in LDS.
+template <typename T>
+__global__ void __launch_bounds__(256) col_permute_kernel(T *__restrict__ X, int64_t ldx, int m, int n,
+                                                          const int *__restrict__ perm) {
+  __shared__ T s[64][33];
+  const int r0 = blockIdx.x * 32, k0 = blockIdx.y * 64, w = min(64, n - k0);
+  const int row = threadIdx.x & 31, j0 = threadIdx.x >> 5;
+  const bool rin = r0 + row < m;
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int j = j0 + 8 * q;
+    if (rin && j < w) s[j][row] = X[(int64_t)(k0 + j) * ldx + r0 + row];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int j = j0 + 8 * q;
+    if (rin && j < w) X[(int64_t)(k0 + j) * ldx + r0 + row] = s[perm[k0 + j] - k0][row];
+  }
+}
+
 // Row interchanges: for p in [k0, k1): swap rows p and ipiv[p] in every column
 // of [c0, c1) except [skip0, skip1).
 template <typename T>
@@ -1300,6 +1360,30 @@ void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t 
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+// Block-local interchanges ipiv[0..n) applied to the rows of X (n x r) or,
+// cols = 1, to the columns of X (m x n) as X P^T, through the permutation of
+// each 64-block (perm: n ints of workspace).
+template <typename T>
+static int permute_local(T *X, int64_t ldx, int m, int n, int r, int cols, const int *ipiv, int *perm,
+                         hipStream_t st) {
+  if (n <= 0) return 0;
+  const int nb = (n + 63) / 64;
+  if (cols) hipLaunchKernelGGL(block_perm_kernel<true>, dim3(nb), dim3(64), 0, st, ipiv, n, perm);
+  else hipLaunchKernelGGL(block_perm_kernel<false>, dim3(nb), dim3(64), 0, st, ipiv, n, perm);
+  if (cols) {
+    if (m <= 0) return 0;
+    if (nb > 65535) return -1;
+    hipLaunchKernelGGL(col_permute_kernel<T>, dim3((m + 31) / 32, nb), dim3(256), 0, st, X, ldx, m, n,
+                       (const int *)perm);
+  } else {
+    if (r <= 0) return 0;
+    if ((r + 3) / 4 > 65535) return -1;
+    hipLaunchKernelGGL(row_permute_kernel<T>, dim3(nb, (r + 3) / 4), dim3(256), 0, st, X, ldx, n, r,
+                       (const int *)perm);
+  }
+  return (int)hipGetLastError();
+}
+
 extern "C" {
 
 // Tile variants (complex128): 32 / 64 are the four-product 32 x 32 and 64 x 64
@@ -1425,6 +1509,19 @@ int bem_laswp(double2 *A, int64_t lda, int c0, int c1, int skip0, int skip1, int
   hipLaunchKernelGGL(laswp_kernel<double2>, dim3((c1 - c0 + 127) / 128), dim3(128), 0, st, A, lda, c0, c1, skip0, skip1,
                      k0, k1, ipiv);
   return (int)hipGetLastError();
+}
+
+int bem_permute_rows_local(double2 *X, int64_t ldx, int n, int r, const int *ipiv, int *perm, hipStream_t st) {
+  return permute_local(X, ldx, 0, n, r, 0, ipiv, perm, st);
+}
+int bem_permute_cols_local(double2 *X, int64_t ldx, int m, int n, const int *ipiv, int *perm, hipStream_t st) {
+  return permute_local(X, ldx, m, n, 0, 1, ipiv, perm, st);
+}
+int bem_permute_rows_local_c(float2 *X, int64_t ldx, int n, int r, const int *ipiv, int *perm, hipStream_t st) {
+  return permute_local(X, ldx, 0, n, r, 0, ipiv, perm, st);
+}
+int bem_permute_cols_local_c(float2 *X, int64_t ldx, int m, int n, const int *ipiv, int *perm, hipStream_t st) {
+  return permute_local(X, ldx, m, n, 0, 1, ipiv, perm, st);
 }
 
 int bem_symm(double2 *G, int64_t ld, int n, hipStream_t st) {
