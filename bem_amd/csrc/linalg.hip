@@ -628,35 +628,73 @@ template <int TT, int END, int J, int SKIP, typename T>
 __device__ __forceinline__ void lu_update(LuRow<T> &r, int wave, int p, T l) {
   if constexpr (TT < END) {
     if (TT != SKIP && wave + 4 * TT > J) {
+      // rows already chosen as pivots (and the pivot row) carry l = 0, so
+      // their U entries come out unchanged without a select
       const T uc = Cplx<T>::mk(lane_read(r.a[TT].x, p), lane_read(r.a[TT].y, p));
       T x = r.a[TT];
       x.x -= l.x * uc.x - l.y * uc.y;
       x.y -= l.x * uc.y + l.y * uc.x;
-      // select, not branch: rows already chosen as pivots keep their U entries
-      r.a[TT].x = r.done ? r.a[TT].x : x.x;
-      r.a[TT].y = r.done ? r.a[TT].y : x.y;
+      r.a[TT] = x;
     }
     lu_update<TT + 1, END, J, SKIP>(r, wave, p, l);
   }
+}
+
+// One step of an inclusive max-scan across the wave through DPP: lanes whose
+// source lies outside the row (or whose row is masked off) take ident.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ double dpp_max(double v, double ident) {
+  const long long b = __double_as_longlong(v), ib = __double_as_longlong(ident);
+  const int lo = __builtin_amdgcn_update_dpp((int)(ib & 0xffffffffll), (int)(b & 0xffffffffll), CTRL, ROWM, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(ib >> 32), (int)(b >> 32), CTRL, ROWM, 0xf, false);
+  return fmax(v, __longlong_as_double(((long long)hi << 32) | (unsigned int)lo));
+}
+template <int CTRL, int ROWM>
+__device__ __forceinline__ float dpp_max(float v, float ident) {
+  return fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(ident), __float_as_int(v), CTRL, ROWM,
+                                                             0xf, false)));
+}
+// Maximum of v over the 64 lanes (all active), wave-uniform: row_shr 1, 2,
+// 4, 8 scan each row of 16 lanes, row_bcast 15 and 31 carry the row maxima
+// up, and lane 63 ends with the whole wave's.
+template <typename R>
+__device__ __forceinline__ R wave_max(R v, R ident) {
+  v = dpp_max<0x111, 0xf>(v, ident);
+  v = dpp_max<0x112, 0xf>(v, ident);
+  v = dpp_max<0x114, 0xf>(v, ident);
+  v = dpp_max<0x118, 0xf>(v, ident);
+  v = dpp_max<0x142, 0xa>(v, ident);
+  v = dpp_max<0x143, 0xc>(v, ident);
+  return lane_read(v, 63);
 }
 
 // Column J's pivot by the wave holding it (column J already updated): max
 // |re|+|im| over the rows not yet chosen, ties to the lowest position; the
 // multipliers replace the column below the pivot and go to lbuf with the
 // pivot's lane to pbuf (buffers of J's parity); the interchange to ipiv.
+// The maximum comes from a DPP scan, the lanes holding it from a ballot; a
+// tie between several lanes is settled by a scalar loop over their positions.
 template <int J, typename T>
 __device__ __forceinline__ void lu_pivot(LuRow<T> &r, int ri, int k0, int *__restrict__ ipiv, int *__restrict__ info,
                                          T (*lbuf)[64], int *pbuf) {
   using R = typename Cplx<T>::R;
   constexpr int t = J >> 2, b = J & 1;
-  R v = r.done ? (R)-1.0 : std::fabs(r.a[t].x) + std::fabs(r.a[t].y);
-  int pp = r.pos, who = ri;
-  for (int off = 32; off >= 1; off >>= 1) {
-    R ov = __shfl_xor(v, off, 64);
-    int op = __shfl_xor(pp, off, 64), ow = __shfl_xor(who, off, 64);
-    if (ov > v || (ov == v && op < pp)) { v = ov; pp = op; who = ow; }
-  }
-  if (!(v > 0.0)) {   // zero column: keep the row at position J, flag it
+  const R v = r.done ? (R)-1.0 : std::fabs(r.a[t].x) + std::fabs(r.a[t].y);
+  const R vmax = wave_max(v, (R)-1.0);
+  int who, pp;
+  if (vmax > (R)0.0) {
+    const unsigned long long m = __ballot(v == vmax);
+    who = __builtin_ctzll(m);
+    pp = __builtin_amdgcn_readlane(r.pos, who);
+    for (unsigned long long mm = m & (m - 1); mm; mm &= mm - 1) {
+      const int l = __builtin_ctzll(mm);
+      const int q = __builtin_amdgcn_readlane(r.pos, l);
+      if (q < pp) {
+        pp = q;
+        who = l;
+      }
+    }
+  } else {   // zero column: keep the row at position J, flag it
     who = __builtin_ctzll(__ballot(r.pos == J));
     pp = J;
     if (ri == 0 && info && *info == 0) *info = k0 + J + 1;
@@ -665,9 +703,10 @@ __device__ __forceinline__ void lu_pivot(LuRow<T> &r, int ri, int k0, int *__res
   const T u = Cplx<T>::mk(lane_read(r.a[t].x, who), lane_read(r.a[t].y, who));
   const R den = u.x * u.x + u.y * u.y;
   T l = Cplx<T>::mk(0.0, 0.0);
-  if (!r.done && ri != who && den > 0.0) {
+  if (!r.done && ri != who && den > (R)0.0) {
     T x = r.a[t];
-    R ur = u.x / den, ui = -u.y / den;
+    const R rden = (R)1.0 / den;
+    const R ur = u.x * rden, ui = -u.y * rden;
     l = Cplx<T>::mk(x.x * ur - x.y * ui, x.x * ui + x.y * ur);
   }
   if (!r.done && ri != who) r.a[t] = l;   // L below the pivot, in place
