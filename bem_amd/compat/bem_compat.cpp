@@ -1,5 +1,6 @@
 // Reference-shaped C++ API over libbem_hip (see bem_compat.h).
 #include "bem_compat.h"
+#include "x3d.h"
 
 #include <cstdio>
 #include <cstdlib>
@@ -18,7 +19,7 @@ bem_ctx *context() {
   });
   return ctx;
 }
-static void check(int rc) {
+void check(int rc) {
   if (rc) throw Error(rc, bem_last_error(context()));
 }
 }  // namespace bem
@@ -148,6 +149,25 @@ Real CalcHeatTransfer(Real a, Real eV, Real T1, Real T2, RWGTree *top, RWGTree *
   s->g->init(eps1, mu1, Complex(omega, 0.0));
   t->g->init(eps2, mu2, Complex(omega, 0.0));
   return r.P;
+}
+
+// src/ShapeParser.cpp:17-63
+void parseShapes(RWGTree *&top, RWGTree *&s, RWGTree *&t, const char *body1x3d, const char *body2x3d,
+                 const Vector3 &gap, Options &) {
+  X3DShape b1, b2;
+  read_x3d(body1x3d, b1);
+  read_x3d(body2x3d, b2);
+  for (const X3DShape *b : {&b1, &b2})
+    if (!b->dielectric) throw bem::Error(BEM_E_INVALID, "parseShapes: a body file has no <Dielectric> element");
+  auto diel = [](const X3DShape &b) {
+    Params p;
+    p.params = b.params;
+    return Dielectrics::getDielectric(b.material, p);
+  };
+  const Real shift[3] = {gap.x, gap.y, gap.z};
+  s = new RWGTree(b1.coord, b1.face, diel(b1), shift);
+  t = new RWGTree(b2.coord, b2.face, diel(b2));
+  top = new RWGTree(s, t);
 }
 
 // src/FrequencySweep.cpp:4-17

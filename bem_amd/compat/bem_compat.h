@@ -8,10 +8,12 @@
 //   src/util/Dielectric.h:7-13 Dielectric (eps(eV), mu(eV), setTemperature)
 //   src/Options.h              Options (farThresh6/10/15/21)
 //   src/util/DenseMatrix.h     ComplexDenseMatrix (column-major, operator()(i,j), lda)
+//   src/ShapeParser.h:47       parseShapes (X3D files, x3d.h)
+//   src/util/Dielectrics.h     Dielectrics::getDielectric (material library)
 // compiles against this header and links libbem_hip.so instead.  RWGTree is
-// built from vertex/triangle arrays (mesh ingestion from X3D is not part of
-// this library); a two-body tree is RWGTree(s, t) with s, t the bodies, as
-// the reference's top tree holds sub[0] = body 1 and sub[1] = body 2.
+// built from vertex/triangle arrays or by parseShapes; a two-body tree is
+// RWGTree(s, t) with s, t the bodies, as the reference's top tree holds
+// sub[0] = body 1 and sub[1] = body 2.
 //
 // Error behaviour follows the reference: no return codes on these calls.
 // Where the reference would abort (dimension mismatch) or silently continue
@@ -22,6 +24,7 @@
 #include <complex>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/bem_hip.h"
@@ -36,6 +39,8 @@ struct Error : public std::runtime_error {
 };
 // Process-wide device context used by the reference-shaped calls.
 bem_ctx *context();
+// Throws Error with the context's last message when rc is not 0.
+void check(int rc);
 }  // namespace bem
 
 class Options {
@@ -62,6 +67,55 @@ class ConstantDielectric : public Dielectric {
  private:
   Complex e_, m_;
 };
+
+// Material library (src/util/Dielectrics.cpp:15-63 and the models it
+// registers): "SiN" (tabulated n, k, src/util/SiN.cpp:3352-3358), "SiC"
+// (temperature-dependent Lorentz, src/util/SiC.cpp:4-19), "Al" (Lorentz,
+// src/util/Al.cpp:4-12), "Au" (Drude, src/util/Au.cpp:4-12),
+// "SiliconDoped" with carrier density N (src/util/SiliconDoped.cpp), and for
+// any other name a constant medium from the complex parameters eps and mu
+// (default 1).  One instance per (name, parameters), owned by the registry.
+class SiN : public Dielectric {
+ public:
+  Complex eps(Real eV);
+  Complex mu(Real eV);
+};
+class SiC : public Dielectric {
+ public:
+  SiC();
+  void setTemperature(Real T);
+  Complex eps(Real eV);
+  Complex mu(Real eV);
+  Real epsinf, gamma, wL, wT;
+};
+class Al : public Dielectric {
+ public:
+  Complex eps(Real eV);
+  Complex mu(Real eV);
+};
+class Au : public Dielectric {
+ public:
+  Complex eps(Real eV);
+  Complex mu(Real eV);
+};
+class SiliconDoped : public Dielectric {
+ public:
+  explicit SiliconDoped(Real N) : N(N) {}
+  Complex eps(Real eV);
+  Complex mu(Real eV);
+  Real N;   // carriers per m^3: < 0 n-type, > 0 p-type
+};
+class Params {
+ public:
+  std::vector<std::pair<std::string, std::string>> params;
+  const std::string *find(const std::string &k) const;
+};
+namespace Dielectrics {
+Dielectric *getDielectric(const std::string &name, const Params &params = Params());
+}
+// "re+im" or "re-im" with an optional trailing i or j; "re" alone is real.
+// Throws bem::Error(BEM_E_INVALID) on anything else.
+Complex parseComplex(const std::string &s);
 
 class GreensFunction {
  public:
@@ -105,6 +159,21 @@ class RWGTree {
  private:
   bool owns_g_;
 };
+
+struct Vector3 {
+  Real x, y, z;
+  Vector3(Real x = 0, Real y = 0, Real z = 0) : x(x), y(y), z(z) {}
+};
+
+// src/ShapeParser.cpp:17-63: both bodies read from X3D files (x3d.h), body 1
+// translated by gap, each with the medium of its <Dielectric> element (a
+// file without one is an error), the two-body tree in top.  The trees come
+// back through references (the reference's pointer arguments are by value).
+// Meshes are used as read: the reference's transfer drivers refine no edge
+// (refineEdgeFuncBody returns false, src/HeatTransfer.h:28-31), and its
+// segmentation and tree order only permute the unknowns.
+void parseShapes(RWGTree *&top, RWGTree *&s, RWGTree *&t, const char *body1x3d, const char *body2x3d,
+                 const Vector3 &gap, Options &options);
 
 ComplexDenseMatrix *Zts(RWGTree *t, RWGTree *s, GreensFunction *g, Options &options);
 void Zdense(RWGTree *t, RWGTree *s, ComplexDenseMatrix *dense, GreensFunction *gExt, Options &options);

@@ -1824,6 +1824,20 @@ int bem_body_upload(bem_ctx *c, const double *xyz, int nv, const int32_t *tri, i
     delete b;
     return fail(c, BEM_E_MESH, "mesh is not closed: %d edges for %d triangles", ne, nt);
   }
+  // a triangle of zero area (or a non-finite vertex) has no RWG basis: its
+  // integrals divide by the area
+  for (int t = 0; t < nt; t++) {
+    const double *p0 = xyz + 3 * (size_t)tri[3 * t], *p1 = xyz + 3 * (size_t)tri[3 * t + 1],
+                 *p2 = xyz + 3 * (size_t)tri[3 * t + 2];
+    const double u[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+    const double v[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+    const double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+    const double a2 = cx * cx + cy * cy + cz * cz;
+    if (!(a2 > 0.0) || !std::isfinite(a2)) {
+      delete b;
+      return fail(c, BEM_E_MESH, "triangle %d has zero area or a non-finite vertex", t);
+    }
+  }
   M.ne = ne;
   bemgeo::build_close_pairs(M);
   uint64_t h = 1469598103934665603ull;

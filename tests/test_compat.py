@@ -21,9 +21,13 @@ def test_compat_library_exports_reference_api():
     for sym in ["Zts(RWGTree*, RWGTree*, GreensFunction*, Options&)",
                 "Zdense(RWGTree*, RWGTree*, ComplexDenseMatrix*, GreensFunction*, Options&)",
                 "CalcHeatTransfer(double, double, double, double, RWGTree*, RWGTree*, RWGTree*, Options&)",
-                "FrequencySweep::FrequencySweep(char*)"]:
+                "FrequencySweep::FrequencySweep(char*)",
+                "parseShapes(RWGTree*&, RWGTree*&, RWGTree*&, char const*, char const*, Vector3 const&, Options&)",
+                "Dielectrics::getDielectric(std::__cxx11::basic_string<char, std::char_traits<char>, "
+                "std::allocator<char> > const&, Params const&)"]:
         assert sym in out, sym
     assert os.access(os.path.join(COMPAT, "transfer_sweep"), os.X_OK)
+    assert os.access(os.path.join(COMPAT, "transfer_farm"), os.X_OK)
 
 
 def _write_mesh(path, m):
@@ -60,3 +64,70 @@ def test_transfer_driver_matches_oracle(tmp_path):
         ref = O.spectral_power(s.translated(gap), c, f, 1e-6, 300.0, 0.0, eps, eps, return_parts=True)
         ReS = P / (ref["theta1"] - ref["theta2"])
         assert abs(ReS - ref["ReS"]) <= 1e-8 * abs(ref["ReS"]), (f, ReS, ref["ReS"])
+
+
+def _write_x3d(path, m, material, translation=(0.0, 0.0, 0.0)):
+    """A shape file as Blender's exporter writes them: one Transform, one
+    Shape with its <Dielectric>, comma-separated faces and points."""
+    faces = ", ".join(f"{t[0]} {t[1]} {t[2]} -1" for t in m.tri)
+    pts = ", ".join(" ".join(repr(float(v)) for v in x) for x in m.xyz)
+    path.write_text(f"""<?xml version="1.0" encoding="UTF-8"?>
+<X3D version="3.0"><Scene>
+ <Transform translation="{translation[0]} {translation[1]} {translation[2]}" rotation="1 0 0 0">
+  <Shape><Dielectric material="{material}"/>
+   <IndexedFaceSet coordIndex="{faces}, "><Coordinate point="{pts}, "/></IndexedFaceSet>
+  </Shape>
+ </Transform>
+</Scene></X3D>
+""")
+
+
+@pytest.mark.gpu
+def test_transfer_farm_matches_oracle(tmp_path):
+    """transfer_farm (the reference's farm inputs and ./output format,
+    src/transfer-mpi.cpp) on X3D bodies with library media: every line
+    within 1e-8 of the oracle, and a restart runs only the jobs missing from
+    ./output."""
+    import ctypes
+    from bem_amd import mesh
+    from oracle import oracle as O
+    s, c = mesh.icosphere(1), mesh.cube(2)
+    _write_x3d(tmp_path / "body1.x3d", s, "SiC")
+    _write_x3d(tmp_path / "body2.x3d", c, "SiN")
+    (tmp_path / "a.txt").write_text("1e-6\n")
+    (tmp_path / "gaps.txt").write_text("0 0 2.1\n")
+    (tmp_path / "freqs.txt").write_text("0.095\n0.11\n")
+    (tmp_path / "temps.txt").write_text("300 0\n")
+    farm = os.path.join(COMPAT, "transfer_farm")
+    print("running transfer_farm", flush=True)
+    out = subprocess.run([farm], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    lines = (tmp_path / "output").read_text().splitlines()
+    assert len(lines) == 2
+    lib = ctypes.CDLL(os.path.join(ROOT, "bem_amd", "libbem_compat.so"))
+    lib.bemc_dielectric.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_double)]
+
+    def eps(name, f, T):
+        o = (ctypes.c_double * 4)()
+        assert lib.bemc_dielectric(name.encode(), None, f, T, o) == 0
+        return complex(o[0], o[1])
+
+    got = {}
+    for l in lines:
+        a, f, T1, T2, g, P = (float(v) for v in l.split())
+        assert (a, T1, T2, g) == (1e-6, 300.0, 0.0, 0.0)
+        got[f] = P
+        ref = O.spectral_power(s.translated((0.0, 0.0, 2.1)), c, f, 1e-6, 300.0, 0.0, eps("SiC", f, 300.0),
+                               eps("SiN", f, 0.0))
+        assert abs(P - ref) <= 1e-8 * abs(ref), (f, P, ref)
+    assert sorted(got) == [0.095, 0.11]
+    # restart: nothing left to run; then one line removed is run again
+    out = subprocess.run([farm], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "0 jobs" in out.stderr, out.stderr
+    assert (tmp_path / "output").read_text().splitlines() == lines
+    (tmp_path / "output").write_text(lines[0] + "\n")
+    out = subprocess.run([farm], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0 and "1 jobs" in out.stderr, out.stderr
+    again = (tmp_path / "output").read_text().splitlines()
+    assert again[0] == lines[0] and again[1] == lines[1]

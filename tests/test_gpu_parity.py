@@ -111,6 +111,28 @@ def test_singular_request_reports_error(gpu_ctx):
         pass
 
 
+def test_mesh_checks_at_upload(gpu_ctx):
+    """bem_body_upload rejects what has no RWG basis: an open surface, a
+    vertex index out of range, a zero-area triangle, a non-finite vertex."""
+    from bem_amd import mesh, BemError
+    from bem_amd.mesh import Mesh
+    c = mesh.cube(1)
+    cases = [Mesh(c.xyz, c.tri[:-1]),                       # one triangle missing: open
+             Mesh(c.xyz, np.where(c.tri == 0, c.nv, c.tri))]  # index nv
+    flat = c.xyz.copy()
+    flat[c.tri[0, 2]] = flat[c.tri[0, 0]]                   # first triangle collapses to an edge
+    cases.append(Mesh(flat, c.tri))
+    bad = c.xyz.copy()
+    bad[3, 1] = np.nan
+    cases.append(Mesh(bad, c.tri))
+    codes = []
+    for m in cases:
+        with pytest.raises(BemError) as e:
+            gpu_ctx.body(m)
+        codes.append(e.value.code)
+    assert codes == [-5, -1, -5, -5]   # BEM_E_MESH, BEM_E_INVALID, BEM_E_MESH, BEM_E_MESH
+
+
 def test_sweep_matches_single_solves(gpu_ctx):
     """A sweep keeps several solves in flight on separate streams; each result
     must be bit-identical to the same request solved alone."""
