@@ -630,10 +630,13 @@ __device__ __forceinline__ void lu_update(LuRow<T> &r, int wave, int p, T l) {
     if (TT != SKIP && wave + 4 * TT > J) {
       // rows already chosen as pivots (and the pivot row) carry l = 0, so
       // their U entries come out unchanged without a select
+      // x - l u as four fused multiply-adds
       const T uc = Cplx<T>::mk(lane_read(r.a[TT].x, p), lane_read(r.a[TT].y, p));
       T x = r.a[TT];
-      x.x -= l.x * uc.x - l.y * uc.y;
-      x.y -= l.x * uc.y + l.y * uc.x;
+      x.x = std::fma(-l.x, uc.x, x.x);
+      x.x = std::fma(l.y, uc.y, x.x);
+      x.y = std::fma(-l.x, uc.y, x.y);
+      x.y = std::fma(-l.y, uc.x, x.y);
       r.a[TT] = x;
     }
     lu_update<TT + 1, END, J, SKIP>(r, wave, p, l);
