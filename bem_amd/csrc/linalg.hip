@@ -819,16 +819,20 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
         for (int k = jj; k < b; k++) {
           T l = sA[(o + k) * 64 + o + b + i];
           T x = (k == jj) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + jj) * 64 + o + k];
-          acc.x += l.x * x.x - l.y * x.y;
-          acc.y += l.x * x.y + l.y * x.x;
+          acc.x = std::fma(l.x, x.x, acc.x);
+          acc.x = std::fma(-l.y, x.y, acc.x);
+          acc.y = std::fma(l.x, x.y, acc.y);
+          acc.y = std::fma(l.y, x.x, acc.y);
         }
       } else {
         // T[i][jj] = sum_k U12[i][k] Uinv22[k][jj], U12 = sA[o+i][o+b+k], Uinv22 (k <= jj)
         for (int k = 0; k <= jj; k++) {
           T u = sA[(o + b + k) * 64 + o + i];
           T x = sX[(o + b + jj) * 64 + o + b + k];
-          acc.x += u.x * x.x - u.y * x.y;
-          acc.y += u.x * x.y + u.y * x.x;
+          acc.x = std::fma(u.x, x.x, acc.x);
+          acc.x = std::fma(-u.y, x.y, acc.x);
+          acc.y = std::fma(u.x, x.y, acc.y);
+          acc.y = std::fma(u.y, x.x, acc.y);
         }
       }
       t[s] = acc;
@@ -858,8 +862,10 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
         for (int k = 0; k <= i; k++) {
           T x = (k == i) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + b + k) * 64 + o + b + i];
           T tt = sA[(o + jj) * 64 + o + b + k];
-          acc.x += x.x * tt.x - x.y * tt.y;
-          acc.y += x.x * tt.y + x.y * tt.x;
+          acc.x = std::fma(x.x, tt.x, acc.x);
+          acc.x = std::fma(-x.y, tt.y, acc.x);
+          acc.y = std::fma(x.x, tt.y, acc.y);
+          acc.y = std::fma(x.y, tt.x, acc.y);
         }
         sX[(o + jj) * 64 + o + b + i] = Cplx<T>::mk(-acc.x, -acc.y);
       } else {
@@ -867,8 +873,10 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
         for (int k = i; k < b; k++) {
           T x = sX[(o + k) * 64 + o + i];
           T tt = sA[(o + b + jj) * 64 + o + k];
-          acc.x += x.x * tt.x - x.y * tt.y;
-          acc.y += x.x * tt.y + x.y * tt.x;
+          acc.x = std::fma(x.x, tt.x, acc.x);
+          acc.x = std::fma(-x.y, tt.y, acc.x);
+          acc.y = std::fma(x.x, tt.y, acc.y);
+          acc.y = std::fma(x.y, tt.x, acc.y);
         }
         sX[(o + b + jj) * 64 + o + i] = Cplx<T>::mk(-acc.x, -acc.y);
       }
