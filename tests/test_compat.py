@@ -28,6 +28,13 @@ def test_compat_library_exports_reference_api():
         assert sym in out, sym
     assert os.access(os.path.join(COMPAT, "transfer_sweep"), os.X_OK)
     assert os.access(os.path.join(COMPAT, "transfer_farm"), os.X_OK)
+    assert os.access(os.path.join(COMPAT, "transfer"), os.X_OK)
+
+
+def test_transfer_usage():
+    """transfer keeps the reference's command line (src/transfer.cpp:7-10)."""
+    out = subprocess.run([os.path.join(COMPAT, "transfer")], capture_output=True, text=True)
+    assert out.returncode == 2 and out.stderr.startswith("Usage: transfer <x3d file for body 1>")
 
 
 def _write_mesh(path, m):
@@ -131,3 +138,9 @@ def test_transfer_farm_matches_oracle(tmp_path):
     assert out.returncode == 0 and "1 jobs" in out.stderr, out.stderr
     again = (tmp_path / "output").read_text().splitlines()
     assert again[0] == lines[0] and again[1] == lines[1]
+    # the single-frequency driver on the same files gives the same P
+    out = subprocess.run([os.path.join(COMPAT, "transfer"), "body1.x3d", "body2.x3d", "0.11", "1e-6", "300", "0",
+                          "0", "0", "2.1"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    eV, P = out.stdout.split()
+    assert float(eV) == 0.11 and float(P) == got[0.11]
