@@ -138,7 +138,11 @@ int main(int argc, char **argv) {
       q.eps1[0] = e1.real(), q.eps1[1] = e1.imag(), q.mu1[0] = m1.real(), q.mu1[1] = m1.imag();
       q.eps2[0] = e2.real(), q.eps2[1] = e2.imag(), q.mu2[0] = m2.real(), q.mu2[1] = m2.imag();
       q.precision = precision;
-      bem::check(bem_submit(ctx, body1, body2, &q, &j.ticket));
+      if (bem_submit(ctx, body1, body2, &q, &j.ticket) != 0) {   // e.g. larger than the device: this job only
+        std::fprintf(stderr, "transfer_farm: job a=%g f=%g gap=%g not submitted: %s\n", j.a, j.f, j.gap[0],
+                     bem_last_error(ctx));
+        j.ticket = 0;
+      }
     }
     FILE *out = std::fopen("output", "a");
     if (!out) {
@@ -150,6 +154,12 @@ int main(int argc, char **argv) {
     std::vector<bool> finished(jobs.size(), false);
     size_t left = jobs.size();
     int failures = 0;
+    for (size_t i = 0; i < jobs.size(); i++)
+      if (jobs[i].ticket == 0) {
+        finished[i] = true;
+        left--;
+        failures++;
+      }
     while (left) {
       bool first = true;
       for (size_t i = 0; i < jobs.size(); i++) {
