@@ -270,13 +270,8 @@ void read_x3d(const char *path, X3DShape &shape) {
   read_x3d_text(ss.str(), shape);
 }
 
-// C entry points for callers outside C++ (bindings, tests): the shapes of
-// one file, then a medium of the material library.  Both return 0 or
-// BEM_E_INVALID and need no device.
+// C entry points (declared in x3d.h).
 extern "C" {
-// Counts go to *nv / *nt; points (nv x 3) and triangles (nt x 3) are written
-// when xyz / tri are non-null and *nv / *nt on entry hold their capacity;
-// the material name (empty without a <Dielectric>) to material[0..matcap).
 int bemc_x3d_read(const char *path, double *xyz, int *nv, int *tri, int *nt, char *material, int matcap) {
   try {
     X3DShape s;
@@ -298,8 +293,6 @@ int bemc_x3d_read(const char *path, double *xyz, int *nv, int *tri, int *nt, cha
   }
 }
 
-// eps and mu (re, im, re, im) of the named medium at eV and temperature T;
-// params as "key=value" pairs separated by ';' (may be null).
 int bemc_dielectric(const char *name, const char *params, double eV, double T, double *epsmu4) {
   try {
     Params p;

@@ -44,3 +44,16 @@ struct X3DShape {
 // (BEM_E_INVALID) on an unreadable file or malformed XML.
 void read_x3d(const char *path, X3DShape &shape);
 void read_x3d_text(const std::string &text, X3DShape &shape);
+
+// C entry points of libbem_compat.so for callers outside C++ (bindings,
+// tests); neither needs a device.  Both return 0 or BEM_E_INVALID.
+extern "C" {
+// Shapes of one file: the counts go to *nv / *nt; points (nv x 3) and
+// triangles (nt x 3) are written when xyz / tri are non-null, in which case
+// *nv / *nt hold their capacity on entry; the <Dielectric> material name
+// (empty without one) to material[0..matcap).
+int bemc_x3d_read(const char *path, double *xyz, int *nv, int *tri, int *nt, char *material, int matcap);
+// eps and mu (re, im, re, im) of a medium of the material library at eV and
+// temperature T; params as "key=value" pairs separated by ';' (or null).
+int bemc_dielectric(const char *name, const char *params, double eV, double T, double *epsmu4);
+}
