@@ -6,6 +6,8 @@ Tolerances (stated by the task for the plain FP64 path):
 """
 import sys
 
+import os
+
 import numpy as np
 import pytest
 
@@ -231,3 +233,25 @@ def test_lookahead_matches_plain_lu():
             out.append(ctx.calc_heat_transfer(b1, b2, Request(eV=eV, gap=gap, eps1=eps, eps2=eps))["ReS"])
     assert abs(out[0] - out[1]) <= 1e-12 * abs(out[1])
     assert abs(out[0] - ref["ReS"]) <= S_TOL * abs(ref["ReS"])
+
+
+C3_ANCHOR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_anchor.json")
+
+
+@pytest.mark.skipif(not os.path.exists(C3_ANCHOR), reason="c3_anchor.json not generated")
+def test_c3_anchor_vs_oracle(gpu_ctx):
+    """The sphere-over-plate configuration at full size (M = 33360) against
+    the oracle's value for one (gap, frequency) point, computed on the CPU by
+    tests/golden/make_c3_anchor.py: Re S within 1e-8."""
+    import json
+    from bem_amd import mesh, Request, lorentz_eps
+    d = json.load(open(C3_ANCHOR))
+    sph = mesh.icosphere(4)
+    plate = mesh.cuboid(4.0, 4.0, 0.5, 30, 30, 10, origin=(-2.0, -2.0, -0.5))
+    b1, b2 = gpu_ctx.body(sph), gpu_ctx.body(plate)
+    eps = lorentz_eps(d["eV"])
+    assert abs(eps - complex(*d["eps"])) == 0
+    print(f"c3 anchor: eV={d['eV']} gap={d['gap']} M={d['M']}", flush=True)
+    r = gpu_ctx.calc_heat_transfer(b1, b2, Request(eV=d["eV"], a=d["a"], T1=d["T1"], T2=d["T2"],
+                                                   gap=(0.0, 0.0, 1.0 + d["gap"]), eps1=eps, eps2=eps))
+    assert abs(r["ReS"] - d["ReS"]) <= 1e-8 * abs(d["ReS"]), (r["ReS"], d["ReS"])
