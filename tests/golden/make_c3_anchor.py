@@ -1,6 +1,6 @@
 """Anchor value of the sphere-over-plate configuration (M = 33360) from the
 oracle: one (gap, frequency) point, computed on the CPU with the C
-restatement's assembly and LAPACK, written to c3_anchor.json.
+restatement's assembly and LAPACK, added to the points of c3_anchor.json.
 
 Composition as CalcHeatTransfer (src/HeatTransfer.cpp:8-59): W = Zts(top) with
 the interior blocks added, X = W^-1[0:m1, m1:M] (the only block the two
@@ -71,7 +71,9 @@ def main():
            "eV": eV, "gap": gap, "a": a, "T1": T1, "T2": T2, "eps": [eps.real, eps.imag], "M": m1 + m2,
            "ReS": S.real, "ImS": S.imag, "P": S.real * (th1 - th2), "seconds": time.time() - t0}
     path = os.path.join(HERE, "c3_anchor.json")
-    json.dump(out, open(path, "w"), indent=1)
+    points = json.load(open(path))["points"] if os.path.exists(path) else []
+    points = [p for p in points if (p["eV"], p["gap"]) != (eV, gap)] + [out]
+    json.dump({"points": points}, open(path, "w"), indent=1)
     say("wrote", path, out["ReS"])
 
 
