@@ -151,7 +151,7 @@ struct PairGeom {
 // Host-visible status of a self entry, written by its producing stream.
 struct SelfStatus {
   int info[4];                    // [0] zero pivot, [1] 36-point queue overflow
-  unsigned long long growth[2];   // max|U|, max|A| of the LU (role 0)
+  unsigned long long growth[3];   // max|U|, max|A|, max|L| (strictly lower) of the LU (role 0)
 };
 
 // Gap-independent blocks of one body at one frequency and precision (src/HeatTransfer.cpp:
@@ -620,6 +620,7 @@ static int lu_factor_block2(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *
              A + (int64_t)K1 * lda + K1, lda));
   }
   if (growth && (e = absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  if (growth && (e = absmax(A, lda, n, n, 2, growth + 2, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
   return 0;
 }
 
@@ -631,8 +632,8 @@ static int lu_factor_block2(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *
 // next diagonal block is factored on the lane's auxiliary stream while the
 // rest of the trailing update (disjoint columns) runs on the main stream, so
 // the latency-bound block factorisation hides behind the large GEMM.
-// growth (optional, device) receives max|A| before in growth[1] and
-// max|U| after in growth[0].
+// growth (optional, device) receives max|A| before in growth[1], max|U|
+// after in growth[0] and max|L| (strictly lower part) in growth[2].
 template <typename T>
 static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *ipiv, int *info, T *inv,
                            unsigned long long *growth) {
@@ -671,6 +672,7 @@ static int lu_factor_block(bem_ctx *c, Lane &L, T *A, int64_t lda, int n, int *i
     }
   }
   if (growth && (e = absmax(A, lda, n, n, 1, growth, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
+  if (growth && (e = absmax(A, lda, n, n, 2, growth + 2, L.st))) return fail(c, BEM_E_DEVICE, "absmax: %d", e);
   return 0;
 }
 
@@ -1055,7 +1057,7 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   };
   int r = self_alloc(c, e, (void **)&e.Z, nn);
   if (!r) r = self_alloc(c, e, (void **)&e.Gs, nn);
-  if (!r) r = self_alloc(c, e, (void **)&e.dinfo, sizeof(int) * 4 + sizeof(unsigned long long) * 2);
+  if (!r) r = self_alloc(c, e, (void **)&e.dinfo, sizeof(int) * 4 + sizeof(unsigned long long) * 3);
   const bool mixed = prec == BEM_PREC_MIXED;
   if (!r && role == 0 && !mixed) r = self_alloc(c, e, (void **)&e.inv, inv_bytes(n));
   if (!r && role == 0) r = self_alloc(c, e, (void **)&e.ipiv, sizeof(int) * (size_t)(n + 64));
@@ -1074,7 +1076,7 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   double *tiles;
   size_t np = (size_t)T * T;
   r = ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * np, (void **)&tiles);
-  if (!r && hipMemsetAsync(e.dinfo, 0, sizeof(int) * 4 + sizeof(unsigned long long) * 2, L.st) != hipSuccess)
+  if (!r && hipMemsetAsync(e.dinfo, 0, sizeof(int) * 4 + sizeof(unsigned long long) * 3, L.st) != hipSuccess)
     r = fail(c, BEM_E_DEVICE, "memset");
   if (twin) {
     // same geometry, medium and omega: the blocks are bitwise those of twin
@@ -1282,10 +1284,10 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
   CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
   CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
-  CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 2, (void **)&gro));
+  CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 3, (void **)&gro));
   CHK(ws_get(c, L, "trace_out", sizeof(double2) * 2, (void **)&trace_out));
   DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
-  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 2, L.st));
+  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 3, L.st));
   CHK(assemble_cross(c, L, g, mext, Bm, Cm, info + 1));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
@@ -1325,16 +1327,22 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   DEV(c, hipEventRecord(L.ph[3], L.st));
   DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
-  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, L.st));
   L.busy = true;
   return 0;
 }
 
+// Growth of the block-local LU: max(1, max|L|) max|U| / max|A| (|re|+|im|
+// norms), the factor by which |L||U| can exceed |A| in the backward error.
+// Partial pivoting over whole columns keeps |L| <= 1; pivots confined to a
+// diagonal block can leave large multipliers below it without any growth of
+// U, which the max|L| term catches.
 static double growth_ratio(const unsigned long long *g) {
-  double u, a;
+  double u, a, l;
   std::memcpy(&u, &g[0], sizeof u);
   std::memcpy(&a, &g[1], sizeof a);
-  double r = a > 0 ? u / a : 0.0;
+  std::memcpy(&l, &g[2], sizeof l);
+  double r = a > 0 ? std::max(1.0, l) * u / a : 0.0;
   return std::isfinite(r) ? r : INFINITY;
 }
 
@@ -2193,13 +2201,13 @@ int bem_zgetrf_local(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv, dou
   unsigned long long *gro;
   double2 *inv;
   CHK(ws_get(c, L, "info_getrf", sizeof(int), (void **)&info));
-  CHK(ws_get(c, L, "growth_getrf", sizeof(unsigned long long) * 2, (void **)&gro));
+  CHK(ws_get(c, L, "growth_getrf", sizeof(unsigned long long) * 3, (void **)&gro));
   CHK(ws_get(c, L, "inv_getrf", inv_bytes(n), (void **)&inv));
   DEV(c, hipMemsetAsync(info, 0, sizeof(int), L.st));
-  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 2, L.st));
+  DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 3, L.st));
   CHK(lu_factor_block(c, L, (double2 *)A, lda, n, ipiv, info, inv, gro));
   int h = 0;
-  unsigned long long g[2] = {0, 0};
+  unsigned long long g[3] = {0, 0, 0};
   DEV(c, hipMemcpyAsync(&h, info, sizeof(int), hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(g, gro, sizeof g, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipStreamSynchronize(L.st));

@@ -892,7 +892,8 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
   }
 }
 
-// max |re|+|im| over an m x n block (upper=1: only entries with row <= col),
+// max |re|+|im| over an m x n block (upper=1: only entries with row <= col,
+// upper=2: only row > col),
 // folded into *out as the bit pattern of a non-negative double (atomic max).
 template <typename T>
 __global__ void absmax_kernel(const T *__restrict__ A, int64_t lda, int m, int n, int upper,
@@ -902,9 +903,9 @@ __global__ void absmax_kernel(const T *__restrict__ A, int64_t lda, int m, int n
   __shared__ double wmax[4];
   double v = 0.0;
   for (int c = blockIdx.y; c < n; c += gridDim.y) {
-    int rmax = upper ? min(m, c + 1) : m;
+    const int rmax = upper == 1 ? min(m, c + 1) : m, rmin = upper == 2 ? c + 1 : 0;
 #pragma unroll 4
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < rmax; r += gridDim.x * blockDim.x) {
+    for (int r = rmin + blockIdx.x * blockDim.x + threadIdx.x; r < rmax; r += gridDim.x * blockDim.x) {
       T x = A[(int64_t)c * lda + r];
       v = fmax(v, std::fabs((double)x.x) + std::fabs((double)x.y));
     }

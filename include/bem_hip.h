@@ -79,12 +79,15 @@ typedef struct {
 
 /* Pivoting of the dense solve.  Default: partial pivoting inside each 64 x 64
  * diagonal block (as the reference's block LU over tree blocks,
- * src/util/HMatrix.h:277-316), with the growth factor max|U|/max|A| measured
- * and reported; a solve whose growth exceeds BEM_GROWTH_LIMIT is repeated
- * with tournament (communication-avoiding) partial pivoting over whole
- * columns.  BEM_SOLVE_TOURNAMENT requests the latter from the start. */
+ * src/util/HMatrix.h:277-316), with the growth factor
+ * max(1, max|L|) max|U| / max|A| measured and reported (|L| <= 1 under
+ * partial pivoting over whole columns; block-local pivots can leave large
+ * multipliers below the block); a solve whose growth exceeds
+ * BEM_GROWTH_LIMIT is repeated with tournament (communication-avoiding)
+ * partial pivoting over whole columns.  BEM_SOLVE_TOURNAMENT requests the
+ * latter from the start. */
 #define BEM_SOLVE_TOURNAMENT 1
-#define BEM_GROWTH_LIMIT 1e4
+#define BEM_GROWTH_LIMIT 16.0
 /* BEM_PREC_MIXED only: one step of iterative refinement -- the residual of
  * W [X; Sc^-1] = [0; I] against the FP64-assembled blocks in FP64, the
  * correction through the complex64 factors, and FP64 products and trace of

@@ -1,15 +1,19 @@
-"""Anchor value of the sphere-over-plate configuration (M = 33360) from the
-oracle: one (gap, frequency) point, computed on the CPU with the C
-restatement's assembly and LAPACK, added to the points of c3_anchor.json.
+"""Anchor values of full-size configurations from the oracle, one (gap,
+frequency) point per run, computed on the CPU with the C restatement's
+assembly and LAPACK and added to the points of <workload>_anchor.json:
+  c3   level-4 icosphere over a 4 x 4 x 0.5 plate of 6000 triangles (M = 33360),
+       sphere centre at z = 1 + gap;
+  tip  the cone tip of the serving mix (apex radius 0.05, 3000 triangles) over
+       a 4 x 4 x 0.5 plate of 1920 triangles (M = 14760), apex at z = gap.
 
 Composition as CalcHeatTransfer (src/HeatTransfer.cpp:8-59): W = Zts(top) with
 the interior blocks added, X = W^-1[0:m1, m1:M] (the only block the two
 products use), B1 = sym(G1) X, B2 = X sym(G2), S = tr(B2^H B1) / 4.  X comes
 from an LU solve of W against [0; I] rather than the full inverse (same
-block, a third of the memory).  Needs about 45 GB of host memory and an hour
-or more on 8 cores.
+block, a third of the memory).  c3 needs about 45 GB of host memory and
+20 minutes on 8 cores, tip a few minutes.
 
-    python tests/golden/make_c3_anchor.py [eV] [gap]
+    python tests/golden/make_anchor.py <c3|tip> <eV> <gap>
 """
 import json
 import os
@@ -25,12 +29,24 @@ from oracle import oracle as O          # noqa: E402
 from bem_amd import mesh, lorentz_eps   # noqa: E402
 
 
+WORKLOADS = {
+    "c3": ("icosphere level 4 (5120 tri, radius 1) over a 4 x 4 x 0.5 plate (6000 tri), "
+           "sphere centre at z = 1 + gap, plate top at z = 0",
+           lambda: mesh.icosphere(4), lambda: mesh.cuboid(4.0, 4.0, 0.5, 30, 30, 10, origin=(-2.0, -2.0, -0.5)),
+           lambda gap: 1.0 + gap),
+    "tip": ("cone tip (apex radius 0.05, 3000 tri) over a 4 x 4 x 0.5 plate (1920 tri), apex at z = gap",
+            lambda: mesh.cone_tip(apex_radius=0.05, n_around=50, n_cap=6, n_along=19, n_base=6),
+            lambda: mesh.cuboid(4.0, 4.0, 0.5, 20, 20, 2, origin=(-2.0, -2.0, -0.5)),
+            lambda gap: gap),
+}
+
+
 def main():
-    eV = float(sys.argv[1]) if len(sys.argv) > 1 else 0.112
-    gap = float(sys.argv[2]) if len(sys.argv) > 2 else 0.05
+    name, eV, gap = sys.argv[1], float(sys.argv[2]), float(sys.argv[3])
+    desc, body1, body2, shift = WORKLOADS[name]
     a, T1, T2 = 1e-6, 300.0, 0.0
-    sph = mesh.icosphere(4).translated((0.0, 0.0, 1.0 + gap))
-    plate = mesh.cuboid(4.0, 4.0, 0.5, 30, 30, 10, origin=(-2.0, -2.0, -0.5))
+    sph = body1().translated((0.0, 0.0, shift(gap)))
+    plate = body2()
     eps = lorentz_eps(eV)
     omega = O.omega_tilde(eV, a)
     t0 = time.time()
@@ -66,11 +82,9 @@ def main():
     del S2, X
     S = 0.25 * np.vdot(B2, B1)
     th1, th2 = O.theta(omega, T1), O.theta(omega, T2)
-    out = {"workload": "icosphere level 4 (5120 tri, radius 1) over a 4 x 4 x 0.5 plate (6000 tri), "
-                       "sphere centre at z = 1 + gap, plate top at z = 0",
-           "eV": eV, "gap": gap, "a": a, "T1": T1, "T2": T2, "eps": [eps.real, eps.imag], "M": m1 + m2,
+    out = {"workload": desc, "eV": eV, "gap": gap, "shift": [0.0, 0.0, shift(gap)], "a": a, "T1": T1, "T2": T2, "eps": [eps.real, eps.imag], "M": m1 + m2,
            "ReS": S.real, "ImS": S.imag, "P": S.real * (th1 - th2), "seconds": time.time() - t0}
-    path = os.path.join(HERE, "c3_anchor.json")
+    path = os.path.join(HERE, f"{name}_anchor.json")
     points = json.load(open(path))["points"] if os.path.exists(path) else []
     points = [p for p in points if (p["eV"], p["gap"]) != (eV, gap)] + [out]
     json.dump({"points": points}, open(path, "w"), indent=1)

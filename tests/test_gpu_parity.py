@@ -235,26 +235,31 @@ def test_lookahead_matches_plain_lu():
     assert abs(out[0] - ref["ReS"]) <= S_TOL * abs(ref["ReS"])
 
 
-C3_ANCHOR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_anchor.json")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.skipif(not os.path.exists(C3_ANCHOR), reason="c3_anchor.json not generated")
-def test_c3_anchor_vs_oracle(gpu_ctx):
-    """The sphere-over-plate configuration at full size (M = 33360) against
-    the oracle's values for (gap, frequency) points computed on the CPU by
-    tests/golden/make_c3_anchor.py: Re S within 1e-8."""
+@pytest.mark.parametrize("workload", ["c3", "tip"])
+def test_full_size_anchor_vs_oracle(gpu_ctx, workload):
+    """Full-size configurations against the oracle's values at (gap,
+    frequency) points computed on the CPU by tests/golden/make_anchor.py:
+    the sphere over the plate (M = 33360) and the cone tip over the plate of
+    the serving mix (M = 14760).  Re S within 1e-8."""
     import json
-    from bem_amd import mesh, Request, lorentz_eps
-    points = json.load(open(C3_ANCHOR))["points"]
-    sph = mesh.icosphere(4)
-    plate = mesh.cuboid(4.0, 4.0, 0.5, 30, 30, 10, origin=(-2.0, -2.0, -0.5))
-    b1, b2 = gpu_ctx.body(sph), gpu_ctx.body(plate)
+    path = os.path.join(GOLDEN, f"{workload}_anchor.json")
+    if not os.path.exists(path):
+        pytest.skip(f"{workload}_anchor.json not generated")
+    sys.path.insert(0, GOLDEN)
+    from make_anchor import WORKLOADS
+    from bem_amd import Request, lorentz_eps
+    points = json.load(open(path))["points"]
+    _, body1, body2, _ = WORKLOADS[workload]
+    b1, b2 = gpu_ctx.body(body1()), gpu_ctx.body(body2())
     for d in points:
         eps = lorentz_eps(d["eV"])
         assert abs(eps - complex(*d["eps"])) == 0
-        print(f"c3 anchor: eV={d['eV']} gap={d['gap']} M={d['M']}", flush=True)
+        print(f"{workload} anchor: eV={d['eV']} gap={d['gap']} M={d['M']}", flush=True)
         r = gpu_ctx.calc_heat_transfer(b1, b2, Request(eV=d["eV"], a=d["a"], T1=d["T1"], T2=d["T2"],
-                                                       gap=(0.0, 0.0, 1.0 + d["gap"]), eps1=eps, eps2=eps))
+                                                       gap=tuple(d["shift"]), eps1=eps, eps2=eps))
         rel = abs(r["ReS"] - d["ReS"]) / abs(d["ReS"])
         print(f"  ReS gpu {r['ReS']:.15e} oracle {d['ReS']:.15e} rel {rel:.2e}", flush=True)
         assert rel <= 1e-8, (d["eV"], d["gap"], r["ReS"], d["ReS"])
