@@ -191,7 +191,7 @@ struct Lane {
   hipEvent_t ph[4] = {nullptr, nullptr, nullptr, nullptr};
   double2 *h_trace = nullptr;   // pinned host slots of the results
   int *h_info = nullptr;        // [0] zero pivot in the Schur LU, [1] queue overflow
-  unsigned long long *h_growth = nullptr;   // max|U|, max|A| of LU(S)
+  unsigned long long *h_growth = nullptr;   // max|U|, max|A|, max|L| of LU(S)
   bool busy = false;
   bem_result *res = nullptr;
   bem_request req;

@@ -105,7 +105,7 @@ typedef struct {
   int iters;         /* refinement steps (mixed), 0 for FP64 */
   int status;        /* 0 or BEM_E_* for this request */
   double ms_assemble, ms_factor, ms_products, ms_total;  /* device-event times */
-  double growth;     /* max |U| / max |A| over the two LU factorisations */
+  double growth;     /* max(1, max|L|) max|U| / max|A|, the larger of the two LU factorisations */
   int pivoting;      /* 0 block-local, 1 tournament */
   double ms_queue;   /* served requests: submit to issue (ms); ms_total is then submit to result */
   int precision;     /* BEM_PREC_* the result was computed in */
@@ -179,7 +179,7 @@ int bem_zgetrf(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv);
  * diagonal block: the factorisation the solve uses by default, as the
  * reference's tree-block LU factors each diagonal block on its own
  * (src/util/HMatrix.h:276-307).  ipiv as bem_zgetrf, every interchange
- * within its block; growth (optional, host) receives max|U| / max|A|.
+ * within its block; growth (optional, host) receives max(1, max|L|) max|U| / max|A|.
  * Returns BEM_E_SINGULAR on a zero pivot. */
 int bem_zgetrf_local(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv, double *growth);
 /* B (n x nrhs, ldb) <- A^-1 B from the factors of bem_zgetrf. */
