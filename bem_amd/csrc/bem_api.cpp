@@ -40,6 +40,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <unordered_map>
 #include <unordered_set>
 #include <vector>
@@ -106,6 +107,11 @@ int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st);
 int bem_signed_transpose(const double2 *B, int64_t ldb, int m, int n, double2 *C, int64_t ldc, hipStream_t st);
 int bem_fill_hash(void *p, size_t bytes, int as_float, unsigned long long seed, hipStream_t st);
 int bem_c2z_add(const float2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
+size_t bem_sumsq_partials(int m, int n);
+int bem_sumsq(const double2 *A, int64_t lda, int m, int n, double *partial, int add, double *out, hipStream_t st);
+int bem_zaxpy(double2 alpha, const double2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
+int bem_zdotc_cols(const double2 *G, int64_t ldg, const double2 *D, int64_t ldd, int m, int n, double2 *out,
+                   hipStream_t st);
 }
 
 // src/util/SI.cpp:3-9
@@ -166,6 +172,10 @@ struct SelfEntry {
   int role = 0, pivot = 0, prec = BEM_PREC_FP64;
   int n = 0;
   double2 *Z = nullptr, *Gs = nullptr, *inv = nullptr;
+  // the unfactored FP64 self block (residuals of the error probe and of
+  // refinement): a copy taken before the LU for an FP64 entry in role 0, Z
+  // itself otherwise
+  double2 *Araw = nullptr;
   // BEM_PREC_MIXED: Z stays the raw FP64 block (refinement residuals) and
   // Z32 holds its complex64 copy (packed LU with inv32 in role 0)
   float2 *Z32 = nullptr, *Gs32 = nullptr, *inv32 = nullptr;
@@ -178,6 +188,18 @@ struct SelfEntry {
   int pins = 0;
   uint64_t stamp = 0;
 };
+
+// Error probe of a solve, written on the device (see bem_hip.h, Error probe):
+// max|d_1| and max|x_1| as the bit patterns of non-negative doubles, and the
+// squared Frobenius norms of the probe residual r and of w.
+struct ProbeOut {
+  unsigned long long dmax, xmax;
+  double rr, ww;
+  double2 h[BEM_PROBE_COLS];   // g_j^H d_1j, g_j = sym G1 X sym G2 w_j (first-order change of Re S)
+};
+
+// Stages of the request in flight on a lane.
+enum { ST_IDLE = 0, ST_SOLVE = 1, ST_REFINE = 2 };
 
 // One stream and its workspaces; holds at most one solve in flight.
 struct Lane {
@@ -201,6 +223,11 @@ struct Lane {
   size_t ws_bytes = 0;            // bytes of this lane's named workspaces
   int prec = BEM_PREC_FP64;       // precision of the solve in flight
   bool refine = false;
+  int stage = ST_IDLE;            // what the lane's stream is running for the request
+  int iters = 0;                  // refinement steps issued for the request
+  int restarts = 0;               // re-solves of the request so far
+  ProbeOut *h_probe = nullptr;    // pinned copy of the last stage's probe
+  double *h_rfull = nullptr;      // pinned: squared norm of the last refinement residual
   // served request in flight (ticket 0: none)
   bem_ctx *ctx = nullptr;
   uint64_t ticket = 0;
@@ -216,6 +243,7 @@ struct Pending {
   bem_body *b1, *b2;
   bem_request req;
   std::chrono::steady_clock::time_point t_submit;
+  int restarts = 0;
 };
 
 }  // namespace
@@ -968,7 +996,7 @@ static size_t self_entry_bytes(int n, int role, int prec) {
   size_t b = 2 * nn + 256;
   if (role == 0) b += sizeof(int) * (size_t)(n + 64);
   if (prec == BEM_PREC_MIXED) b += nn + (role == 0 ? inv_bytes<float2>(n) : 0);
-  else if (role == 0) b += inv_bytes(n);
+  else if (role == 0) b += inv_bytes(n) + nn;   // inverses and the unfactored copy
   return b;
 }
 
@@ -1060,6 +1088,8 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   if (!r) r = self_alloc(c, e, (void **)&e.dinfo, sizeof(int) * 4 + sizeof(unsigned long long) * 3);
   const bool mixed = prec == BEM_PREC_MIXED;
   if (!r && role == 0 && !mixed) r = self_alloc(c, e, (void **)&e.inv, inv_bytes(n));
+  if (!r && role == 0 && !mixed) r = self_alloc(c, e, (void **)&e.Araw, nn);
+  if (!e.Araw) e.Araw = e.Z;
   if (!r && role == 0) r = self_alloc(c, e, (void **)&e.ipiv, sizeof(int) * (size_t)(n + 64));
   if (!r && mixed) r = self_alloc(c, e, (void **)&e.Z32, nn / 2);
   if (!r && mixed) r = self_alloc(c, e, (void **)&e.Gs32, nn / 2);
@@ -1098,7 +1128,9 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
       r = fail(c, BEM_E_DEVICE, "complex64 copy launch");
     if (!r && role == 0) r = lu_factor_block(c, L, e.Z32, n, n, e.ipiv, e.dinfo, e.inv32, e.dgrowth);
   } else if (!r && role == 0) {
-    if (pivot) r = lu_factor(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv);
+    if (hipMemcpyAsync(e.Araw, e.Z, nn, hipMemcpyDeviceToDevice, L.st) != hipSuccess)
+      r = fail(c, BEM_E_DEVICE, "self block copy");
+    else if (pivot) r = lu_factor(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv);
     else r = lu_factor_block(c, L, e.Z, n, n, e.ipiv, e.dinfo, e.inv, e.dgrowth);
   }
   if (!r && hipMemcpyAsync(e.h, e.dinfo, sizeof(SelfStatus), hipMemcpyDeviceToHost, L.st) != hipSuccess)
@@ -1139,15 +1171,18 @@ static void self_drop_body(bem_ctx *c, const void *b) {
 
 // ------------------------------------------------------------- one solve
 // Cross blocks of Z_ext for the pair's gap: B = Z_ext(1,2) (m1 x m2) and
-// C = Z_ext(2,1) (m2 x m1), complex128.
-static int assemble_cross(bem_ctx *c, Lane &L, PairGeom &g, const Medium &mext, double2 *Bm, double2 *Cm, int *flag) {
+// C = Z_ext(2,1) (m2 x m1), complex128.  Braw (optional) receives a second
+// copy of B for the residuals of the error probe and of refinement, since the
+// FP64 elimination overwrites B.
+static int assemble_cross(bem_ctx *c, Lane &L, PairGeom &g, const Medium &mext, double2 *Bm, double2 *Cm,
+                          double2 *Braw, int *flag) {
   const int T1 = g.T1, T2 = g.top.T - g.T1, Eb1 = g.E1, E = g.top.E;
   const int m1 = 2 * Eb1, m2 = 2 * (E - Eb1);
   double *tiles;
   size_t npc = (size_t)T1 * T2;
   CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(npc, 1), (void **)&tiles));
   CHK(assemble_tiles(c, L, g.top, mext, tiles, 0, T1, T1, T1 + T2, flag));
-  CHK(gather(c, L, tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, nullptr, 0, 0, 0));
+  CHK(gather(c, L, tiles, npc, T2, 0, T1, g.top.rw, 0, Eb1, Eb1, E, &mext, Bm, m1, 0, 0, 0, Braw, m1, 0, 0));
   // C = Z_ext(2,1) = S B^T S (reciprocity of the far-pair rules, S = +1 on J
   // and -1 on M unknowns) instead of a second assembly
   int e = bem_signed_transpose(Bm, m1, m1, m2, Cm, m2, L.st);
@@ -1155,19 +1190,211 @@ static int assemble_cross(bem_ctx *c, Lane &L, PairGeom &g, const Medium &mext, 
   return 0;
 }
 
-// Mixed-precision solve (BEM_PREC_MIXED) after the FP64 assembly of B, C:
-// the block elimination Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1 and the
-// products run in complex64 on FP32 MFMA from the cached complex64 LU of A
-// (the trace sums in FP64).  With BEM_SOLVE_REFINE one step of iterative
-// refinement follows: the residual of W [X; Sc^-1] = [0; I] against the FP64
-// blocks A, B, C, D in FP64, the correction through the complex64 factors,
-// and the products and trace of the corrected X in FP64; the relative change
-// of S is reported as the error estimate of the complex64 result.
+// Factors of a square block for the triangular solves: packed LU, its
+// interchanges, the diagonal-block inverses, and whether every interchange
+// stays inside its 64-block (block-local pivoting).
+template <typename T>
+struct LuRef {
+  const T *lu;
+  int n;
+  const int *ipiv;
+  const T *inv;
+  bool local;
+};
+
+// FP64 <-> working-precision copies of an m x n block (leading dimension m):
+// a plain copy in complex128 (none when src and dst coincide), rounding to
+// complex64 and widening back in the mixed path.
+static int to_work(bem_ctx *c, Lane &L, const double2 *src, double2 *dst, int m, int n) {
+  if (src != dst) DEV(c, hipMemcpyAsync(dst, src, sizeof(double2) * (size_t)m * n, hipMemcpyDeviceToDevice, L.st));
+  return 0;
+}
+static int to_work(bem_ctx *c, Lane &L, const double2 *src, float2 *dst, int m, int n) {
+  if (bem_z2c(src, m, dst, m, m, n, L.st)) return fail(c, BEM_E_DEVICE, "complex64 copy launch");
+  return 0;
+}
+static int from_work(bem_ctx *c, Lane &L, const double2 *src, double2 *dst, int m, int n) {
+  return to_work(c, L, src, dst, m, n);
+}
+static int from_work(bem_ctx *c, Lane &L, const float2 *src, double2 *dst, int m, int n) {
+  if (bem_c2z(src, m, dst, m, m, n, L.st)) return fail(c, BEM_E_DEVICE, "complex128 copy launch");
+  return 0;
+}
+// dst (FP64) += src (working precision)
+static int add_work(bem_ctx *c, Lane &L, const double2 *src, double2 *dst, int m, int n) {
+  if (bem_zaxpy(make_double2(1, 0), src, m, dst, m, m, n, L.st)) return fail(c, BEM_E_DEVICE, "update launch");
+  return 0;
+}
+static int add_work(bem_ctx *c, Lane &L, const float2 *src, double2 *dst, int m, int n) {
+  if (bem_c2z_add(src, m, dst, m, m, n, L.st)) return fail(c, BEM_E_DEVICE, "update launch");
+  return 0;
+}
+
+// Seed of the probe's columns (fixed: every solve of a request, alone or in a
+// mix, draws the same ones).
+static const unsigned long long PROBE_SEED = 0x5EED0B5Eull;
+
+// Error probe of a solution held as Xn = -X (m1 x m2, FP64) and, when X2 is
+// given, Z2 = Sc^-1 (m2 x m2, FP64; otherwise Sc^-1 w comes from the factors
+// of Sc).  w: BEM_PROBE_COLS seeded columns with complex64 values (exact in
+// both precisions); x = [X w; Z2 w]; r = [0; w] - W x in FP64 against the
+// unfactored blocks; d = W^-1 r by block elimination through the factors of
+// A and Sc in the working precision T.  The top rows of W^-1 [0; w] are
+// X_exact w, so d_1 = dX w with dX = X_exact - X, up to the accuracy of the
+// correction and whatever the error of Z2 w.
+// Re S = 1/4 tr(sym G2 X^H sym G1 X) changes to first order by 1/2 Re t,
+// t = tr(F dX), F = sym G2 X^H sym G1; with E[w w^H] = s^2 I,
+// E[w^H F dX w] = s^2 t, and w^H F = g^H for g = sym G1 X sym G2 w, so
+// h_j = g_j^H d_1j are the terms of a trace estimate of t.
+// out: max|d_1|, max|x_1|, ||r||^2, ||w||^2, h_j.
+template <typename T>
+static int probe(bem_ctx *c, Lane &L, const LuRef<T> &A, const LuRef<T> &S, const double2 *Araw,
+                 const double2 *Braw, const double2 *Cm, const double2 *Draw, const double2 *Gs1, const double2 *Gs2,
+                 const double2 *Xn, const double2 *X2, ProbeOut *out) {
+  const int m1 = A.n, m2 = S.n, k = BEM_PROBE_COLS;
+  const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  float2 *w32;
+  double2 *w, *x1, *x2, *r1, *r2, *t1, *t2;
+  T *a, *b;
+  double *part;
+  CHK(ws_get(c, L, "pr_w32", sizeof(float2) * (size_t)m2 * k, (void **)&w32));
+  CHK(ws_get(c, L, "pr_w", sizeof(double2) * (size_t)m2 * k, (void **)&w));
+  CHK(ws_get(c, L, "pr_x1", sizeof(double2) * (size_t)m1 * k, (void **)&x1));
+  CHK(ws_get(c, L, "pr_x2", sizeof(double2) * (size_t)m2 * k, (void **)&x2));
+  CHK(ws_get(c, L, "pr_r1", sizeof(double2) * (size_t)m1 * k, (void **)&r1));
+  CHK(ws_get(c, L, "pr_r2", sizeof(double2) * (size_t)m2 * k, (void **)&r2));
+  CHK(ws_get(c, L, "pr_t1", sizeof(double2) * (size_t)m1 * k, (void **)&t1));
+  CHK(ws_get(c, L, "pr_t2", sizeof(double2) * (size_t)m2 * k, (void **)&t2));
+  CHK(ws_get(c, L, "pr_a", sizeof(T) * (size_t)m1 * k, (void **)&a));
+  CHK(ws_get(c, L, "pr_b", sizeof(T) * (size_t)m2 * k, (void **)&b));
+  CHK(ws_get(c, L, "pr_part", sizeof(double) * std::max(bem_sumsq_partials(m1, k), bem_sumsq_partials(m2, k)),
+             (void **)&part));
+  const size_t wb1 = sizeof(double2) * (size_t)m1 * k, wb2 = sizeof(double2) * (size_t)m2 * k;
+  if (bem_fill_hash(w32, sizeof(float2) * (size_t)m2 * k, 1, PROBE_SEED, L.st) ||
+      bem_c2z(w32, m2, w, m2, m2, k, L.st))
+    return fail(c, BEM_E_DEVICE, "probe fill launch");
+  DEV(c, hipMemsetAsync(out, 0, 2 * sizeof(unsigned long long), L.st));
+  // x = [X w; Z2 w]
+  if (X2) {
+    CHK(gemm(c, L, m2, k, m2, one, X2, m2, w, m2, zero, x2, m2));
+  } else {
+    CHK(to_work(c, L, w, b, m2, k));
+    CHK(lu_solve(c, L, S.lu, m2, m2, S.ipiv, S.inv, b, m2, k, S.local));
+    CHK(from_work(c, L, b, x2, m2, k));
+  }
+  CHK(gemm(c, L, m1, k, m2, mone, Xn, m1, w, m2, zero, x1, m1));
+  // r1 = -(A x1 + B x2), r2 = w - C x1 - D x2
+  CHK(gemm(c, L, m1, k, m1, mone, Araw, m1, x1, m1, zero, r1, m1));
+  CHK(gemm(c, L, m1, k, m2, mone, Braw, m1, x2, m2, one, r1, m1));
+  DEV(c, hipMemcpyAsync(r2, w, wb2, hipMemcpyDeviceToDevice, L.st));
+  CHK(gemm(c, L, m2, k, m1, mone, Cm, m2, x1, m1, one, r2, m2));
+  CHK(gemm(c, L, m2, k, m2, mone, Draw, m2, x2, m2, one, r2, m2));
+  if (bem_sumsq(r1, m1, m1, k, part, 0, &out->rr, L.st) || bem_sumsq(r2, m2, m2, k, part, 1, &out->rr, L.st) ||
+      bem_sumsq(w, m2, m2, k, part, 0, &out->ww, L.st))
+    return fail(c, BEM_E_DEVICE, "probe norm launch");
+  // d = W^-1 r: u = A^-1 r1, d2 = Sc^-1 (r2 - C u), d1 = A^-1 (r1 - B d2)
+  CHK(to_work(c, L, r1, a, m1, k));
+  CHK(lu_solve(c, L, A.lu, m1, m1, A.ipiv, A.inv, a, m1, k, A.local));
+  CHK(from_work(c, L, a, t1, m1, k));
+  DEV(c, hipMemcpyAsync(t2, r2, wb2, hipMemcpyDeviceToDevice, L.st));
+  CHK(gemm(c, L, m2, k, m1, mone, Cm, m2, t1, m1, one, t2, m2));
+  CHK(to_work(c, L, t2, b, m2, k));
+  CHK(lu_solve(c, L, S.lu, m2, m2, S.ipiv, S.inv, b, m2, k, S.local));
+  CHK(from_work(c, L, b, t2, m2, k));
+  DEV(c, hipMemcpyAsync(t1, r1, wb1, hipMemcpyDeviceToDevice, L.st));
+  CHK(gemm(c, L, m1, k, m2, mone, Braw, m1, t2, m2, one, t1, m1));
+  CHK(to_work(c, L, t1, a, m1, k));
+  CHK(lu_solve(c, L, A.lu, m1, m1, A.ipiv, A.inv, a, m1, k, A.local));
+  int e = absmax(a, m1, m1, k, 0, &out->dmax, L.st);
+  if (!e) e = bem_absmax(x1, m1, m1, k, 0, &out->xmax, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "probe absmax launch: %d", e);
+  // h_j = g_j^H d_1j with g = sym G1 X sym G2 w = -sym G1 Xn sym G2 w
+  CHK(from_work(c, L, a, t1, m1, k));
+  CHK(gemm(c, L, m2, k, m2, one, Gs2, m2, w, m2, zero, r2, m2));
+  CHK(gemm(c, L, m1, k, m2, one, Xn, m1, r2, m2, zero, r1, m1));
+  CHK(gemm(c, L, m1, k, m1, mone, Gs1, m1, r1, m1, zero, x1, m1));
+  if ((e = bem_zdotc_cols(x1, m1, t1, m1, m1, k, out->h, L.st))) return fail(c, BEM_E_DEVICE, "probe dot launch: %d", e);
+  return 0;
+}
+
+// One step of iterative refinement of [X; Z2] = W^-1 [0; I]: the full
+// residual R = [0; I] - W [X; Z2] in FP64 against the unfactored blocks (its
+// squared Frobenius norm to *rfull), the correction [dX; dZ2] = W^-1 R through
+// the factors in the working precision T, with Y = A^-1 B kept from the
+// elimination:  u = A^-1 T1 (T1 = -R1),  dZ2 = Sc^-1 (R2 + C u),
+// -dX = u + Y dZ2;  then the FP64 products and trace of the corrected X and
+// the probe.  Xn holds -X and X2 holds Z2 (FP64); `first` forms Z2 from the
+// factors of Sc before the residual.
+template <typename T>
+static int refine_step(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry &E2, const LuRef<T> &A,
+                       const LuRef<T> &S, const T *Ct, const T *Yt, const double2 *Braw, const double2 *Cm,
+                       double2 *Xn, double2 *X2, bool first, double2 *trace_out, ProbeOut *pout, double *rfull) {
+  const int m1 = A.n, m2 = S.n;
+  const double2 d1 = make_double2(1, 0), dm1 = make_double2(-1, 0), d0 = make_double2(0, 0);
+  const T one = cnum<T>(1, 0);
+  double2 *R1, *R2, *B1, *partial;
+  T *R1t, *R2t;
+  double *part;
+  CHK(ws_get(c, L, "R1_64", sizeof(double2) * (size_t)m1 * m2, (void **)&R1));
+  CHK(ws_get(c, L, "R2_64", sizeof(double2) * (size_t)m2 * m2, (void **)&R2));
+  if (std::is_same<T, double2>::value) {
+    R1t = (T *)R1;
+    R2t = (T *)R2;
+  } else {
+    CHK(ws_get(c, L, "R1_w", sizeof(T) * (size_t)m1 * m2, (void **)&R1t));
+    CHK(ws_get(c, L, "R2_w", sizeof(T) * (size_t)m2 * m2, (void **)&R2t));
+  }
+  CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
+  CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (bem_htrace_partials(m2) + 1), (void **)&partial));
+  CHK(ws_get(c, L, "rf_part", sizeof(double) * std::max(bem_sumsq_partials(m1, m2), bem_sumsq_partials(m2, m2)),
+             (void **)&part));
+  int e;
+  if (first) {   // Z2 = Sc^-1: the identity through the factors
+    DEV(c, hipMemsetAsync(R2, 0, sizeof(double2) * (size_t)m2 * m2, L.st));
+    if ((e = bem_identity_minus(R2, m2, m2, L.st))) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
+    CHK(to_work(c, L, R2, R2t, m2, m2));
+    CHK(lu_solve(c, L, S.lu, m2, m2, S.ipiv, S.inv, R2t, m2, m2, S.local));
+    CHK(from_work(c, L, R2t, X2, m2, m2));
+  }
+  // T1 = B Z2 - A Xn (= -R1);  R2 = I - (D Z2 - C Xn)
+  CHK(gemm(c, L, m1, m2, m2, d1, Braw, m1, X2, m2, d0, R1, m1));
+  CHK(gemm(c, L, m1, m2, m1, dm1, E1.Araw, m1, Xn, m1, d1, R1, m1));
+  CHK(gemm(c, L, m2, m2, m2, d1, E2.Araw, m2, X2, m2, d0, R2, m2));
+  CHK(gemm(c, L, m2, m2, m1, dm1, Cm, m2, Xn, m1, d1, R2, m2));
+  if ((e = bem_identity_minus(R2, m2, m2, L.st))) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
+  if (bem_sumsq(R1, m1, m1, m2, part, 0, rfull, L.st) || bem_sumsq(R2, m2, m2, m2, part, 1, rfull, L.st))
+    return fail(c, BEM_E_DEVICE, "residual norm launch");
+  // correction through the factors
+  CHK(to_work(c, L, R1, R1t, m1, m2));
+  CHK(lu_solve(c, L, A.lu, m1, m1, A.ipiv, A.inv, R1t, m1, m2, A.local));    // u
+  CHK(to_work(c, L, R2, R2t, m2, m2));
+  CHK(gemm(c, L, m2, m2, m1, one, Ct, m2, R1t, m1, one, R2t, m2));           // R2 + C u
+  CHK(lu_solve(c, L, S.lu, m2, m2, S.ipiv, S.inv, R2t, m2, m2, S.local));    // dZ2
+  CHK(gemm(c, L, m1, m2, m2, one, Yt, m1, R2t, m2, one, R1t, m1));           // u + Y dZ2 = -dX
+  CHK(add_work(c, L, R1t, Xn, m1, m2));
+  CHK(add_work(c, L, R2t, X2, m2, m2));
+  // FP64 products and trace of the corrected X
+  CHK(gemm(c, L, m1, m2, m1, d1, E1.Gs, m1, Xn, m1, d0, B1, m1));
+  EventRec *r;
+  CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
+  if ((e = bem_zgemm_htrace(m2, m1, Xn, m1, B1, m1, E2.Gs, m2, partial, trace_out, L.st)))
+    return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
+  CHK(ev_end(c, L, r));
+  return probe<T>(c, L, A, S, E1.Araw, Braw, Cm, E2.Araw, E1.Gs, E2.Gs, Xn, X2, pout);
+}
+
+// Mixed-precision elimination (BEM_PREC_MIXED) after the FP64 assembly of B,
+// C: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1 and the products in complex64 on
+// FP32 MFMA from the cached complex64 LU of A (the trace sums in FP64), then
+// the probe of the complex64 X (widened to FP64 in Xn64) against the FP64
+// blocks.  With refinement requested, Y is kept (Y32) for the correction
+// steps.
 static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry &E2, double2 *Bm, double2 *Cm,
-                       int *info, unsigned long long *gro, double2 *trace_out, bool refine) {
+                       int *info, unsigned long long *gro, double2 *trace_out, ProbeOut *pout, bool refine) {
   const int m1 = E1.n, m2 = E2.n;
   const float2 one = cnum<float2>(1, 0), mone = cnum<float2>(-1, 0), zero = cnum<float2>(0, 0);
   float2 *B32, *C32, *S32, *B1, *invS, *Y32 = nullptr;
+  double2 *Xn;
   int *ipS;
   double2 *partial;
   CHK(ws_get(c, L, "B32", sizeof(float2) * (size_t)m1 * m2, (void **)&B32));
@@ -1176,6 +1403,7 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   CHK(ws_get(c, L, "B1_32", sizeof(float2) * (size_t)m1 * m2, (void **)&B1));
   CHK(ws_get(c, L, "invS32", inv_bytes<float2>(m2), (void **)&invS));
   CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
+  CHK(ws_get(c, L, "Xn64", sizeof(double2) * (size_t)m1 * m2, (void **)&Xn));
   size_t np = std::max(bem_chtrace_partials(m2), bem_htrace_partials(m2));
   CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
   if (bem_z2c(Bm, m1, B32, m1, m1, m2, L.st) || bem_z2c(Cm, m2, C32, m2, m2, m1, L.st))
@@ -1199,100 +1427,69 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
     if (e) return fail(c, BEM_E_DEVICE, "complex64 trace gemm: %d", e);
     CHK(ev_end(c, L, r));
   }
-  if (!refine) return 0;
-
-  // ---- one FP64 refinement step.  Xn = -X (m1 x m2), X2 = Sc^-1 (m2 x m2).
-  const double2 d1 = make_double2(1, 0), dm1 = make_double2(-1, 0), d0 = make_double2(0, 0);
-  float2 *X2, *R1c, *R2c;
-  double2 *Xn, *X2d, *R1, *R2, *B1d;
-  CHK(ws_get(c, L, "X2_32", sizeof(float2) * (size_t)m2 * m2, (void **)&X2));
-  CHK(ws_get(c, L, "R1_32", sizeof(float2) * (size_t)m1 * m2, (void **)&R1c));
-  CHK(ws_get(c, L, "R2_32", sizeof(float2) * (size_t)m2 * m2, (void **)&R2c));
-  CHK(ws_get(c, L, "Xn64", sizeof(double2) * (size_t)m1 * m2, (void **)&Xn));
-  CHK(ws_get(c, L, "X2_64", sizeof(double2) * (size_t)m2 * m2, (void **)&X2d));
-  CHK(ws_get(c, L, "R1_64", sizeof(double2) * (size_t)m1 * m2, (void **)&R1));
-  CHK(ws_get(c, L, "R2_64", sizeof(double2) * (size_t)m2 * m2, (void **)&R2));
-  CHK(ws_get(c, L, "B1_64", sizeof(double2) * (size_t)m1 * m2, (void **)&B1d));
-  // X2 = Sc^-1 = I Sc^-1 (I formed as I - 0)
-  DEV(c, hipMemsetAsync(R2, 0, sizeof(double2) * (size_t)m2 * m2, L.st));
-  int e = bem_identity_minus(R2, m2, m2, L.st);
-  if (!e) e = bem_z2c(R2, m2, X2, m2, m2, m2, L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
-  CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, X2, m2, m2, true));
-  if (bem_c2z(B32, m1, Xn, m1, m1, m2, L.st) || bem_c2z(X2, m2, X2d, m2, m2, m2, L.st))
-    return fail(c, BEM_E_DEVICE, "complex128 copy launch");
-  // T1 = B X2 - A Xn = -(R1); T2 = D X2 - C Xn, R2 = I - T2
-  CHK(gemm(c, L, m1, m2, m2, d1, Bm, m1, X2d, m2, d0, R1, m1));
-  CHK(gemm(c, L, m1, m2, m1, dm1, E1.Z, m1, Xn, m1, d1, R1, m1));
-  CHK(gemm(c, L, m2, m2, m2, d1, E2.Z, m2, X2d, m2, d0, R2, m2));
-  CHK(gemm(c, L, m2, m2, m1, dm1, Cm, m2, Xn, m1, d1, R2, m2));
-  if ((e = bem_identity_minus(R2, m2, m2, L.st))) return fail(c, BEM_E_DEVICE, "identity launch: %d", e);
-  // correction [dX; dX2] = W^-1 [R1; R2] with the complex64 factors, R1 = -T1:
-  //   u = A^-1 T1 (= -A^-1 R1), dX2 = Sc^-1 (R2 + C u), dX = -u - Y dX2
-  if (bem_z2c(R1, m1, R1c, m1, m1, m2, L.st) || bem_z2c(R2, m2, R2c, m2, m2, m2, L.st))
-    return fail(c, BEM_E_DEVICE, "complex64 copy launch");
-  CHK(lu_solve(c, L, E1.Z32, m1, m1, E1.ipiv, E1.inv32, R1c, m1, m2, true));   // R1c <- u
-  CHK(gemm(c, L, m2, m2, m1, one, C32, m2, R1c, m1, one, R2c, m2));      // R2c <- R2 + C u
-  CHK(lu_solve(c, L, S32, m2, m2, ipS, invS, R2c, m2, m2, true));        // R2c <- dX2
-  CHK(gemm(c, L, m1, m2, m2, one, Y32, m1, R2c, m2, one, R1c, m1));      // R1c <- u + Y dX2 = -dX
-  // corrected -X = Xn - dX = Xn + (u + Y dX2), in FP64
-  if ((e = bem_c2z_add(R1c, m1, Xn, m1, m1, m2, L.st))) return fail(c, BEM_E_DEVICE, "update launch: %d", e);
-  // FP64 products and trace of the corrected X into trace_out[1]
-  CHK(gemm(c, L, m1, m2, m1, d1, E1.Gs, m1, Xn, m1, d0, B1d, m1));
-  EventRec *r;
-  CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
-  e = bem_zgemm_htrace(m2, m1, Xn, m1, B1d, m1, E2.Gs, m2, partial, trace_out + 1, L.st);
-  if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
-  return ev_end(c, L, r);
+  if (bem_c2z(B32, m1, Xn, m1, m1, m2, L.st)) return fail(c, BEM_E_DEVICE, "complex128 copy launch");
+  const LuRef<float2> A{E1.Z32, m1, E1.ipiv, E1.inv32, true}, S{S32, m2, ipS, invS, true};
+  return probe<float2>(c, L, A, S, E1.Araw, Bm, Cm, E2.Araw, E1.Gs, E2.Gs, Xn, nullptr, pout);
 }
 
-// Issues every kernel of one spectral-power solve on the lane's stream and
-// returns without waiting; lane_finish() collects the result.
+// Issues every kernel of the solve stage of one spectral-power request on the
+// lane's stream and returns without waiting; lane_step() collects the result
+// and decides what follows.
 //   A = Z_ext(1,1) + Z_int(1) (cached, factored), D = Z_ext(2,2) + Z_int(2)
 //   (cached), B = Z_ext(1,2), C = Z_ext(2,1) (assembled for this gap);
-//   Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1, S = 1/4 tr((X sym G2)^H sym G1 X).
+//   Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1, S = 1/4 tr((X sym G2)^H sym G1 X),
+//   then the error probe of X.
 // BEM_PREC_MIXED runs the elimination in complex64 (solve_mixed); a mixed
 // request with tournament pivoting runs the FP64 path.
 static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *b2, const bem_request *q,
                       bem_result *res) {
-  L.t0 = std::chrono::steady_clock::now();
-  L.req = *q;
+  const bem_request qq = *q;
+  if (L.restarts == 0) L.t0 = std::chrono::steady_clock::now();
+  L.req = qq;
   L.res = res;
+  L.sb1 = b1;
+  L.sb2 = b2;
   L.pg = &g;
   g.pins++;
-  double omegaSI = q->eV * SI_EV / SI_HBAR;
-  double omega = omegaSI * q->a / SI_C;
-  const int pivot = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
-  const int prec = pivot ? BEM_PREC_FP64 : q->precision;
+  L.stage = ST_SOLVE;
+  L.iters = 0;
+  double omegaSI = qq.eV * SI_EV / SI_HBAR;
+  double omega = omegaSI * qq.a / SI_C;
+  const int pivot = (qq.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  const int prec = pivot ? BEM_PREC_FP64 : qq.precision;
   L.prec = prec;
-  L.refine = prec == BEM_PREC_MIXED && (q->flags & BEM_SOLVE_REFINE);
+  L.refine = (qq.flags & BEM_SOLVE_REFINE) != 0;
   Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
   DEV(c, hipEventRecord(L.ph[0], L.st));
   // body 2 (raw) first: when both bodies share one mesh and medium, body 1's
   // factored entry is then a copy of it instead of a second assembly
-  CHK(self_get(c, L, b2, q->eps2, q->mu2, omega, 1, 0, prec, nullptr, &L.e2));
-  const bool twins = same_mesh(b1, b2) && std::memcmp(q->eps1, q->eps2, sizeof q->eps1) == 0 &&
-                     std::memcmp(q->mu1, q->mu2, sizeof q->mu1) == 0;
-  CHK(self_get(c, L, b1, q->eps1, q->mu1, omega, 0, pivot, prec, twins ? L.e2 : nullptr, &L.e1));
+  CHK(self_get(c, L, b2, qq.eps2, qq.mu2, omega, 1, 0, prec, nullptr, &L.e2));
+  const bool twins = same_mesh(b1, b2) && std::memcmp(qq.eps1, qq.eps2, sizeof qq.eps1) == 0 &&
+                     std::memcmp(qq.mu1, qq.mu2, sizeof qq.mu1) == 0;
+  CHK(self_get(c, L, b1, qq.eps1, qq.mu1, omega, 0, pivot, prec, twins ? L.e2 : nullptr, &L.e1));
   const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
   const int m1 = E1.n, m2 = E2.n;
   if (m1 != 2 * g.E1 || m2 != 2 * (g.top.E - g.E1)) return fail(c, BEM_E_INVALID, "body sizes do not match the pair tree");
 
-  double2 *Bm, *Cm, *trace_out;
+  double2 *Bm, *Cm, *Braw, *trace_out;
   int *info;
   unsigned long long *gro;
+  ProbeOut *pout;
   CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
   CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
+  if (prec == BEM_PREC_FP64) CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
+  else Braw = Bm;   // the mixed elimination works on complex64 copies: B stays unfactored
   CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
   CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 3, (void **)&gro));
   CHK(ws_get(c, L, "trace_out", sizeof(double2) * 2, (void **)&trace_out));
+  CHK(ws_get(c, L, "probe_out", sizeof(ProbeOut), (void **)&pout));
   DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
   DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 3, L.st));
-  CHK(assemble_cross(c, L, g, mext, Bm, Cm, info + 1));
+  CHK(assemble_cross(c, L, g, mext, Bm, Cm, prec == BEM_PREC_FP64 ? Braw : nullptr, info + 1));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
   if (prec == BEM_PREC_MIXED) {
-    CHK(solve_mixed(c, L, E1, E2, Bm, Cm, info, gro, trace_out, L.refine));
+    CHK(solve_mixed(c, L, E1, E2, Bm, Cm, info, gro, trace_out, pout, L.refine));
     DEV(c, hipEventRecord(L.ph[2], L.st));
   } else {
     double2 *Sc, *B1, *partial, *invS;
@@ -1310,6 +1507,11 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
     if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
     else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
+    if (L.refine) {   // Y for the refinement's corrections
+      double2 *Y;
+      CHK(ws_get(c, L, "Y64", sizeof(double2) * (size_t)m1 * m2, (void **)&Y));
+      DEV(c, hipMemcpyAsync(Y, Bm, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
+    }
     // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
     CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1, !pivot));
     double2 *X = Bm;
@@ -1323,11 +1525,60 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     int e = bem_zgemm_htrace(m2, m1, X, m1, B1, m1, E2.Gs, m2, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
     CHK(ev_end(c, L, r));
+    const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, S{Sc, m2, ipS, invS, !pivot};
+    CHK(probe<double2>(c, L, A, S, E1.Araw, Braw, Cm, E2.Araw, E1.Gs, E2.Gs, Bm, nullptr, pout));
   }
   DEV(c, hipEventRecord(L.ph[3], L.st));
   DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_probe, pout, sizeof(ProbeOut), hipMemcpyDeviceToHost, L.st));
+  L.busy = true;
+  return 0;
+}
+
+// Issues one refinement step (refine_step) of the lane's request, whose solve
+// stage left its factors and X in the lane's workspaces.
+static int lane_refine(bem_ctx *c, Lane &L) {
+  const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
+  const int m1 = E1.n, m2 = E2.n;
+  const bool first = L.iters == 0;
+  L.iters++;
+  L.stage = ST_REFINE;
+  double2 *Bm, *Cm, *X2, *trace_out;
+  double *rfull;
+  ProbeOut *pout;
+  int *ipS;
+  CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
+  CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
+  CHK(ws_get(c, L, "X2_64", sizeof(double2) * (size_t)m2 * m2, (void **)&X2));
+  CHK(ws_get(c, L, "trace_out", sizeof(double2) * 2, (void **)&trace_out));
+  CHK(ws_get(c, L, "probe_out", sizeof(ProbeOut), (void **)&pout));
+  CHK(ws_get(c, L, "rfull", sizeof(double), (void **)&rfull));
+  CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
+  if (L.prec == BEM_PREC_MIXED) {
+    float2 *S32, *invS, *C32, *Y32;
+    double2 *Xn;
+    CHK(ws_get(c, L, "S32", sizeof(float2) * (size_t)m2 * m2, (void **)&S32));
+    CHK(ws_get(c, L, "invS32", inv_bytes<float2>(m2), (void **)&invS));
+    CHK(ws_get(c, L, "C32", sizeof(float2) * (size_t)m2 * m1, (void **)&C32));
+    CHK(ws_get(c, L, "Y32", sizeof(float2) * (size_t)m1 * m2, (void **)&Y32));
+    CHK(ws_get(c, L, "Xn64", sizeof(double2) * (size_t)m1 * m2, (void **)&Xn));
+    const LuRef<float2> A{E1.Z32, m1, E1.ipiv, E1.inv32, true}, S{S32, m2, ipS, invS, true};
+    CHK(refine_step<float2>(c, L, E1, E2, A, S, C32, Y32, Bm, Cm, Xn, X2, first, trace_out, pout, rfull));
+  } else {
+    const int pivot = (L.req.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+    double2 *Sc, *invS, *Y, *Braw;
+    CHK(ws_get(c, L, "Sc", sizeof(double2) * (size_t)m2 * m2, (void **)&Sc));
+    CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
+    CHK(ws_get(c, L, "Y64", sizeof(double2) * (size_t)m1 * m2, (void **)&Y));
+    CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
+    const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, S{Sc, m2, ipS, invS, !pivot};
+    CHK(refine_step<double2>(c, L, E1, E2, A, S, Cm, Y, Braw, Cm, Bm, X2, first, trace_out, pout, rfull));
+  }
+  DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_probe, pout, sizeof(ProbeOut), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_rfull, rfull, sizeof(double), hipMemcpyDeviceToHost, L.st));
   L.busy = true;
   return 0;
 }
@@ -1346,28 +1597,28 @@ static double growth_ratio(const unsigned long long *g) {
   return std::isfinite(r) ? r : INFINITY;
 }
 
-static int lane_finish(bem_ctx *c, Lane &L) {
-  if (!L.busy) {
-    self_release(c, L);
-    return 0;
-  }
+static double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+// Waits for the lane's stage to finish and reads it into *L.res: the solve
+// stage fills every field, a refinement step updates the trace, the probe and
+// the residual.  Returns 0 or the stage's error (also in res->status).
+static int lane_collect(bem_ctx *c, Lane &L) {
   L.busy = false;
   hipError_t se = hipStreamSynchronize(L.st);
   const SelfEntry *E1 = L.e1, *E2 = L.e2;
   bem_result *res = L.res;
   const bem_request *q = &L.req;
   if (se != hipSuccess) {
-    self_release(c, L);
     res->status = BEM_E_DEVICE;
     return fail(c, BEM_E_DEVICE, "solve failed on the device: %s", hipGetErrorString(se));
   }
-  float ta = 0, tf = 0, tp = 0;
-  (void)hipEventElapsedTime(&ta, L.ph[0], L.ph[1]);
-  (void)hipEventElapsedTime(&tf, L.ph[1], L.ph[2]);
-  (void)hipEventElapsedTime(&tp, L.ph[2], L.ph[3]);
   ev_collect(c, L);
-  double omega = q->eV * SI_EV / SI_HBAR * q->a / SI_C;
-  double2 tr = L.refine ? L.h_trace[1] : L.h_trace[0];
+  const double omega = q->eV * SI_EV / SI_HBAR * q->a / SI_C;
+  const double2 tr = L.h_trace[0];
   res->omega = omega;
   res->ReS = 0.25 * tr.x;
   res->ImS = 0.25 * tr.y;
@@ -1375,24 +1626,48 @@ static int lane_finish(bem_ctx *c, Lane &L) {
   res->theta2 = theta(omega, q->T2);
   res->P = res->ReS * (res->theta1 - res->theta2);
   res->precision = L.prec;
-  res->rel_err_est = 0;
-  res->iters = 0;
-  if (L.prec == BEM_PREC_MIXED) {
-    // refined: the relative change of Re S from the complex64 result;
-    // complex64 alone: not estimated (-1)
-    res->rel_err_est = L.refine ? std::fabs(L.h_trace[1].x - L.h_trace[0].x) / std::fabs(L.h_trace[1].x) : -1.0;
-    res->iters = L.refine ? 1 : 0;
+  res->iters = L.iters;
+  res->restarts = L.restarts;
+  res->pivoting = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  const ProbeOut &p = *L.h_probe;
+  const double xm = bits_to_double(p.xmax), dm = bits_to_double(p.dmax);
+  res->probe_err = xm > 0 ? dm / xm : INFINITY;
+  if (!std::isfinite(res->probe_err)) res->probe_err = INFINITY;
+  res->residual = p.ww > 0 ? std::sqrt(p.rr / p.ww) : INFINITY;
+  // first-order change of Re S: 1/2 Re t, t ~ (1 / (k s^2)) sum_j h_j with
+  // s^2 = ||w||^2 / (m2 k) the sample's mean |w_i|^2; the stated error sums
+  // |h_j| (an estimate that errs on the high side of |t|)
+  {
+    const int m2 = E2 ? E2->n : 1;
+    double habs = 0.0, hre = 0.0;
+    for (int j = 0; j < BEM_PROBE_COLS; j++) {
+      habs += std::hypot(p.h[j].x, p.h[j].y);
+      hre += p.h[j].x;
+    }
+    const double scale = p.ww > 0 ? (double)m2 / p.ww : INFINITY;   // 1 / (k s^2)
+    const double aS = std::fabs(res->ReS);
+    res->rel_err_est = aS > 0 ? 0.5 * habs * scale / aS : INFINITY;
+    res->err_first_order = aS > 0 ? 0.5 * hre * scale / aS : INFINITY;
+    if (!std::isfinite(res->rel_err_est)) res->rel_err_est = INFINITY;
   }
+  res->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - L.t0).count();
+  if (L.stage == ST_REFINE) {
+    res->full_residual = std::sqrt(*L.h_rfull / (double)(E2 ? E2->n : 1));
+    res->status = 0;
+    return 0;
+  }
+  float ta = 0, tf = 0, tp = 0;
+  (void)hipEventElapsedTime(&ta, L.ph[0], L.ph[1]);
+  (void)hipEventElapsedTime(&tf, L.ph[1], L.ph[2]);
+  (void)hipEventElapsedTime(&tp, L.ph[2], L.ph[3]);
   res->ms_assemble = ta;
   res->ms_factor = tf;
   res->ms_products = tp;
-  res->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - L.t0).count();
+  res->full_residual = -1.0;
   res->status = 0;
-  res->pivoting = (q->flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
   res->growth = 0.0;
   int sing_a = E1 ? E1->h->info[0] : 0, ovf = L.h_info[1] || (E1 && E1->h->info[1]) || (E2 && E2->h->info[1]);
   if (!res->pivoting && E1) res->growth = std::max(growth_ratio(E1->h->growth), growth_ratio(L.h_growth));
-  self_release(c, L);
   c->t_last[0] = ta;
   c->t_last[4] = tf;
   c->t_last[5] = tp;
@@ -1404,7 +1679,97 @@ static int lane_finish(bem_ctx *c, Lane &L) {
     res->status = BEM_E_SINGULAR;
     return fail(c, BEM_E_SINGULAR, "exactly singular pivot (A: %d, Schur: %d)", sing_a, L.h_info[0]);
   }
+  if (!std::isfinite(res->ReS) || !std::isfinite(res->ImS)) {
+    res->status = BEM_E_SINGULAR;
+    return fail(c, BEM_E_SINGULAR, "non-finite spectral power: the system is singular or not finite "
+                                   "(coincident bodies?)");
+  }
   return 0;
+}
+
+// What follows a finished stage (lane_step).
+enum { STEP_DONE = 0, STEP_MORE = 1, STEP_RESTART = 2 };
+
+// Collects the lane's finished stage and decides what follows:
+//  - Recovery: a block-local solve that met an exactly zero pivot, whose
+//    growth exceeds BEM_GROWTH_LIMIT, or (FP64) whose probe estimates an
+//    error of Re S above BEM_PROBE_LIMIT is solved again with tournament
+//    pivoting and refinement (STEP_RESTART: the lane is released and L.req
+//    holds the request with its new flags);
+//  - with refinement requested, a result whose relative residual is above
+//    BEM_REFINE_TOL gets another refinement step on the lane (STEP_MORE),
+//    at most BEM_REFINE_MAX; a mixed request that reaches the cap is solved
+//    again on the FP64 path (STEP_RESTART);
+//  - otherwise the result is final and the lane is released (STEP_DONE, *err
+//    the request's status).
+static int lane_step(bem_ctx *c, Lane &L, int *err) {
+  int r = lane_collect(c, L);
+  bem_result *res = L.res;
+  *err = r;
+  const bool tourn = (L.req.flags & BEM_SOLVE_TOURNAMENT) != 0;
+  bool restart = false;
+  if (L.stage == ST_SOLVE && !tourn && L.restarts < 2 &&
+      (r == BEM_E_SINGULAR ||
+       (!r && (!(res->growth <= BEM_GROWTH_LIMIT) ||
+               (L.prec == BEM_PREC_FP64 && !(res->rel_err_est <= BEM_PROBE_LIMIT)))))) {
+    L.req.flags |= BEM_SOLVE_TOURNAMENT | BEM_SOLVE_REFINE;
+    L.req.precision = BEM_PREC_FP64;
+    restart = true;
+  } else if (!r && L.refine && !(res->residual <= BEM_REFINE_TOL)) {
+    if (L.iters < BEM_REFINE_MAX) {
+      int e = lane_refine(c, L);
+      if (!e) return STEP_MORE;
+      (void)hipStreamSynchronize(L.st);
+      L.busy = false;
+      res->status = e;
+      *err = e;
+    } else if (L.prec == BEM_PREC_MIXED && L.restarts < 2) {
+      L.req.precision = BEM_PREC_FP64;
+      restart = true;
+    }
+  }
+  self_release(c, L);
+  L.stage = ST_IDLE;
+  if (restart) {
+    L.restarts++;
+    *err = 0;
+    return STEP_RESTART;
+  }
+  return STEP_DONE;
+}
+
+// Abandons a partially issued stage: waits for what was issued and releases
+// the lane.
+static void lane_abort(bem_ctx *c, Lane &L) {
+  (void)hipStreamSynchronize(L.st);
+  L.busy = false;
+  L.stage = ST_IDLE;
+  self_release(c, L);
+}
+
+// Drives a synchronous caller's lane (no ticket) until its request is final:
+// collects each stage, issues refinement steps, and solves restarted requests
+// again on the same lane.  Returns the request's status.
+static int lane_drive(bem_ctx *c, Lane &L) {
+  for (;;) {
+    if (!L.busy) {
+      self_release(c, L);
+      return 0;
+    }
+    int err = 0;
+    int st = lane_step(c, L, &err);
+    if (st == STEP_MORE) continue;
+    if (st == STEP_DONE) return err;
+    const bem_request q = L.req;
+    PairGeom *g = nullptr;
+    int e = pair_geom(c, L.sb1, L.sb2, q.gap, &g);
+    if (!e) e = lane_issue(c, L, *g, L.sb1, L.sb2, &q, L.res);
+    if (e) {
+      lane_abort(c, L);
+      L.res->status = e;
+      return e;
+    }
+  }
 }
 
 static int check_request(bem_ctx *c, const bem_request *q) {
@@ -1439,7 +1804,15 @@ static size_t footprint(bem_ctx *c, const Lane &L, const Pending &p) {
   const bool h2 = self_cached(c, p.b2, q.eps2, q.mu2, omega, 1, 0, prec);
   size_t selfb = (h1 ? 0 : self_entry_bytes((int)m1, 0, prec)) + (h2 ? 0 : self_entry_bytes((int)m2, 1, prec));
   size_t pairs = std::max(T1 * T2, std::max(h1 ? 0 : T1 * T1, h2 ? 0 : T2 * T2));
-  size_t ws = sizeof(double2) * (3 * m1 * m2 + m2 * m2) * (prec == BEM_PREC_MIXED ? 3 : 1) + 8 * (size_t)BEM_TILE_PLANES * pairs +
+  // lane workspaces in complex128 units: B, C, the unfactored B, B1 and Sc
+  // (FP64); B, C, -X in FP64 plus complex64 B, C, B1 and Sc (mixed); Y,
+  // Sc^-1 and the residual blocks with refinement (complex64 copies too when
+  // mixed); the probe's columns are negligible
+  const bool refine = (q.flags & BEM_SOLVE_REFINE) != 0;
+  const double mm = (double)m1 * m2, m22 = (double)m2 * m2;
+  const double w16 = prec == BEM_PREC_MIXED ? 4.5 * mm + 0.5 * m22 + (refine ? 3.0 * mm + 2.5 * m22 : 0.0)
+                                            : 4.0 * mm + m22 + (refine ? 2.0 * mm + 2.0 * m22 : 0.0);
+  size_t ws = (size_t)(sizeof(double2) * w16) + 8 * (size_t)BEM_TILE_PLANES * pairs +
               sizeof(int2) * std::min<size_t>(pairs, (size_t)1 << 24) + inv_bytes((int)m2) +
               sizeof(double2) * 64 * std::max(m1, m2) + sizeof(double2) * (bem_htrace_partials((int)m2) + 1) +
               sizeof(double) * bem_close_mom_doubles((unsigned int)CLOSE_CHUNK);
@@ -1471,35 +1844,47 @@ static double ms_between(std::chrono::steady_clock::time_point a, std::chrono::s
   return std::chrono::duration<double, std::milli>(b - a).count();
 }
 
-// Finishes the lane's solve.  A served request's result goes to the table of
-// finished tickets (or, when block-local pivoting grew too much, back to the
-// head of the queue with tournament pivoting); its errors are reported
-// through the ticket, not as this call's return value.
-static int lane_retire(bem_ctx *c, Lane &L) {
-  const uint64_t t = L.ticket;
-  int r = lane_finish(c, L);
-  if (!t) return r;
-  L.ticket = 0;
-  bem_result res = L.sres;
-  if (r && !res.status) res.status = r;
-  auto now = std::chrono::steady_clock::now();
-  const bool retry = !res.status && !res.pivoting && !(res.growth <= BEM_GROWTH_LIMIT);
-  std::lock_guard<std::mutex> lk(c->qmu);
-  c->running--;
-  if (retry) {
-    Pending p{t, L.sb1, L.sb2, L.req, L.t_submit};
-    p.req.flags |= BEM_SOLVE_TOURNAMENT;
-    c->queue.push_front(p);
-    c->kick = true;
-    c->qcv.notify_one();
+static void lane_done_cb(void *arg);
+
+// Finishes what the lane has in flight.  A synchronous caller's lane is
+// driven to its final result (lane_drive).  A served request's stage is
+// collected: a refinement step that follows is issued on the lane (with
+// `block`, waited for in turn; without, its completion callback is enqueued
+// and the call returns), a restarted request goes back to the head of the
+// queue, and a final result goes to the table of finished tickets; its errors
+// are reported through the ticket, not as this call's return value.
+static int lane_retire(bem_ctx *c, Lane &L, bool block = true) {
+  if (!L.ticket) return lane_drive(c, L);
+  for (;;) {
+    const uint64_t t = L.ticket;
+    int err = 0;
+    const int st = L.busy ? lane_step(c, L, &err) : STEP_DONE;
+    if (st == STEP_MORE) {
+      if (block) continue;
+      __atomic_store_n(&L.done_flag, 0, __ATOMIC_RELEASE);
+      if (hipLaunchHostFunc(L.st, lane_done_cb, &L) == hipSuccess) return 0;
+      continue;   // no callback: wait for the step here
+    }
+    L.ticket = 0;
+    bem_result res = L.sres;
+    if (err && !res.status) res.status = err;
+    auto now = std::chrono::steady_clock::now();
+    std::lock_guard<std::mutex> lk(c->qmu);
+    c->running--;
+    if (st == STEP_RESTART) {
+      Pending p{t, L.sb1, L.sb2, L.req, L.t_submit, L.restarts};
+      c->queue.push_front(p);
+      c->kick = true;
+      c->qcv.notify_one();
+      return 0;
+    }
+    res.ms_queue = ms_between(L.t_submit, L.t_issue);
+    res.ms_total = ms_between(L.t_submit, now);
+    c->done[t] = res;
+    c->live.erase(t);
+    c->done_cv.notify_all();
     return 0;
   }
-  res.ms_queue = ms_between(L.t_submit, L.t_issue);
-  res.ms_total = ms_between(L.t_submit, now);
-  c->done[t] = res;
-  c->live.erase(t);
-  c->done_cv.notify_all();
-  return 0;
 }
 
 // Host callback enqueued behind a served solve: marks the lane done and wakes
@@ -1607,14 +1992,13 @@ static void admit(bem_ctx *c) {
       L->t_issue = std::chrono::steady_clock::now();
       std::memset(&L->sres, 0, sizeof L->sres);
       __atomic_store_n(&L->done_flag, 0, __ATOMIC_RELEASE);
+      L->restarts = p.restarts;
       e = lane_issue(c, *L, *g, p.b1, p.b2, &p.req, &L->sres);
       if (!e && hipLaunchHostFunc(L->st, lane_done_cb, L) != hipSuccess)
         e = fail(c, BEM_E_DEVICE, "host callback could not be enqueued");
       if (e) {
-        (void)hipStreamSynchronize(L->st);
-        L->busy = false;
+        lane_abort(c, *L);
         L->ticket = 0;
-        self_release(c, *L);
       }
     }
     if (e) {
@@ -1638,7 +2022,7 @@ static void sched_main(bem_ctx *c) {
     }
     std::lock_guard<std::recursive_mutex> g(c->mu);
     for (auto &L : c->lanes)
-      if (L.ticket && __atomic_load_n(&L.done_flag, __ATOMIC_ACQUIRE)) (void)lane_retire(c, L);
+      if (L.ticket && __atomic_load_n(&L.done_flag, __ATOMIC_ACQUIRE)) (void)lane_retire(c, L, false);
     admit(c);
   }
 }
@@ -1761,6 +2145,8 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     ok = ok && hipHostMalloc((void **)&L.h_trace, sizeof(double2) * 2, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_info, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_growth, sizeof(unsigned long long) * 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_probe, sizeof(ProbeOut), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_rfull, sizeof(double), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
       bem_ctx_destroy(c);
       return BEM_E_DEVICE;
@@ -1802,6 +2188,8 @@ void bem_ctx_destroy(bem_ctx *c) {
     if (L.h_trace) (void)hipHostFree(L.h_trace);
     if (L.h_info) (void)hipHostFree(L.h_info);
     if (L.h_growth) (void)hipHostFree(L.h_growth);
+    if (L.h_probe) (void)hipHostFree(L.h_probe);
+    if (L.h_rfull) (void)hipHostFree(L.h_rfull);
     if (L.st) (void)hipStreamDestroy(L.st);
     if (L.aux) (void)hipStreamDestroy(L.aux);
     if (L.la_a) (void)hipEventDestroy(L.la_a);
@@ -1991,16 +2379,14 @@ int bem_spectral_power(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request
   CHK(pair_geom(c, b1, b2, q->gap, &g));
   Lane &L = c->lanes[0];
   CHK(lane_retire(c, L));
+  L.restarts = 0;
   int e = lane_issue(c, L, *g, b1, b2, q, res);
-  int f = lane_retire(c, L);
-  if (e) return e;
-  if (!f && !(res->growth <= BEM_GROWTH_LIMIT) && !(q->flags & BEM_SOLVE_TOURNAMENT)) {
-    bem_request q2 = *q;
-    q2.flags |= BEM_SOLVE_TOURNAMENT;
-    e = lane_issue(c, L, *g, b1, b2, &q2, res);
-    f = lane_retire(c, L);
-    if (e) return e;
+  if (e) {
+    lane_abort(c, L);
+    res->status = e;
+    return e;
   }
+  int f = lane_drive(c, L);
   c->t_last[6] = c->gemm_ms;
   c->t_last[7] = c->gemm_n;
   c->t_last[8] = c->gemm_flops;
@@ -2014,6 +2400,9 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
   stats_reset(c);
   int first_err = 0;
   const int nl = (int)c->lanes.size();
+  // request i runs on lane i % lanes; a lane's previous request is driven to
+  // its final result (refinement steps, re-solves) before the lane takes the
+  // next one, while the other lanes' kernels keep running
   for (int i = 0; i < n; i++) {
     std::memset(&res[i], 0, sizeof res[i]);
     Lane &L = c->lanes[i % nl];
@@ -2022,34 +2411,17 @@ int bem_sweep(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *reqs, i
     int e = check_request(c, &reqs[i]);
     PairGeom *g = nullptr;
     if (!e) e = pair_geom(c, b1, b2, reqs[i].gap, &g);
+    L.restarts = 0;
     if (!e) e = lane_issue(c, L, *g, b1, b2, &reqs[i], &res[i]);
     if (e) {
       res[i].status = e;
       if (!first_err) first_err = e;
-      (void)hipStreamSynchronize(L.st);
-      L.busy = false;
-      self_release(c, L);
+      lane_abort(c, L);
     }
   }
   for (auto &L : c->lanes) {
     int f = lane_retire(c, L);
     if (f && !first_err) first_err = f;
-  }
-  // requests whose block-local pivoting grew too much are solved again with
-  // tournament pivoting
-  for (int i = 0; i < n; i++) {
-    if (res[i].status || res[i].pivoting || res[i].growth <= BEM_GROWTH_LIMIT) continue;
-    bem_request q2 = reqs[i];
-    q2.flags |= BEM_SOLVE_TOURNAMENT;
-    Lane &L = c->lanes[0];
-    PairGeom *g = nullptr;
-    int e = pair_geom(c, b1, b2, q2.gap, &g);
-    if (!e) e = lane_issue(c, L, *g, b1, b2, &q2, &res[i]);
-    int f = e ? e : lane_retire(c, L);
-    if (f) {
-      res[i].status = f;
-      if (!first_err) first_err = f;
-    }
   }
   c->t_last[6] = c->gemm_ms;
   c->t_last[7] = c->gemm_n;
@@ -2061,7 +2433,7 @@ int bem_submit(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *q, uin
   if (!c || !b1 || !b2 || !q || !ticket) return BEM_E_INVALID;
   std::lock_guard<std::mutex> lk(c->qmu);
   if (c->stop) return BEM_E_INVALID;
-  Pending p{c->next_ticket++, b1, b2, *q, std::chrono::steady_clock::now()};
+  Pending p{c->next_ticket++, b1, b2, *q, std::chrono::steady_clock::now(), 0};
   c->queue.push_back(p);
   c->live.insert(p.ticket);
   *ticket = p.ticket;

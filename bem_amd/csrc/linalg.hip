@@ -1382,6 +1382,81 @@ __global__ void add_z_to_c_kernel(const double2 *__restrict__ src, int64_t lds, 
   }
 }
 
+// Sum of |re|^2 + |im|^2 over an m x n complex128 block: each block of 256
+// threads writes one partial (grid (gx, gy), partial[by * gx + bx]);
+// sumsq_finish adds them in index order, so the norm does not depend on
+// scheduling (no floating-point atomics).
+__global__ void __launch_bounds__(256) sumsq_kernel(const double2 *__restrict__ A, int64_t lda, int m, int n,
+                                                    double *__restrict__ partial) {
+  __shared__ double ws[4];
+  double v = 0.0;
+  for (int c = blockIdx.y; c < n; c += gridDim.y)
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < m; r += gridDim.x * blockDim.x) {
+      const double2 x = A[(int64_t)c * lda + r];
+      v = fma(x.x, x.x, fma(x.y, x.y, v));
+    }
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.y * gridDim.x + blockIdx.x] = (ws[0] + ws[1]) + (ws[2] + ws[3]);
+}
+
+// out[0] <- (add ? out[0] : 0) + sum of partial[0..np) in a fixed order.
+__global__ void __launch_bounds__(256) sumsq_finish_kernel(const double *__restrict__ partial, int np, int add,
+                                                           double *__restrict__ out) {
+  __shared__ double s[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) v += partial[i];
+  s[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) s[threadIdx.x] += s[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = (add ? out[0] : 0.0) + s[0];
+}
+
+// out[j] <- sum_i conj(G(i, j)) D(i, j) for the n columns of two m-row
+// complex128 blocks: one 256-thread block per column, summed in a fixed
+// order.
+__global__ void __launch_bounds__(256) zdotc_cols_kernel(const double2 *__restrict__ G, int64_t ldg,
+                                                         const double2 *__restrict__ D, int64_t ldd, int m,
+                                                         double2 *__restrict__ out) {
+  __shared__ double sr[256], si[256];
+  const int j = blockIdx.x;
+  double re = 0.0, im = 0.0;
+  for (int i = threadIdx.x; i < m; i += 256) {
+    const double2 g = G[(int64_t)j * ldg + i], d = D[(int64_t)j * ldd + i];
+    re = fma(g.x, d.x, fma(g.y, d.y, re));
+    im = fma(g.x, d.y, fma(-g.y, d.x, im));
+  }
+  sr[threadIdx.x] = re;
+  si[threadIdx.x] = im;
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = make_double2(sr[0], si[0]);
+}
+
+// dst (complex128) <- dst + alpha src over an m x n block.
+__global__ void zaxpy_kernel(double2 alpha, const double2 *__restrict__ src, int64_t lds, double2 *__restrict__ dst,
+                             int64_t ldd, int m, int n) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  for (int c = blockIdx.y; c < n; c += gridDim.y) {
+    const double2 v = src[(int64_t)c * lds + r];
+    double2 o = dst[(int64_t)c * ldd + r];
+    o.x = fma(alpha.x, v.x, fma(-alpha.y, v.y, o.x));
+    o.y = fma(alpha.x, v.y, fma(alpha.y, v.x, o.y));
+    dst[(int64_t)c * ldd + r] = o;
+  }
+}
+
 // C (m x n, complex128) <- I - C on the square part (the residual of an
 // identity right-hand side).
 __global__ void identity_minus_kernel(double2 *__restrict__ Cm, int64_t ldc, int n) {
@@ -1709,6 +1784,44 @@ int bem_signed_transpose(const double2 *B, int64_t ldb, int m, int n, double2 *C
   if ((n + 31) / 32 > 65535) return -1;
   hipLaunchKernelGGL(signed_transpose_kernel, dim3((m + 31) / 32, (n + 31) / 32), dim3(256), 0, st, B, ldb, m, n, C,
                      ldc);
+  return (int)hipGetLastError();
+}
+
+// Partials bem_sumsq needs for an m x n block (its fixed launch grid).
+static dim3 sumsq_grid(int m, int n) {
+  const int gx = (m + 255) / 256 < 8 ? (m + 255) / 256 : 8;
+  return dim3(gx > 0 ? gx : 1, n < 256 ? (n > 0 ? n : 1) : 256);
+}
+size_t bem_sumsq_partials(int m, int n) {
+  dim3 g = sumsq_grid(m, n);
+  return (size_t)g.x * g.y;
+}
+
+// *out <- (add ? *out : 0) + ||A||_F^2 of an m x n complex128 block
+// (partial: bem_sumsq_partials(m, n) doubles); deterministic.
+int bem_sumsq(const double2 *A, int64_t lda, int m, int n, double *partial, int add, double *out, hipStream_t st) {
+  dim3 g = sumsq_grid(m, n);
+  if (m > 0 && n > 0) {
+    hipLaunchKernelGGL(sumsq_kernel, g, dim3(256), 0, st, A, lda, m, n, partial);
+    hipLaunchKernelGGL(sumsq_finish_kernel, dim3(1), dim3(256), 0, st, (const double *)partial, (int)(g.x * g.y), add,
+                       out);
+  } else {
+    hipLaunchKernelGGL(sumsq_finish_kernel, dim3(1), dim3(256), 0, st, (const double *)partial, 0, add, out);
+  }
+  return (int)hipGetLastError();
+}
+
+int bem_zdotc_cols(const double2 *G, int64_t ldg, const double2 *D, int64_t ldd, int m, int n, double2 *out,
+                   hipStream_t st) {
+  if (n <= 0) return 0;
+  if (n > 65535) return -1;
+  hipLaunchKernelGGL(zdotc_cols_kernel, dim3(n), dim3(256), 0, st, G, ldg, D, ldd, m, out);
+  return (int)hipGetLastError();
+}
+
+int bem_zaxpy(double2 alpha, const double2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  hipLaunchKernelGGL(zaxpy_kernel, copy_grid(m, n), dim3(256), 0, st, alpha, src, lds, dst, ldd, m, n);
   return (int)hipGetLastError();
 }
 

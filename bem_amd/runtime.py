@@ -28,6 +28,11 @@ BEM_PREC_MIXED = 1
 BEM_ASM_DEVICE_PTR = 1
 BEM_SOLVE_TOURNAMENT = 1
 BEM_SOLVE_REFINE = 2
+BEM_GROWTH_LIMIT = 16.0
+BEM_PROBE_COLS = 8
+BEM_PROBE_LIMIT = 1e-9
+BEM_REFINE_TOL = 1e-10
+BEM_REFINE_MAX = 3
 
 # src/util/SI.cpp:3-9
 SI_HBAR = 1.05457148e-34
@@ -60,7 +65,9 @@ class _Res(ctypes.Structure):
                 ("rel_err_est", ctypes.c_double), ("iters", ctypes.c_int), ("status", ctypes.c_int),
                 ("ms_assemble", ctypes.c_double), ("ms_factor", ctypes.c_double),
                 ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double), ("growth", ctypes.c_double),
-                ("pivoting", ctypes.c_int), ("ms_queue", ctypes.c_double), ("precision", ctypes.c_int)]
+                ("pivoting", ctypes.c_int), ("ms_queue", ctypes.c_double), ("precision", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("probe_err", ctypes.c_double), ("residual", ctypes.c_double),
+                ("full_residual", ctypes.c_double), ("err_first_order", ctypes.c_double)]
 
 
 def lib_path():

@@ -82,33 +82,64 @@ typedef struct {
  * src/util/HMatrix.h:277-316), with the growth factor
  * max(1, max|L|) max|U| / max|A| measured and reported (|L| <= 1 under
  * partial pivoting over whole columns; block-local pivots can leave large
- * multipliers below the block); a solve whose growth exceeds
- * BEM_GROWTH_LIMIT is repeated with tournament (communication-avoiding)
- * partial pivoting over whole columns.  BEM_SOLVE_TOURNAMENT requests the
- * latter from the start. */
+ * multipliers below the block).  BEM_SOLVE_TOURNAMENT requests tournament
+ * (communication-avoiding) partial pivoting over whole columns.
+ *
+ * Error probe.  Every solve ends with a probe of its result: for
+ * BEM_PROBE_COLS seeded random vectors w, x = [X w; Sc^-1 w] is checked
+ * against the unfactored FP64 blocks -- r = [0; w] - W x in FP64 -- and one
+ * correction d = W^-1 r through the solve's own factors gives d_1 ~ dX w,
+ * dX the error of X.  probe_err = max|d_1| / max|x_1|; residual =
+ * ||r||_F / ||w||_F estimates the relative residual
+ * ||[0; I] - W [X; Sc^-1]||_F / ||I||_F.  The error of Re S is, to first
+ * order, 1/2 Re tr(sym G2 X^H sym G1 dX); the probe estimates that trace from
+ * the same columns (g_j^H d_1j with g_j = sym G1 X sym G2 w_j): rel_err_est
+ * sums the terms' magnitudes, err_first_order keeps their signs.  Where W is
+ * ill-conditioned, dX holds the rounding of the residual amplified by W^-1
+ * in directions Re S hardly sees, so probe_err can exceed the error of Re S
+ * by orders of magnitude while rel_err_est does not.
+ *
+ * Recovery.  A block-local FP64 solve is solved again with tournament
+ * pivoting and BEM_SOLVE_REFINE when its growth exceeds BEM_GROWTH_LIMIT,
+ * when it met an exactly zero pivot, or when rel_err_est exceeds
+ * BEM_PROBE_LIMIT; `restarts` counts such re-solves. */
 #define BEM_SOLVE_TOURNAMENT 1
 #define BEM_GROWTH_LIMIT 16.0
-/* BEM_PREC_MIXED only: one step of iterative refinement -- the residual of
- * W [X; Sc^-1] = [0; I] against the FP64-assembled blocks in FP64, the
- * correction through the complex64 factors, and FP64 products and trace of
- * the corrected X.  rel_err_est then reports the relative change of Re S
- * from the complex64 result.  A mixed request that needs tournament
- * pivoting (growth above the limit) is solved on the FP64 path. */
+#define BEM_PROBE_COLS 8
+#define BEM_PROBE_LIMIT 1e-9
+/* Iterative refinement: each step forms the full residual
+ * R = [0; I] - W [X; Z2] of the current X and Z2 = Sc^-1 in FP64 against the
+ * FP64-assembled blocks, the correction through the solve's factors
+ * (complex64 for BEM_PREC_MIXED, complex128 otherwise), and the FP64
+ * products and trace of the corrected X, followed by the probe.  Steps
+ * repeat until the probe's relative residual is at most BEM_REFINE_TOL, at
+ * most BEM_REFINE_MAX of them; a BEM_PREC_MIXED request that reaches the
+ * cap is solved again on the FP64 path.  A mixed request that needs
+ * tournament pivoting (growth above the limit) is solved on the FP64 path. */
 #define BEM_SOLVE_REFINE 2
+#define BEM_REFINE_TOL 1e-10
+#define BEM_REFINE_MAX 3
 
 typedef struct {
   double P;          /* Re(S) (Theta(omega,T1) - Theta(omega,T2)) */
   double ReS, ImS;   /* S = 1/4 tr(B2^H B1) */
   double theta1, theta2;
   double omega;      /* dimensionless omega a / c */
-  double rel_err_est;/* mixed + refine: |Re S - Re S_complex64| / |Re S|; mixed alone: -1; FP64: 0 */
-  int iters;         /* refinement steps (mixed), 0 for FP64 */
+  double rel_err_est;/* estimated relative error of Re S (see Error probe) */
+  int iters;         /* refinement steps taken */
   int status;        /* 0 or BEM_E_* for this request */
   double ms_assemble, ms_factor, ms_products, ms_total;  /* device-event times */
   double growth;     /* max(1, max|L|) max|U| / max|A|, the larger of the two LU factorisations */
   int pivoting;      /* 0 block-local, 1 tournament */
   double ms_queue;   /* served requests: submit to issue (ms); ms_total is then submit to result */
   int precision;     /* BEM_PREC_* the result was computed in */
+  int restarts;      /* re-solves with tournament pivoting and refinement (Recovery), or on the FP64
+                        path after the refinement cap of a mixed request */
+  double probe_err;  /* max|d_1| / max|x_1| of the probe of the returned X */
+  double residual;   /* the probe's relative residual of the returned [X; Sc^-1] */
+  double full_residual; /* ||R||_F / ||I||_F of the last refinement step's full residual
+                           (of the X before that step's correction); -1 without refinement */
+  double err_first_order; /* the probe's signed estimate of (Re S_exact - Re S) / |Re S| */
 } bem_result;
 
 /* ---- context ---------------------------------------------------------- */

@@ -13,7 +13,13 @@ from an LU solve of W against [0; I] rather than the full inverse (same
 block, a third of the memory).  c3 needs about 45 GB of host memory and
 20 minutes on 8 cores, tip a few minutes.
 
-    python tests/golden/make_anchor.py <c3|tip> <eV> <gap>
+With --refine the point also carries the value after two steps of FP64
+iterative refinement of [X; Z2] = W^-1 [0; I] against the unfactored W
+(ReS_refined), LAPACK's reciprocal 1-norm condition estimate of W (rcond,
+zgecon) and the relative change of Re S per step, so that the accuracy of
+the plain LU value itself is known where W is ill-conditioned.
+
+    python tests/golden/make_anchor.py <c3|tip> <eV> <gap> [--refine]
 """
 import json
 import os
@@ -42,7 +48,9 @@ WORKLOADS = {
 
 
 def main():
-    name, eV, gap = sys.argv[1], float(sys.argv[2]), float(sys.argv[3])
+    args = [x for x in sys.argv[1:] if not x.startswith("--")]
+    refine = "--refine" in sys.argv
+    name, eV, gap = args[0], float(args[1]), float(args[2])
     desc, body1, body2, shift = WORKLOADS[name]
     a, T1, T2 = 1e-6, 300.0, 0.0
     sph = body1().translated((0.0, 0.0, shift(gap)))
@@ -65,25 +73,45 @@ def main():
     W[m1:, m1:] += G
     S2 = G + G.conj().T
     del G
+    W0 = np.array(W, order="F", copy=True) if refine else None
+    anorm = np.abs(W).sum(axis=0).max() if refine else 0.0
     say("G2 done; LU of W")
     lu, piv = sla.lu_factor(W, overwrite_a=True, check_finite=False)
     del W
+    rcond = None
+    if refine:
+        rcond = float(sla.lapack.zgecon(lu, anorm, norm="1")[0])
+        say(f"rcond (1-norm estimate) {rcond:.3e}")
     E = np.zeros((m1 + m2, m2), dtype=np.complex128, order="F")
     E[m1:, :] = np.eye(m2)
     say("solve")
-    Z = sla.lu_solve((lu, piv), E, overwrite_b=True, check_finite=False)
-    del lu, E
-    X = np.asfortranarray(Z[:m1, :])
-    del Z
+    Z = sla.lu_solve((lu, piv), E, overwrite_b=False, check_finite=False)
+
+    def trace_of(Z):
+        X = np.asfortranarray(Z[:m1, :])
+        B1 = S1 @ X
+        B2 = X @ S2
+        return 0.25 * np.vdot(B2, B1)
+
     say("products")
-    B1 = S1 @ X
-    del S1
-    B2 = X @ S2
-    del S2, X
-    S = 0.25 * np.vdot(B2, B1)
+    S = trace_of(Z)
     th1, th2 = O.theta(omega, T1), O.theta(omega, T2)
     out = {"workload": desc, "eV": eV, "gap": gap, "shift": [0.0, 0.0, shift(gap)], "a": a, "T1": T1, "T2": T2, "eps": [eps.real, eps.imag], "M": m1 + m2,
-           "ReS": S.real, "ImS": S.imag, "P": S.real * (th1 - th2), "seconds": time.time() - t0}
+           "ReS": S.real, "ImS": S.imag, "P": S.real * (th1 - th2)}
+    if refine:
+        steps, Sk = [], S
+        for it in range(2):
+            R = E - W0 @ Z
+            rel_res = float(np.linalg.norm(R) / np.linalg.norm(E))
+            Z = Z + sla.lu_solve((lu, piv), R, check_finite=False)
+            Sn = trace_of(Z)
+            steps.append({"residual_before": rel_res, "rel_change_ReS": abs(Sn.real - Sk.real) / abs(Sn.real)})
+            say(f"refinement {it + 1}: ||R||/||E|| before {rel_res:.3e}, ReS {Sn.real:.15e}, "
+                f"change {steps[-1]['rel_change_ReS']:.3e}")
+            Sk = Sn
+        out.update({"ReS_refined": Sk.real, "rcond": rcond, "refinement": steps,
+                    "lu_vs_refined": abs(S.real - Sk.real) / abs(Sk.real)})
+    out["seconds"] = time.time() - t0
     path = os.path.join(HERE, f"{name}_anchor.json")
     points = json.load(open(path))["points"] if os.path.exists(path) else []
     points = [p for p in points if (p["eV"], p["gap"]) != (eV, gap)] + [out]

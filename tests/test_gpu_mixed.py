@@ -1,12 +1,15 @@
 """Mixed-precision path (BEM_PREC_MIXED) through the C ABI: complex64 GEMM on
-FP32 MFMA, the complex64 block elimination, and one FP64 refinement step.
+FP32 MFMA, the complex64 block elimination, the error probe of every result
+and FP64 iterative refinement to a relative residual of BEM_REFINE_TOL.
 
 Tolerances:
   cgemm:                |C - C_ref| <= 4e-6 sqrt(k) max|C_ref| (complex64 inputs, FP32 accumulation)
   complex64 solve:      |ReS - ReS_oracle| / |ReS_oracle| <= 2e-5 (no refinement; FP32 rounding
                         amplified by the conditioning of W, ~1e-6 measured on these meshes)
-  refined (1 step):     <= 1e-8, the FP64 path's bound
-  error estimate:       |rel_err_est - actual complex64 error| <= 1e-8
+  its error statement:  rel_err_est >= actual error / 10 (the probe estimates the error of X in
+                        random directions; Re S is quadratic in X)
+  refined:              <= 1e-8, the FP64 path's bound, with the probe's relative residual
+                        <= BEM_REFINE_TOL (1e-10)
 """
 import numpy as np
 import pytest
@@ -57,14 +60,17 @@ def test_mixed_spectral_power_vs_oracle(gpu_ctx, eV):
                                                       flags=BEM_SOLVE_REFINE))
     e_fast = abs(fast["ReS"] - ref["ReS"]) / abs(ref["ReS"])
     e_ref = abs(refd["ReS"] - ref["ReS"]) / abs(ref["ReS"])
-    say(f"eV={eV}: complex64 err {e_fast:.2e}, refined err {e_ref:.2e}, est {refd['rel_err_est']:.2e}")
-    assert fast["precision"] == BEM_PREC_MIXED and fast["iters"] == 0 and fast["rel_err_est"] == -1.0
-    assert refd["precision"] == BEM_PREC_MIXED and refd["iters"] == 1
+    say(f"eV={eV}: complex64 err {e_fast:.2e} (est {fast['rel_err_est']:.2e}, residual {fast['residual']:.2e}), "
+        f"refined err {e_ref:.2e} after {refd['iters']} steps (est {refd['rel_err_est']:.2e}, "
+        f"residual {refd['residual']:.2e}, full residual before the last step {refd['full_residual']:.2e})")
+    assert fast["precision"] == BEM_PREC_MIXED and fast["iters"] == 0 and fast["full_residual"] == -1.0
     assert e_fast <= MIX_TOL
+    assert e_fast / 10 <= fast["rel_err_est"] <= 1e-3
+    assert refd["precision"] == BEM_PREC_MIXED and 1 <= refd["iters"] <= 3 and refd["restarts"] == 0
+    assert refd["residual"] <= 1e-10 and refd["full_residual"] > refd["residual"]
     assert e_ref <= REF_TOL
-    assert abs(refd["rel_err_est"] - e_fast) <= 1e-8
+    assert refd["rel_err_est"] <= 1e-8
     assert fast["P"] == fast["ReS"] * (fast["theta1"] - fast["theta2"])
-    assert refd["precision"] != BEM_PREC_FP64
 
 
 def test_mixed_refined_matches_reference_fixture(gpu_ctx):
