@@ -112,6 +112,13 @@ int bem_sumsq(const double2 *A, int64_t lda, int m, int n, double *partial, int 
 int bem_zaxpy(double2 alpha, const double2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st);
 int bem_zdotc_cols(const double2 *G, int64_t ldg, const double2 *D, int64_t ldd, int m, int n, double2 *out,
                    hipStream_t st);
+int bem_trsm_few(const double2 *A, int64_t lda, int n, const double2 *inv, double2 *X, int64_t ldx, int r, int lower,
+                 double2 *Y, int64_t ldy, hipStream_t st);
+int bem_trsm_few_c(const float2 *A, int64_t lda, int n, const float2 *inv, float2 *X, int64_t ldx, int r, int lower,
+                   float2 *Y, int64_t ldy, hipStream_t st);
+size_t bem_zgemm_skinny_partials(int m, int r, int K);
+int bem_zgemm_skinny(int m, int r, int K, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                     double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st);
 }
 
 // src/util/SI.cpp:3-9
@@ -226,6 +233,7 @@ struct Lane {
   int stage = ST_IDLE;            // what the lane's stream is running for the request
   int iters = 0;                  // refinement steps issued for the request
   int restarts = 0;               // re-solves of the request so far
+  bool probe_async = false;       // the stage's probe runs on aux (probe_join pending)
   ProbeOut *h_probe = nullptr;    // pinned copy of the last stage's probe
   double *h_rfull = nullptr;      // pinned: squared norm of the last refinement residual
   // served request in flight (ticket 0: none)
@@ -801,6 +809,59 @@ static int lu_solve_right(bem_ctx *c, Lane &L, const T *A, int64_t lda, int n, c
   return 0;
 }
 
+// Few-column substitutions (a launch per 64-row block) and the split-K
+// skinny product serve up to FEW_COLS columns.
+static const int FEW_COLS = 8;
+static int trsm_few(const double2 *A, int n, const double2 *inv, double2 *X, int64_t ldx, int r, int lower,
+                    double2 *Y, hipStream_t st) {
+  return bem_trsm_few(A, n, n, inv, X, ldx, r, lower, Y, n, st);
+}
+static int trsm_few(const float2 *A, int n, const float2 *inv, float2 *X, int64_t ldx, int r, int lower, float2 *Y,
+                    hipStream_t st) {
+  return bem_trsm_few_c(A, n, n, inv, X, ldx, r, lower, Y, n, st);
+}
+
+// X (n x r) <- A^-1 X for few right-hand sides (r <= FEW_COLS): the
+// interchanges, then the forward and backward substitutions with one launch
+// per 64-row block each (A packed n x n, leading dimension n), instead of the
+// blocked GEMMs; wider systems take lu_solve.
+template <typename T>
+static int lu_solve_few(bem_ctx *c, Lane &L, const T *A, int n, const int *ipiv, const T *inv, T *X, int64_t ldx,
+                        int r, bool local) {
+  if (r > FEW_COLS) return lu_solve(c, L, A, n, n, ipiv, inv, X, ldx, r, local);
+  int e;
+  T *Y;
+  CHK(ws_get(c, L, "few_y", sizeof(T) * (size_t)n * FEW_COLS, (void **)&Y));
+  if (local) {
+    int *perm;
+    CHK(ws_get(c, L, "perm", sizeof(int) * (size_t)(n + 64), (void **)&perm));
+    e = permute_rows_local(X, ldx, n, r, ipiv, perm, L.st);
+  } else {
+    e = laswp(X, ldx, 0, r, 0, 0, 0, n, ipiv, L.st);
+  }
+  if (!e) e = trsm_few(A, n, inv, X, ldx, r, 1, Y, L.st);
+  if (!e) e = trsm_few(A, n, inv, X, ldx, r, 0, Y, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "few-column substitution launch: %d", e);
+  return 0;
+}
+
+// C (m x n) <- alpha A B + beta C for n <= FEW_COLS columns (the probe's
+// products) by the split-K skinny kernel; wider products take gemm().
+static int gemm_few(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda,
+                    const double2 *B, int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
+  if (n > FEW_COLS) return gemm(c, L, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc);
+  if (m <= 0 || n <= 0) return 0;
+  double2 *part;
+  CHK(ws_get(c, L, "skinny_part", sizeof(double2) * bem_zgemm_skinny_partials(m, FEW_COLS, k), (void **)&part));
+  EventRec *r;
+  const bool rd_c = beta.x != 0.0 || beta.y != 0.0;
+  CHK(ev_begin(c, L, BEM_KS_ZGEMM_NARROW, 8.0 * m * (double)n * k,
+               16.0 * ((double)m * k + (double)k * n + (rd_c ? 2.0 : 1.0) * (double)m * n), &r));
+  int e = bem_zgemm_skinny(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, part, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "skinny zgemm launch: %d", e);
+  return ev_end(c, L, r);
+}
+
 // ------------------------------------------------------------------ assembly
 static Medium make_medium(double er, double ei, double mr, double mi, double omega) {
   // src/GreensFunction.cpp:17-30 (complex arithmetic written out)
@@ -1238,10 +1299,10 @@ static const unsigned long long PROBE_SEED = 0x5EED0B5Eull;
 // given, Z2 = Sc^-1 (m2 x m2, FP64; otherwise Sc^-1 w comes from the factors
 // of Sc).  w: BEM_PROBE_COLS seeded columns with complex64 values (exact in
 // both precisions); x = [X w; Z2 w]; r = [0; w] - W x in FP64 against the
-// unfactored blocks; d = W^-1 r by block elimination through the factors of
-// A and Sc in the working precision T.  The top rows of W^-1 [0; w] are
+// unfactored blocks; d_1 = [W^-1 r]_1 through the factors of A in the
+// working precision T and the solve's X.  The top rows of W^-1 [0; w] are
 // X_exact w, so d_1 = dX w with dX = X_exact - X, up to the accuracy of the
-// correction and whatever the error of Z2 w.
+// correction.
 // Re S = 1/4 tr(sym G2 X^H sym G1 X) changes to first order by 1/2 Re t,
 // t = tr(F dX), F = sym G2 X^H sym G1; with E[w w^H] = s^2 I,
 // E[w^H F dX w] = s^2 t, and w^H F = g^H for g = sym G1 X sym G2 w, so
@@ -1269,51 +1330,87 @@ static int probe(bem_ctx *c, Lane &L, const LuRef<T> &A, const LuRef<T> &S, cons
   CHK(ws_get(c, L, "pr_b", sizeof(T) * (size_t)m2 * k, (void **)&b));
   CHK(ws_get(c, L, "pr_part", sizeof(double) * std::max(bem_sumsq_partials(m1, k), bem_sumsq_partials(m2, k)),
              (void **)&part));
-  const size_t wb1 = sizeof(double2) * (size_t)m1 * k, wb2 = sizeof(double2) * (size_t)m2 * k;
+  {   // workspaces at their largest, so none grows between the probe's launches
+    T *fy;
+    CHK(ws_get(c, L, "few_y", sizeof(T) * (size_t)std::max(m1, m2) * FEW_COLS, (void **)&fy));
+    double2 *sp;
+    size_t np = 0;
+    for (int mm : {m1, m2})
+      for (int kk : {m1, m2}) np = std::max(np, bem_zgemm_skinny_partials(mm, FEW_COLS, kk));
+    CHK(ws_get(c, L, "skinny_part", sizeof(double2) * np, (void **)&sp));
+  }
+  const size_t wb2 = sizeof(double2) * (size_t)m2 * k;
   if (bem_fill_hash(w32, sizeof(float2) * (size_t)m2 * k, 1, PROBE_SEED, L.st) ||
       bem_c2z(w32, m2, w, m2, m2, k, L.st))
     return fail(c, BEM_E_DEVICE, "probe fill launch");
   DEV(c, hipMemsetAsync(out, 0, 2 * sizeof(unsigned long long), L.st));
   // x = [X w; Z2 w]
   if (X2) {
-    CHK(gemm(c, L, m2, k, m2, one, X2, m2, w, m2, zero, x2, m2));
+    CHK(gemm_few(c, L, m2, k, m2, one, X2, m2, w, m2, zero, x2, m2));
   } else {
     CHK(to_work(c, L, w, b, m2, k));
-    CHK(lu_solve(c, L, S.lu, m2, m2, S.ipiv, S.inv, b, m2, k, S.local));
+    CHK(lu_solve_few(c, L, S.lu, m2, S.ipiv, S.inv, b, m2, k, S.local));
     CHK(from_work(c, L, b, x2, m2, k));
   }
-  CHK(gemm(c, L, m1, k, m2, mone, Xn, m1, w, m2, zero, x1, m1));
+  CHK(gemm_few(c, L, m1, k, m2, mone, Xn, m1, w, m2, zero, x1, m1));
   // r1 = -(A x1 + B x2), r2 = w - C x1 - D x2
-  CHK(gemm(c, L, m1, k, m1, mone, Araw, m1, x1, m1, zero, r1, m1));
-  CHK(gemm(c, L, m1, k, m2, mone, Braw, m1, x2, m2, one, r1, m1));
+  CHK(gemm_few(c, L, m1, k, m1, mone, Araw, m1, x1, m1, zero, r1, m1));
+  CHK(gemm_few(c, L, m1, k, m2, mone, Braw, m1, x2, m2, one, r1, m1));
   DEV(c, hipMemcpyAsync(r2, w, wb2, hipMemcpyDeviceToDevice, L.st));
-  CHK(gemm(c, L, m2, k, m1, mone, Cm, m2, x1, m1, one, r2, m2));
-  CHK(gemm(c, L, m2, k, m2, mone, Draw, m2, x2, m2, one, r2, m2));
+  CHK(gemm_few(c, L, m2, k, m1, mone, Cm, m2, x1, m1, one, r2, m2));
+  CHK(gemm_few(c, L, m2, k, m2, mone, Draw, m2, x2, m2, one, r2, m2));
   if (bem_sumsq(r1, m1, m1, k, part, 0, &out->rr, L.st) || bem_sumsq(r2, m2, m2, k, part, 1, &out->rr, L.st) ||
       bem_sumsq(w, m2, m2, k, part, 0, &out->ww, L.st))
     return fail(c, BEM_E_DEVICE, "probe norm launch");
-  // d = W^-1 r: u = A^-1 r1, d2 = Sc^-1 (r2 - C u), d1 = A^-1 (r1 - B d2)
+  // d_1 = [W^-1 r]_1 = u + X (r2 - C u) with u = A^-1 r1: the top block row
+  // of W^-1 is [A^-1 - X C A^-1, X], and the X of the solve stands in for the
+  // exact one (the difference times r2 - C u is of second order)
   CHK(to_work(c, L, r1, a, m1, k));
-  CHK(lu_solve(c, L, A.lu, m1, m1, A.ipiv, A.inv, a, m1, k, A.local));
-  CHK(from_work(c, L, a, t1, m1, k));
+  CHK(lu_solve_few(c, L, A.lu, m1, A.ipiv, A.inv, a, m1, k, A.local));
+  CHK(from_work(c, L, a, t1, m1, k));                                // u
   DEV(c, hipMemcpyAsync(t2, r2, wb2, hipMemcpyDeviceToDevice, L.st));
-  CHK(gemm(c, L, m2, k, m1, mone, Cm, m2, t1, m1, one, t2, m2));
-  CHK(to_work(c, L, t2, b, m2, k));
-  CHK(lu_solve(c, L, S.lu, m2, m2, S.ipiv, S.inv, b, m2, k, S.local));
-  CHK(from_work(c, L, b, t2, m2, k));
-  DEV(c, hipMemcpyAsync(t1, r1, wb1, hipMemcpyDeviceToDevice, L.st));
-  CHK(gemm(c, L, m1, k, m2, mone, Braw, m1, t2, m2, one, t1, m1));
-  CHK(to_work(c, L, t1, a, m1, k));
-  CHK(lu_solve(c, L, A.lu, m1, m1, A.ipiv, A.inv, a, m1, k, A.local));
-  int e = absmax(a, m1, m1, k, 0, &out->dmax, L.st);
+  CHK(gemm_few(c, L, m2, k, m1, mone, Cm, m2, t1, m1, one, t2, m2));    // r2 - C u
+  CHK(gemm_few(c, L, m1, k, m2, mone, Xn, m1, t2, m2, one, t1, m1));    // d1 = u - Xn (r2 - C u)
+  int e = bem_absmax(t1, m1, m1, k, 0, &out->dmax, L.st);
   if (!e) e = bem_absmax(x1, m1, m1, k, 0, &out->xmax, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "probe absmax launch: %d", e);
   // h_j = g_j^H d_1j with g = sym G1 X sym G2 w = -sym G1 Xn sym G2 w
-  CHK(from_work(c, L, a, t1, m1, k));
-  CHK(gemm(c, L, m2, k, m2, one, Gs2, m2, w, m2, zero, r2, m2));
-  CHK(gemm(c, L, m1, k, m2, one, Xn, m1, r2, m2, zero, r1, m1));
-  CHK(gemm(c, L, m1, k, m1, mone, Gs1, m1, r1, m1, zero, x1, m1));
+  CHK(gemm_few(c, L, m2, k, m2, one, Gs2, m2, w, m2, zero, r2, m2));
+  CHK(gemm_few(c, L, m1, k, m2, one, Xn, m1, r2, m2, zero, r1, m1));
+  CHK(gemm_few(c, L, m1, k, m1, mone, Gs1, m1, r1, m1, zero, x1, m1));
   if ((e = bem_zdotc_cols(x1, m1, t1, m1, m1, k, out->h, L.st))) return fail(c, BEM_E_DEVICE, "probe dot launch: %d", e);
+  return 0;
+}
+
+// Starts the probe of a solve stage.  When look-ahead gave the lane a second
+// stream (the lanes fit the hardware queues twice), the probe runs there,
+// overlapping the products and trace, which read the same X; probe_join()
+// then makes the lane's stream wait for it.  Otherwise it runs in line.  Every
+// workspace the probe uses is sized before its first launch, so none is
+// reallocated while its kernels run on the other stream.
+template <typename T>
+static int probe_start(bem_ctx *c, Lane &L, const LuRef<T> &A, const LuRef<T> &S, const double2 *Araw,
+                       const double2 *Braw, const double2 *Cm, const double2 *Draw, const double2 *Gs1,
+                       const double2 *Gs2, const double2 *Xn, ProbeOut *out) {
+  if (!c->lookahead) return probe<T>(c, L, A, S, Araw, Braw, Cm, Draw, Gs1, Gs2, Xn, nullptr, out);
+  DEV(c, hipEventRecord(L.la_a, L.st));
+  DEV(c, hipStreamWaitEvent(L.aux, L.la_a, 0));
+  std::swap(L.st, L.aux);
+  int e = probe<T>(c, L, A, S, Araw, Braw, Cm, Draw, Gs1, Gs2, Xn, nullptr, out);
+  std::swap(L.st, L.aux);
+  if (e) {
+    (void)hipStreamSynchronize(L.aux);
+    return e;
+  }
+  DEV(c, hipEventRecord(L.la_b, L.aux));
+  L.probe_async = true;
+  return 0;
+}
+
+static int probe_join(bem_ctx *c, Lane &L) {
+  if (!L.probe_async) return 0;
+  L.probe_async = false;
+  DEV(c, hipStreamWaitEvent(L.st, L.la_b, 0));
   return 0;
 }
 
@@ -1417,19 +1514,17 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
     DEV(c, hipMemcpyAsync(Y32, B32, sizeof(float2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
   }
   CHK(lu_solve_right(c, L, S32, m2, m2, ipS, invS, B32, m1, m1, true));  // B32 <- Y Sc^-1 = -X
+  if (bem_c2z(B32, m1, Xn, m1, m1, m2, L.st)) return fail(c, BEM_E_DEVICE, "complex128 copy launch");
+  const LuRef<float2> A{E1.Z32, m1, E1.ipiv, E1.inv32, true}, S{S32, m2, ipS, invS, true};
+  CHK(probe_start<float2>(c, L, A, S, E1.Araw, Bm, Cm, E2.Araw, E1.Gs, E2.Gs, Xn, pout));
   // products: 4 S = tr(sym G2 X^H sym G1 X), even in X, over the Hermitian
   // X^H (sym G1 X): half of it is formed
   CHK(gemm(c, L, m1, m2, m1, one, E1.Gs32, m1, B32, m1, zero, B1, m1));
-  {
-    EventRec *r;
-    CHK(ev_begin(c, L, BEM_KS_CGEMM, 4.0 * m1 * (double)m2 * m2, 8.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
-    int e = bem_cgemm_htrace(m2, m1, B32, m1, B1, m1, E2.Gs32, m2, partial, trace_out, L.st);
-    if (e) return fail(c, BEM_E_DEVICE, "complex64 trace gemm: %d", e);
-    CHK(ev_end(c, L, r));
-  }
-  if (bem_c2z(B32, m1, Xn, m1, m1, m2, L.st)) return fail(c, BEM_E_DEVICE, "complex128 copy launch");
-  const LuRef<float2> A{E1.Z32, m1, E1.ipiv, E1.inv32, true}, S{S32, m2, ipS, invS, true};
-  return probe<float2>(c, L, A, S, E1.Araw, Bm, Cm, E2.Araw, E1.Gs, E2.Gs, Xn, nullptr, pout);
+  EventRec *r;
+  CHK(ev_begin(c, L, BEM_KS_CGEMM, 4.0 * m1 * (double)m2 * m2, 8.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
+  int e = bem_cgemm_htrace(m2, m1, B32, m1, B1, m1, E2.Gs32, m2, partial, trace_out, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "complex64 trace gemm: %d", e);
+  return ev_end(c, L, r);
 }
 
 // Issues every kernel of the solve stage of one spectral-power request on the
@@ -1516,6 +1611,8 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1, !pivot));
     double2 *X = Bm;
     DEV(c, hipEventRecord(L.ph[2], L.st));
+    const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, S{Sc, m2, ipS, invS, !pivot};
+    CHK(probe_start<double2>(c, L, A, S, E1.Araw, Braw, Cm, E2.Araw, E1.Gs, E2.Gs, Bm, pout));
     // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
     // = 1/4 tr(sym(G2) P) with P = X^H sym(G1) X Hermitian (sym(G) is G + G^H),
     // so only the upper half of P is formed
@@ -1525,9 +1622,8 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     int e = bem_zgemm_htrace(m2, m1, X, m1, B1, m1, E2.Gs, m2, partial, trace_out, L.st);
     if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
     CHK(ev_end(c, L, r));
-    const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, S{Sc, m2, ipS, invS, !pivot};
-    CHK(probe<double2>(c, L, A, S, E1.Araw, Braw, Cm, E2.Araw, E1.Gs, E2.Gs, Bm, nullptr, pout));
   }
+  CHK(probe_join(c, L));
   DEV(c, hipEventRecord(L.ph[3], L.st));
   DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
   DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
@@ -1741,6 +1837,8 @@ static int lane_step(bem_ctx *c, Lane &L, int *err) {
 // Abandons a partially issued stage: waits for what was issued and releases
 // the lane.
 static void lane_abort(bem_ctx *c, Lane &L) {
+  (void)hipStreamSynchronize(L.aux);
+  L.probe_async = false;
   (void)hipStreamSynchronize(L.st);
   L.busy = false;
   L.stage = ST_IDLE;
@@ -2600,7 +2698,10 @@ int bem_zgetrs(bem_ctx *c, const void *A, int64_t lda, int n, const int32_t *ipi
   CHK(ws_get(c, L, "inv_getrs", inv_bytes(n), (void **)&inv));
   int e = bem_diag_block((double2 *)A, lda, n, 0, (n + LU_NB - 1) / LU_NB, LU_NB, 0, inv, nullptr, L.st);
   if (e) return fail(c, BEM_E_DEVICE, "diag block inverses: %d", e);
-  CHK(lu_solve(c, L, (const double2 *)A, lda, n, ipiv, inv, (double2 *)B, ldb, nrhs));
+  if (nrhs <= FEW_COLS && lda == n)   // few right-hand sides: a launch per 64-row block
+    CHK(lu_solve_few(c, L, (const double2 *)A, n, ipiv, inv, (double2 *)B, ldb, nrhs, false));
+  else
+    CHK(lu_solve(c, L, (const double2 *)A, lda, n, ipiv, inv, (double2 *)B, ldb, nrhs));
   DEV(c, hipStreamSynchronize(L.st));
   ev_collect(c, L);
   return 0;
@@ -2614,8 +2715,9 @@ int bem_zgemm(bem_ctx *c, int m, int n, int k, const double *alpha, const void *
   Lane &L = c->lanes[0];
   CHK(lane_retire(c, L));
   stats_reset(c);
-  CHK(gemm(c, L, m, n, k, make_double2(alpha[0], alpha[1]), (const double2 *)A, lda, (const double2 *)B, ldb,
-           make_double2(beta[0], beta[1]), (double2 *)C, ldc));
+  // at most FEW_COLS columns: the split-K skinny kernel (the probe's products)
+  CHK(gemm_few(c, L, m, n, k, make_double2(alpha[0], alpha[1]), (const double2 *)A, lda, (const double2 *)B, ldb,
+               make_double2(beta[0], beta[1]), (double2 *)C, ldc));
   DEV(c, hipStreamSynchronize(L.st));
   ev_collect(c, L);
   c->t_last[6] = c->gemm_ms;

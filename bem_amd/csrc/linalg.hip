@@ -1416,6 +1416,179 @@ __global__ void __launch_bounds__(256) sumsq_finish_kernel(const double *__restr
   if (threadIdx.x == 0) out[0] = (add ? out[0] : 0.0) + s[0];
 }
 
+// --------------------------------------------------- few-column kernels
+// Substitutions and products with at most FEW_MAX_COLS columns: the error
+// probe's.
+constexpr int FEW_MAX_COLS = 8;
+
+template <typename T>
+__device__ __forceinline__ T cfma(T a, T b, T c) {   // c + a b
+  c.x = fma(a.x, b.x, fma(-a.y, b.y, c.x));
+  c.y = fma(a.x, b.y, fma(a.y, b.x, c.y));
+  return c;
+}
+
+// ------------------------------------------- few-column substitution step
+// Step k of a substitution with a packed LU for r <= 8 right-hand sides, as
+// one launch with no waiting between workgroups: every workgroup forms
+// x_k = inv_k b_k from the rows of block k in Bw (only read in this launch),
+// workgroup 0 writes x_k to Xo, and workgroup g >= 1 subtracts the coupling
+// A(rows of its chunk, block k) x_k from its 64 rows of Bw -- the chunks
+// below block k (LOWER: forward with the unit-lower L) or above it (upper:
+// backward with U).  Thread (i, q) takes row i and the quarter q of the inner
+// index for all columns; its slice of the inverse and of the coupling block
+// are loaded before anything waits, and the quarters meet in LDS.
+template <typename T, bool LOWER>
+__global__ void __launch_bounds__(256) trsm_step_kernel(const T *__restrict__ A, int64_t lda, int n,
+                                                        const T *__restrict__ inv, T *__restrict__ Bw, int64_t ldb,
+                                                        T *__restrict__ Xo, int64_t ldx, int r, int k) {
+  using R = typename Cplx<T>::R;
+  __shared__ T xs[64][FEW_MAX_COLS];
+  __shared__ T part[4][FEW_MAX_COLS][64];
+  const int t = threadIdx.x, i = t & 63, q = t >> 6, l0 = q * 16;
+  const int g = blockIdx.x;
+  const int r0 = k * 64, w = min(64, n - r0);
+  const int row0 = LOWER ? (k + g) * 64 : (g - 1) * 64;
+  const bool live = g > 0 && row0 + i < n;
+  const T *M = inv + (size_t)k * 2 * 64 * 64 + (LOWER ? 0 : 64 * 64);
+  T m[16], a[16];
+#pragma unroll
+  for (int l = 0; l < 16; l++) {
+    m[l] = M[(l0 + l) * 64 + i];
+    a[l] = (live && l0 + l < w) ? A[(int64_t)(r0 + l0 + l) * lda + row0 + i] : Cplx<T>::mk((R)0, (R)0);
+  }
+  for (int e = t; e < 64 * r; e += 256) {
+    const int l = e & 63, c = e >> 6;
+    xs[l][c] = l < w ? Bw[(int64_t)c * ldb + r0 + l] : Cplx<T>::mk((R)0, (R)0);
+  }
+  __syncthreads();
+  T acc[FEW_MAX_COLS];
+#pragma unroll
+  for (int c = 0; c < FEW_MAX_COLS; c++) {
+    acc[c] = Cplx<T>::mk((R)0, (R)0);
+    if (c < r) {
+#pragma unroll
+      for (int l = 0; l < 16; l++) acc[c] = cfma(m[l], xs[l0 + l][c], acc[c]);
+      part[q][c][i] = acc[c];
+    }
+  }
+  __syncthreads();
+  T xk[2];
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int c = q + 4 * u;
+    if (c < r) {
+      const T s0 = part[0][c][i], s1 = part[1][c][i], s2 = part[2][c][i], s3 = part[3][c][i];
+      xk[u] = Cplx<T>::mk((s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y));
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int c = q + 4 * u;
+    if (c < r) xs[i][c] = i < w ? xk[u] : Cplx<T>::mk((R)0, (R)0);
+  }
+  if (g == 0) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int c = q + 4 * u;
+      if (c < r && i < w) Xo[(int64_t)c * ldx + r0 + i] = xk[u];
+    }
+    return;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < FEW_MAX_COLS; c++) {
+    acc[c] = Cplx<T>::mk((R)0, (R)0);
+    if (c < r) {
+#pragma unroll
+      for (int l = 0; l < 16; l++) acc[c] = cfma(a[l], xs[l0 + l][c], acc[c]);
+      part[q][c][i] = acc[c];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 2; u++) {
+    const int c = q + 4 * u;
+    if (c < r && live) {
+      const T s0 = part[0][c][i], s1 = part[1][c][i], s2 = part[2][c][i], s3 = part[3][c][i];
+      const T b = Bw[(int64_t)c * ldb + row0 + i];
+      Bw[(int64_t)c * ldb + row0 + i] =
+          Cplx<T>::mk(b.x - ((s0.x + s1.x) + (s2.x + s3.x)), b.y - ((s0.y + s1.y) + (s2.y + s3.y)));
+    }
+  }
+}
+
+// ------------------------------------------------------------ skinny ZGEMM
+// C (m x r, r <= 8) <- alpha A (m x K) B (K x r) + beta C in complex128: the
+// probe's products, bound by reading A once.  Workgroup (bx, by) takes rows
+// [64 bx, 64 bx + 64) and the by-th of nsplit ranges of K; thread (i, q) row i
+// and every fourth k of the range (coalesced column reads of A), the B rows
+// staged in LDS.  The quarters meet in LDS and each workgroup writes its
+// partial; skinny_finish adds the partials in split order (deterministic).
+constexpr int SKINNY_KCHUNK = 64;
+
+__global__ void __launch_bounds__(256) zgemm_skinny_kernel(int m, int K, int r, const double2 *__restrict__ A,
+                                                           int64_t lda, const double2 *__restrict__ B, int64_t ldb,
+                                                           int kper, double2 *__restrict__ part) {
+  __shared__ double2 bs[SKINNY_KCHUNK][FEW_MAX_COLS];
+  __shared__ double2 red[4][FEW_MAX_COLS][64];
+  const int t = threadIdx.x, i = t & 63, q = t >> 6;
+  const int row = blockIdx.x * 64 + i;
+  const bool live = row < m;
+  const int kb0 = blockIdx.y * kper, kb1 = min(K, kb0 + kper);
+  double2 acc[FEW_MAX_COLS];
+#pragma unroll
+  for (int c = 0; c < FEW_MAX_COLS; c++) acc[c] = make_double2(0.0, 0.0);
+  for (int k0 = kb0; k0 < kb1; k0 += SKINNY_KCHUNK) {
+    const int kw = min(SKINNY_KCHUNK, kb1 - k0);
+    __syncthreads();
+    for (int e = t; e < SKINNY_KCHUNK * r; e += 256) {
+      const int l = e % SKINNY_KCHUNK, c = e / SKINNY_KCHUNK;
+      bs[l][c] = l < kw ? B[(int64_t)c * ldb + k0 + l] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2 a[SKINNY_KCHUNK / 4];
+#pragma unroll
+    for (int u = 0; u < SKINNY_KCHUNK / 4; u++) {
+      const int l = q + 4 * u;
+      a[u] = (live && l < kw) ? A[(int64_t)(k0 + l) * lda + row] : make_double2(0.0, 0.0);
+    }
+#pragma unroll
+    for (int u = 0; u < SKINNY_KCHUNK / 4; u++)
+#pragma unroll
+      for (int c = 0; c < FEW_MAX_COLS; c++)
+        if (c < r) acc[c] = cfma(a[u], bs[q + 4 * u][c], acc[c]);
+  }
+#pragma unroll
+  for (int c = 0; c < FEW_MAX_COLS; c++)
+    if (c < r) red[q][c][i] = acc[c];
+  __syncthreads();
+  for (int c = q; c < r; c += 4) {
+    const double2 s0 = red[0][c][i], s1 = red[1][c][i], s2 = red[2][c][i], s3 = red[3][c][i];
+    if (live)
+      part[((int64_t)blockIdx.y * r + c) * m + row] =
+          make_double2((s0.x + s1.x) + (s2.x + s3.x), (s0.y + s1.y) + (s2.y + s3.y));
+  }
+}
+
+__global__ void zgemm_skinny_finish_kernel(int m, int r, int nsplit, const double2 *__restrict__ part,
+                                           double2 alpha, double2 beta, double2 *__restrict__ C, int64_t ldc) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+  if (row >= m) return;
+  double2 s = make_double2(0.0, 0.0);
+  for (int p = 0; p < nsplit; p++) {
+    const double2 v = part[((int64_t)p * r + c) * m + row];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  double2 o = make_double2(0.0, 0.0);
+  if (beta.x != 0.0 || beta.y != 0.0) o = C[(int64_t)c * ldc + row];
+  double2 v = make_double2(beta.x * o.x - beta.y * o.y, beta.x * o.y + beta.y * o.x);
+  v = cfma(alpha, s, v);
+  C[(int64_t)c * ldc + row] = v;
+}
+
 // out[j] <- sum_i conj(G(i, j)) D(i, j) for the n columns of two m-row
 // complex128 blocks: one 256-thread block per column, summed in a fixed
 // order.
@@ -1486,6 +1659,30 @@ void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t 
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
+// X (n x r, r <= 8) <- L^-1 X (lower = 1) or U^-1 X (lower = 0) with a
+// packed LU and its diagonal-block inverses: one trsm_step_kernel launch per
+// 64-row block, the solution gathered in Y (n x r, leading dimension ldy) and
+// copied back to X.
+template <typename T>
+static int trsm_few(const T *A, int64_t lda, int n, const T *inv, T *X, int64_t ldx, int r, int lower, T *Y,
+                    int64_t ldy, hipStream_t st) {
+  if (n <= 0 || r <= 0) return 0;
+  if (r > FEW_MAX_COLS) return -1;
+  const int nb = (n + 63) / 64;
+  for (int s = 0; s < nb; s++) {
+    const int k = lower ? s : nb - 1 - s;
+    const int grid = 1 + (lower ? nb - 1 - k : k);
+    if (lower)
+      hipLaunchKernelGGL((trsm_step_kernel<T, true>), dim3(grid), dim3(256), 0, st, A, lda, n, inv, X, ldx, Y, ldy, r,
+                         k);
+    else
+      hipLaunchKernelGGL((trsm_step_kernel<T, false>), dim3(grid), dim3(256), 0, st, A, lda, n, inv, X, ldx, Y, ldy,
+                         r, k);
+  }
+  if (hipError_t e = hipGetLastError()) return (int)e;
+  return (int)hipMemcpy2DAsync(X, sizeof(T) * ldx, Y, sizeof(T) * ldy, sizeof(T) * n, r, hipMemcpyDeviceToDevice, st);
+}
+
 // Block-local interchanges ipiv[0..n) applied to the rows of X (n x r) or,
 // cols = 1, to the columns of X (m x n) as X P^T, through the permutation of
 // each 64-block (perm: n ints of workspace).
@@ -1808,6 +2005,41 @@ int bem_sumsq(const double2 *A, int64_t lda, int m, int n, double *partial, int 
   } else {
     hipLaunchKernelGGL(sumsq_finish_kernel, dim3(1), dim3(256), 0, st, (const double *)partial, 0, add, out);
   }
+  return (int)hipGetLastError();
+}
+
+int bem_trsm_few(const double2 *A, int64_t lda, int n, const double2 *inv, double2 *X, int64_t ldx, int r, int lower,
+                 double2 *Y, int64_t ldy, hipStream_t st) {
+  return trsm_few(A, lda, n, inv, X, ldx, r, lower, Y, ldy, st);
+}
+int bem_trsm_few_c(const float2 *A, int64_t lda, int n, const float2 *inv, float2 *X, int64_t ldx, int r, int lower,
+                   float2 *Y, int64_t ldy, hipStream_t st) {
+  return trsm_few(A, lda, n, inv, X, ldx, r, lower, Y, ldy, st);
+}
+
+// Partials zgemm_skinny needs for C (m x r) with inner dimension K.
+static int skinny_split(int m, int K) {
+  const int mb = (m + 63) / 64, kc = (K + SKINNY_KCHUNK - 1) / SKINNY_KCHUNK;
+  int ns = (1024 + mb - 1) / mb;
+  return ns < 1 ? 1 : (ns > kc ? (kc > 0 ? kc : 1) : ns);
+}
+size_t bem_zgemm_skinny_partials(int m, int r, int K) { return (size_t)skinny_split(m, K) * r * (size_t)m; }
+
+// C (m x r, r <= 8) <- alpha A B + beta C (A m x K, B K x r) in complex128
+// with part holding bem_zgemm_skinny_partials(m, r, K) values.
+int bem_zgemm_skinny(int m, int r, int K, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                     double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st) {
+  if (m <= 0 || r <= 0) return 0;
+  if (r > FEW_MAX_COLS) return -1;
+  const int ns = skinny_split(m, K);
+  const int kc = (K + SKINNY_KCHUNK - 1) / SKINNY_KCHUNK;
+  const int kper = ((kc + ns - 1) / ns) * SKINNY_KCHUNK;
+  const int nsplit = K > 0 ? (K + kper - 1) / kper : 1;
+  if (K > 0)
+    hipLaunchKernelGGL(zgemm_skinny_kernel, dim3((m + 63) / 64, nsplit), dim3(256), 0, st, m, K, r, A, lda, B, ldb,
+                       kper, part);
+  hipLaunchKernelGGL(zgemm_skinny_finish_kernel, dim3((m + 255) / 256, r), dim3(256), 0, st, m, r, K > 0 ? nsplit : 0,
+                     (const double2 *)part, alpha, beta, C, ldc);
   return (int)hipGetLastError();
 }
 
