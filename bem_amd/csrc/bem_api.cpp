@@ -117,6 +117,7 @@ int bem_trsm_few(const double2 *A, int64_t lda, int n, const double2 *inv, doubl
 int bem_trsm_few_c(const float2 *A, int64_t lda, int n, const float2 *inv, float2 *X, int64_t ldx, int r, int lower,
                    float2 *Y, int64_t ldy, hipStream_t st);
 size_t bem_zgemm_skinny_partials(int m, int r, int K);
+int bem_fill_zero(void *p, size_t bytes, hipStream_t st);
 int bem_zgemm_skinny(int m, int r, int K, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                      double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st);
 }
@@ -258,6 +259,8 @@ struct Pending {
 
 struct bem_ctx {
   int device = 0;
+  void *scratch = nullptr;          // device address of the allocation-free entry point's array
+  int *h_noalloc = nullptr;         // pinned: its zero-pivot flag and growth maxima
   size_t budget = 0;
   size_t bytes = 0;
   std::string err;
@@ -2203,6 +2206,32 @@ static void tune_store(const char *path, const std::string &arch, const int *til
   std::fclose(f);
 }
 
+// ------------------------------------------------- allocation-free LU entry
+// Workspace of bem_zgetrf_local_noalloc in a fixed device array: the
+// diagonal-block inverses of an n <= BEM_NOALLOC_MAX_N factorisation, then its
+// zero-pivot flag (at +0) and growth maxima (at +64).  Each GPU holds its own
+// copy until the process ends; host code reaches it only through the address
+// that hipGetSymbolAddress returns with that device current (the array's name
+// is a host-side stand-in), looked up once per device by bem_ctx_create.
+static const size_t NOALLOC_INV_BYTES = sizeof(double2) * 2 * 64 * 64 * (size_t)((BEM_NOALLOC_MAX_N + 63) / 64);
+static const size_t NOALLOC_BYTES = NOALLOC_INV_BYTES + 256;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64];   // per device ordinal
+
+static int scratch_address(int device, void **out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0 || device >= 64) return BEM_E_INVALID;
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return BEM_E_DEVICE;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+
 // ------------------------------------------------------------------- C ABI
 extern "C" {
 
@@ -2223,6 +2252,11 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
   c->lookahead = la > 0 || (la == 0 && 2 * nl <= hwq);
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= c->device || hipSetDevice(c->device) != hipSuccess) {
+    delete c;
+    return BEM_E_DEVICE;
+  }
+  if (scratch_address(c->device, &c->scratch) ||
+      hipHostMalloc((void **)&c->h_noalloc, 128, hipHostMallocDefault) != hipSuccess) {
     delete c;
     return BEM_E_DEVICE;
   }
@@ -2276,6 +2310,7 @@ void bem_ctx_destroy(bem_ctx *c) {
     b->up = false;
     b->ctx = nullptr;   // bem_body_free then only releases the host object
   }
+  if (c->h_noalloc) (void)hipHostFree(c->h_noalloc);
   for (auto &L : c->lanes) {
     for (auto &r : L.ev) {
       (void)hipEventDestroy(r.a);
@@ -2684,6 +2719,44 @@ int bem_zgetrf_local(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv, dou
   ev_collect(c, L);
   if (growth) *growth = growth_ratio(g);
   if (h) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", h - 1);
+  return 0;
+}
+
+int bem_zgetrf_local_noalloc(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv, double *growth) {
+  if (!c || !A || !ipiv || n < 0 || lda < n) return fail(c, BEM_E_INVALID, "bad arguments");
+  if (n > BEM_NOALLOC_MAX_N)
+    return fail(c, BEM_E_UNSUPPORTED, "n = %d exceeds BEM_NOALLOC_MAX_N = %d of the allocation-free factorisation", n,
+                BEM_NOALLOC_MAX_N);
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane *L = nullptr;
+  for (auto &l : c->lanes)
+    if (!l.busy) {
+      L = &l;
+      break;
+    }
+  if (!L) return fail(c, BEM_E_UNSUPPORTED, "every lane has a request in flight; the allocation-free factorisation "
+                                            "runs on an idle lane");
+  unsigned char *base = (unsigned char *)c->scratch;
+  double2 *inv = (double2 *)base;
+  int *info = (int *)(base + NOALLOC_INV_BYTES);
+  unsigned long long *gro = (unsigned long long *)(base + NOALLOC_INV_BYTES + 64);
+  if (bem_fill_zero(base + NOALLOC_INV_BYTES, 256, L->st)) return fail(c, BEM_E_DEVICE, "scratch clear launch");
+  // the same kernels in the same order as bem_zgetrf_local; no timing events
+  // (nothing is created)
+  const bool prof = c->profile;
+  c->profile = false;
+  int e = lu_factor_block(c, *L, (double2 *)A, lda, n, ipiv, info, inv, gro);
+  c->profile = prof;
+  if (e) {
+    (void)hipStreamSynchronize(L->st);
+    return e;
+  }
+  DEV(c, hipMemcpyAsync(c->h_noalloc, info, sizeof(int), hipMemcpyDeviceToHost, L->st));
+  DEV(c, hipMemcpyAsync(c->h_noalloc + 16, gro, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, L->st));
+  DEV(c, hipStreamSynchronize(L->st));
+  if (growth) *growth = growth_ratio((const unsigned long long *)(c->h_noalloc + 16));
+  if (c->h_noalloc[0]) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", c->h_noalloc[0] - 1);
   return 0;
 }
 

@@ -21,6 +21,8 @@
 //  symm_kernel      G <- G + G^H (src/util/DenseMatrix.cpp:477-486)
 #include "bem_common.h"
 
+#include <algorithm>
+
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
 
@@ -1630,6 +1632,11 @@ __global__ void zaxpy_kernel(double2 alpha, const double2 *__restrict__ src, int
   }
 }
 
+// p[0..n) <- 0 (32-bit words): clears counters without a memset node.
+__global__ void fill_zero_kernel(unsigned int *__restrict__ p, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0u;
+}
+
 // C (m x n, complex128) <- I - C on the square part (the residual of an
 // identity right-hand side).
 __global__ void identity_minus_kernel(double2 *__restrict__ Cm, int64_t ldc, int n) {
@@ -2054,6 +2061,15 @@ int bem_zdotc_cols(const double2 *G, int64_t ldg, const double2 *D, int64_t ldd,
 int bem_zaxpy(double2 alpha, const double2 *src, int64_t lds, double2 *dst, int64_t ldd, int m, int n, hipStream_t st) {
   if (m <= 0 || n <= 0) return 0;
   hipLaunchKernelGGL(zaxpy_kernel, copy_grid(m, n), dim3(256), 0, st, alpha, src, lds, dst, ldd, m, n);
+  return (int)hipGetLastError();
+}
+
+// bytes (a multiple of 4) of p <- 0 by a kernel.
+int bem_fill_zero(void *p, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / 4;
+  if (n == 0) return 0;
+  const unsigned int blocks = (unsigned int)std::min<size_t>((n + 255) / 256, 1024);
+  hipLaunchKernelGGL(fill_zero_kernel, dim3(blocks), dim3(256), 0, st, (unsigned int *)p, n);
   return (int)hipGetLastError();
 }
 

@@ -33,6 +33,7 @@ BEM_PROBE_COLS = 8
 BEM_PROBE_LIMIT = 1e-9
 BEM_REFINE_TOL = 1e-10
 BEM_REFINE_MAX = 3
+BEM_NOALLOC_MAX_N = 4096
 
 # src/util/SI.cpp:3-9
 SI_HBAR = 1.05457148e-34
@@ -82,7 +83,7 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_sweep", "bem_zgetrf", "bem_zgetrf_local", "bem_zgetrs", "bem_zgemm", "bem_device_alloc",
            "bem_device_free", "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
            "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune",
-           "bem_synchronize"]
+           "bem_synchronize", "bem_zgetrf_local_noalloc"]
 
 
 def load_library():
@@ -114,6 +115,7 @@ def load_library():
     L.bem_sweep.argtypes = [vp, vp, vp, P(_Req), i32, P(_Res)]
     L.bem_zgetrf.argtypes = [vp, vp, i64, i32, vp]
     L.bem_zgetrf_local.argtypes = [vp, vp, i64, i32, vp, vp]
+    L.bem_zgetrf_local_noalloc.argtypes = [vp, vp, i64, i32, vp, vp]
     L.bem_zgetrs.argtypes = [vp, vp, i64, i32, vp, vp, i64, i32]
     L.bem_zgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]
     L.bem_cgemm.argtypes = [vp, i32, i32, i32, vp, vp, i64, vp, i64, vp, vp, i64]

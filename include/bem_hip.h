@@ -213,6 +213,20 @@ int bem_zgetrf(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv);
  * within its block; growth (optional, host) receives max(1, max|L|) max|U| / max|A|.
  * Returns BEM_E_SINGULAR on a zero pivot. */
 int bem_zgetrf_local(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv, double *growth);
+/* Allocation-free variant of bem_zgetrf_local.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same kernels in the same order as bem_zgetrf_local, so the
+ * factors, interchanges, growth and status are the same; its workspace --
+ * the diagonal-block inverses, the zero-pivot flag and the growth maxima --
+ * sits in a fixed device array of the library (8 MiB + 256 bytes per GPU,
+ * held until the process ends), and the call allocates nothing.
+ * n <= BEM_NOALLOC_MAX_N, else BEM_E_UNSUPPORTED.  Every call on one GPU, from
+ * any context, shares that array and finds the last call's contents there:
+ * calls that overlap on one GPU race.  It runs on an idle lane of the
+ * context (BEM_E_UNSUPPORTED when every lane has a request in flight). */
+#define BEM_NOALLOC_MAX_N 4096
+int bem_zgetrf_local_noalloc(bem_ctx *ctx, void *A, int64_t lda, int n, int32_t *ipiv, double *growth);
 /* B (n x nrhs, ldb) <- A^-1 B from the factors of bem_zgetrf. */
 int bem_zgetrs(bem_ctx *ctx, const void *A, int64_t lda, int n, const int32_t *ipiv, void *B, int64_t ldb, int nrhs);
 /* C <- alpha A B + beta C (no transposes), alpha/beta as (re, im). */
