@@ -170,3 +170,67 @@ def test_heat_transfer_composition_closed_form():
         det = (a + g1) * (d + g2) - b * c
         ref = g1.real * g2.real * abs(b) ** 2 / abs(det) ** 2
         assert abs(S - ref) <= 1e-12 * abs(ref)
+
+
+# ------------------------------------------------ omega-tilde and Theta pins
+REF_SI = "/root/reference/src/util/SI.cpp"
+# CODATA 2018 values (hbar, k_B, e, c); the reference's constants are older
+# roundings of these, within 1e-6
+CODATA = {"hbar": 1.054571817e-34, "kb": 1.380649e-23, "eV": 1.602176634e-19, "c": 2.99792458e8}
+
+
+def test_si_constants_pinned():
+    """The SI constants of the oracle and of the library's binding are the
+    reference's (src/util/SI.cpp:3-6, parsed from the file where the reference
+    tree is mounted) and lie within 1e-6 of CODATA: a dropped digit or a
+    swapped constant fails."""
+    import os
+    import re
+    from bem_amd import runtime as R
+    ours = {"hbar": O.SI_HBAR, "kb": O.SI_KB, "eV": O.SI_EV, "c": O.SI_C}
+    assert ours == {"hbar": R.SI_HBAR, "kb": R.SI_KB, "eV": R.SI_EV, "c": R.SI_C}
+    for k, v in ours.items():
+        assert abs(v / CODATA[k] - 1) < 1e-6, k
+    if os.path.exists(REF_SI):
+        text = open(REF_SI).read()
+        for k, v in ours.items():
+            m = re.search(r"Real\s+SI::" + k + r"\s*=\s*([-+0-9.eE]+)\s*;", text)
+            assert m and float(m.group(1)) == v, k
+
+
+def test_omega_tilde_closed_form():
+    """omega-tilde = omega a / c (src/HeatTransfer.cpp:10-11) is 2 pi a /
+    lambda: a 1 eV photon has lambda = 1.23984e-6 m, so at a = 1e-6 m
+    omega-tilde = 5.06773 (an h for hbar, or a missing a, fails by 2 pi or 1e6);
+    the oracle's and the binding's copies agree bitwise and scale linearly in
+    eV and in a."""
+    from bem_amd import runtime as R
+    lam = 1.2398419843320026e-6        # h c / (1 eV), CODATA 2018
+    want = 2 * np.pi * 1e-6 / lam
+    got = O.omega_tilde(1.0, 1e-6)
+    assert abs(got / want - 1) < 1e-6
+    for eV in (0.08, 0.11, 0.14):
+        for a in (1e-6, 2.5e-7):
+            w = O.omega_tilde(eV, a)
+            assert w == R.omega_tilde(eV, a)
+            assert abs(w / (want * eV * a / 1e-6) - 1) < 1e-6
+
+
+def test_theta_pinned():
+    """Theta (src/HeatTransfer.cpp:3-6) with the dimensionless omega-tilde, as
+    CalcHeatTransfer calls it: at the workloads' frequencies x = hbar w / (k_B
+    T) is about 1e-14, so Theta is k_B T to within the ~1% that exp(x) - 1
+    loses to cancellation (3% allowed; an SI omega inside Theta gives x ~ 4 and
+    fails); Theta(w, 0) = 0; and at x = 1 (w = k_B T / hbar) the expression is
+    k_B T / (e - 1), which an error in hbar inside Theta would move."""
+    from bem_amd import runtime as R
+    for eV in np.linspace(0.08, 0.14, 7):
+        w = O.omega_tilde(eV, 1e-6)
+        for T in (300.0, 77.0):
+            th = O.theta(w, T)
+            assert th == R.theta(w, T)
+            assert abs(th / (O.SI_KB * T) - 1) < 0.03
+        assert O.theta(w, 0.0) == 0.0 and R.theta(w, 0.0) == 0.0
+    T = 300.0
+    w1 = O.SI_KB * T / O.SI_HBAR
+    assert abs(O.theta(w1, T) / (O.SI_KB * T / (np.e - 1)) - 1) < 1e-12
