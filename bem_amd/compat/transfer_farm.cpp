@@ -17,7 +17,9 @@
 //
 // Restart: lines already in ./output are read first and the jobs whose
 // first five columns they hold are not run again, so an interrupted sweep
-// resumes where it stopped.
+// resumes where it stopped.  ./output records only the x component of a gap,
+// so a restart is refused when gaps.txt holds distinct gaps with equal x
+// (their lines could not be told apart).
 //
 // usage: transfer_farm [fp64|mixed]
 #include <cstdint>
@@ -71,6 +73,18 @@ struct Job {
   uint64_t ticket;
 };
 
+const char *status_name(int st) {
+  switch (st) {
+    case BEM_E_INVALID: return "invalid argument";
+    case BEM_E_DEVICE: return "device error";
+    case BEM_E_NOMEM: return "out of device memory or over the HBM budget";
+    case BEM_E_SINGULAR: return "singular system";
+    case BEM_E_MESH: return "mesh rejected";
+    case BEM_E_UNSUPPORTED: return "unsupported size or mode";
+    default: return "error";
+  }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -109,6 +123,17 @@ int main(int argc, char **argv) {
           done.insert(key(v[0], v[1], v[2], v[3], v[4]));
       }
     }
+    if (!done.empty())
+      for (size_t i = 0; i < gaps.size(); i++)
+        for (size_t k = i + 1; k < gaps.size(); k++)
+          if (gaps[i][0] == gaps[k][0] && (gaps[i][1] != gaps[k][1] || gaps[i][2] != gaps[k][2])) {
+            std::fprintf(stderr,
+                         "transfer_farm: gaps.txt holds distinct gaps with the same x (%g %g %g and %g %g %g); "
+                         "./output records only x, so a restart cannot tell their jobs apart: move ./output away "
+                         "to run all jobs\n",
+                         gaps[i][0], gaps[i][1], gaps[i][2], gaps[k][0], gaps[k][1], gaps[k][2]);
+            return 2;
+          }
     std::vector<Job> jobs;
     for (auto &T : temps)
       for (auto &a : as)
@@ -171,8 +196,9 @@ int main(int argc, char **argv) {
         finished[i] = true;
         left--;
         if (rc != 0 || r.status != 0) {
-          std::fprintf(stderr, "transfer_farm: job a=%g f=%g gap=%g failed: %s\n", jobs[i].a, jobs[i].f,
-                       jobs[i].gap[0], bem_last_error(ctx));
+          const int st = r.status ? r.status : rc;   // the ticket's own status, not the context's last message
+          std::fprintf(stderr, "transfer_farm: job a=%g f=%g gap=%g %g %g failed: status %d (%s)\n", jobs[i].a,
+                       jobs[i].f, jobs[i].gap[0], jobs[i].gap[1], jobs[i].gap[2], st, status_name(st));
           failures++;
           continue;
         }

@@ -144,3 +144,24 @@ def test_transfer_farm_matches_oracle(tmp_path):
     assert out.returncode == 0, out.stderr
     eV, P = out.stdout.split()
     assert float(eV) == 0.11 and float(P) == got[0.11]
+
+
+def test_transfer_farm_refuses_ambiguous_restart(tmp_path):
+    """./output keeps only a gap's x component: with distinct gaps of equal x in
+    gaps.txt a restart could skip jobs that never ran, so transfer_farm
+    refuses it (before any device work) and says why."""
+    from bem_amd import mesh
+    farm = os.path.join(COMPAT, "transfer_farm")
+    if not os.path.exists(farm):
+        pytest.skip("compat drivers not built")
+    s, c = mesh.icosphere(1), mesh.cube(2)
+    _write_x3d(tmp_path / "body1.x3d", s, "SiC")
+    _write_x3d(tmp_path / "body2.x3d", c, "SiN")
+    (tmp_path / "a.txt").write_text("1e-6\n")
+    (tmp_path / "gaps.txt").write_text("0 0 2.1\n0 0.5 2.1\n")
+    (tmp_path / "freqs.txt").write_text("0.095\n")
+    (tmp_path / "temps.txt").write_text("300 0\n")
+    (tmp_path / "output").write_text("1e-06 0.095 300 0 0 1.0e-24\n")
+    out = subprocess.run([farm], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert out.returncode == 2
+    assert "same x" in out.stderr and "move ./output away" in out.stderr
