@@ -267,8 +267,9 @@ __device__ __forceinline__ void outer_far(const Medium &md, const double *r, dou
 }
 
 // EHNonSmoothIntegral (src/Source.cpp:16-86) on the frame half of triangle S.
+// ILn1 (optional) receives sum L_e m_e, which the dipole source needs.
 __device__ __forceinline__ void eh_nonsmooth(const TriGeom &S, const double *rt, double &ISn3h, double &ISn1, double &IS1,
-                             double *IL1, double *IL3) {
+                             double *IL1, double *IL3, double *ILn1 = nullptr) {
   double tol = 1e-6 * S.elen;
   double p0[3], p1[3], p2[3];
   for (int c = 0; c < 3; c++) { p0[c] = rt[c] - S.p0[c]; p1[c] = rt[c] - S.p1[c]; p2[c] = rt[c] - S.p2[c]; }
@@ -316,6 +317,8 @@ __device__ __forceinline__ void eh_nonsmooth(const TriGeom &S, const double *rt,
     IL1[c] = IL1_01 * S.m01[c] + IL1_12 * S.m12[c] + IL1_20 * S.m20[c];
     IL3[c] = IL3_01 * S.m01[c] + IL3_12 * S.m12[c] + IL3_20 * S.m20[c];
   }
+  if (ILn1)
+    for (int c = 0; c < 3; c++) ILn1[c] = L01 * S.m01[c] + L12 * S.m12[c] + L20 * S.m20[c];
 }
 
 // LKEdgeIntegral (src/Impedance.cpp:13-73) on the frame half of triangle T.
@@ -442,6 +445,232 @@ __device__ __forceinline__ void store_tile(double *tiles, size_t npairs, size_t 
 }
 
 }  // namespace
+
+// ------------------------------------------------------------ dipole source
+
+// Per triangle of a source's region, the two dot products of
+// DipolePointSource::EHTriangle (src/Source.cpp:121-165) for each half on
+// the triangle, before the per-RWG factor c = +-len/(4 pi) (and Zeps for E):
+//   dE_i = j . (E1 - k^2 (Gphi_i + IL13 + ISn11 (rho - p0_i)))
+//   dH_i = j . -(DGxphi_i + K31 x (p - p0_i))
+// The smooth part is the 21-point rule on the frame half with the smooth
+// Green's function and its gradient (src/GreensFunction.cpp:54-69, R = |p -
+// r'|), the singular part EHNonSmoothIntegral at p.  tb: nsrc x T x 3 pairs
+// (dE, dH); blockIdx.y = source.
+__global__ void __launch_bounds__(256) dipole_tri_kernel(const TriGeom *__restrict__ tg, int T,
+                                                         const DipoleSrc *__restrict__ src,
+                                                         double2 *__restrict__ tb) {
+  const DipoleSrc &S = src[blockIdx.y];
+  const int t = S.t_lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= S.t_hi) return;
+  const TriGeom &G = tg[t];
+  const Medium md = S.md;
+  const double *p = S.p, *jv = S.j;
+  const int nh = G.nh;
+  double DGr[3] = {0, 0, 0}, DGi[3] = {0, 0, 0};
+  double Gpr[3][3] = {}, Gpi[3][3] = {}, Xr[3][3] = {}, Xi[3][3] = {};
+  for (int q = 0; q < 21; q++) {
+    double v[3];
+    quad_point(G, BEM_QUAD_OFF_21 + q, v);
+    const double w = c_quad[BEM_QUAD_OFF_21 + q][2];
+    double d[3] = {p[0] - v[0], p[1] - v[1], p[2] - v[2]};
+    double R2 = dot3(d, d), R, invR;
+    root_and_inverse(R2, R, invR);
+    double ar = md.ik_re * R, ai = md.ik_im * R;
+    double er, ei;
+    cexp_i(ar, ai, er, ei);
+    // gs = (e - 1)/R - k2 R ; q = (e (1 - ikR) + k2 R^2 - 1)/R^3 ; grad = q d
+    C2 gs = cmk((er - 1.0) * invR - md.k2_re * R, ei * invR - md.k2_im * R);
+    C2 tq = cmul(cmk(er, ei), cmk(1.0 - ar, -ai));
+    double invR3 = invR * invR * invR;
+    C2 qq = cmk((tq.re + md.k2_re * R2 - 1.0) * invR3, (tq.im + md.k2_im * R2) * invR3);
+    gs.re *= w;
+    gs.im *= w;
+    double ggr[3], ggi[3];
+    for (int c = 0; c < 3; c++) {
+      ggr[c] = w * (qq.re * d[c]);
+      ggi[c] = w * (qq.im * d[c]);
+      DGr[c] += ggr[c];
+      DGi[c] += ggi[c];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      if (i >= nh) break;
+      double phi[3] = {0.5 * (v[0] - G.hp0[i][0]), 0.5 * (v[1] - G.hp0[i][1]), 0.5 * (v[2] - G.hp0[i][2])};
+      double xr[3], xi[3];
+      cross3(ggr, phi, xr);
+      cross3(ggi, phi, xi);
+      for (int c = 0; c < 3; c++) {
+        Gpr[i][c] += gs.re * phi[c];
+        Gpi[i][c] += gs.im * phi[c];
+        Xr[i][c] += xr[c];
+        Xi[i][c] += xi[c];
+      }
+    }
+  }
+  double ISn3h, ISn1, IS1, IL1[3], IL3[3], ILn1[3];
+  eh_nonsmooth(G, p, ISn3h, ISn1, IS1, IL1, IL3, ILn1);
+  const C2 k2 = cmk(md.k2_re, md.k2_im), ksq = cmk(md.ksq_re, md.ksq_im);
+  const double Ai = G.Ai, Ai2 = 0.5 * Ai;
+  const C2 c13 = cmk(BEM_ONETHIRD * k2.re, BEM_ONETHIRD * k2.im);
+  const C2 ISn11 = cmk(Ai2 * (ISn1 + k2.re * IS1), Ai2 * (k2.im * IS1));
+  const double h = (p[0] - G.p0[0]) * G.n[0] + (p[1] - G.p0[1]) * G.n[1] + (p[2] - G.p0[2]) * G.n[2];
+  const C2 kh = cmk(k2.re * h * ISn1, k2.im * h * ISn1);   // (k2 h) ISn1
+  double IL13r[3], IL13i[3], K31r[3], K31i[3], E1r[3], E1i[3], rho[3];
+  for (int c = 0; c < 3; c++) {
+    IL13r[c] = Ai2 * (IL1[c] + c13.re * IL3[c]);
+    IL13i[c] = Ai2 * (c13.im * IL3[c]);
+    rho[c] = p[c] - h * G.n[c];
+    const double u = IL1[c] - h * ISn1 * G.n[c];
+    K31r[c] = Ai2 * ((ILn1[c] + ISn3h * G.n[c]) + k2.re * u);
+    K31i[c] = Ai2 * (k2.im * u);
+    E1r[c] = DGr[c] + Ai * ((ISn3h - kh.re) * G.n[c] + (ILn1[c] + k2.re * IL1[c]));
+    E1i[c] = DGi[c] + Ai * ((-kh.im) * G.n[c] + k2.im * IL1[c]);
+  }
+  double2 *o = tb + ((size_t)blockIdx.y * T + t) * 6;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (i >= nh) break;
+    double dEr = 0, dEi = 0, dHr = 0, dHi = 0;
+    double pq[3] = {p[0] - G.hp0[i][0], p[1] - G.hp0[i][1], p[2] - G.hp0[i][2]};
+    double kr[3], ki[3];
+    cross3(K31r, pq, kr);
+    cross3(K31i, pq, ki);
+    for (int c = 0; c < 3; c++) {
+      const double rc = rho[c] - G.hp0[i][c];
+      const double ar = (Gpr[i][c] + IL13r[c]) + (ISn11.re * rc);
+      const double ai = (Gpi[i][c] + IL13i[c]) + (ISn11.im * rc);
+      const double er = E1r[c] - (ksq.re * ar - ksq.im * ai);
+      const double eim = E1i[c] - (ksq.re * ai + ksq.im * ar);
+      dEr += jv[c] * er;
+      dEi += jv[c] * eim;
+      dHr -= jv[c] * (Xr[i][c] + kr[c]);
+      dHi -= jv[c] * (Xi[i][c] + ki[c]);
+    }
+    o[2 * i] = make_double2(dEr, dEi);
+    o[2 * i + 1] = make_double2(dHr, dHi);
+  }
+}
+
+// B(2e) = sum over the RWG's halves in the region of (c Zeps) dE,
+// B(2e+1) = sum of c dH, c = len/(4 pi), negated when "external" differs
+// from "positive half" (src/Source.cpp:150-161).  B: column blockIdx.y of a
+// 2E x nsrc matrix (leading dimension ldb).
+__global__ void __launch_bounds__(256) dipole_gather_kernel(const RwgRec *__restrict__ rw, int E, int T,
+                                                            const DipoleSrc *__restrict__ src,
+                                                            const double2 *__restrict__ tb,
+                                                            double2 *__restrict__ B, int64_t ldb) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const DipoleSrc &S = src[blockIdx.y];
+  const RwgRec r = rw[e];
+  const double2 *base = tb + (size_t)blockIdx.y * T * 6;
+  const double c0 = BEM_ONEOVER4PI * r.len;
+  double2 bE = make_double2(0.0, 0.0), bH = make_double2(0.0, 0.0);
+  const int th[2] = {r.tp, r.tn}, sl[2] = {r.sp, r.sn};
+  for (int hh = 0; hh < 2; hh++) {
+    const int t = th[hh];
+    if (t < S.t_lo || t >= S.t_hi || sl[hh] >= 3) continue;
+    const double c = (S.external != 0) != (hh == 0) ? -c0 : c0;
+    const double2 dE = base[(size_t)t * 6 + 2 * sl[hh]], dH = base[(size_t)t * 6 + 2 * sl[hh] + 1];
+    const double czr = c * S.md.Zeps_re, czi = c * S.md.Zeps_im;
+    bE.x += czr * dE.x - czi * dE.y;
+    bE.y += czr * dE.y + czi * dE.x;
+    bH.x += c * dH.x;
+    bH.y += c * dH.y;
+  }
+  double2 *col = B + (int64_t)blockIdx.y * ldb;
+  col[2 * (int64_t)e] = bE;
+  col[2 * (int64_t)e + 1] = bH;
+}
+
+// ------------------------------------------------------------------- flux
+
+#define FLUX_BLOCK 256
+// Per triangle, dS of flux(top, t, X) (src/Flux.cpp:100-219):
+//   j_k, m_k   vertex currents from the three RWGs, m / omega, j conjugated
+//   dS = n . Re(j1 x m1 + j2 x m2 + j3 x m3 + (j1+j2+j3) x (m1+m2+m3)) / (48 A2)
+// then a block sum in a fixed tree order.  part: one sum per block, for each
+// column (blockIdx.y) of X (2E x ncol, leading dimension ldx).
+__global__ void __launch_bounds__(FLUX_BLOCK) flux_kernel(const FluxTri *__restrict__ ft, int nt, double omega,
+                                                          const double2 *__restrict__ X, int64_t ldx,
+                                                          double *__restrict__ part) {
+  __shared__ double red[FLUX_BLOCK];
+  const int t = blockIdx.x * FLUX_BLOCK + threadIdx.x;
+  double dS = 0.0;
+  if (t < nt) {
+    const FluxTri &F = ft[t];
+    const double2 *x = X + (int64_t)blockIdx.y * ldx;
+    double v12[3], v23[3], v31[3], n[3];
+    for (int c = 0; c < 3; c++) {
+      v12[c] = F.P[1][c] - F.P[0][c];
+      v23[c] = F.P[2][c] - F.P[1][c];
+      v31[c] = F.P[0][c] - F.P[2][c];
+    }
+    double v13[3] = {F.P[2][0] - F.P[0][0], F.P[2][1] - F.P[0][1], F.P[2][2] - F.P[0][2]};
+    cross3(v12, v13, n);
+    const double A2 = sqrt(dot3(n, n));
+    for (int c = 0; c < 3; c++) n[c] *= 1.0 / A2;
+    // currents at the three vertices: jr/ji, mr/mi [vertex][component]
+    double jr[3][3] = {}, ji[3][3] = {}, mr[3][3] = {}, mi[3][3] = {};
+    const double *ve[3][2] = {{v31, v23}, {v12, v31}, {v23, v12}};   // (+ at vertex k, - at vertex k+1)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const double2 xj = x[2 * (int64_t)F.e[k]], xm = x[2 * (int64_t)F.e[k] + 1];
+      const double cj_r = F.c[k] * xj.x, cj_i = F.c[k] * xj.y, cm_r = F.c[k] * xm.x, cm_i = F.c[k] * xm.y;
+      const int a = k, b = (k + 1) % 3;
+      for (int c = 0; c < 3; c++) {
+        jr[a][c] += cj_r * ve[k][0][c];
+        ji[a][c] += cj_i * ve[k][0][c];
+        mr[a][c] += cm_r * ve[k][0][c];
+        mi[a][c] += cm_i * ve[k][0][c];
+        jr[b][c] -= cj_r * ve[k][1][c];
+        ji[b][c] -= cj_i * ve[k][1][c];
+        mr[b][c] -= cm_r * ve[k][1][c];
+        mi[b][c] -= cm_i * ve[k][1][c];
+      }
+    }
+    const double oi = 1.0 / omega;
+    double s[3] = {0, 0, 0}, Jr[3] = {0, 0, 0}, Ji[3] = {0, 0, 0}, Mr[3] = {0, 0, 0}, Mi[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+      for (int c = 0; c < 3; c++) {
+        mr[k][c] *= oi;
+        mi[k][c] *= oi;
+        ji[k][c] = -ji[k][c];
+        Jr[c] += jr[k][c];
+        Ji[c] += ji[k][c];
+        Mr[c] += mr[k][c];
+        Mi[c] += mi[k][c];
+      }
+      // Re(j x m) = jr x mr - ji x mi
+      double a1[3], a2[3];
+      cross3(jr[k], mr[k], a1);
+      cross3(ji[k], mi[k], a2);
+      for (int c = 0; c < 3; c++) s[c] += a1[c] - a2[c];
+    }
+    double a1[3], a2[3];
+    cross3(Jr, Mr, a1);
+    cross3(Ji, Mi, a2);
+    for (int c = 0; c < 3; c++) s[c] += a1[c] - a2[c];
+    dS = dot3(n, s) / (48.0 * A2);
+  }
+  red[threadIdx.x] = dS;
+  __syncthreads();
+  for (int w = FLUX_BLOCK / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+// flux[col] = sum of the column's block sums in block order.
+__global__ void flux_finish_kernel(const double *__restrict__ part, int nblk, int ncol, double *__restrict__ out) {
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= ncol) return;
+  double S = 0.0;
+  for (int b = 0; b < nblk; b++) S += part[(size_t)col * nblk + b];
+  out[col] = S;
+}
 
 // ---------------------------------------------------------------------------
 // Dense pair grid: every 6-point pair computed here; 36-point pairs queued.
@@ -847,6 +1076,38 @@ __global__ void q36_check_kernel(const unsigned int *count, unsigned int cap, in
 
 int bem_launch_q36_check(const unsigned int *count, unsigned int cap, int *flag, hipStream_t st) {
   hipLaunchKernelGGL(q36_check_kernel, dim3(1), dim3(64), 0, st, count, cap, flag);
+  return (int)hipGetLastError();
+}
+
+// Dipole right-hand sides: src (nsrc device records), tb (nsrc x T x 6
+// double2 workspace), B (2E x nsrc, leading dimension ldb).  tmax: the
+// largest t_hi - t_lo over the sources.
+int bem_launch_dipole_rhs(const TriGeom *tg, int T, const RwgRec *rw, int E, const void *src, int nsrc, int tmax,
+                          double2 *tb, double2 *B, int64_t ldb, hipStream_t st) {
+  if (nsrc <= 0) return 0;
+  if (tmax > 0) {
+    dim3 g1((tmax + 255) / 256, nsrc);
+    hipLaunchKernelGGL(dipole_tri_kernel, g1, dim3(256), 0, st, tg, T, (const DipoleSrc *)src, tb);
+  }
+  if (E > 0) {
+    dim3 g2((E + 255) / 256, nsrc);
+    hipLaunchKernelGGL(dipole_gather_kernel, g2, dim3(256), 0, st, rw, E, T, (const DipoleSrc *)src, tb, B, ldb);
+  }
+  return (int)hipGetLastError();
+}
+
+// Flux of ncol solution columns through a surface of nt triangles: part
+// holds ncol x ceil(nt / 256) block sums, out ncol values (device).
+size_t bem_flux_partials(int nt, int ncol) { return (size_t)((nt + FLUX_BLOCK - 1) / FLUX_BLOCK) * ncol; }
+
+int bem_launch_flux(const void *ft, int nt, double omega, const double2 *X, int64_t ldx, int ncol, double *part,
+                    double *out, hipStream_t st) {
+  if (ncol <= 0) return 0;
+  const int nblk = (nt + FLUX_BLOCK - 1) / FLUX_BLOCK;
+  if (nblk > 0)
+    hipLaunchKernelGGL(flux_kernel, dim3(nblk, ncol), dim3(FLUX_BLOCK), 0, st, (const FluxTri *)ft, nt, omega, X,
+                       ldx, part);
+  hipLaunchKernelGGL(flux_finish_kernel, dim3((ncol + 63) / 64), dim3(64), 0, st, part, nblk, ncol, out);
   return (int)hipGetLastError();
 }
 

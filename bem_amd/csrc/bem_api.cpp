@@ -120,6 +120,11 @@ size_t bem_zgemm_skinny_partials(int m, int r, int K);
 int bem_fill_zero(void *p, size_t bytes, hipStream_t st);
 int bem_zgemm_skinny(int m, int r, int K, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                      double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st);
+int bem_launch_dipole_rhs(const TriGeom *tg, int T, const RwgRec *rw, int E, const void *src, int nsrc, int tmax,
+                          double2 *tb, double2 *B, int64_t ldb, hipStream_t st);
+size_t bem_flux_partials(int nt, int ncol);
+int bem_launch_flux(const void *ft, int nt, double omega, const double2 *X, int64_t ldx, int ncol, double *part,
+                    double *out, hipStream_t st);
 }
 
 // src/util/SI.cpp:3-9
@@ -2232,6 +2237,67 @@ static int scratch_address(int device, void **out) {
   return 0;
 }
 
+// ------------------------------------------------- dipole sources and flux
+// Right-hand sides of ns dipoles on tree d (B: 2E x ns, leading dimension
+// ldb, device): toff[b]..toff[b+1] are body b's triangles, eps/mu (re, im)
+// per body; the exterior is vacuum.
+static int dipole_rhs(bem_ctx *c, Lane &L, const DevTree &d, const std::vector<int> &toff, const double *eps,
+                      const double *mu, double omega, int ns, const bem_dipole *src, double2 *B, int64_t ldb) {
+  const int nb = (int)toff.size() - 1;
+  std::vector<DipoleSrc> h(ns);
+  int tmax = 0;
+  for (int s = 0; s < ns; s++) {
+    const bem_dipole &q = src[s];
+    if (q.region < -1 || q.region >= nb) return fail(c, BEM_E_INVALID, "dipole %d: region %d", s, (int)q.region);
+    DipoleSrc &o = h[s];
+    std::memset(&o, 0, sizeof o);
+    for (int k = 0; k < 3; k++) {
+      o.p[k] = q.p[k];
+      o.j[k] = q.j[k];
+    }
+    if (q.region < 0) {
+      o.md = make_medium(1.0, 0.0, 1.0, 0.0, omega);
+      o.t_lo = 0;
+      o.t_hi = d.T;
+      o.external = 1;
+    } else {
+      const int b = q.region;
+      o.md = make_medium(eps[2 * b], eps[2 * b + 1], mu[2 * b], mu[2 * b + 1], omega);
+      o.t_lo = toff[b];
+      o.t_hi = toff[b + 1];
+      o.external = 0;
+    }
+    tmax = std::max(tmax, o.t_hi - o.t_lo);
+  }
+  DipoleSrc *dsrc;
+  double2 *tb;
+  CHK(ws_get(c, L, "dip_src", sizeof(DipoleSrc) * (size_t)std::max(ns, 1), (void **)&dsrc));
+  CHK(ws_get(c, L, "dip_tb", sizeof(double2) * 6 * (size_t)std::max(ns, 1) * std::max(d.T, 1), (void **)&tb));
+  DEV(c, hipMemcpyAsync(dsrc, h.data(), sizeof(DipoleSrc) * ns, hipMemcpyHostToDevice, L.st));
+  int e = bem_launch_dipole_rhs(d.tg, d.T, d.rw, d.E, dsrc, ns, tmax, tb, B, ldb, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "dipole launch: %s", hipGetErrorString((hipError_t)e));
+  // the host records must outlive the copy
+  DEV(c, hipStreamSynchronize(L.st));
+  return 0;
+}
+
+// Flux of ncol columns of X (device) through the surface ft: out (host).
+static int flux_columns(bem_ctx *c, Lane &L, const std::vector<FluxTri> &ft, double omega,
+                        const double2 *X, int64_t ldx, int ncol, double *out) {
+  const int nt = (int)ft.size();
+  FluxTri *dft;
+  double *part, *dout;
+  CHK(ws_get(c, L, "flux_tri", sizeof(FluxTri) * (size_t)std::max(nt, 1), (void **)&dft));
+  CHK(ws_get(c, L, "flux_part", sizeof(double) * std::max<size_t>(bem_flux_partials(nt, ncol), 1), (void **)&part));
+  CHK(ws_get(c, L, "flux_out", sizeof(double) * (size_t)std::max(ncol, 1), (void **)&dout));
+  if (nt) DEV(c, hipMemcpyAsync(dft, ft.data(), sizeof(FluxTri) * nt, hipMemcpyHostToDevice, L.st));
+  int e = bem_launch_flux(dft, nt, omega, X, ldx, ncol, part, dout, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "flux launch: %s", hipGetErrorString((hipError_t)e));
+  DEV(c, hipMemcpyAsync(out, dout, sizeof(double) * ncol, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  return 0;
+}
+
 // ------------------------------------------------------------------- C ABI
 extern "C" {
 
@@ -2499,6 +2565,149 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   if (!e && hflag[1]) e = fail(c, BEM_E_DEVICE, "36-point pair queue overflowed");
   free_allocs(c, allocs, tb);
   return e;
+}
+
+int bem_dipole_rhs(bem_ctx *c, int nb, bem_body *const *bodies, const double *shifts, const double *eps,
+                   const double *mu, double omega, int ns, const bem_dipole *src, void *B, int64_t ldb, int flags) {
+  if (!c || nb < 1 || nb > 2 || !bodies || !shifts || !eps || !mu || ns < 0 || (ns && (!src || !B)))
+    return fail(c, BEM_E_INVALID, "bad arguments");
+  if (!(omega > 0)) return fail(c, BEM_E_INVALID, "omega must be positive");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  std::vector<const bemgeo::BodyMesh *> bm;
+  for (int i = 0; i < nb; i++) bm.push_back(&bodies[i]->mesh);
+  bemgeo::Tree t;
+  bemgeo::build_tree(bm, shifts, t);
+  const int M = 2 * t.E;
+  if (ldb < M) return fail(c, BEM_E_INVALID, "ldb %lld < %d", (long long)ldb, M);
+  if (ns == 0) return 0;
+  t.close.clear();
+  DevTree d;
+  std::vector<void *> allocs;
+  size_t tb = 0;
+  int e = upload_tree(c, t, d, allocs, tb);
+  std::vector<int> toff(t.tri_off);
+  toff.push_back(t.T);
+  const bool dev = flags & BEM_ASM_DEVICE_PTR;
+  double2 *Bd = dev ? (double2 *)B : nullptr;
+  const int64_t ld = dev ? ldb : M;
+  if (!e && !dev) e = ws_get(c, L, "dip_B", sizeof(double2) * (size_t)M * ns, (void **)&Bd);
+  if (!e) e = dipole_rhs(c, L, d, toff, eps, mu, omega, ns, src, Bd, ld);
+  if (!e && !dev && hipMemcpy2DAsync(B, sizeof(double2) * ldb, Bd, sizeof(double2) * M, sizeof(double2) * M, ns,
+                                     hipMemcpyDeviceToHost, L.st) != hipSuccess)
+    e = fail(c, BEM_E_DEVICE, "copy of B failed");
+  if (!e && hipStreamSynchronize(L.st) != hipSuccess) e = fail(c, BEM_E_DEVICE, "dipole sources failed on the device");
+  free_allocs(c, allocs, tb);
+  return e;
+}
+
+int bem_flux(bem_ctx *c, int nb, bem_body *const *bodies, const double *shifts, int body, double omega,
+             const void *X, int64_t ldx, int ncol, int flags, double *flux) {
+  if (!c || nb < 1 || nb > 2 || !bodies || !shifts || body < 0 || body >= nb || ncol < 0 || (ncol && (!X || !flux)))
+    return fail(c, BEM_E_INVALID, "bad arguments");
+  if (!(omega > 0)) return fail(c, BEM_E_INVALID, "omega must be positive");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  int off = 0, E = 0;
+  for (int i = 0; i < nb; i++) {
+    if (i < body) off += bodies[i]->mesh.ne;
+    E += bodies[i]->mesh.ne;
+  }
+  const int M = 2 * E;
+  if (ldx < M) return fail(c, BEM_E_INVALID, "ldx %lld < %d", (long long)ldx, M);
+  if (ncol == 0) return 0;
+  std::vector<FluxTri> ft;
+  if (bemgeo::flux_table(bodies[body]->mesh, shifts + 3 * body, off, ft))
+    return fail(c, BEM_E_INVALID, "body %d has a triangle edge without an RWG", body);
+  const double2 *Xd = (const double2 *)X;
+  int64_t ld = ldx;
+  if (!(flags & BEM_ASM_DEVICE_PTR)) {
+    double2 *tmp;
+    CHK(ws_get(c, L, "flux_X", sizeof(double2) * (size_t)M * ncol, (void **)&tmp));
+    DEV(c, hipMemcpy2DAsync(tmp, sizeof(double2) * M, X, sizeof(double2) * ldx, sizeof(double2) * M, ncol,
+                            hipMemcpyHostToDevice, L.st));
+    Xd = tmp;
+    ld = M;
+  }
+  return flux_columns(c, L, ft, omega, Xd, ld, ncol, flux);
+}
+
+int bem_dipole_flux(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *q, int ns, const bem_dipole *src,
+                    double *flux) {
+  if (!c || !b1 || !b2 || !q || ns < 0 || (ns && (!src || !flux))) return fail(c, BEM_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  CHK(check_request(c, q));
+  if (ns == 0) return 0;
+  stats_reset(c);
+  PairGeom *g;
+  CHK(pair_geom(c, b1, b2, q->gap, &g));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  const double omega = q->eV * SI_EV / SI_HBAR * q->a / SI_C;
+  const int T1 = g->T1, T = g->top.T, E1 = g->E1, E = g->top.E;
+  const int m1 = 2 * E1, m2 = 2 * (E - E1), M = m1 + m2;
+  const Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
+  double2 *W, *Bx, *inv;
+  int *ipiv, *info;
+  CHK(ws_get(c, L, "dflux_W", sizeof(double2) * (size_t)M * M, (void **)&W));
+  CHK(ws_get(c, L, "dflux_B", sizeof(double2) * (size_t)M * ns, (void **)&Bx));
+  CHK(ws_get(c, L, "dflux_ipiv", sizeof(int) * (size_t)(M + 64), (void **)&ipiv));
+  CHK(ws_get(c, L, "dflux_inv", inv_bytes(M), (void **)&inv));
+  CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
+  DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
+  // diagonal blocks: Z_ext(b, b) + Z_int(b) on the body's own tree
+  bem_body *bs[2] = {b1, b2};
+  const double *eps[2] = {q->eps1, q->eps2}, *mu[2] = {q->mu1, q->mu2};
+  for (int b = 0; b < 2; b++) {
+    const DevTree *dt;
+    CHK(body_tree(c, bs[b], &dt));
+    const int Tb = dt->T, o = b ? m1 : 0;
+    const size_t np = (size_t)Tb * Tb;
+    double *tiles;
+    CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(np, 1), (void **)&tiles));
+    const Medium mint = make_medium(eps[b][0], eps[b][1], mu[b][0], mu[b][1], omega);
+    CHK(assemble_tiles(c, L, *dt, mext, tiles, 0, Tb, 0, Tb, info + 1));
+    CHK(gather(c, L, tiles, np, Tb, 0, 0, dt->rw, 0, dt->E, 0, dt->E, &mext, W, M, o, o, 0, nullptr, 0, 0, 0));
+    CHK(assemble_tiles(c, L, *dt, mint, tiles, 0, Tb, 0, Tb, info + 1));
+    CHK(gather(c, L, tiles, np, Tb, 0, 0, dt->rw, 0, dt->E, 0, dt->E, &mint, W, M, o, o, 1, nullptr, 0, 0, 0));
+  }
+  // cross blocks: Z_ext(1, 2) assembled, Z_ext(2, 1) = S Z_ext(1, 2)^T S
+  {
+    const size_t npc = (size_t)T1 * (T - T1);
+    double *tiles;
+    CHK(ws_get(c, L, "tiles", sizeof(double) * BEM_TILE_PLANES * std::max<size_t>(npc, 1), (void **)&tiles));
+    CHK(assemble_tiles(c, L, g->top, mext, tiles, 0, T1, T1, T, info + 1));
+    CHK(gather(c, L, tiles, npc, T - T1, 0, T1, g->top.rw, 0, E1, E1, E, &mext, W, M, 0, m1, 0, nullptr, 0, 0, 0));
+    int e = bem_signed_transpose(W + (size_t)m1 * M, M, m1, m2, W + m1, M, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "transpose launch: %d", e);
+  }
+  std::vector<int> toff = {0, T1, T};
+  const double epsv[4] = {q->eps1[0], q->eps1[1], q->eps2[0], q->eps2[1]};
+  const double muv[4] = {q->mu1[0], q->mu1[1], q->mu2[0], q->mu2[1]};
+  CHK(dipole_rhs(c, L, g->top, toff, epsv, muv, omega, ns, src, Bx, M));
+  CHK(lu_factor(c, L, W, M, M, ipiv, info, inv));
+  int hinfo[4] = {0, 0, 0, 0};
+  DEV(c, hipMemcpyAsync(hinfo, info, sizeof hinfo, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  if (hinfo[1]) return fail(c, BEM_E_DEVICE, "36-point pair queue overflowed");
+  if (hinfo[0]) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", hinfo[0] - 1);
+  CHK(lu_solve_few(c, L, (const double2 *)W, M, ipiv, (const double2 *)inv, Bx, M, ns, false));
+  const double shift1[3] = {q->gap[0], q->gap[1], q->gap[2]}, shift2[3] = {0, 0, 0};
+  std::vector<FluxTri> ft;
+  std::vector<double> out(ns);
+  for (int b = 0; b < 2; b++) {
+    if (bemgeo::flux_table(bs[b]->mesh, b ? shift2 : shift1, b ? E1 : 0, ft))
+      return fail(c, BEM_E_INVALID, "body %d has a triangle edge without an RWG", b + 1);
+    CHK(flux_columns(c, L, ft, omega, Bx, M, ns, out.data()));
+    for (int s = 0; s < ns; s++) flux[2 * s + b] = out[s];
+  }
+  ev_collect(c, L);
+  return 0;
 }
 
 int bem_spectral_power(bem_ctx *c, bem_body *b1, bem_body *b2, const bem_request *q, bem_result *res) {

@@ -55,6 +55,25 @@ struct RwgRec {
   double len;
 };
 
+// One point dipole: moment j (real) at p in the medium md of its region; it
+// drives the triangles [t_lo, t_hi) of the tree (every triangle for the
+// exterior, one body's for an interior dipole).
+struct DipoleSrc {
+  double p[3], j[3];
+  Medium md;
+  int t_lo, t_hi, external, pad;
+};
+
+// One triangle of a flux surface: its vertices P1, P2, P3 in the triangle's
+// own order, and for the edges e12, e23, e31 the RWG index and the signed
+// edge length (+len when the triangle is the RWG's positive half).
+struct FluxTri {
+  double P[3][3];
+  double c[3];
+  int32_t e[3];
+  int32_t pad;
+};
+
 #define HIP_CHECK_RET(expr)                                   \
   do {                                                        \
     hipError_t _e = (expr);                                   \

@@ -183,6 +183,42 @@ void build_tree(const std::vector<const BodyMesh *> &bodies, const double *shift
   out.E = eoff;
 }
 
+int flux_table(const BodyMesh &B, const double *shift, int rwg_off, std::vector<FluxTri> &out) {
+  // translated coordinates, rounded as in build_tree
+  std::vector<double> X(3 * (size_t)B.nv);
+  for (int v = 0; v < B.nv; v++)
+    for (int c = 0; c < 3; c++) X[3 * v + c] = B.xyz[3 * v + c] + shift[c];
+  out.assign(B.nt, FluxTri());
+  for (int t = 0; t < B.nt; t++) {
+    FluxTri &f = out[t];
+    std::memset(&f, 0, sizeof f);
+    for (int k = 0; k < 3; k++) {
+      for (int c = 0; c < 3; c++) f.P[k][c] = X[3 * B.tri[3 * (size_t)t + k] + c];
+      f.e[k] = -1;
+    }
+  }
+  for (int e = 0; e < B.ne; e++) {
+    const int32_t *E = &B.edges[4 * (size_t)e];
+    double d[3];
+    sub3(&X[3 * E[1]], &X[3 * E[0]], d);
+    const double len = std::sqrt(norm2(d));
+    for (int h = 0; h < 2; h++) {
+      const int t = E[2 + h];
+      const int32_t *tv = &B.tri[3 * (size_t)t];
+      for (int k = 0; k < 3; k++) {
+        const int a = tv[k], b = tv[(k + 1) % 3];
+        if ((a == E[0] && b == E[1]) || (a == E[1] && b == E[0])) {
+          out[t].e[k] = rwg_off + e;
+          out[t].c[k] = h == 0 ? len : -len;   // + on the RWG's positive (creating) triangle
+        }
+      }
+    }
+  }
+  for (const FluxTri &f : out)
+    if (f.e[0] < 0 || f.e[1] < 0 || f.e[2] < 0) return -1;
+  return 0;
+}
+
 void build_close_pairs(BodyMesh &B) {
   // pairs (a, b) sharing at least one vertex (a == b included),
   // src/Triangle.h:23-30, grouped by test triangle a

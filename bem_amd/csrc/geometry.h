@@ -31,5 +31,10 @@ int build_edges(int nv, int nt, const int32_t *tri, std::vector<int32_t> &edges)
 void build_close_pairs(BodyMesh &B);
 // Union of bodies (body k translated by shifts[3k..3k+2]) in one index space.
 void build_tree(const std::vector<const BodyMesh *> &bodies, const double *shifts, Tree &out);
+// Flux surface of one body (translated by shift, its RWGs numbered from
+// rwg_off): per triangle its vertices in the triangle's own order and the
+// RWGs of its edges e12, e23, e31 with the signed edge length.  Returns -1
+// when an edge of a triangle has no RWG (an open surface).
+int flux_table(const BodyMesh &B, const double *shift, int rwg_off, std::vector<FluxTri> &out);
 
 }  // namespace bemgeo

@@ -71,6 +71,23 @@ class _Res(ctypes.Structure):
                 ("full_residual", ctypes.c_double), ("err_first_order", ctypes.c_double)]
 
 
+class _Dipole(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_double * 3), ("j", ctypes.c_double * 3), ("region", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
+
+
+def _dipoles(src):
+    """[(region, p, j), ...] -> a bem_dipole array."""
+    n = len(src)
+    arr = (_Dipole * max(n, 1))()
+    for i, (region, p, j) in enumerate(src):
+        arr[i].region = int(region)
+        for k in range(3):
+            arr[i].p[k] = float(p[k])
+            arr[i].j[k] = float(j[k])
+    return arr
+
+
 def lib_path():
     return os.path.join(HERE, "libbem_hip.so")
 
@@ -83,7 +100,7 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_sweep", "bem_zgetrf", "bem_zgetrf_local", "bem_zgetrs", "bem_zgemm", "bem_device_alloc",
            "bem_device_free", "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
            "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune",
-           "bem_synchronize", "bem_zgetrf_local_noalloc"]
+           "bem_synchronize", "bem_zgetrf_local_noalloc", "bem_dipole_rhs", "bem_flux", "bem_dipole_flux"]
 
 
 def load_library():
@@ -111,6 +128,9 @@ def load_library():
     L.bem_body_free.restype = None
     L.bem_assemble.argtypes = [vp, i32, vp, vp, vp, vp, dbl, vp, i64, i32]
     L.bem_pair_histogram.argtypes = [vp, i32, vp, vp, vp]
+    L.bem_dipole_rhs.argtypes = [vp, i32, vp, vp, vp, vp, dbl, i32, vp, vp, i64, i32]
+    L.bem_flux.argtypes = [vp, i32, vp, vp, i32, dbl, vp, i64, i32, i32, vp]
+    L.bem_dipole_flux.argtypes = [vp, vp, vp, P(_Req), i32, vp, vp]
     L.bem_spectral_power.argtypes = [vp, vp, vp, P(_Req), P(_Res)]
     L.bem_sweep.argtypes = [vp, vp, vp, P(_Req), i32, P(_Res)]
     L.bem_zgetrf.argtypes = [vp, vp, i64, i32, vp]
@@ -280,6 +300,45 @@ class Context:
         h = np.zeros(6, dtype=np.int64)
         self._check(self.lib.bem_pair_histogram(self.h, nb, arr, sh.ctypes.data, h.ctypes.data))
         return h
+
+    def dipole_rhs(self, bodies, shifts, eps, mu, omega, src) -> np.ndarray:
+        """Right-hand sides of point dipoles (Bdense, src/Source.cpp:167-193)
+        over the union of bodies: (2E, ns) complex128.  src: [(region, p, j)]
+        with region -1 (exterior vacuum) or a body index; eps, mu: per body."""
+        nb = len(bodies)
+        arr = (ctypes.c_void_p * nb)(*[b.handle for b in bodies])
+        sh = np.ascontiguousarray(np.asarray(shifts, dtype=np.float64).reshape(nb, 3))
+        e = np.array([[complex(x).real, complex(x).imag] for x in eps], dtype=np.float64).ravel()
+        m = np.array([[complex(x).real, complex(x).imag] for x in mu], dtype=np.float64).ravel()
+        M = 2 * sum(b.ne for b in bodies)
+        B = np.zeros((M, max(len(src), 1)), dtype=np.complex128, order="F")
+        self._check(self.lib.bem_dipole_rhs(self.h, nb, arr, sh.ctypes.data, e.ctypes.data, m.ctypes.data,
+                                            float(omega), len(src), _dipoles(src), B.ctypes.data, M, 0))
+        return B[:, :len(src)]
+
+    def flux(self, bodies, shifts, body, omega, X) -> np.ndarray:
+        """Flux through body `body`'s surface of each column of X (flux(top,
+        t, X), src/Flux.cpp:100-219)."""
+        nb = len(bodies)
+        arr = (ctypes.c_void_p * nb)(*[b.handle for b in bodies])
+        sh = np.ascontiguousarray(np.asarray(shifts, dtype=np.float64).reshape(nb, 3))
+        X = np.asarray(X, dtype=np.complex128)
+        if X.ndim == 1:
+            X = X[:, None]
+        X = np.asfortranarray(X)
+        out = np.zeros(X.shape[1])
+        self._check(self.lib.bem_flux(self.h, nb, arr, sh.ctypes.data, int(body), float(omega), X.ctypes.data,
+                                      X.shape[0], X.shape[1], 0, out.ctypes.data))
+        return out
+
+    def dipole_flux(self, b1: Body, b2: Body, req: Request, src) -> np.ndarray:
+        """flux(ZLU, j, green, p, ...) (src/Flux.cpp:221-236) for each dipole
+        of src through both bodies: (ns, 2)."""
+        r = req.to_c()
+        out = np.zeros((max(len(src), 1), 2))
+        self._check(self.lib.bem_dipole_flux(self.h, b1.handle, b2.handle, ctypes.byref(r), len(src), _dipoles(src),
+                                             out.ctypes.data))
+        return out[:len(src)]
 
     def calc_heat_transfer(self, b1: Body, b2: Body, req: Request) -> dict:
         """CalcHeatTransfer: spectral power for one (frequency, gap) request."""

@@ -170,6 +170,41 @@ int bem_assemble(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *sh
 /* Pair-class histogram of the same union: {close, g6, g10, g15, g21, g36}. */
 int bem_pair_histogram(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *shifts, int64_t *hist6);
 
+/* ---- dipole sources and flux ------------------------------------------ */
+/* A point dipole: moment j (real) at p, in the exterior medium (vacuum) for
+ * region -1 or inside body `region` (0-based), whose medium it then takes. */
+typedef struct {
+  double p[3];
+  double j[3];
+  int32_t region;
+  int32_t pad;
+} bem_dipole;
+
+/* Right-hand sides of ns dipoles over the union of nb bodies, body k
+ * translated by shifts[3k..3k+2]: DipolePointSource::EHTriangle summed over
+ * the triangles of each body's tree as Bdense does (src/Source.cpp:121-193),
+ * the leaf level of BH (src/Source.cpp:199-215).  B is 2E x ns (leading
+ * dimension ldb >= 2E), rows [J_0, M_0, J_1, M_1, ...]; column s is zero on
+ * the RWGs outside dipole s's region.  eps, mu: 2 nb doubles, (re, im) of
+ * each body's medium.  B is a host pointer unless flags & BEM_ASM_DEVICE_PTR. */
+int bem_dipole_rhs(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *shifts, const double *eps,
+                   const double *mu, double omega, int ns, const bem_dipole *src, void *B, int64_t ldb, int flags);
+/* Power flux through the surface of body `body` of the same union for each
+ * of ncol solution columns X (2E x ncol, leading dimension ldx, [J, M] per
+ * RWG): flux(top, t, X) (src/Flux.cpp:100-219), summed over the body's
+ * triangles.  flux: ncol doubles (host).  X is a host pointer unless
+ * flags & BEM_ASM_DEVICE_PTR. */
+int bem_flux(bem_ctx *ctx, int nb, bem_body *const *bodies, const double *shifts, int body, double omega,
+             const void *X, int64_t ldx, int ncol, int flags, double *flux);
+/* flux(ZLU, j, green, p, rwgTree, t, greenExt) (src/Flux.cpp:221-236) for ns
+ * dipoles at once: their right-hand sides B, X = W^-1 B with W = Z_ext +
+ * blkdiag(Z_int(1), Z_int(2)) of the pair at the request's omega, gap and
+ * media (the matrix of bem_spectral_power), factored with tournament partial
+ * pivoting, then the flux of each column through each body.
+ * flux: ns x 2 doubles, flux[2s + b] through body b+1. */
+int bem_dipole_flux(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *req, int ns,
+                    const bem_dipole *src, double *flux);
+
 /* ---- spectral power (CalcHeatTransfer) -------------------------------- */
 /* Gap-independent blocks -- Z_ext(b,b) + Z_int(b) (factored for body 1) and
  * sym(Z_int(b)) -- are cached on the device per (body, medium, omega) and

@@ -19,6 +19,9 @@
  *   LKTriangle (tiers, scatter)  src/Impedance.cpp:276-385
  *   Zts ........................ src/Impedance.cpp:388-413
  *   Options far thresholds ..... src/Options.cpp:23-26
+ *   dipole right-hand side ..... src/Source.cpp:88-193 (EHSmoothIntegrand,
+ *                                DipolePointSource::EHTriangle, Bdense)
+ *   flux ....................... src/Flux.cpp:100-219
  *
  * Parity pins (see tests/): every function here is checked against the
  * reference's own Zts compiled from its sources (oracle/_ref, golden fixtures
@@ -641,6 +644,170 @@ int oracle_pair_histogram(int nv, const double *xyz, int nt, const int *tri, con
                               body_of_tri[a] == body_of_tri[b])]++;
   free(temax);
   return 0;
+}
+
+/* ------------------------------------------------------ dipole and flux */
+static inline cvec3 cvsub(cvec3 a, cvec3 b) { return cv3(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline cvec3 cvcross(cvec3 a, cvec3 b) {
+  return cv3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static inline vec3 cvreal(cvec3 a) { return v3(creal(a.x), creal(a.y), creal(a.z)); }
+static inline cvec3 cvconj(cvec3 a) { return cv3(conj(a.x), conj(a.y), conj(a.z)); }
+
+/*
+ * Right-hand side of a point dipole (moment j, real, at p) in the medium
+ * (eps, mu) -- DipolePointSource::EHTriangle (src/Source.cpp:121-165) summed
+ * over the triangles of a tree as Bdense does (src/Source.cpp:167-193): per
+ * triangle, the smooth part of the field by the 21-point rule on the
+ * triangle's first half-RWG, the singular part by EHNonSmoothIntegral, then
+ * B(2i) += E_i (J rows) and B(2i+1) += H_i (M rows) for each of its halves,
+ * with the sign of the exterior rule (:150-153) when external is set.  Only
+ * triangles with tri_on[t] != 0 take part (those bordering the dipole's
+ * region).  B: 2ne complex128, overwritten.
+ */
+int oracle_dipole_rhs(int nv, const double *xyz, int nt, const int *tri, int ne, const int *edges, double eps_re,
+                      double eps_im, double mu_re, double mu_im, double omega, int external, const double *p,
+                      const double *jv, const int *tri_on, double *B) {
+  tree T;
+  tree_build(&T, nv, xyz, nt, tri, ne, edges);
+  green g;
+  green_init(&g, eps_re + I * eps_im, mu_re + I * mu_im, omega);
+  cplx *Bc = (cplx *)B;
+  memset(Bc, 0, sizeof(cplx) * 2 * (size_t)ne);
+  const vec3 pp = v3(p[0], p[1], p[2]), j = v3(jv[0], jv[1], jv[2]);
+  for (int t = 0; t < nt; t++) {
+    const halflist *tl = &T.tlist[t];
+    if (!tri_on[t] || tl->n == 0) continue;
+    const halfrwg *th = tl->h[0];
+    /* EHSmoothIntegrand (src/Source.cpp:88-119) over the 21-point rule */
+    cvec3 DG = cvzero(), Gphi[3], DGxphi[3];
+    for (int i = 0; i < 3; i++) Gphi[i] = DGxphi[i] = cvzero();
+    for (int q = 0; q < 21; q++) {
+      const double *abw = bem_quad_abw[BEM_QUAD_OFF_21 + q];
+      vec3 v = quad_point(th->p0, th->v01, th->v02, abw[0], abw[1]);
+      cplx gs;
+      cvec3 gg;
+      scalar_gradient_smooth(&g, pp, v, &gs, &gg);
+      gs *= abw[2];
+      gg = cvscale_r(abw[2], gg);
+      DG = cvadd(DG, gg);
+      for (int i = 0; i < tl->n; i++) {
+        vec3 phi = vscale(0.5, vsub(v, tl->h[i]->p0));
+        Gphi[i] = cvadd(Gphi[i], cv_from(gs, phi));
+        DGxphi[i] = cvadd(DGxphi[i], cvcross_r(gg, phi));
+      }
+    }
+    double ISn3h, ISn1, IS1;
+    vec3 ILn1, IL1, IL3;
+    oracle_eh_nonsmooth(th, pp, &ISn3h, &ISn1, &IS1, &ILn1, &IL1, &IL3);
+    const cplx k2 = g.k2;
+    const double Ai = th->Ai, Ai2 = 0.5 * Ai;
+    cvec3 IL13 = cvscale_r(Ai2, cvadd(cv_real(IL1), cv_from(ONETHIRD * k2, IL3)));
+    cplx ISn11 = Ai2 * (ISn1 + k2 * IS1);
+    vec3 n = th->n;
+    double h = vdot(vsub(pp, th->p0), n);
+    vec3 rho = vsub(pp, vscale(h, n));
+    cvec3 K31 = cvscale_r(Ai2, cvadd(cv_real(vadd(ILn1, vscale(ISn3h, n))),
+                                     cv_from(k2, vsub(IL1, vscale(h * ISn1, n)))));
+    cvec3 E1 = cvadd(DG, cvscale_r(Ai, cvadd(cv_from(ISn3h - k2 * h * ISn1, n),
+                                             cvadd(cv_real(ILn1), cv_from(k2, IL1)))));
+    for (int i = 0; i < tl->n; i++) {
+      const halfrwg *hi = tl->h[i];
+      int64_t ii = 2 * (int64_t)hi->rwg;
+      double c = ONEOVER4PI * hi->elen;
+      if ((external != 0) != (hi->positive != 0)) c = -c;
+      cvec3 a = cvadd(cvadd(Gphi[i], IL13), cv_from(ISn11, vsub(rho, hi->p0)));
+      cplx Ei = c * g.Zeps * cvdot_r(j, cvsub(E1, cvscale(g.ksq, a)));
+      cvec3 hv = cvadd(DGxphi[i], cvcross_r(K31, vsub(pp, hi->p0)));
+      cplx Hi = c * cvdot_r(j, cvscale_r(-1.0, hv));
+      Bc[ii] += Ei;
+      Bc[ii + 1] += Hi;
+    }
+  }
+  tree_free(&T);
+  return 0;
+}
+
+/*
+ * Power flux through the triangles [t_lo, t_hi) of a tree from the solution X
+ * (2ne complex128, [J, M] interleaved) -- flux(top, t, X)
+ * (src/Flux.cpp:100-219): per triangle the surface currents at its vertices
+ * from its three RWGs (signed by whether the triangle is the RWG's positive
+ * half), M scaled by 1/omega and J conjugated, then
+ * dS = n . Re(j1 x m1 + j2 x m2 + j3 x m3 + (j1+j2+j3) x (m1+m2+m3)) / (48 A2),
+ * A2 twice the triangle's area.  Returns the sum of dS in triangle order.
+ */
+double oracle_flux(int nv, const double *xyz, int nt, const int *tri, int ne, const int *edges, int t_lo, int t_hi,
+                   double omega, const double *X) {
+  (void)nv;
+  const cplx *x = (const cplx *)X;
+  int *tedge = malloc(sizeof(int) * 3 * (size_t)nt);
+  for (int i = 0; i < 3 * nt; i++) tedge[i] = -1;
+  for (int e = 0; e < ne; e++) {
+    const int *E = edges + 4 * e;
+    for (int side = 2; side <= 3; side++) {
+      const int t = E[side], *f = tri + 3 * t;
+      for (int k = 0; k < 3; k++) {   /* e12, e23, e31 */
+        int a = f[k], b = f[(k + 1) % 3];
+        if ((a == E[0] && b == E[1]) || (a == E[1] && b == E[0])) tedge[3 * t + k] = e;
+      }
+    }
+  }
+  double S = 0.;
+  for (int t = t_lo; t < t_hi; t++) {
+    const int *f = tri + 3 * t;
+    vec3 P1 = v3(xyz[3 * f[0]], xyz[3 * f[0] + 1], xyz[3 * f[0] + 2]);
+    vec3 P2 = v3(xyz[3 * f[1]], xyz[3 * f[1] + 1], xyz[3 * f[1] + 2]);
+    vec3 P3 = v3(xyz[3 * f[2]], xyz[3 * f[2] + 1], xyz[3 * f[2] + 2]);
+    vec3 n = vcross(vsub(P2, P1), vsub(P3, P1));   /* normalUnnormalized, src/Triangle.cpp:5-12 */
+    double A2 = sqrt(vnorm2(n));
+    n = vscale(1. / A2, n);
+    vec3 v12 = vsub(P2, P1), v23 = vsub(P3, P2), v31 = vsub(P1, P3);
+    cvec3 j1 = cvzero(), j2 = cvzero(), j3 = cvzero(), m1 = cvzero(), m2 = cvzero(), m3 = cvzero();
+    int e12 = tedge[3 * t], e23 = tedge[3 * t + 1], e31 = tedge[3 * t + 2];
+    if (e12 < 0 || e23 < 0 || e31 < 0) continue;   /* open edge: no RWG */
+    const int *E;
+    double l;
+    E = edges + 4 * e12;
+    l = sqrt(vnorm2(vsub(v3(xyz[3 * E[1]], xyz[3 * E[1] + 1], xyz[3 * E[1] + 2]),
+                         v3(xyz[3 * E[0]], xyz[3 * E[0] + 1], xyz[3 * E[0] + 2]))));
+    double c12 = t == E[2] ? l : -l;
+    cplx j12 = c12 * x[2 * e12], m12 = c12 * x[2 * e12 + 1];
+    j1 = cvadd(j1, cv_from(j12, v31));
+    j2 = cvsub(j2, cv_from(j12, v23));
+    m1 = cvadd(m1, cv_from(m12, v31));
+    m2 = cvsub(m2, cv_from(m12, v23));
+    E = edges + 4 * e23;
+    l = sqrt(vnorm2(vsub(v3(xyz[3 * E[1]], xyz[3 * E[1] + 1], xyz[3 * E[1] + 2]),
+                         v3(xyz[3 * E[0]], xyz[3 * E[0] + 1], xyz[3 * E[0] + 2]))));
+    double c23 = t == E[2] ? l : -l;
+    cplx j23 = c23 * x[2 * e23], m23 = c23 * x[2 * e23 + 1];
+    j2 = cvadd(j2, cv_from(j23, v12));
+    j3 = cvsub(j3, cv_from(j23, v31));
+    m2 = cvadd(m2, cv_from(m23, v12));
+    m3 = cvsub(m3, cv_from(m23, v31));
+    E = edges + 4 * e31;
+    l = sqrt(vnorm2(vsub(v3(xyz[3 * E[1]], xyz[3 * E[1] + 1], xyz[3 * E[1] + 2]),
+                         v3(xyz[3 * E[0]], xyz[3 * E[0] + 1], xyz[3 * E[0] + 2]))));
+    double c31 = t == E[2] ? l : -l;
+    cplx j31 = c31 * x[2 * e31], m31 = c31 * x[2 * e31 + 1];
+    j3 = cvadd(j3, cv_from(j31, v23));
+    j1 = cvsub(j1, cv_from(j31, v12));
+    m3 = cvadd(m3, cv_from(m31, v23));
+    m1 = cvsub(m1, cv_from(m31, v12));
+    cplx omegai = 1.0 / (cplx)omega;
+    m1 = cvscale(omegai, m1);
+    m2 = cvscale(omegai, m2);
+    m3 = cvscale(omegai, m3);
+    j1 = cvconj(j1);
+    j2 = cvconj(j2);
+    j3 = cvconj(j3);
+    vec3 s = vadd(vadd(vadd(cvreal(cvcross(j1, m1)), cvreal(cvcross(j2, m2))), cvreal(cvcross(j3, m3))),
+                  cvreal(cvcross(cvadd(cvadd(j1, j2), j3), cvadd(cvadd(m1, m2), m3))));
+    S += vdot(n, s) / (48. * A2);
+  }
+  free(tedge);
+  return S;
 }
 
 /* Test hooks for the analytic integrals: half-RWG built from a triangle

@@ -13,6 +13,8 @@ baseline, never as the product path.
   symm ................... src/util/DenseMatrix.cpp:477-486
   multiply_Adagger_B ..... src/util/DenseMatrix.cpp:468-475
   trace .................. src/util/DenseMatrix.h:489-497
+  dipole RHS (C) ......... src/Source.cpp:88-193
+  flux (C) ............... src/Flux.cpp:100-219
 """
 from __future__ import annotations
 
@@ -54,6 +56,12 @@ def lib():
         L.oracle_eh_nonsmooth_tri.argtypes = [f64p, ctypes.c_int, ctypes.c_int, f64p, f64p]
         L.oracle_green.argtypes = [ctypes.c_double] * 5 + [f64p, f64p, ctypes.c_int, f64p]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_dipole_rhs.argtypes = [ctypes.c_int, f64p, ctypes.c_int, i32p, ctypes.c_int, i32p] + \
+            [ctypes.c_double] * 5 + [ctypes.c_int, f64p, f64p, i32p, ctypes.c_void_p]
+        L.oracle_dipole_rhs.restype = ctypes.c_int
+        L.oracle_flux.argtypes = [ctypes.c_int, f64p, ctypes.c_int, i32p, ctypes.c_int, i32p, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_double, ctypes.c_void_p]
+        L.oracle_flux.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -109,6 +117,34 @@ class Tree:
                          self.edges, eps.real, eps.imag, mu.real, mu.imag, float(omega), lo, hi,
                          Z.ctypes.data, M)
         return Z
+
+    def body_triangles(self, b):
+        """Triangle range [lo, hi) of body b."""
+        lo = int(np.searchsorted(self.body, b, side="left"))
+        return lo, int(np.searchsorted(self.body, b, side="right"))
+
+    def dipole_rhs(self, eps, mu, omega, region, p, j) -> np.ndarray:
+        """Right-hand side of a point dipole j at p (src/Source.cpp:121-193):
+        2ne complex128.  region -1: the dipole is in the exterior medium
+        (eps, mu) and every body's triangles take part; region b: it is inside
+        body b, whose medium (eps, mu) is, and only body b's triangles do."""
+        eps, mu = complex(eps), complex(mu)
+        on = np.ones(self.tri.shape[0], dtype=np.int32) if region < 0 else (self.body == region).astype(np.int32)
+        B = np.zeros(2 * self.ne, dtype=np.complex128)
+        lib().oracle_dipole_rhs(self.xyz.shape[0], self.xyz, self.tri.shape[0], self.tri, self.ne, self.edges,
+                                eps.real, eps.imag, mu.real, mu.imag, float(omega), 1 if region < 0 else 0,
+                                np.ascontiguousarray(p, dtype=np.float64), np.ascontiguousarray(j, dtype=np.float64),
+                                on, B.ctypes.data)
+        return B
+
+    def flux(self, omega, X, body) -> float:
+        """Power through the surface of body `body` from the solution X
+        (2ne complex128, [J, M] per RWG) -- src/Flux.cpp:100-219."""
+        X = np.ascontiguousarray(X, dtype=np.complex128)
+        assert X.shape == (2 * self.ne,)
+        lo, hi = self.body_triangles(body)
+        return lib().oracle_flux(self.xyz.shape[0], self.xyz, self.tri.shape[0], self.tri, self.ne, self.edges,
+                                 lo, hi, float(omega), X.ctypes.data)
 
     def pair_histogram(self):
         h = np.zeros(6, dtype=np.int64)
@@ -209,3 +245,35 @@ def run_reference(meshes, eps_list, mu_list, omega, workdir):
         pos += 16 * cnt
         mats.append(np.array(z, order="F"))
     return edges, mats[0], mats[1:]
+
+
+def run_reference_source(meshes, eps_list, mu_list, omega, region, p, j, X, workdir):
+    """Run the reference's Bdense and flux (oracle/_ref/zts_ref --source).
+
+    Returns (B, [flux over body b ...]): B is the dipole right-hand side
+    (2E complex128) of each body's tree in body order, the flux is
+    flux(top, body b, X).  Fixture generation only."""
+    if not os.path.exists(REF_DRIVER):
+        raise RuntimeError("reference driver not built (make -C oracle ref)")
+    lines = [repr(float(omega)), str(len(meshes))]
+    for m, e, mu in zip(meshes, eps_list, mu_list):
+        e, mu = complex(e), complex(mu)
+        lines.append(f"{e.real!r} {e.imag!r} {mu.real!r} {mu.imag!r} {m.nv} {m.nt}")
+        lines += [f"{x!r} {y!r} {z!r}" for x, y, z in m.xyz.tolist()]
+        lines += [f"{a} {b} {c}" for a, b, c in m.tri.tolist()]
+    lines.append(f"{int(region)} " + " ".join(repr(float(v)) for v in list(p) + list(j)))
+    lines += [f"{z.real!r} {z.imag!r}" for z in np.asarray(X, dtype=np.complex128).tolist()]
+    out = os.path.join(workdir, "source_ref.bin")
+    subprocess.run([REF_DRIVER, out, "--source"], input="\n".join(lines) + "\n", text=True, check=True,
+                   stdout=subprocess.DEVNULL)
+    raw = open(out, "rb").read()
+    pos = 0
+    E = 0
+    for _ in meshes:
+        ne = int(np.frombuffer(raw, dtype=np.int64, count=1, offset=pos)[0])
+        pos += 8 + 32 * ne
+        E += ne
+    B = np.frombuffer(raw, dtype=np.complex128, count=2 * E, offset=pos).copy()
+    pos += 16 * 2 * E
+    S = np.frombuffer(raw, dtype=np.float64, count=len(meshes), offset=pos).copy()
+    return B, [float(v) for v in S]

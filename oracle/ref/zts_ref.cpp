@@ -9,16 +9,26 @@
 //   nbody
 //   per body: eps_re eps_im mu_re mu_im nv nt
 //             nv lines "x y z", nt lines "a b c" (0-based vertex ids)
+//   with argv[2] == "--source", after the bodies:
+//     region px py pz jx jy jz    (dipole in the exterior for region -1, else
+//                                  inside body `region`)
+//     2E lines "re im"            (a solution vector X, [J, M] per RWG)
 // Output (binary, file named by argv[1]; the reference's mesher prints to stdout):
 //   per body: int64 ne, then ne x 4 int64 {p1, p2, t1, t2} of the reference's
 //             ShapeMesh edge table (body-local ids), in Edge::index order
 //   then complex128 column-major matrices:
 //     Zts(top, top, exterior g)   (2E x 2E, E = total RWGs, body 1 first)
 //     Zts(body_b, body_b, g_b)    for each body b
+//   or, with --source:
+//     B (2E complex128): Bdense of each body's tree in body order, the
+//       leaf level of BH (src/Source.cpp:199-215)
+//     float64 flux(top, body_b, X) for each body b (src/Flux.cpp:100)
 //
 // Reference entry points used: ShapeMesh::ShapeMesh (src/ShapeMesh.cpp:821),
 // RWG::RWG (src/RWG.cpp:5), RWGTree protected ctor (src/RWGTree.cpp:590),
-// GreensFunction (src/GreensFunction.cpp:11), Zts (src/Impedance.cpp:388).
+// GreensFunction (src/GreensFunction.cpp:11), Zts (src/Impedance.cpp:388),
+// DipolePointSource (src/Source.cpp:14), Bdense (src/Source.cpp:167),
+// flux (src/Flux.cpp:100).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -28,6 +38,11 @@
 #include "Impedance.h"
 #include "Options.h"
 #include "ReferenceCountingPointer.h"
+#include "Source.h"
+#include "Flux.h"
+#include "Progress.h"
+#include <iostream>
+#include <sstream>
 
 struct FlatTree : public RWGTree {
   FlatTree(int n, PointEdgeVectorMap &m, Dielectric *d, GreensFunction *g) : RWGTree(n, m, d, g) {}
@@ -49,7 +64,8 @@ static void write_matrix(ComplexDenseMatrix *Z) {
 }
 
 int main(int argc, char **argv) {
-  if (argc != 2) return 2;
+  if (argc != 2 && !(argc == 3 && std::string(argv[2]) == "--source")) return 2;
+  const bool source_mode = argc == 3;
   out = fopen(argv[1], "wb");
   if (!out) return 2;
   initReferenceCountingPointerLock();
@@ -59,6 +75,7 @@ int main(int argc, char **argv) {
   Options options;
   PointEdgeVectorMap eap;
   std::vector<ShapeMesh *> meshes;
+  std::vector<std::vector<Triangle *> > body_tris(nbody);
   std::vector<std::vector<RWG *> > rwgs(nbody);
   std::vector<Complex> eps(nbody), mu(nbody);
   for (int b = 0; b < nbody; b++) {
@@ -86,6 +103,7 @@ int main(int argc, char **argv) {
     Array<Point> &pts = *m->refinedPoint[0];
     Array<Edge> &edges = *m->refinedEdge[0];
     Array<Triangle> &tris = *m->refinedTriangle[0];
+    for (int i = 0; i < tris.n; i++) body_tris[b].push_back(tris + i);
     int64_t ne = edges.n;
     fwrite(&ne, sizeof ne, 1, out);
     for (int i = 0; i < edges.n; i++) {
@@ -109,6 +127,43 @@ int main(int argc, char **argv) {
       sub[b]->rwg[sub[b]->rwgs++] = rwgs[b][i];
       top.rwg[top.rwgs++] = rwgs[b][i];
     }
+  }
+  if (source_mode) {
+    int region;
+    double p[3], j[3];
+    if (scanf("%d %lf %lf %lf %lf %lf %lf", &region, p, p + 1, p + 2, j, j + 1, j + 2) != 7) return 2;
+    if (region < -1 || region >= nbody) return 2;
+    ComplexDenseMatrix X(2 * (int)total, 1, 0);
+    for (size_t i = 0; i < 2 * total; i++) {
+      double re, im;
+      if (scanf("%lf %lf", &re, &im) != 2) return 2;
+      X(i, 0) = Complex(re, im);
+    }
+    Point pp(p[0], p[1], p[2]);
+    Vector3 jj(j[0], j[1], j[2]);
+    DipolePointSource dip(region < 0 ? &gext : gb[region], pp, jj);
+    SourceSet sources;
+    sources.insert(&dip);
+    Progress progress;
+    int off = 0;
+    for (int b = 0; b < nbody; b++) {
+      for (int i = 0; i < sub[b]->rwgs; i++) sub[b]->index.push_back(off + i);
+      off += sub[b]->rwgs;
+      sub[b]->triangle.insert(body_tris[b].begin(), body_tris[b].end());
+      ComplexDenseMatrix *B = Bdense(sub[b], &gext, sources, progress, options);
+      write_matrix(B);
+      delete B;
+    }
+    // flux() prints a line per triangle on stdout: keep it out of the result
+    std::ostringstream sink;
+    std::streambuf *keep = std::cout.rdbuf(sink.rdbuf());
+    for (int b = 0; b < nbody; b++) {
+      double S = flux(&top, sub[b], X);
+      fwrite(&S, sizeof S, 1, out);
+    }
+    std::cout.rdbuf(keep);
+    fclose(out);
+    return 0;
   }
   ComplexDenseMatrix *W = Zts(&top, &top, &gext, options);
   write_matrix(W);

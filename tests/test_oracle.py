@@ -234,3 +234,71 @@ def test_theta_pinned():
     T = 300.0
     w1 = O.SI_KB * T / O.SI_HBAR
     assert abs(O.theta(w1, T) / (O.SI_KB * T / (np.e - 1)) - 1) < 1e-12
+
+
+# ------------------------------------------------------ dipole source, flux
+def _dipole_case():
+    g = load_golden("sphere1_cube1_gap0p05")
+    d = load_golden("dipole_flux")
+    return g, d, O.Tree(golden_meshes(g))
+
+
+def test_dipole_rhs_matches_reference():
+    """Oracle Bdense (src/Source.cpp:121-193) against the reference's own for
+    an exterior dipole and one inside each body; rows outside an interior
+    dipole's body are exactly zero."""
+    g, d, T = _dipole_case()
+    om = float(g["omega"])
+    m1 = 2 * T.ne_body[0]
+    for s in range(3):
+        r = int(d["region"][s])
+        eps, mu = (1.0, 1.0) if r < 0 else (complex(g[f"eps_b{r}"]), complex(g[f"mu_b{r}"]))
+        B = T.dipole_rhs(eps, mu, om, r, d["p"][s], d["j"][s])
+        assert nerr(B, d["B"][s]) < 1e-13, s
+        if r == 0:
+            assert not B[m1:].any()
+        if r == 1:
+            assert not B[:m1].any()
+
+
+def test_flux_matches_reference():
+    """Oracle flux (src/Flux.cpp:100-219) of a random solution vector through
+    each body, against the reference's own."""
+    g, d, T = _dipole_case()
+    om = float(g["omega"])
+    for b in range(2):
+        f = T.flux(om, d["X_rand"], b)
+        assert abs(f - d["flux_rand"][b]) <= 1e-13 * abs(d["flux_rand"][b])
+
+
+def test_flux_quadratic_form():
+    """flux is a real quadratic form in X: f(aX) = |a|^2 f(X) for complex a,
+    and f(X + Y) + f(X - Y) = 2 f(X) + 2 f(Y)."""
+    g, d, T = _dipole_case()
+    om = float(g["omega"])
+    rng = np.random.default_rng(3)
+    X = d["X_rand"]
+    Y = rng.normal(size=X.shape) + 1j * rng.normal(size=X.shape)
+    for b in range(2):
+        fx, fy = T.flux(om, X, b), T.flux(om, Y, b)
+        a = 0.7 - 1.3j
+        assert abs(T.flux(om, a * X, b) - abs(a) ** 2 * fx) <= 1e-12 * abs(fx)
+        par = T.flux(om, X + Y, b) + T.flux(om, X - Y, b)
+        assert abs(par - 2 * (fx + fy)) <= 1e-12 * (abs(fx) + abs(fy))
+
+
+def test_dipole_flux_chain():
+    """The dipole's flux through each body: oracle B, X = W_full^-1 B with the
+    reference's W, G1, G2, oracle flux -- against the reference's flux of
+    the same solve."""
+    g, d, T = _dipole_case()
+    om = float(g["omega"])
+    m1 = g["G1"].shape[0]
+    Wf = g["W"].copy()
+    Wf[:m1, :m1] += g["G1"]
+    Wf[m1:, m1:] += g["G2"]
+    for s in range(3):
+        X = np.linalg.solve(Wf, d["B"][s])
+        for b in range(2):
+            ref = d["flux_phys"][s][b]
+            assert abs(T.flux(om, X, b) - ref) <= 1e-10 * abs(ref), (s, b)
