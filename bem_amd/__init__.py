@@ -13,6 +13,8 @@ from .runtime import (  # noqa: F401
     BEM_PENDING,
     BEM_PREC_FP64,
     BEM_PREC_MIXED,
+    BEM_SOLVE_LOWRANK,
+    BEM_LOWRANK_ACA,
     BemError,
     Body,
     Context,

@@ -120,6 +120,25 @@ size_t bem_zgemm_skinny_partials(int m, int r, int K);
 int bem_fill_zero(void *p, size_t bytes, hipStream_t st);
 int bem_zgemm_skinny(int m, int r, int K, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
                      double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st);
+size_t bem_zgemm_ta_partials(int m, int n, int k);
+int bem_zgemm_ta(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                 double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st);
+int bem_chol_inv64(const double2 *G, int64_t ldg, int w, double2 *Rinv, int *info, hipStream_t st);
+int bem_transpose_op(const double2 *src, int64_t lds, int m, int n, double2 *dst, int64_t ldd, int conj, int sign_rows,
+                     hipStream_t st);
+int bem_conj_sign(const double2 *src, int64_t lds, int m, int n, double2 *dst, int64_t ldd, int sign_rows,
+                  hipStream_t st);
+size_t bem_aca_state_bytes(void);
+size_t bem_aca_partial_bytes(int m, int n);
+int bem_aca_init(void *state, int m, int n, hipStream_t st);
+int bem_aca_read(const void *state, int *nu, int *rank, int *done, int *full);
+int bem_aca_steps(double2 *M, int64_t ldm, int m, int n, double eps2, int kmax, double2 *U, int64_t ldu, double2 *V,
+                  int64_t ldv, int *rows, int *cols, void *state, void *partial, int steps, hipStream_t st);
+int bem_jacobi_sweep(double2 *A, int64_t lda, int m, int k, double2 *W, int64_t ldw, int kw, double tol, int *rotated,
+                     hipStream_t st);
+int bem_col_norms(const double2 *A, int64_t lda, int m, int k, double *sigma, hipStream_t st);
+int bem_gather_cols(const double2 *src, int64_t lds, int m, const int *idx, const double *scale, int nc, double2 *dst,
+                    int64_t ldd, hipStream_t st);
 int bem_launch_dipole_rhs(const TriGeom *tg, int T, const RwgRec *rw, int E, const void *src, int nsrc, int tmax,
                           double2 *tb, double2 *B, int64_t ldb, hipStream_t st);
 size_t bem_flux_partials(int nt, int ncol);
@@ -242,6 +261,10 @@ struct Lane {
   bool probe_async = false;       // the stage's probe runs on aux (probe_join pending)
   ProbeOut *h_probe = nullptr;    // pinned copy of the last stage's probe
   double *h_rfull = nullptr;      // pinned: squared norm of the last refinement residual
+  int rank = 0;                   // rank of the coupling block of a low-rank solve (0: dense)
+  double ms_compress = 0.0;       // its compression time (host clock)
+  double *h_lr = nullptr;         // pinned: norms read back by the low-rank compression
+  int *h_lri = nullptr;           // pinned: its status words
   // served request in flight (ticket 0: none)
   bem_ctx *ctx = nullptr;
   uint64_t ticket = 0;
@@ -1535,6 +1558,376 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
   return ev_end(c, L, r);
 }
 
+// ------------------------------------------------- low-rank coupling solve
+// Columns per block of the randomized range finder (one CholQR2 block).
+static const int LR_BLK = 64;
+// Seed of the range finder's test columns (fixed, so a request gives the same
+// factors alone and in a mix).
+static const unsigned long long LR_SEED = 0x10C0FFEEull;
+
+// C (m x n) <- alpha A^H B + beta C (A k x m, B k x n) on the lane's stream.
+static int gemm_ta(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda,
+                   const double2 *B, int64_t ldb, double2 beta, double2 *C, int64_t ldc) {
+  if (m <= 0 || n <= 0) return 0;
+  double2 *part = nullptr;
+  const size_t np = bem_zgemm_ta_partials(m, n, k);
+  if (np) CHK(ws_get(c, L, "lr_tapart", sizeof(double2) * np, (void **)&part));
+  EventRec *r;
+  CHK(ev_begin(c, L, BEM_KS_ZGEMM_NARROW, 8.0 * m * (double)n * k,
+               16.0 * ((double)m * k + (double)k * n + (double)m * n), &r));
+  int e = bem_zgemm_ta(m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, part, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "zgemm (A^H B) launch: %d", e);
+  return ev_end(c, L, r);
+}
+
+// Squared Frobenius norm of an m x n block, read back to the host (waits for
+// the lane's stream).
+static int lr_sumsq_host(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int m, int n, double *out) {
+  double *part, *dst;
+  CHK(ws_get(c, L, "lr_ssq", sizeof(double) * (bem_sumsq_partials(m, n) + 1), (void **)&part));
+  dst = part + bem_sumsq_partials(m, n);
+  if (bem_sumsq(A, lda, m, n, part, 0, dst, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
+  DEV(c, hipMemcpyAsync(L.h_lr, dst, sizeof(double), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  *out = L.h_lr[0];
+  return 0;
+}
+
+// Blocked randomized range finder of B (m1 x m2): Q (m1 x r, orthonormal
+// columns, r a multiple of LR_BLK, leading dimension m1) and Vh = Q^H B
+// (r x m2, leading dimension ldv) with B ~ Q Vh.  Block k draws LR_BLK seeded
+// test columns W_k (entries with real and imaginary parts uniform in
+// [-1, 1), E|w|^2 = 2/3), Y = B W_k, projects out the range found so far
+// twice (Y -= Q (Q^H Y)), and orthonormalises Y by CholQR2 (G = Y^H Y = R^H
+// R, Y <- Y R^-1, twice).  E||Y_k||_F^2 = LR_BLK (2/3) ||B - Q Q^H B||_F^2
+// before the block joins, so the loop ends after the first block whose
+// estimate is at most tol^2 ||B||_F^2 (that block still joins Q, which only
+// lowers the error).  *rank = 0 when B is zero; -1 when rmax columns do not
+// reach the tolerance (the caller then solves densely).  Waits on the lane's
+// stream once per block.
+static int lr_compress_rand(bem_ctx *c, Lane &L, const double2 *B, int m1, int m2, double tol, int rmax, double2 *Q,
+                            double2 *Vh, int ldv, int *rank) {
+  const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  *rank = -1;
+  double bn2;
+  CHK(lr_sumsq_host(c, L, B, m1, m1, m2, &bn2));
+  if (!(bn2 > 0)) {
+    *rank = bn2 == 0 ? 0 : -1;
+    return 0;
+  }
+  double2 *W, *Cb, *G, *Rinv;
+  int *info;
+  CHK(ws_get(c, L, "lr_W", sizeof(double2) * (size_t)m2 * LR_BLK, (void **)&W));
+  CHK(ws_get(c, L, "lr_Cb", sizeof(double2) * (size_t)std::max(rmax, 1) * LR_BLK, (void **)&Cb));
+  CHK(ws_get(c, L, "lr_G", sizeof(double2) * LR_BLK * LR_BLK, (void **)&G));
+  CHK(ws_get(c, L, "lr_Rinv", sizeof(double2) * 64 * 64, (void **)&Rinv));
+  CHK(ws_get(c, L, "lr_info", sizeof(int) * 4, (void **)&info));
+  int r = 0;
+  bool done = false;
+  for (int blk = 0; r + LR_BLK <= rmax; blk++) {
+    double2 *Y = Q + (int64_t)r * m1;
+    if (bem_fill_hash(W, sizeof(double2) * (size_t)m2 * LR_BLK, 0, LR_SEED + 0x9E37ull * blk, L.st))
+      return fail(c, BEM_E_DEVICE, "test column fill launch");
+    CHK(gemm(c, L, m1, LR_BLK, m2, one, B, m1, W, m2, zero, Y, m1));
+    if (r > 0)
+      for (int pass = 0; pass < 2; pass++) {
+        CHK(gemm_ta(c, L, r, LR_BLK, m1, one, Q, m1, Y, m1, zero, Cb, r));
+        CHK(gemm(c, L, m1, LR_BLK, r, mone, Q, m1, Cb, r, one, Y, m1));
+      }
+    double y2;
+    CHK(lr_sumsq_host(c, L, Y, m1, m1, LR_BLK, &y2));
+    const bool last = y2 / (LR_BLK * (2.0 / 3.0)) <= tol * tol * bn2;
+    DEV(c, hipMemsetAsync(info, 0, sizeof(int), L.st));
+    for (int pass = 0; pass < 2; pass++) {
+      CHK(gemm_ta(c, L, LR_BLK, LR_BLK, m1, one, Y, m1, Y, m1, zero, G, LR_BLK));
+      if (bem_chol_inv64(G, LR_BLK, LR_BLK, Rinv, info, L.st)) return fail(c, BEM_E_DEVICE, "cholesky launch");
+      CHK(gemm_inplace(c, L, 'R', m1, LR_BLK, Rinv, Y, m1));
+    }
+    DEV(c, hipMemcpyAsync(L.h_lri, info, sizeof(int), hipMemcpyDeviceToHost, L.st));
+    DEV(c, hipStreamSynchronize(L.st));
+    if (L.h_lri[0]) {
+      // the block's columns are dependent to working precision: the range is
+      // exhausted when the residual estimate is already under the tolerance
+      done = last;
+      break;
+    }
+    r += LR_BLK;
+    if (last) {
+      done = true;
+      break;
+    }
+  }
+  if (!done || r == 0) return 0;
+  CHK(gemm_ta(c, L, r, m2, m1, one, Q, m1, B, m1, zero, Vh, ldv));
+  *rank = r;
+  return 0;
+}
+
+// Adaptive cross approximation with full pivoting on the device (see
+// bem_aca): batches of 16 steps, the state read back after each batch.
+// rows/cols: device arrays of kmax ints.  *full: kmax columns were used up
+// before the loop ended.
+static int aca_run(bem_ctx *c, Lane &L, double2 *M, int64_t ldm, int m, int n, double eps2, int kmax, double2 *U,
+                   int64_t ldu, double2 *V, int64_t ldv, int *rows, int *cols, int *rank, int *full) {
+  void *state, *part;
+  CHK(ws_get(c, L, "aca_state", bem_aca_state_bytes(), &state));
+  CHK(ws_get(c, L, "aca_part", bem_aca_partial_bytes(m, n), &part));
+  if (bem_aca_init(state, m, n, L.st)) return fail(c, BEM_E_DEVICE, "aca state upload");
+  const int smax = std::min(std::min(m, n), kmax) + 1;
+  int nu = 0, done = 0;
+  *rank = 0;
+  *full = 0;
+  for (int s0 = 0; s0 < smax && !done; s0 += 16) {
+    if (bem_aca_steps(M, ldm, m, n, eps2, kmax, U, ldu, V, ldv, rows, cols, state, part, std::min(16, smax - s0),
+                      L.st))
+      return fail(c, BEM_E_DEVICE, "aca launch");
+    DEV(c, hipStreamSynchronize(L.st));
+    if (bem_aca_read(state, &nu, rank, &done, full)) return fail(c, BEM_E_DEVICE, "aca state read");
+  }
+  return 0;
+}
+
+// Rk::truncate (see bem_rk_truncate) of U (m x k, ldu) V (k x n, ldv) on the
+// lane's stream.  Jacobi sweeps run until a sweep rotates no pair (at most
+// 60).  sigma_host (optional) receives the k singular values, decreasing.
+static int rk_truncate(bem_ctx *c, Lane &L, double2 *U, int64_t ldu, double2 *V, int64_t ldv, int m, int n, int k,
+                       double eps, int *knew, double *sigma_host) {
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+  *knew = 0;
+  if (k <= 0) return 0;
+  const int ke = k + (k & 1);   // even column count for the round-robin pairs (a zero column pads)
+  double2 *Vt, *W, *U1, *QW;
+  double *sig;
+  int *rot, *idx;
+  CHK(ws_get(c, L, "rk_Vt", sizeof(double2) * (size_t)n * ke, (void **)&Vt));
+  CHK(ws_get(c, L, "rk_W", sizeof(double2) * (size_t)ke * ke, (void **)&W));
+  CHK(ws_get(c, L, "rk_U1", sizeof(double2) * (size_t)m * ke, (void **)&U1));
+  CHK(ws_get(c, L, "rk_QW", sizeof(double2) * (size_t)n * ke, (void **)&QW));
+  CHK(ws_get(c, L, "rk_sig", sizeof(double) * 2 * (size_t)ke, (void **)&sig));
+  CHK(ws_get(c, L, "rk_rot", sizeof(int) * 4, (void **)&rot));
+  CHK(ws_get(c, L, "rk_idx", sizeof(int) * (size_t)ke, (void **)&idx));
+  std::vector<double2> eye((size_t)ke * ke, zero);
+  for (int i = 0; i < ke; i++) eye[(size_t)i * ke + i] = one;
+  auto sweeps = [&](double2 *A, int rows_a) -> int {
+    for (int sw = 0; sw < 60; sw++) {
+      DEV(c, hipMemsetAsync(rot, 0, sizeof(int), L.st));
+      if (bem_jacobi_sweep(A, rows_a, rows_a, ke, W, ke, ke, 1e-15, rot, L.st))
+        return fail(c, BEM_E_DEVICE, "jacobi launch");
+      DEV(c, hipMemcpyAsync(L.h_lri, rot, sizeof(int), hipMemcpyDeviceToHost, L.st));
+      DEV(c, hipStreamSynchronize(L.st));
+      if (!L.h_lri[0]) return 0;
+    }
+    return 0;
+  };
+  // pass 1: Vt = V^H (n x k), Vt W1 = Y1 with orthogonal columns; Q1 = Y1 / |Y1|
+  DEV(c, hipMemsetAsync(Vt, 0, sizeof(double2) * (size_t)n * ke, L.st));
+  if (bem_transpose_op(V, ldv, k, n, Vt, n, 1, 0, L.st)) return fail(c, BEM_E_DEVICE, "transpose launch");
+  DEV(c, hipMemcpyAsync(W, eye.data(), sizeof(double2) * eye.size(), hipMemcpyHostToDevice, L.st));
+  CHK(sweeps(Vt, n));
+  if (bem_col_norms(Vt, n, n, ke, sig, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
+  // U1 = U W1 Sigma1 (the zero pad column of U contributes nothing)
+  DEV(c, hipMemsetAsync(U1, 0, sizeof(double2) * (size_t)m * ke, L.st));
+  CHK(gemm(c, L, m, ke, k, one, U, ldu, W, ke, zero, U1, m));
+  std::vector<double> s1(ke);
+  DEV(c, hipMemcpyAsync(s1.data(), sig, sizeof(double) * ke, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  std::vector<int> id(ke);
+  std::vector<double> inv1(ke);
+  for (int j = 0; j < ke; j++) {
+    id[j] = j;
+    inv1[j] = s1[j] > 0 ? 1.0 / s1[j] : 0.0;
+  }
+  DEV(c, hipMemcpyAsync(idx, id.data(), sizeof(int) * ke, hipMemcpyHostToDevice, L.st));
+  DEV(c, hipMemcpyAsync(sig + ke, inv1.data(), sizeof(double) * ke, hipMemcpyHostToDevice, L.st));
+  // U1 <- U1 Sigma1 (column scale), Vt <- Q1 (normalised columns), in place
+  if (bem_gather_cols(U1, m, m, idx, sig, ke, U1, m, L.st) ||
+      bem_gather_cols(Vt, n, n, idx, sig + ke, ke, Vt, n, L.st))
+    return fail(c, BEM_E_DEVICE, "column scale launch");
+  // pass 2: U1 W2 = Y2 orthogonal columns, sigma = |Y2_j|; U V = Y2 (Q1 W2)^H
+  DEV(c, hipMemcpyAsync(W, eye.data(), sizeof(double2) * eye.size(), hipMemcpyHostToDevice, L.st));
+  CHK(sweeps(U1, m));
+  if (bem_col_norms(U1, m, m, ke, sig, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
+  CHK(gemm(c, L, n, ke, ke, one, Vt, n, W, ke, zero, QW, n));
+  std::vector<double> s2(ke);
+  DEV(c, hipMemcpyAsync(s2.data(), sig, sizeof(double) * ke, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipStreamSynchronize(L.st));
+  std::stable_sort(id.begin(), id.end(), [&](int a, int b) { return s2[a] > s2[b]; });
+  const double smax = s2[id[0]];
+  int kn = 0;
+  for (int j = 0; j < k; j++)
+    if (s2[id[j]] >= eps * smax && s2[id[j]] > 0) kn++;
+  if (sigma_host)
+    for (int j = 0; j < k; j++) sigma_host[j] = s2[id[j]];
+  DEV(c, hipMemcpyAsync(idx, id.data(), sizeof(int) * ke, hipMemcpyHostToDevice, L.st));
+  // U <- Y2(:, kept) (= U_svd S), V <- (Q1 W2)(:, kept)^H
+  if (kn > 0) {
+    if (bem_gather_cols(U1, m, m, idx, nullptr, kn, U, ldu, L.st) ||
+        bem_gather_cols(QW, n, n, idx, nullptr, kn, Vt, n, L.st) ||
+        bem_transpose_op(Vt, n, n, kn, V, ldv, 1, 0, L.st))
+      return fail(c, BEM_E_DEVICE, "truncation gather launch");
+  }
+  DEV(c, hipStreamSynchronize(L.st));
+  *knew = kn;
+  return 0;
+}
+
+// Compression of B by the reference's route: ACA with full pivoting on a
+// copy of B, then Rk::truncate.  ACA stops on the size of its last increment
+// and the truncation on singular values relative to the largest, neither a
+// bound on ||B - U Vh||_F, so both run a decade tighter than tol:
+// eps2 = (tol / 10)^2 and eps = tol / 10.  U (m1 x rmax) and Vh (rmax x m2,
+// leading dimension ldv) receive the factors; *rank as lr_compress_rand.
+static int lr_compress_aca(bem_ctx *c, Lane &L, const double2 *B, int m1, int m2, double tol, int rmax, double2 *U,
+                           double2 *Vh, int ldv, int *rank) {
+  *rank = -1;
+  double2 *M;
+  int *rc;
+  CHK(ws_get(c, L, "lr_M", sizeof(double2) * (size_t)m1 * m2, (void **)&M));
+  CHK(ws_get(c, L, "lr_rc", sizeof(int) * 2 * (size_t)rmax, (void **)&rc));
+  DEV(c, hipMemcpyAsync(M, B, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
+  int r = 0, full = 0;
+  const double t = 0.1 * tol;
+  CHK(aca_run(c, L, M, m1, m1, m2, t * t, rmax, U, m1, Vh, ldv, rc, rc + rmax, &r, &full));
+  if (full) return 0;
+  if (r == 0) {
+    *rank = 0;
+    return 0;
+  }
+  int kn = 0;
+  CHK(rk_truncate(c, L, U, m1, Vh, ldv, m1, m2, r, t, &kn, nullptr));
+  *rank = kn;
+  return 0;
+}
+
+// Error probe of a low-rank solve, as probe() for the dense one: w seeded
+// (m2 x k); x1 = X w = -P (Z w); x2 = Sc~^-1 w = D^-1 (w + U2 K Z w) (the
+// Woodbury form of the compressed Schur complement); r = [0; w] - W x with
+// the dense A, B, C, D; d_1 = u + X (r2 - C u), u = A^-1 r1; h_j = g_j^H d_1j,
+// g = sym G1 X sym G2 w.  Since W holds the dense B and C, the residual and
+// the estimate include the truncation of B.
+static int probe_lowrank(bem_ctx *c, Lane &L, const LuRef<double2> &A, const LuRef<double2> &D, const double2 *Araw,
+                         const double2 *Draw, const double2 *Braw, const double2 *Cm, const double2 *Gs1,
+                         const double2 *Gs2, const double2 *P, const double2 *Z, const double2 *K, const double2 *U2,
+                         int r, ProbeOut *out) {
+  const int m1 = A.n, m2 = D.n, k = BEM_PROBE_COLS;
+  const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+  float2 *w32;
+  double2 *w, *x1, *x2, *r1, *r2, *t1, *t2, *zr, *yr;
+  double *part;
+  CHK(ws_get(c, L, "pr_w32", sizeof(float2) * (size_t)m2 * k, (void **)&w32));
+  CHK(ws_get(c, L, "pr_w", sizeof(double2) * (size_t)m2 * k, (void **)&w));
+  CHK(ws_get(c, L, "pr_x1", sizeof(double2) * (size_t)m1 * k, (void **)&x1));
+  CHK(ws_get(c, L, "pr_x2", sizeof(double2) * (size_t)m2 * k, (void **)&x2));
+  CHK(ws_get(c, L, "pr_r1", sizeof(double2) * (size_t)m1 * k, (void **)&r1));
+  CHK(ws_get(c, L, "pr_r2", sizeof(double2) * (size_t)m2 * k, (void **)&r2));
+  CHK(ws_get(c, L, "pr_t1", sizeof(double2) * (size_t)m1 * k, (void **)&t1));
+  CHK(ws_get(c, L, "pr_t2", sizeof(double2) * (size_t)m2 * k, (void **)&t2));
+  CHK(ws_get(c, L, "pr_zr", sizeof(double2) * (size_t)r * k, (void **)&zr));
+  CHK(ws_get(c, L, "pr_yr", sizeof(double2) * (size_t)r * k, (void **)&yr));
+  CHK(ws_get(c, L, "pr_part", sizeof(double) * std::max(bem_sumsq_partials(m1, k), bem_sumsq_partials(m2, k)),
+             (void **)&part));
+  const size_t wb2 = sizeof(double2) * (size_t)m2 * k;
+  if (bem_fill_hash(w32, sizeof(float2) * (size_t)m2 * k, 1, PROBE_SEED, L.st) ||
+      bem_c2z(w32, m2, w, m2, m2, k, L.st))
+    return fail(c, BEM_E_DEVICE, "probe fill launch");
+  DEV(c, hipMemsetAsync(out, 0, 2 * sizeof(unsigned long long), L.st));
+  // x1 = -P (Z w);  x2 = D^-1 (w + U2 K (Z w))
+  CHK(gemm_few(c, L, r, k, m2, one, Z, r, w, m2, zero, zr, r));
+  CHK(gemm_few(c, L, m1, k, r, mone, P, m1, zr, r, zero, x1, m1));
+  CHK(gemm_few(c, L, r, k, r, one, K, r, zr, r, zero, yr, r));
+  DEV(c, hipMemcpyAsync(x2, w, wb2, hipMemcpyDeviceToDevice, L.st));
+  CHK(gemm_few(c, L, m2, k, r, one, U2, m2, yr, r, one, x2, m2));
+  CHK(lu_solve_few(c, L, D.lu, m2, D.ipiv, D.inv, x2, m2, k, D.local));
+  // r1 = -(A x1 + B x2), r2 = w - C x1 - D x2
+  CHK(gemm_few(c, L, m1, k, m1, mone, Araw, m1, x1, m1, zero, r1, m1));
+  CHK(gemm_few(c, L, m1, k, m2, mone, Braw, m1, x2, m2, one, r1, m1));
+  DEV(c, hipMemcpyAsync(r2, w, wb2, hipMemcpyDeviceToDevice, L.st));
+  CHK(gemm_few(c, L, m2, k, m1, mone, Cm, m2, x1, m1, one, r2, m2));
+  CHK(gemm_few(c, L, m2, k, m2, mone, Draw, m2, x2, m2, one, r2, m2));
+  if (bem_sumsq(r1, m1, m1, k, part, 0, &out->rr, L.st) || bem_sumsq(r2, m2, m2, k, part, 1, &out->rr, L.st) ||
+      bem_sumsq(w, m2, m2, k, part, 0, &out->ww, L.st))
+    return fail(c, BEM_E_DEVICE, "probe norm launch");
+  // d_1 = u + X (r2 - C u) = u - P (Z (r2 - C u)), u = A^-1 r1
+  DEV(c, hipMemcpyAsync(t1, r1, sizeof(double2) * (size_t)m1 * k, hipMemcpyDeviceToDevice, L.st));
+  CHK(lu_solve_few(c, L, A.lu, m1, A.ipiv, A.inv, t1, m1, k, A.local));   // u
+  DEV(c, hipMemcpyAsync(t2, r2, wb2, hipMemcpyDeviceToDevice, L.st));
+  CHK(gemm_few(c, L, m2, k, m1, mone, Cm, m2, t1, m1, one, t2, m2));
+  CHK(gemm_few(c, L, r, k, m2, one, Z, r, t2, m2, zero, zr, r));
+  CHK(gemm_few(c, L, m1, k, r, mone, P, m1, zr, r, one, t1, m1));
+  int e = bem_absmax(t1, m1, m1, k, 0, &out->dmax, L.st);
+  if (!e) e = bem_absmax(x1, m1, m1, k, 0, &out->xmax, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "probe absmax launch: %d", e);
+  // g = sym G1 X sym G2 w = -sym G1 P Z sym G2 w
+  CHK(gemm_few(c, L, m2, k, m2, one, Gs2, m2, w, m2, zero, r2, m2));
+  CHK(gemm_few(c, L, r, k, m2, one, Z, r, r2, m2, zero, zr, r));
+  CHK(gemm_few(c, L, m1, k, r, one, P, m1, zr, r, zero, r1, m1));
+  CHK(gemm_few(c, L, m1, k, m1, mone, Gs1, m1, r1, m1, zero, x1, m1));
+  if ((e = bem_zdotc_cols(x1, m1, t1, m1, m1, k, out->h, L.st))) return fail(c, BEM_E_DEVICE, "probe dot launch: %d", e);
+  return 0;
+}
+
+// Elimination on the compressed coupling B ~ U Vh (U m1 x r, Vh r x m2,
+// leading dimension ldv), C = S B^T S ~ U2 V2^H with U2 = S Vh^T and
+// V2 = S conj(U), against LU(A) (E1) and LU(D) (E2):
+//   P = A^-1 U, K = V2^H P, Rh = Vh D^-1, F = Rh U2, T = I - F K,
+//   Z = T^-1 Rh, X = -P Z,
+//   4 S = tr(M1 M2), M1 = P^H sym G1 P, M2 = Z sym G2 Z^H (both Hermitian, so
+//   the trace is the Hermitian-trace GEMM of Zt^H (sym G2 Zt) against M1,
+//   Zt = Z^H),
+// then the probe against the dense B and C.
+static int solve_lowrank(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry &E2, const double2 *Bm,
+                         const double2 *Cm, const double2 *U, const double2 *Vh, int ldv, int r, int *info,
+                         double2 *trace_out, ProbeOut *pout) {
+  const int m1 = E1.n, m2 = E2.n;
+  const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
+  double2 *P, *Qc, *K, *Rh, *U2, *F, *T, *invT, *Zt, *H1, *M1, *W2, *partial;
+  int *ipT;
+  CHK(ws_get(c, L, "lr_P", sizeof(double2) * (size_t)m1 * r, (void **)&P));
+  CHK(ws_get(c, L, "lr_Qc", sizeof(double2) * (size_t)m1 * r, (void **)&Qc));
+  CHK(ws_get(c, L, "lr_K", sizeof(double2) * (size_t)r * r, (void **)&K));
+  CHK(ws_get(c, L, "lr_Rh", sizeof(double2) * (size_t)r * m2, (void **)&Rh));
+  CHK(ws_get(c, L, "lr_U2", sizeof(double2) * (size_t)m2 * r, (void **)&U2));
+  CHK(ws_get(c, L, "lr_F", sizeof(double2) * (size_t)r * r, (void **)&F));
+  CHK(ws_get(c, L, "lr_T", sizeof(double2) * (size_t)r * r, (void **)&T));
+  CHK(ws_get(c, L, "lr_invT", inv_bytes(r), (void **)&invT));
+  CHK(ws_get(c, L, "lr_ipT", sizeof(int) * (size_t)(r + 64), (void **)&ipT));
+  CHK(ws_get(c, L, "lr_Zt", sizeof(double2) * (size_t)m2 * r, (void **)&Zt));
+  CHK(ws_get(c, L, "lr_H1", sizeof(double2) * (size_t)m1 * r, (void **)&H1));
+  CHK(ws_get(c, L, "lr_M1", sizeof(double2) * (size_t)r * r, (void **)&M1));
+  CHK(ws_get(c, L, "lr_W2", sizeof(double2) * (size_t)m2 * r, (void **)&W2));
+  CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (bem_htrace_partials(r) + 1), (void **)&partial));
+  // P = A^-1 U;  K = (S conj U)^H P
+  DEV(c, hipMemcpyAsync(P, U, sizeof(double2) * (size_t)m1 * r, hipMemcpyDeviceToDevice, L.st));
+  CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, P, m1, r, !E1.pivot));
+  if (bem_conj_sign(U, m1, m1, r, Qc, m1, 1, L.st)) return fail(c, BEM_E_DEVICE, "conjugate launch");
+  CHK(gemm_ta(c, L, r, r, m1, one, Qc, m1, P, m1, zero, K, r));
+  // Rh = Vh D^-1;  U2 = S Vh^T;  F = Rh U2;  T = I - F K
+  DEV(c, hipMemcpy2DAsync(Rh, sizeof(double2) * r, Vh, sizeof(double2) * ldv, sizeof(double2) * r, m2,
+                          hipMemcpyDeviceToDevice, L.st));
+  CHK(lu_solve_right(c, L, E2.Z, m2, m2, E2.ipiv, E2.inv, Rh, r, r, !E2.pivot));
+  if (bem_transpose_op(Vh, ldv, r, m2, U2, m2, 0, 1, L.st)) return fail(c, BEM_E_DEVICE, "transpose launch");
+  CHK(gemm(c, L, r, r, m2, one, Rh, r, U2, m2, zero, F, r));
+  CHK(gemm(c, L, r, r, r, one, F, r, K, r, zero, T, r));
+  if (bem_identity_minus(T, r, r, L.st)) return fail(c, BEM_E_DEVICE, "identity launch");
+  // Z = T^-1 Rh (tournament pivoting: T is small and carries the coupling's conditioning)
+  CHK(lu_factor(c, L, T, r, r, ipT, info, invT));
+  CHK(lu_solve(c, L, T, r, r, ipT, invT, Rh, r, m2, false));
+  double2 *Z = Rh;
+  DEV(c, hipEventRecord(L.ph[2], L.st));
+  // 4 S = tr(M1 M2)
+  if (bem_transpose_op(Z, r, r, m2, Zt, m2, 1, 0, L.st)) return fail(c, BEM_E_DEVICE, "transpose launch");
+  CHK(gemm(c, L, m1, r, m1, one, E1.Gs, m1, P, m1, zero, H1, m1));
+  CHK(gemm_ta(c, L, r, r, m1, one, P, m1, H1, m1, zero, M1, r));
+  CHK(gemm(c, L, m2, r, m2, one, E2.Gs, m2, Zt, m2, zero, W2, m2));
+  EventRec *er;
+  CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m2 * (double)r * r, 16.0 * (2.0 * m2 * r + 0.5 * r * r), &er));
+  int e = bem_zgemm_htrace(r, m2, Zt, m2, W2, m2, M1, r, partial, trace_out, L.st);
+  if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
+  CHK(ev_end(c, L, er));
+  const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, D{E2.Z, m2, E2.ipiv, E2.inv, !E2.pivot};
+  return probe_lowrank(c, L, A, D, E1.Araw, E2.Araw, Bm, Cm, E1.Gs, E2.Gs, P, Z, K, U2, r, pout);
+}
+
 // Issues every kernel of the solve stage of one spectral-power request on the
 // lane's stream and returns without waiting; lane_step() collects the result
 // and decides what follows.
@@ -1561,15 +1954,32 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   const int pivot = (qq.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
   const int prec = pivot ? BEM_PREC_FP64 : qq.precision;
   L.prec = prec;
-  L.refine = (qq.flags & BEM_SOLVE_REFINE) != 0;
+  // the low-rank coupling runs in FP64 with block-local pivoting and no
+  // refinement (the probe checks it against the dense blocks instead)
+  const bool lowrank = (qq.flags & BEM_SOLVE_LOWRANK) && prec == BEM_PREC_FP64 && !pivot;
+  L.refine = !lowrank && (qq.flags & BEM_SOLVE_REFINE) != 0;
+  L.rank = 0;
+  L.ms_compress = 0.0;
   Medium mext = make_medium(1.0, 0.0, 1.0, 0.0, omega);
   DEV(c, hipEventRecord(L.ph[0], L.st));
-  // body 2 (raw) first: when both bodies share one mesh and medium, body 1's
-  // factored entry is then a copy of it instead of a second assembly
-  CHK(self_get(c, L, b2, qq.eps2, qq.mu2, omega, 1, 0, prec, nullptr, &L.e2));
   const bool twins = same_mesh(b1, b2) && std::memcmp(qq.eps1, qq.eps2, sizeof qq.eps1) == 0 &&
                      std::memcmp(qq.mu1, qq.mu2, sizeof qq.mu1) == 0;
-  CHK(self_get(c, L, b1, qq.eps1, qq.mu1, omega, 0, pivot, prec, twins ? L.e2 : nullptr, &L.e1));
+  if (lowrank) {
+    // both self blocks factored (LU(D) serves the Woodbury solves); twin
+    // bodies share one entry, since A and D are then the same matrix
+    CHK(self_get(c, L, b2, qq.eps2, qq.mu2, omega, 0, 0, prec, nullptr, &L.e2));
+    if (twins) {
+      L.e1 = L.e2;
+      L.e2->pins++;
+    } else {
+      CHK(self_get(c, L, b1, qq.eps1, qq.mu1, omega, 0, 0, prec, nullptr, &L.e1));
+    }
+  } else {
+    // body 2 (raw) first: when both bodies share one mesh and medium, body 1's
+    // factored entry is then a copy of it instead of a second assembly
+    CHK(self_get(c, L, b2, qq.eps2, qq.mu2, omega, 1, 0, prec, nullptr, &L.e2));
+    CHK(self_get(c, L, b1, qq.eps1, qq.mu1, omega, 0, pivot, prec, twins ? L.e2 : nullptr, &L.e1));
+  }
   const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
   const int m1 = E1.n, m2 = E2.n;
   if (m1 != 2 * g.E1 || m2 != 2 * (g.top.E - g.E1)) return fail(c, BEM_E_INVALID, "body sizes do not match the pair tree");
@@ -1580,18 +1990,43 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   ProbeOut *pout;
   CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
   CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
-  if (prec == BEM_PREC_FP64) CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
-  else Braw = Bm;   // the mixed elimination works on complex64 copies: B stays unfactored
+  if (prec == BEM_PREC_FP64 && !lowrank) CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
+  else Braw = Bm;   // the mixed and low-rank eliminations leave B unfactored
   CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
   CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 3, (void **)&gro));
   CHK(ws_get(c, L, "trace_out", sizeof(double2) * 2, (void **)&trace_out));
   CHK(ws_get(c, L, "probe_out", sizeof(ProbeOut), (void **)&pout));
   DEV(c, hipMemsetAsync(info, 0, sizeof(int) * 4, L.st));
   DEV(c, hipMemsetAsync(gro, 0, sizeof(unsigned long long) * 3, L.st));
-  CHK(assemble_cross(c, L, g, mext, Bm, Cm, prec == BEM_PREC_FP64 ? Braw : nullptr, info + 1));
+  CHK(assemble_cross(c, L, g, mext, Bm, Cm, prec == BEM_PREC_FP64 && !lowrank ? Braw : nullptr, info + 1));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
-  if (prec == BEM_PREC_MIXED) {
+  bool solved = false;
+  if (lowrank) {
+    const double tol = qq.rank_tol > 0 ? qq.rank_tol : BEM_LOWRANK_TOL;
+    const int rmax = ((int)(BEM_LOWRANK_MAX_FRAC * std::min(m1, m2)) / LR_BLK) * LR_BLK;
+    int r = -1;
+    if (rmax >= LR_BLK) {
+      double2 *Q, *Vh;
+      CHK(ws_get(c, L, "lr_Q", sizeof(double2) * (size_t)m1 * rmax, (void **)&Q));
+      CHK(ws_get(c, L, "lr_Vh", sizeof(double2) * (size_t)rmax * m2, (void **)&Vh));
+      const auto tc = std::chrono::steady_clock::now();
+      if (qq.flags & BEM_LOWRANK_ACA) CHK(lr_compress_aca(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, &r));
+      else CHK(lr_compress_rand(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, &r));
+      L.ms_compress = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
+      if (r > 0) {
+        L.rank = r;
+        CHK(solve_lowrank(c, L, E1, E2, Bm, Cm, Q, Vh, rmax, r, info, trace_out, pout));
+        solved = true;
+      }
+    }
+    if (!solved) {   // not compressible within rmax: the dense solve on the same blocks
+      CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
+      DEV(c, hipMemcpyAsync(Braw, Bm, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
+    }
+  }
+  if (solved) {
+  } else if (prec == BEM_PREC_MIXED) {
     CHK(solve_mixed(c, L, E1, E2, Bm, Cm, info, gro, trace_out, pout, L.refine));
     DEV(c, hipEventRecord(L.ph[2], L.st));
   } else {
@@ -1603,7 +2038,7 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
     CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
     size_t np = bem_htrace_partials(m2);
     CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
-    DEV(c, hipMemcpyAsync(Sc, E2.Z, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
+    DEV(c, hipMemcpyAsync(Sc, E2.Araw, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
     // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
     const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
     CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2, !E1.pivot));   // B <- Y
@@ -1770,8 +2205,14 @@ static int lane_collect(bem_ctx *c, Lane &L) {
   res->full_residual = -1.0;
   res->status = 0;
   res->growth = 0.0;
+  res->rank = L.rank;
+  res->ms_compress = L.ms_compress;
   int sing_a = E1 ? E1->h->info[0] : 0, ovf = L.h_info[1] || (E1 && E1->h->info[1]) || (E2 && E2->h->info[1]);
   if (!res->pivoting && E1) res->growth = std::max(growth_ratio(E1->h->growth), growth_ratio(L.h_growth));
+  if (E2 && E2 != E1 && E2->role == 0) {   // a factored D (low-rank coupling)
+    sing_a = sing_a || E2->h->info[0];
+    res->growth = std::max(res->growth, growth_ratio(E2->h->growth));
+  }
   c->t_last[0] = ta;
   c->t_last[4] = tf;
   c->t_last[5] = tp;
@@ -1812,7 +2253,13 @@ static int lane_step(bem_ctx *c, Lane &L, int *err) {
   *err = r;
   const bool tourn = (L.req.flags & BEM_SOLVE_TOURNAMENT) != 0;
   bool restart = false;
-  if (L.stage == ST_SOLVE && !tourn && L.restarts < 2 &&
+  if (L.stage == ST_SOLVE && L.rank > 0 && L.restarts < 2 &&
+      (r == BEM_E_SINGULAR ||
+       (!r && (!(res->growth <= BEM_GROWTH_LIMIT) || !(res->rel_err_est <= BEM_PROBE_LIMIT))))) {
+    // a low-rank result the probe (or a factorisation) rejects: the dense solve
+    L.req.flags &= ~BEM_SOLVE_LOWRANK;
+    restart = true;
+  } else if (L.stage == ST_SOLVE && !tourn && L.restarts < 2 &&
       (r == BEM_E_SINGULAR ||
        (!r && (!(res->growth <= BEM_GROWTH_LIMIT) ||
                (L.prec == BEM_PREC_FP64 && !(res->rel_err_est <= BEM_PROBE_LIMIT)))))) {
@@ -2345,6 +2792,8 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     ok = ok && hipHostMalloc((void **)&L.h_growth, sizeof(unsigned long long) * 4, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_probe, sizeof(ProbeOut), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_rfull, sizeof(double), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_lr, sizeof(double) * 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_lri, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
     if (!ok) {
       bem_ctx_destroy(c);
       return BEM_E_DEVICE;
@@ -2389,6 +2838,8 @@ void bem_ctx_destroy(bem_ctx *c) {
     if (L.h_growth) (void)hipHostFree(L.h_growth);
     if (L.h_probe) (void)hipHostFree(L.h_probe);
     if (L.h_rfull) (void)hipHostFree(L.h_rfull);
+    if (L.h_lr) (void)hipHostFree(L.h_lr);
+    if (L.h_lri) (void)hipHostFree(L.h_lri);
     if (L.st) (void)hipStreamDestroy(L.st);
     if (L.aux) (void)hipStreamDestroy(L.aux);
     if (L.la_a) (void)hipEventDestroy(L.la_a);
@@ -2929,6 +3380,38 @@ int bem_zgetrf_local(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv, dou
   if (growth) *growth = growth_ratio(g);
   if (h) return fail(c, BEM_E_SINGULAR, "zero pivot at column %d", h - 1);
   return 0;
+}
+
+int bem_aca(bem_ctx *c, void *A, int64_t lda, int m, int n, double eps2, int kmax, void *U, int64_t ldu, void *V,
+            int64_t ldv, int32_t *rows, int32_t *cols, int *rank) {
+  if (!c || !A || !U || !V || !rows || !cols || !rank || m <= 0 || n <= 0 || kmax <= 0 || lda < m || ldu < m ||
+      ldv < kmax)
+    return fail(c, BEM_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  int *rc;
+  CHK(ws_get(c, L, "aca_rc", sizeof(int) * 2 * (size_t)kmax, (void **)&rc));
+  int r = 0, full = 0;
+  CHK(aca_run(c, L, (double2 *)A, lda, m, n, eps2, kmax, (double2 *)U, ldu, (double2 *)V, ldv, rc, rc + kmax, &r,
+              &full));
+  DEV(c, hipMemcpy(rows, rc, sizeof(int) * (size_t)kmax, hipMemcpyDeviceToHost));
+  DEV(c, hipMemcpy(cols, rc + kmax, sizeof(int) * (size_t)kmax, hipMemcpyDeviceToHost));
+  *rank = r;
+  if (full) return fail(c, BEM_E_UNSUPPORTED, "aca: %d columns used up before the tolerance was met", kmax);
+  return 0;
+}
+
+int bem_rk_truncate(bem_ctx *c, void *U, int64_t ldu, void *V, int64_t ldv, int m, int n, int k, double eps, int *knew,
+                    double *sigma) {
+  if (!c || !U || !V || !knew || m <= 0 || n <= 0 || k < 0 || ldu < m || ldv < k || !(eps >= 0))
+    return fail(c, BEM_E_INVALID, "bad arguments");
+  std::lock_guard<std::recursive_mutex> lk(c->mu);
+  DEV(c, hipSetDevice(c->device));
+  Lane &L = c->lanes[0];
+  CHK(lane_retire(c, L));
+  return rk_truncate(c, L, (double2 *)U, ldu, (double2 *)V, ldv, m, n, k, eps, knew, sigma);
 }
 
 int bem_zgetrf_local_noalloc(bem_ctx *c, void *A, int64_t lda, int n, int32_t *ipiv, double *growth) {

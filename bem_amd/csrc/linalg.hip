@@ -1663,6 +1663,78 @@ void launch_zgemm(int m, int n, int k, double2 alpha, const double2 *A, int64_t 
                      lda, B, ldb, beta, C, ldc);
 }
 
+
+// C (m x n) <- alpha A^H B + beta C with A (k x m) and B (k x n) column-major:
+// the projections and small Gram products of the low-rank coupling, whose
+// output is small and whose k is long.  K is split over gridDim.z: with one
+// split the epilogue applies alpha and beta to C; otherwise split z writes its
+// partial tile to part + z m n (leading dimension m) and ta_finish adds the
+// splits in z order (deterministic, no atomics).
+template <int BM, int BN, int WM, int WN, int GBK>
+__global__ void __launch_bounds__(64 * WM * WN)
+zgemm_ta_kernel(int m, int n, int k, int kper, const double2 *__restrict__ A, int64_t lda,
+                const double2 *__restrict__ B, int64_t ldb, double2 alpha, double2 beta, double2 *__restrict__ C,
+                int64_t ldc, double2 *__restrict__ part) {
+  constexpr int FM = BM / WM / 16, FN = BN / WN / 16;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wr = wave / WN, wc = wave % WN;
+  const int li = lane & 15, lk = lane >> 4;
+  const int m0 = blockIdx.x * BM, n0 = blockIdx.y * BN;
+  const int kz0 = blockIdx.z * kper, kl = min(kper, k - kz0);
+  dbl4 acc[3][FM][FN];
+  zgemm_tile<BM, BN, WM, WN, GBK, true, true>(m, n, kl, A + kz0, lda, B + kz0, ldb, m0, n0, acc);
+  const bool split = gridDim.z > 1;
+  double2 *out = split ? part + (int64_t)blockIdx.z * m * n : C;
+  const int64_t ldo = split ? m : ldc;
+  const bool rd = !split && (beta.x != 0.0 || beta.y != 0.0);
+#pragma unroll
+  for (int x = 0; x < FM; x++)
+#pragma unroll
+    for (int y = 0; y < FN; y++)
+#pragma unroll
+      for (int r = 0; r < 4; r++) {
+        // element layout as in zgemm_kernel's epilogue
+        const int row = m0 + wr * (BM / WM) + x * 16 + li;
+        const int col = n0 + wc * (BN / WN) + y * 16 + 4 * r + lk;
+        if (row < m && col < n) {
+          const double t1 = acc[0][x][y][r], t2 = acc[1][x][y][r];
+          const double cr = t1 - t2, ci = acc[2][x][y][r] - t1 - t2;
+          double2 v;
+          if (split) {
+            v = make_double2(cr, ci);
+          } else {
+            v = make_double2(alpha.x * cr - alpha.y * ci, alpha.x * ci + alpha.y * cr);
+            if (rd) {
+              const double2 o = C[(int64_t)col * ldc + row];
+              v.x += beta.x * o.x - beta.y * o.y;
+              v.y += beta.x * o.y + beta.y * o.x;
+            }
+          }
+          out[(int64_t)col * ldo + row] = v;
+        }
+      }
+}
+
+__global__ void zgemm_ta_finish_kernel(int m, int n, int nz, const double2 *__restrict__ part, double2 alpha,
+                                       double2 beta, double2 *__restrict__ C, int64_t ldc) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x, col = blockIdx.y;
+  if (row >= m) return;
+  double2 s = make_double2(0.0, 0.0);
+  for (int z = 0; z < nz; z++) {
+    const double2 v = part[(int64_t)z * m * n + (int64_t)col * m + row];
+    s.x += v.x;
+    s.y += v.y;
+  }
+  double2 v = make_double2(alpha.x * s.x - alpha.y * s.y, alpha.x * s.y + alpha.y * s.x);
+  if (beta.x != 0.0 || beta.y != 0.0) {
+    const double2 o = C[(int64_t)col * ldc + row];
+    v.x += beta.x * o.x - beta.y * o.y;
+    v.y += beta.x * o.y + beta.y * o.x;
+  }
+  C[(int64_t)col * ldc + row] = v;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
@@ -2079,4 +2151,39 @@ int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+// K splits of bem_zgemm_ta for an m x n output over k: enough workgroups to
+// fill the device (about 512), each split at least 64 long.
+static int ta_splits(int m, int n, int k) {
+  const int blocks = ((m + 31) / 32) * ((n + 31) / 32);
+  int nz = (512 + blocks - 1) / blocks;
+  const int kmax = (k + 63) / 64;
+  if (nz > kmax) nz = kmax;
+  return nz < 1 ? 1 : nz;
+}
+size_t bem_zgemm_ta_partials(int m, int n, int k) {
+  const int nz = ta_splits(m, n, k);
+  return nz > 1 ? (size_t)nz * m * n : 0;
+}
+
+// C (m x n) <- alpha A^H B + beta C, A k x m and B k x n (column-major),
+// part holding bem_zgemm_ta_partials(m, n, k) values.
+int bem_zgemm_ta(int m, int n, int k, double2 alpha, const double2 *A, int64_t lda, const double2 *B, int64_t ldb,
+                 double2 beta, double2 *C, int64_t ldc, double2 *part, hipStream_t st) {
+  if (m <= 0 || n <= 0) return 0;
+  if (k <= 0) return -1;
+  int nz = ta_splits(m, n, k);
+  int kper = (k + nz - 1) / nz;
+  kper = (kper + 15) / 16 * 16;
+  nz = (k + kper - 1) / kper;
+  if ((n + 31) / 32 > 65535 || nz > 65535) return -1;
+  dim3 grid((m + 31) / 32, (n + 31) / 32, nz);
+  hipLaunchKernelGGL((zgemm_ta_kernel<32, 32, 2, 2, 16>), grid, dim3(256), 0, st, m, n, k, kper, A, lda, B, ldb, alpha,
+                     beta, C, ldc, part);
+  if (nz > 1)
+    hipLaunchKernelGGL(zgemm_ta_finish_kernel, dim3((m + 255) / 256, n), dim3(256), 0, st, m, n, nz,
+                       (const double2 *)part, alpha, beta, C, ldc);
+  return (int)hipGetLastError();
+}
+
 }  // extern "C"
+

@@ -28,6 +28,8 @@ BEM_PREC_MIXED = 1
 BEM_ASM_DEVICE_PTR = 1
 BEM_SOLVE_TOURNAMENT = 1
 BEM_SOLVE_REFINE = 2
+BEM_SOLVE_LOWRANK = 4
+BEM_LOWRANK_ACA = 8
 BEM_GROWTH_LIMIT = 16.0
 BEM_PROBE_COLS = 8
 BEM_PROBE_LIMIT = 1e-9
@@ -57,7 +59,7 @@ class _Req(ctypes.Structure):
     _fields_ = [("eV", ctypes.c_double), ("a", ctypes.c_double), ("T1", ctypes.c_double), ("T2", ctypes.c_double),
                 ("gap", ctypes.c_double * 3), ("eps1", ctypes.c_double * 2), ("mu1", ctypes.c_double * 2),
                 ("eps2", ctypes.c_double * 2), ("mu2", ctypes.c_double * 2), ("precision", ctypes.c_int),
-                ("flags", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("rank_tol", ctypes.c_double)]
 
 
 class _Res(ctypes.Structure):
@@ -68,7 +70,8 @@ class _Res(ctypes.Structure):
                 ("ms_products", ctypes.c_double), ("ms_total", ctypes.c_double), ("growth", ctypes.c_double),
                 ("pivoting", ctypes.c_int), ("ms_queue", ctypes.c_double), ("precision", ctypes.c_int),
                 ("restarts", ctypes.c_int), ("probe_err", ctypes.c_double), ("residual", ctypes.c_double),
-                ("full_residual", ctypes.c_double), ("err_first_order", ctypes.c_double)]
+                ("full_residual", ctypes.c_double), ("err_first_order", ctypes.c_double),
+                ("rank", ctypes.c_int), ("pad", ctypes.c_int), ("ms_compress", ctypes.c_double)]
 
 
 class _Dipole(ctypes.Structure):
@@ -100,7 +103,8 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_sweep", "bem_zgetrf", "bem_zgetrf_local", "bem_zgetrs", "bem_zgemm", "bem_device_alloc",
            "bem_device_free", "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
            "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune",
-           "bem_synchronize", "bem_zgetrf_local_noalloc", "bem_dipole_rhs", "bem_flux", "bem_dipole_flux"]
+           "bem_synchronize", "bem_zgetrf_local_noalloc", "bem_dipole_rhs", "bem_flux", "bem_dipole_flux",
+           "bem_aca", "bem_rk_truncate"]
 
 
 def load_library():
@@ -150,6 +154,8 @@ def load_library():
     L.bem_submit.argtypes = [vp, vp, vp, P(_Req), P(ctypes.c_uint64)]
     L.bem_poll.argtypes = [vp, ctypes.c_uint64, P(_Res), i32]
     L.bem_queue_depth.argtypes = [vp, P(i32), P(i32)]
+    L.bem_aca.argtypes = [vp, vp, i64, i32, i32, dbl, i32, vp, i64, vp, i64, vp, vp, P(i32)]
+    L.bem_rk_truncate.argtypes = [vp, vp, i64, vp, i64, i32, i32, i32, dbl, P(i32), vp]
     _lib = L
     return L
 
@@ -185,6 +191,7 @@ class Request:
     mu2: complex = 1.0
     precision: int = BEM_PREC_FP64
     flags: int = 0
+    rank_tol: float = 0.0
 
     def to_c(self):
         r = _Req()
@@ -197,6 +204,7 @@ class Request:
             arr[0], arr[1] = z.real, z.imag
         r.precision = self.precision
         r.flags = self.flags
+        r.rank_tol = self.rank_tol
         return r
 
 
@@ -491,6 +499,56 @@ class Context:
         finally:
             self.lib.bem_device_free(self.h, dA)
             self.lib.bem_device_free(self.h, dp)
+
+    def aca_host(self, A, eps2, kmax=None):
+        """Adaptive cross approximation with full pivoting of A (bem_aca,
+        device round trip): (rows, cols, U, V, residual) with A ~ U V."""
+        A = np.asfortranarray(A, dtype=np.complex128)
+        m, n = A.shape
+        kmax = min(m, n) + 1 if kmax is None else kmax
+        dA = self._dev(A)
+        dU = self._dev(np.zeros((m, kmax), dtype=np.complex128, order="F"))
+        dV = self._dev(np.zeros((kmax, n), dtype=np.complex128, order="F"))
+        rows = np.zeros(kmax, dtype=np.int32)
+        cols = np.zeros(kmax, dtype=np.int32)
+        rank = ctypes.c_int(0)
+        try:
+            self._check(self.lib.bem_aca(self.h, dA, m, m, n, float(eps2), kmax, dU, m, dV, kmax, rows.ctypes.data,
+                                         cols.ctypes.data, ctypes.byref(rank)))
+            r = rank.value
+            U = np.zeros((m, kmax), dtype=np.complex128, order="F")
+            V = np.zeros((kmax, n), dtype=np.complex128, order="F")
+            R = np.zeros_like(A)
+            self._check(self.lib.bem_memcpy_d2h(self.h, U.ctypes.data, dU, U.nbytes))
+            self._check(self.lib.bem_memcpy_d2h(self.h, V.ctypes.data, dV, V.nbytes))
+            self._check(self.lib.bem_memcpy_d2h(self.h, R.ctypes.data, dA, R.nbytes))
+            return rows[:r].copy(), cols[:r].copy(), U[:, :r].copy(), V[:r, :].copy(), R
+        finally:
+            for p in (dA, dU, dV):
+                self.lib.bem_device_free(self.h, p)
+
+    def rk_truncate_host(self, U, V, eps):
+        """Rk::truncate of U V (bem_rk_truncate, device round trip):
+        (U', V', sigma) with U' V' the SVD truncation at eps s_max."""
+        U = np.asfortranarray(U, dtype=np.complex128)
+        V = np.asfortranarray(V, dtype=np.complex128)
+        m, k = U.shape
+        n = V.shape[1]
+        dU, dV = self._dev(U), self._dev(V)
+        kn = ctypes.c_int(0)
+        sig = np.zeros(max(k, 1), dtype=np.float64)
+        try:
+            self._check(self.lib.bem_rk_truncate(self.h, dU, m, dV, k, m, n, k, float(eps), ctypes.byref(kn),
+                                                 sig.ctypes.data))
+            Uo = np.zeros_like(U)
+            Vo = np.zeros_like(V)
+            self._check(self.lib.bem_memcpy_d2h(self.h, Uo.ctypes.data, dU, Uo.nbytes))
+            self._check(self.lib.bem_memcpy_d2h(self.h, Vo.ctypes.data, dV, Vo.nbytes))
+            r = kn.value
+            return Uo[:, :r].copy(), Vo[:r, :].copy(), sig[:k].copy()
+        finally:
+            for p in (dU, dV):
+                self.lib.bem_device_free(self.h, p)
 
     def zgesv_host(self, A, B):
         A = np.asfortranarray(A, dtype=np.complex128)

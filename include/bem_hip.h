@@ -75,6 +75,8 @@ typedef struct {
   double eps2[2], mu2[2];  /* body 2 medium (re, im) at eV */
   int precision;     /* BEM_PREC_* */
   int flags;         /* BEM_SOLVE_* bits */
+  double rank_tol;   /* BEM_SOLVE_LOWRANK: relative Frobenius tolerance of the compressed
+                        coupling block (0: BEM_LOWRANK_TOL) */
 } bem_request;
 
 /* Pivoting of the dense solve.  Default: partial pivoting inside each 64 x 64
@@ -119,6 +121,33 @@ typedef struct {
 #define BEM_SOLVE_REFINE 2
 #define BEM_REFINE_TOL 1e-10
 #define BEM_REFINE_MAX 3
+/* Low-rank coupling (BEM_SOLVE_LOWRANK, FP64 with block-local pivoting).  The
+ * cross block B = Z_ext(1,2) of the two disjoint bodies is compressed to
+ * B ~ U V^H with ||B - U V^H||_F <= rank_tol ||B||_F (default
+ * BEM_LOWRANK_TOL), and C = Z_ext(2,1) = S B^T S follows from it.  The
+ * elimination of the reference's block LU over the two-body tree
+ * (src/util/HMatrix.h:277-360: factor the diagonal blocks, substitute into
+ * the coupling blocks, update and factor the Schur complement) then runs on
+ * the factors: with LU(A) and LU(D) of the cached self blocks,
+ *   P = A^-1 U, K = V2^H P, R^H = V^H D^-1, F = R^H U2, T = I - F K,
+ *   X = W^-1[0:m1, m1:] = -P T^-1 R^H        (Woodbury for Sc^-1),
+ *   4 S = tr(M1 M2), M1 = P^H sym G1 P, M2 = Z sym G2 Z^H, Z = T^-1 R^H,
+ * with U2 = S conj(V), V2 = S conj(U); O(m^2 r) per gap instead of O(m^3).
+ * Compression: a blocked randomized range finder (64 seeded columns per
+ * block, block Gram-Schmidt twice and CholQR2 per block, stop when the
+ * residual estimate of the new block is under the tolerance), or, with
+ * BEM_LOWRANK_ACA, the reference's adaptive cross approximation with full
+ * pivoting (bem_aca) followed by SVD truncation (bem_rk_truncate), both at
+ * rank_tol / 10 since neither stopping rule bounds the residual.  The error
+ * probe runs against the dense B and C, so rel_err_est includes the
+ * truncation; a result above BEM_PROBE_LIMIT, a rank above
+ * BEM_LOWRANK_MAX_FRAC min(m1, m2), or a recovery condition of the dense
+ * path sends the request to the dense solve.  bem_result.rank reports the
+ * rank used (0: dense). */
+#define BEM_SOLVE_LOWRANK 4
+#define BEM_LOWRANK_ACA 8
+#define BEM_LOWRANK_TOL 1e-10
+#define BEM_LOWRANK_MAX_FRAC 0.5
 
 typedef struct {
   double P;          /* Re(S) (Theta(omega,T1) - Theta(omega,T2)) */
@@ -140,6 +169,9 @@ typedef struct {
   double full_residual; /* ||R||_F / ||I||_F of the last refinement step's full residual
                            (of the X before that step's correction); -1 without refinement */
   double err_first_order; /* the probe's signed estimate of (Re S_exact - Re S) / |Re S| */
+  int rank;          /* rank of the coupling block of a low-rank solve, 0 for the dense solve */
+  int pad;
+  double ms_compress;/* low-rank solves: compression of B (host clock, ms) */
 } bem_result;
 
 /* ---- context ---------------------------------------------------------- */
@@ -235,6 +267,31 @@ int bem_submit(bem_ctx *ctx, bem_body *b1, bem_body *b2, const bem_request *req,
 int bem_poll(bem_ctx *ctx, uint64_t ticket, bem_result *res, int wait_ms);
 /* Requests waiting in the queue and running on lanes. */
 int bem_queue_depth(bem_ctx *ctx, int *queued, int *running);
+
+/* ---- low-rank blocks (Rk) on device pointers --------------------------- */
+/* Adaptive cross approximation with full pivoting of A (m x n, lda; overwritten
+ * by the residual A - U V), as the reference's aca(DenseMatrix&, r, c, eps2)
+ * (src/util/DenseMatrix.h:213-279): each step takes the entry of largest
+ * |.|^2 of the residual (first in row-major scan order among equal ones),
+ * a = its column, b = its row / the pivot, subtracts a b^T, and continues
+ * while nu < min(m, n) and |a|^2 |b|^2 / sum_steps |a|^2 |b|^2 > eps2; the
+ * step that stops the loop is not kept, as the reference pushes no pivot for
+ * it.  Outputs: rank kept pivots rows[]/cols[] (host, kmax ints each), U
+ * (device, m x kmax, ldu) and V (device, kmax x n, ldv) whose first rank
+ * columns / rows are the kept steps' a and b, so A ~ U(:, :rank) V(:rank, :).
+ * BEM_E_UNSUPPORTED when kmax columns are used up before the loop ends. */
+int bem_aca(bem_ctx *ctx, void *A, int64_t lda, int m, int n, double eps2, int kmax, void *U, int64_t ldu, void *V,
+            int64_t ldv, int32_t *rows, int32_t *cols, int *rank);
+/* Rk::truncate (src/util/Rk.cpp:4-53) of the rank-k product U V (U m x k,
+ * ldu; V k x n, ldv; device): its singular value decomposition -- here two
+ * one-sided Jacobi passes, on V^H and then on U times the first pass's
+ * factors, in place of the reference's QR, RQ and SVD of the core -- keeping
+ * the singular values s_i >= eps s_max in decreasing order; U (m x knew) <-
+ * U_svd S and V (knew x n) <- V_svd^H, as the reference's A and B.  sigma
+ * (optional, host, k doubles) receives every singular value in decreasing
+ * order. */
+int bem_rk_truncate(bem_ctx *ctx, void *U, int64_t ldu, void *V, int64_t ldv, int m, int n, int k, double eps,
+                    int *knew, double *sigma);
 
 /* ---- dense complex128 kernels on device pointers ---------------------- */
 /* A (n x n, lda) <- P L U in place with tournament partial pivoting; ipiv

@@ -15,6 +15,10 @@ baseline, never as the product path.
   trace .................. src/util/DenseMatrix.h:489-497
   dipole RHS (C) ......... src/Source.cpp:88-193
   flux (C) ............... src/Flux.cpp:100-219
+  aca .................... src/util/DenseMatrix.h:213-279
+  Rk::truncate ........... src/util/Rk.cpp:4-53
+  block LU over the two-body tree with Rk coupling blocks (Woodbury form)
+                           src/util/HMatrix.h:277-360
 """
 from __future__ import annotations
 
@@ -27,6 +31,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbem_oracle.so")
 REF_DRIVER = os.path.join(HERE, "_ref", "zts_ref")
+REF_ACA = os.path.join(HERE, "_ref", "aca_ref")
 
 # src/util/SI.cpp:3-9
 SI_HBAR = 1.05457148e-34
@@ -277,3 +282,95 @@ def run_reference_source(meshes, eps_list, mu_list, omega, region, p, j, X, work
     pos += 16 * 2 * E
     S = np.frombuffer(raw, dtype=np.float64, count=len(meshes), offset=pos).copy()
     return B, [float(v) for v in S]
+
+
+# ------------------------------------------------------ low-rank blocks
+def aca(A: np.ndarray, eps2: float):
+    """Adaptive cross approximation with full pivoting, step by step as
+    src/util/DenseMatrix.h:213-279: the pivot is the entry of largest
+    |M(i, j)|^2 of the residual M (first in row-major order among equal
+    ones, the reference's i-outer / j-inner scan with '>'), a = M(:, js),
+    b = M(is, :) / M(is, js), M -= a b^T, and the loop goes on while
+    nu < min(m, n) and |a|^2 |b|^2 / sum |a|^2 |b|^2 > eps2; the step that
+    stops it keeps no pivot; a zero maximum breaks the loop.
+    Returns (rows, cols, U, V, M) with U V the kept steps' product and
+    M = A - U V (the reference's residual is a local copy it drops)."""
+    M = np.array(A, dtype=np.complex128, copy=True)
+    m, n = M.shape
+    k = min(m, n)
+    rows, cols, us, vs = [], [], [], []
+    tot = 0.0
+    nu = 0
+    while True:
+        a2 = M.real * M.real + M.imag * M.imag
+        flat = int(np.argmax(a2))            # first maximum in row-major order
+        i, j = divmod(flat, n)
+        delta = M[i, j]
+        if a2[i, j] == 0.0:
+            break
+        a = M[:, j].copy()
+        b = M[i, :] / delta
+        e = float(np.sum(a.real ** 2 + a.imag ** 2)) * float(np.sum(b.real ** 2 + b.imag ** 2))
+        tot += e
+        nu += 1
+        cont = nu < k and e / tot > eps2
+        if not cont:
+            break        # the reference also subtracts this step from its local copy, which it then drops
+        M -= np.outer(a, b)
+        rows.append(i)
+        cols.append(j)
+        us.append(a)
+        vs.append(b)
+    U = np.array(us, dtype=np.complex128).T.reshape(m, len(us))
+    V = np.array(vs, dtype=np.complex128).reshape(len(vs), n)
+    return np.array(rows, dtype=np.int32), np.array(cols, dtype=np.int32), U, V, M
+
+
+def run_reference_aca(A: np.ndarray, eps2: float, workdir: str):
+    """Pivots of the reference's own aca() (oracle/_ref/aca_ref) on A."""
+    if not os.path.exists(REF_ACA):
+        raise RuntimeError("reference ACA driver not built (make -C oracle/ref)")
+    A = np.asfortranarray(A, dtype=np.complex128)
+    m, n = A.shape
+    payload = np.array([m, n], dtype=np.int32).tobytes() + np.array([eps2], dtype=np.float64).tobytes() + \
+        A.tobytes(order="F")
+    out = os.path.join(workdir, "aca_ref.bin")
+    subprocess.run([REF_ACA, out], input=payload, check=True, stdout=subprocess.DEVNULL)
+    raw = open(out, "rb").read()
+    cnt = int(np.frombuffer(raw, dtype=np.int32, count=1)[0])
+    r = np.frombuffer(raw, dtype=np.int32, count=cnt, offset=4).copy()
+    c = np.frombuffer(raw, dtype=np.int32, count=cnt, offset=4 + 4 * cnt).copy()
+    return r, c
+
+
+def rk_truncate(U: np.ndarray, V: np.ndarray, eps: float):
+    """Rk::truncate (src/util/Rk.cpp:4-53): QR of U, RQ of V, SVD of the core
+    R_U R_V, keep the singular values s >= eps s_max; returns (U S, V^H of the
+    kept part, every singular value in decreasing order)."""
+    Qa, Ra = np.linalg.qr(U)
+    Qb, Rb = np.linalg.qr(V.conj().T)          # V = Rb^H Qb^H (the RQ of V)
+    core = Ra @ Rb.conj().T
+    W, S, Zh = np.linalg.svd(core)
+    keep = int(np.sum(S >= eps * S[0])) if S.size else 0
+    Un = (Qa @ W[:, :keep]) * S[:keep]
+    Vn = Zh[:keep, :] @ Qb.conj().T
+    return Un, Vn, S
+
+
+def heat_transfer_lowrank(A, D, U, Vh, G1, G2):
+    """4 S of the block elimination with the coupling B = U Vh and
+    C = S B^T S (J/M signs), in the Woodbury form the GPU path uses:
+    P = A^-1 U, K = (S conj U)^H P, Rh = Vh D^-1, F = Rh (S Vh^T),
+    Z = (I - F K)^-1 Rh, X = -P Z, 4 S = tr(P^H sym G1 P Z sym G2 Z^H).
+    A and D are the self blocks including the interior impedances."""
+    m1, m2 = A.shape[0], D.shape[0]
+    s1 = np.where(np.arange(m1) % 2 == 0, 1.0, -1.0)
+    s2 = np.where(np.arange(m2) % 2 == 0, 1.0, -1.0)
+    P = np.linalg.solve(A, U)
+    K = (s1[:, None] * U.conj()).conj().T @ P
+    Rh = np.linalg.solve(D.T, Vh.T).T
+    F = Rh @ (s2[:, None] * Vh.T)
+    Z = np.linalg.solve(np.eye(U.shape[1]) - F @ K, Rh)
+    M1 = P.conj().T @ symm(G1) @ P
+    M2 = Z @ symm(G2) @ Z.conj().T
+    return np.trace(M1 @ M2) / 4.0

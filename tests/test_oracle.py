@@ -1,6 +1,8 @@
 """The oracle (CPU restatement) pinned against the reference's own Zts
 (golden fixtures made by tests/golden/make_golden.py from oracle/_ref) and
 against mathematics: closed forms, brute-force quadrature, invariants."""
+import os
+
 import numpy as np
 import pytest
 
@@ -302,3 +304,84 @@ def test_dipole_flux_chain():
         for b in range(2):
             ref = d["flux_phys"][s][b]
             assert abs(T.flux(om, X, b) - ref) <= 1e-10 * abs(ref), (s, b)
+
+
+# ------------------------------------------------------ low-rank blocks
+def _kernel_block(m, n, seed, sep=2.5, k=0.7):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(0, 1, (m, 3))
+    y = rng.uniform(0, 1, (n, 3)) + [sep, 0, 0]
+    d = np.linalg.norm(x[:, None, :] - y[None, :, :], axis=2)
+    return np.exp(1j * k * d) / d
+
+
+@pytest.mark.skipif(not os.path.exists(O.REF_ACA), reason="reference ACA driver not built (needs the reference tree)")
+@pytest.mark.parametrize("eps2", [1e-4, 1e-10, 1e-24])
+def test_aca_matches_reference(tmp_path, eps2):
+    """oracle.aca keeps the same pivots, in the same order, as the
+    reference's own aca() built from src/util/DenseMatrix.h."""
+    A = _kernel_block(60, 45, seed=1)
+    r, c, U, V, M = O.aca(A, eps2)
+    rr, cc = O.run_reference_aca(A, eps2, str(tmp_path))
+    assert np.array_equal(r, rr) and np.array_equal(c, cc)
+    assert len(r) > 0
+
+
+def test_aca_closed_forms():
+    """Exact rank 3: three kept steps reproduce the matrix, and the fourth
+    step's increment (rounding of the zero residual) stops the loop without
+    keeping a pivot; every pivot is the largest entry of the residual
+    at its step (brute force); the residual is A - U V."""
+    rng = np.random.default_rng(2)
+    A = (rng.standard_normal((30, 3)) + 1j * rng.standard_normal((30, 3))) @ \
+        (rng.standard_normal((3, 25)) + 1j * rng.standard_normal((3, 25)))
+    r, c, U, V, M = O.aca(A, 1e-20)
+    assert len(r) == 3
+    assert np.abs(A - U @ V).max() <= 1e-12 * np.abs(A).max()
+    B = _kernel_block(40, 30, seed=5)
+    r, c, U, V, M = O.aca(B, 1e-16)
+    R = B.copy()
+    for k in range(len(r)):
+        i, j = np.unravel_index(np.argmax(np.abs(R)), R.shape)
+        assert (i, j) == (r[k], c[k])
+        R = R - np.outer(R[:, j], R[i, :] / R[i, j])
+    assert np.abs(M - (B - U @ V)).max() <= 1e-13 * np.abs(B).max()
+
+
+def test_rk_truncate_eckart_young():
+    """The truncation keeps the singular values >= eps s_max of U V, and the
+    kept product is its best approximation of that rank: the spectral error
+    is the first dropped singular value."""
+    rng = np.random.default_rng(3)
+    k = 30
+    Q1, _ = np.linalg.qr(rng.standard_normal((80, k)) + 1j * rng.standard_normal((80, k)))
+    Q2, _ = np.linalg.qr(rng.standard_normal((60, k)) + 1j * rng.standard_normal((60, k)))
+    s = np.logspace(0, -12, k)
+    U, V = Q1 * s, Q2.conj().T
+    Un, Vn, S = O.rk_truncate(U, V, 1e-6)
+    keep = int(np.sum(s >= 1e-6))
+    assert Un.shape[1] == keep and np.allclose(S, s, rtol=1e-10, atol=1e-15)
+    err = np.linalg.norm(U @ V - Un @ Vn, 2)
+    assert abs(err - s[keep]) <= 1e-10 * s[0]
+
+
+def test_lowrank_composition_matches_dense():
+    """The Woodbury form of the block elimination with the coupling block
+    factored (B = U Vh, C = S B^T S) gives the dense composition's 4 S: an
+    exact SVD of B to rounding, a truncation at 1e-10 to 1e-10."""
+    from bem_amd import mesh
+    s = mesh.icosphere(1)
+    m1m = s.translated((2.5, 0, 0))
+    eV = 0.11
+    om, eps = O.omega_tilde(eV, 1e-6), O.lorentz_eps(eV)
+    top = O.Tree([m1m, s])
+    W = top.zts(1.0, 1.0, om)
+    G1, G2 = O.Tree([m1m]).zts(eps, 1, om), O.Tree([s]).zts(eps, 1, om)
+    n1 = G1.shape[0]
+    S0 = O.heat_transfer_from_blocks(W, G1, G2)
+    A, D, B = W[:n1, :n1] + G1, W[n1:, n1:] + G2, W[:n1, n1:]
+    Uu, sv, Vh = np.linalg.svd(B)
+    assert abs(O.heat_transfer_lowrank(A, D, Uu * sv, Vh, G1, G2) - S0) <= 1e-13 * abs(S0)
+    keep = int(np.sum(sv >= 1e-10 * sv[0]))
+    S1 = O.heat_transfer_lowrank(A, D, Uu[:, :keep] * sv[:keep], Vh[:keep], G1, G2)
+    assert keep < n1 and abs(S1 - S0) <= 1e-10 * abs(S0)
