@@ -184,6 +184,7 @@ struct PairGeom {
   std::vector<void *> allocs;
   size_t bytes = 0;
   int pins = 0;         // lanes whose solve in flight reads this tree
+  int lr_blocks = 0;    // range-finder blocks the last low-rank solve on this pair needed (issue hint)
 };
 
 // Host-visible status of a self entry, written by its producing stream.
@@ -1559,8 +1560,10 @@ static int solve_mixed(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry
 }
 
 // ------------------------------------------------- low-rank coupling solve
-// Columns per block of the randomized range finder (one CholQR2 block).
+// Columns per block of the randomized range finder (one CholQR2 block), and
+// the most blocks one compression takes (ranks up to 32768).
 static const int LR_BLK = 64;
+static const int LR_MAX_BLOCKS = 512;
 // Seed of the range finder's test columns (fixed, so a request gives the same
 // factors alone and in a mix).
 static const unsigned long long LR_SEED = 0x10C0FFEEull;
@@ -1580,84 +1583,88 @@ static int gemm_ta(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, cons
   return ev_end(c, L, r);
 }
 
-// Squared Frobenius norm of an m x n block, read back to the host (waits for
-// the lane's stream).
-static int lr_sumsq_host(bem_ctx *c, Lane &L, const double2 *A, int64_t lda, int m, int n, double *out) {
-  double *part, *dst;
-  CHK(ws_get(c, L, "lr_ssq", sizeof(double) * (bem_sumsq_partials(m, n) + 1), (void **)&part));
-  dst = part + bem_sumsq_partials(m, n);
-  if (bem_sumsq(A, lda, m, n, part, 0, dst, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
-  DEV(c, hipMemcpyAsync(L.h_lr, dst, sizeof(double), hipMemcpyDeviceToHost, L.st));
-  DEV(c, hipStreamSynchronize(L.st));
-  *out = L.h_lr[0];
-  return 0;
-}
-
 // Blocked randomized range finder of B (m1 x m2): Q (m1 x r, orthonormal
 // columns, r a multiple of LR_BLK, leading dimension m1) and Vh = Q^H B
 // (r x m2, leading dimension ldv) with B ~ Q Vh.  Block k draws LR_BLK seeded
 // test columns W_k (entries with real and imaginary parts uniform in
-// [-1, 1), E|w|^2 = 2/3), Y = B W_k, projects out the range found so far
+// [-1, 1), E|w|^2 = 2/3), Y = B W_k, projects out the blocks before it
 // twice (Y -= Q (Q^H Y)), and orthonormalises Y by CholQR2 (G = Y^H Y = R^H
-// R, Y <- Y R^-1, twice).  E||Y_k||_F^2 = LR_BLK (2/3) ||B - Q Q^H B||_F^2
-// before the block joins, so the loop ends after the first block whose
-// estimate is at most tol^2 ||B||_F^2 (that block still joins Q, which only
-// lowers the error).  *rank = 0 when B is zero; -1 when rmax columns do not
-// reach the tolerance (the caller then solves densely).  Waits on the lane's
-// stream once per block.
+// R, Y <- Y R^-1, twice).  E||Y_k||_F^2 = LR_BLK (2/3) ||B - Q_<k Q_<k^H B||_F^2,
+// so r ends with the first block whose estimate is at most tol^2 ||B||_F^2
+// (that block still joins Q, which only lowers the error); a block whose
+// Gram is singular ends it before that block when its estimate is under the
+// tolerance, else the block is not compressible here.  Blocks are issued in
+// chunks and their estimates read back once per chunk: the first chunk is
+// `hint` blocks (what the pair needed last time), later ones LR_CHUNK.
+// Block k depends only on the blocks before it, so r, Q and Vh do not depend
+// on the chunking.  *rank = 0 when B is zero; -1 when rmax columns do not
+// reach the tolerance (the caller then solves densely).
+static const int LR_CHUNK = 4;
 static int lr_compress_rand(bem_ctx *c, Lane &L, const double2 *B, int m1, int m2, double tol, int rmax, double2 *Q,
-                            double2 *Vh, int ldv, int *rank) {
+                            double2 *Vh, int ldv, int hint, int *rank) {
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
   *rank = -1;
-  double bn2;
-  CHK(lr_sumsq_host(c, L, B, m1, m1, m2, &bn2));
-  if (!(bn2 > 0)) {
-    *rank = bn2 == 0 ? 0 : -1;
-    return 0;
-  }
+  const int nbmax = std::min(rmax / LR_BLK, LR_MAX_BLOCKS);
+  if (nbmax < 1) return 0;
   double2 *W, *Cb, *G, *Rinv;
-  int *info;
+  double *est, *part;
+  int *binfo;
   CHK(ws_get(c, L, "lr_W", sizeof(double2) * (size_t)m2 * LR_BLK, (void **)&W));
-  CHK(ws_get(c, L, "lr_Cb", sizeof(double2) * (size_t)std::max(rmax, 1) * LR_BLK, (void **)&Cb));
+  CHK(ws_get(c, L, "lr_Cb", sizeof(double2) * (size_t)nbmax * LR_BLK * LR_BLK, (void **)&Cb));
   CHK(ws_get(c, L, "lr_G", sizeof(double2) * LR_BLK * LR_BLK, (void **)&G));
   CHK(ws_get(c, L, "lr_Rinv", sizeof(double2) * 64 * 64, (void **)&Rinv));
-  CHK(ws_get(c, L, "lr_info", sizeof(int) * 4, (void **)&info));
-  int r = 0;
+  CHK(ws_get(c, L, "lr_est", sizeof(double) * (LR_MAX_BLOCKS + 1), (void **)&est));
+  CHK(ws_get(c, L, "lr_binfo", sizeof(int) * (LR_MAX_BLOCKS + 1), (void **)&binfo));
+  const size_t np = std::max(bem_sumsq_partials(m1, m2), bem_sumsq_partials(m1, LR_BLK));
+  CHK(ws_get(c, L, "lr_ssq", sizeof(double) * np, (void **)&part));
+  if (bem_sumsq(B, m1, m1, m2, part, 0, est, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");   // est[0] = ||B||^2
+  DEV(c, hipMemsetAsync(binfo, 0, sizeof(int) * (LR_MAX_BLOCKS + 1), L.st));
+  int issued = 0, r = -1;
   bool done = false;
-  for (int blk = 0; r + LR_BLK <= rmax; blk++) {
-    double2 *Y = Q + (int64_t)r * m1;
-    if (bem_fill_hash(W, sizeof(double2) * (size_t)m2 * LR_BLK, 0, LR_SEED + 0x9E37ull * blk, L.st))
-      return fail(c, BEM_E_DEVICE, "test column fill launch");
-    CHK(gemm(c, L, m1, LR_BLK, m2, one, B, m1, W, m2, zero, Y, m1));
-    if (r > 0)
+  while (!done && issued < nbmax) {
+    const int chunk = std::min(nbmax - issued, issued == 0 ? std::max(hint, LR_CHUNK) : LR_CHUNK);
+    for (int k = issued; k < issued + chunk; k++) {
+      double2 *Y = Q + (int64_t)k * LR_BLK * m1;
+      const int rk = k * LR_BLK;
+      if (bem_fill_hash(W, sizeof(double2) * (size_t)m2 * LR_BLK, 0, LR_SEED + 0x9E37ull * k, L.st))
+        return fail(c, BEM_E_DEVICE, "test column fill launch");
+      CHK(gemm(c, L, m1, LR_BLK, m2, one, B, m1, W, m2, zero, Y, m1));
+      if (rk > 0)
+        for (int pass = 0; pass < 2; pass++) {
+          CHK(gemm_ta(c, L, rk, LR_BLK, m1, one, Q, m1, Y, m1, zero, Cb, rk));
+          CHK(gemm(c, L, m1, LR_BLK, rk, mone, Q, m1, Cb, rk, one, Y, m1));
+        }
+      if (bem_sumsq(Y, m1, m1, LR_BLK, part, 0, est + 1 + k, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
       for (int pass = 0; pass < 2; pass++) {
-        CHK(gemm_ta(c, L, r, LR_BLK, m1, one, Q, m1, Y, m1, zero, Cb, r));
-        CHK(gemm(c, L, m1, LR_BLK, r, mone, Q, m1, Cb, r, one, Y, m1));
+        CHK(gemm_ta(c, L, LR_BLK, LR_BLK, m1, one, Y, m1, Y, m1, zero, G, LR_BLK));
+        if (bem_chol_inv64(G, LR_BLK, LR_BLK, Rinv, binfo + k, L.st)) return fail(c, BEM_E_DEVICE, "cholesky launch");
+        CHK(gemm_inplace(c, L, 'R', m1, LR_BLK, Rinv, Y, m1));
       }
-    double y2;
-    CHK(lr_sumsq_host(c, L, Y, m1, m1, LR_BLK, &y2));
-    const bool last = y2 / (LR_BLK * (2.0 / 3.0)) <= tol * tol * bn2;
-    DEV(c, hipMemsetAsync(info, 0, sizeof(int), L.st));
-    for (int pass = 0; pass < 2; pass++) {
-      CHK(gemm_ta(c, L, LR_BLK, LR_BLK, m1, one, Y, m1, Y, m1, zero, G, LR_BLK));
-      if (bem_chol_inv64(G, LR_BLK, LR_BLK, Rinv, info, L.st)) return fail(c, BEM_E_DEVICE, "cholesky launch");
-      CHK(gemm_inplace(c, L, 'R', m1, LR_BLK, Rinv, Y, m1));
     }
-    DEV(c, hipMemcpyAsync(L.h_lri, info, sizeof(int), hipMemcpyDeviceToHost, L.st));
+    DEV(c, hipMemcpyAsync(L.h_lr, est, sizeof(double) * (issued + chunk + 1), hipMemcpyDeviceToHost, L.st));
+    DEV(c, hipMemcpyAsync(L.h_lri, binfo, sizeof(int) * (issued + chunk), hipMemcpyDeviceToHost, L.st));
     DEV(c, hipStreamSynchronize(L.st));
-    if (L.h_lri[0]) {
-      // the block's columns are dependent to working precision: the range is
-      // exhausted when the residual estimate is already under the tolerance
-      done = last;
-      break;
+    const double bn2 = L.h_lr[0];
+    if (!(bn2 > 0)) {
+      *rank = bn2 == 0 ? 0 : -1;
+      return 0;
     }
-    r += LR_BLK;
-    if (last) {
-      done = true;
-      break;
+    for (int k = issued; k < issued + chunk; k++) {
+      const bool last = L.h_lr[1 + k] / (LR_BLK * (2.0 / 3.0)) <= tol * tol * bn2;
+      if (L.h_lri[k]) {   // dependent columns: the range is exhausted if the estimate is under the tolerance
+        done = true;
+        r = last ? k * LR_BLK : -1;
+        break;
+      }
+      if (last) {
+        done = true;
+        r = (k + 1) * LR_BLK;
+        break;
+      }
     }
+    issued += chunk;
   }
-  if (!done || r == 0) return 0;
+  if (!done || r <= 0) return 0;
   CHK(gemm_ta(c, L, r, m2, m1, one, Q, m1, B, m1, zero, Vh, ldv));
   *rank = r;
   return 0;
@@ -1877,7 +1884,7 @@ static int probe_lowrank(bem_ctx *c, Lane &L, const LuRef<double2> &A, const LuR
 // then the probe against the dense B and C.
 static int solve_lowrank(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEntry &E2, const double2 *Bm,
                          const double2 *Cm, const double2 *U, const double2 *Vh, int ldv, int r, int *info,
-                         double2 *trace_out, ProbeOut *pout) {
+                         unsigned long long *gro, double2 *trace_out, ProbeOut *pout) {
   const int m1 = E1.n, m2 = E2.n;
   const double2 one = make_double2(1, 0), zero = make_double2(0, 0);
   double2 *P, *Qc, *K, *Rh, *U2, *F, *T, *invT, *Zt, *H1, *M1, *W2, *partial;
@@ -1909,9 +1916,10 @@ static int solve_lowrank(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEnt
   CHK(gemm(c, L, r, r, m2, one, Rh, r, U2, m2, zero, F, r));
   CHK(gemm(c, L, r, r, r, one, F, r, K, r, zero, T, r));
   if (bem_identity_minus(T, r, r, L.st)) return fail(c, BEM_E_DEVICE, "identity launch");
-  // Z = T^-1 Rh (tournament pivoting: T is small and carries the coupling's conditioning)
-  CHK(lu_factor(c, L, T, r, r, ipT, info, invT));
-  CHK(lu_solve(c, L, T, r, r, ipT, invT, Rh, r, m2, false));
+  // Z = T^-1 Rh; T = I - F K is the identity less the coupling loop, and its
+  // block-local LU reports its growth like the Schur complement's
+  CHK(lu_factor_block(c, L, T, r, r, ipT, info, invT, gro));
+  CHK(lu_solve(c, L, T, r, r, ipT, invT, Rh, r, m2, true));
   double2 *Z = Rh;
   DEV(c, hipEventRecord(L.ph[2], L.st));
   // 4 S = tr(M1 M2)
@@ -2012,11 +2020,12 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
       CHK(ws_get(c, L, "lr_Vh", sizeof(double2) * (size_t)rmax * m2, (void **)&Vh));
       const auto tc = std::chrono::steady_clock::now();
       if (qq.flags & BEM_LOWRANK_ACA) CHK(lr_compress_aca(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, &r));
-      else CHK(lr_compress_rand(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, &r));
+      else CHK(lr_compress_rand(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, g.lr_blocks, &r));
+      if (r > 0 && !(qq.flags & BEM_LOWRANK_ACA)) g.lr_blocks = r / LR_BLK;
       L.ms_compress = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
       if (r > 0) {
         L.rank = r;
-        CHK(solve_lowrank(c, L, E1, E2, Bm, Cm, Q, Vh, rmax, r, info, trace_out, pout));
+        CHK(solve_lowrank(c, L, E1, E2, Bm, Cm, Q, Vh, rmax, r, info, gro, trace_out, pout));
         solved = true;
       }
     }
@@ -2792,8 +2801,8 @@ int bem_ctx_create(bem_ctx **out, const bem_cfg *cfg) {
     ok = ok && hipHostMalloc((void **)&L.h_growth, sizeof(unsigned long long) * 4, hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_probe, sizeof(ProbeOut), hipHostMallocDefault) == hipSuccess;
     ok = ok && hipHostMalloc((void **)&L.h_rfull, sizeof(double), hipHostMallocDefault) == hipSuccess;
-    ok = ok && hipHostMalloc((void **)&L.h_lr, sizeof(double) * 4, hipHostMallocDefault) == hipSuccess;
-    ok = ok && hipHostMalloc((void **)&L.h_lri, sizeof(int) * 4, hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_lr, sizeof(double) * (LR_MAX_BLOCKS + 1), hipHostMallocDefault) == hipSuccess;
+    ok = ok && hipHostMalloc((void **)&L.h_lri, sizeof(int) * (LR_MAX_BLOCKS + 1), hipHostMallocDefault) == hipSuccess;
     if (!ok) {
       bem_ctx_destroy(c);
       return BEM_E_DEVICE;

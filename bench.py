@@ -48,6 +48,43 @@ PEAK_FP32_TFLOPS = 157.3    # MI355X dense FP32 MFMA
 # --precision -> (bem_request.precision, flags): BEM_PREC_FP64 = 0,
 # BEM_PREC_MIXED = 1, BEM_SOLVE_REFINE = 2
 PRECISIONS = {"fp64": (0, 0), "mixed": (1, 0), "mixed-refine": (1, 2)}
+# --solver -> flags: BEM_SOLVE_LOWRANK = 4 (cross block compressed by the
+# randomized range finder), BEM_LOWRANK_ACA = 8 (by ACA + truncation instead)
+SOLVERS = {"dense": 0, "lowrank": 4, "lowrank-aca": 4 | 8}
+
+
+def elimination_flops(m1, m2):
+    """Real flops of the dense block elimination per solve (LU(A), Y = A^-1 B,
+    Sc = D - C Y, LU(Sc), X = -Y Sc^-1, B1 = sym G1 X, the Hermitian trace
+    GEMM): 8/3 (m1^3 + m2^3) + 16 m1^2 m2 + 20 m1 m2^2, 41 1/3 m^3 for m1 = m2."""
+    return 8.0 / 3.0 * (m1 ** 3 + m2 ** 3) + 16.0 * m1 * m1 * m2 + 20.0 * m1 * m2 * m2
+
+
+# flops per triangle pair by class {close, g6, g10, g15, g21, g36}: the close
+# pair's 105 x 91 smooth kernel + analytic terms, and points_out x
+# (points_in x 166 + 342) + 280 for the far rules
+PAIR_FLOPS = np.array([1.83e6, 8308.0, 10 * (10 * 166 + 342) + 280.0, 15 * (15 * 166 + 342) + 280.0,
+                       21 * (21 * 166 + 342) + 280.0, 36 * (36 * 166 + 342) + 280.0])
+
+
+def assembly_flops(ctx, b1, b2, gap, twins):
+    """Assembly flops per solve: both media of each self block (one when the
+    bodies are twins, the other block being a copy) and the exterior cross
+    block, from the pair-class histograms."""
+    h1 = np.asarray(ctx.pair_histogram([b1], [(0.0, 0.0, 0.0)]), dtype=np.float64)
+    h2 = np.asarray(ctx.pair_histogram([b2], [(0.0, 0.0, 0.0)]), dtype=np.float64)
+    ht = np.asarray(ctx.pair_histogram([b1, b2], [tuple(gap), (0.0, 0.0, 0.0)]), dtype=np.float64)
+    cross = ht - h1 - h2
+    return float(2.0 * PAIR_FLOPS @ h2 + (0.0 if twins else 2.0 * PAIR_FLOPS @ h1) + PAIR_FLOPS @ cross)
+
+
+def rank_summary(results):
+    """Ranks of the coupling block over the results (0: solved densely)."""
+    ranks = [int(r.get("rank", 0)) for r in results]
+    comp = [r.get("ms_compress", 0.0) for r in results if r.get("rank", 0) > 0]
+    return {"min": min(ranks), "median": float(np.median(ranks)), "max": max(ranks),
+            "dense": sum(1 for k in ranks if k == 0), "restarts": sum(int(r["restarts"]) for r in results),
+            "ms_compress_median": float(np.median(comp)) if comp else 0.0}
 
 
 def workload(name):
@@ -81,11 +118,12 @@ def run_precision(args):
     gaps = [0.05, 0.1, 0.2, 0.5]
     freqs = [0.08, 0.09, 0.1, 0.105, 0.112, 0.118, 0.13, 0.14]
     prec, flags = PRECISIONS[args.precision]
+    flags |= SOLVERS[args.solver]
     ctx = bem_amd.Context(0, lanes=args.lanes)
     tune(ctx, args)
     b1, b2 = ctx.body(sph), ctx.body(plate)
     reqs = [Request(eV=f, gap=(0.0, 0.0, 1.0 + g), eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f),
-                    precision=prec, flags=flags) for f in freqs for g in gaps]
+                    precision=prec, flags=flags, rank_tol=args.rank_tol) for f in freqs for g in gaps]
     n = args.steps if args.steps < len(reqs) else len(reqs)
     reqs = reqs[:n]
     t0 = time.perf_counter()
@@ -93,8 +131,9 @@ def run_precision(args):
     res = []
     for q, t in zip(reqs, tickets):
         res.append(ctx.poll(t, wait_ms=-1))
-        print(f"c3 {args.precision}: eV={q.eV} gap={q.gap[2] - 1.0:.2f} ReS={res[-1]['ReS']:.12e} "
-              f"est={res[-1]['rel_err_est']:.2e} t={time.perf_counter() - t0:.1f}s", file=sys.stderr, flush=True)
+        print(f"c3 {args.precision} {args.solver}: eV={q.eV} gap={q.gap[2] - 1.0:.2f} ReS={res[-1]['ReS']:.12e} "
+              f"est={res[-1]['rel_err_est']:.2e} rank={res[-1]['rank']} t={time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
     elapsed = time.perf_counter() - t0
     out = {"metric": "spectral-power solves/s", "value": n / elapsed, "unit": "solves/s", "n_gpus": 1, "steps": n,
            "warmup": 0, "ms_per_step": elapsed * 1e3 / n, "higher_is_better": True, "scaling": "weak",
@@ -102,10 +141,13 @@ def run_precision(args):
            "data": "synthetic",
            "config": {"workload": f"c3: icosphere 5120 tri over 4x4x0.5 plate 6000 tri, M={2 * (b1.ne + b2.ne)}, "
                                   "gaps 0.05/0.1/0.2/0.5, a=1e-6 m", "precision": args.precision,
-                      "lanes": args.lanes},
+                      "solver": args.solver, "rank_tol": args.rank_tol or None, "lanes": args.lanes},
            "results": [{"eV": q.eV, "gap": q.gap[2] - 1.0, "ReS": r["ReS"], "P": r["P"],
-                        "rel_err_est": r["rel_err_est"], "growth": r["growth"]} for q, r in zip(reqs, res)]}
-    if args.precision != "fp64":
+                        "rel_err_est": r["rel_err_est"], "growth": r["growth"], "rank": r["rank"]}
+                       for q, r in zip(reqs, res)]}
+    if args.solver != "dense":
+        out["rank"] = rank_summary(res)
+    if args.precision != "fp64" or args.solver != "dense":
         ref = []
         for q in reqs:
             ref.append(ctx.calc_heat_transfer(b1, b2, Request(eV=q.eV, gap=q.gap, eps1=q.eps1, eps2=q.eps2)))
@@ -330,6 +372,12 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed", "mixed-refine"],
                     help="fp64: complex128 throughout; mixed: complex64 elimination on FP32 MFMA; "
                          "mixed-refine: plus one FP64 refinement step")
+    ap.add_argument("--solver", default="dense", choices=list(SOLVERS),
+                    help="dense: block elimination on the dense cross blocks; lowrank: the cross block compressed "
+                         "(randomized range finder) and eliminated in Woodbury form; lowrank-aca: compressed by "
+                         "ACA with full pivoting and SVD truncation")
+    ap.add_argument("--rank-tol", type=float, default=0.0,
+                    help="relative Frobenius tolerance of the compressed cross block (0: the library default, 1e-10)")
     ap.add_argument("--roofline-solves", type=int, default=2, help="single-lane solves timed per kernel class")
     ap.add_argument("--tune", default="cache", choices=["cache", "force", "off"],
                     help="GEMM tiles: from the autotuner cache (measured when missing), re-measured, or built-in rule")
@@ -386,10 +434,11 @@ def main():
     freqs = pick_freqs(wl["freqs"], args.steps)
 
     prec, flags = PRECISIONS[args.precision]
+    flags |= SOLVERS[args.solver]
 
     def req(f):
         return Request(eV=f, gap=wl["gap"], eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f),
-                       precision=prec, flags=flags)
+                       precision=prec, flags=flags, rank_tol=args.rank_tol)
 
     reqs = [req(f) for f in freqs]
     if freqs_w:
@@ -445,7 +494,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f64" if args.precision == "fp64" else "f64 assembly + c64 elimination",
         "data": "synthetic",
-        "config": {"workload": wl["name"], "precision": args.precision, "lanes": args.lanes, "gemm_tiles": tiles,
+        "config": {"workload": wl["name"], "precision": args.precision, "solver": args.solver,
+                   "rank_tol": args.rank_tol or None, "lanes": args.lanes, "gemm_tiles": tiles,
                    "freqs_eV": [round(f, 6) for f in freqs], "T1": 300.0, "T2": 0.0,
                    "permittivity": "6.7(w^2-0.120^2+i0.00059w)/(w^2-0.0983^2+i0.00059w)"},
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
@@ -492,7 +542,19 @@ def main():
                 if key in ref:
                     errs.append(abs(r["ReS"] - ref[key]["ReS"]) / abs(ref[key]["ReS"]))
             out["rel_err_vs_fp64_oracle"] = max(errs) if errs else None
-    if args.precision != "fp64" and not args.no_reference:
+    m1, m2 = 2 * b1.ne, 2 * b2.ne
+    M = m1 + m2
+    f_elim = elimination_flops(m1, m2)
+    f_asm = assembly_flops(ctx, b1, b2, wl["gap"], wl["m1"] is wl["m2"])
+    out["flops_per_solve"] = {
+        "elimination_dense": f_elim, "assembly": f_asm,
+        "survey_26_3_M3": 26.0 / 3.0 * M ** 3, "reference_13_M3": 13.0 * M ** 3,
+        "whole_step_TFLOPs_dense_count": (f_elim + f_asm) * value / 1e12,
+        "note": "elimination_dense counts the dense block elimination (41 1/3 m^3 for equal bodies) that the dense "
+                "solver runs; times value it must stay under the FP64 peak of 78.6 TFLOP/s"}
+    if args.solver != "dense":
+        out["rank"] = rank_summary(results)
+    if (args.precision != "fp64" or args.solver != "dense") and not args.no_reference:
         # error of every result against the FP64 path on the same requests
         # (outside the timed region; the FP64 path meets 1e-8 vs the oracle)
         ref64 = ctx.sweep(b1, b2, [Request(eV=q.eV, gap=q.gap, eps1=q.eps1, eps2=q.eps2) for q in reqs])
