@@ -26,6 +26,11 @@ void check(int rc) {
 
 // src/Options.cpp:23-26 (only the far thresholds act on this path; the
 // library applies the same constants)
+// The reference's drivers call this first (src/transfer.cpp:5) so that its
+// reference-counted matrix storage may be copied; the storage here is the
+// library's device memory, and the device context is created on first use.
+void initReferenceCountingPointerLock() {}
+
 Options::Options() : farThresh6(12.0), farThresh10(8.0), farThresh15(4.0), farThresh21(2.0) {}
 
 GreensFunction::GreensFunction(bool ext) : bExternal(ext) {}

@@ -175,6 +175,10 @@ struct Vector3 {
 void parseShapes(RWGTree *&top, RWGTree *&s, RWGTree *&t, const char *body1x3d, const char *body2x3d,
                  const Vector3 &gap, Options &options);
 
+// src/util/ReferenceCountingPointer.cpp:10-14: the reference's drivers call
+// it before anything else; nothing needs initialising here.
+void initReferenceCountingPointerLock();
+
 ComplexDenseMatrix *Zts(RWGTree *t, RWGTree *s, GreensFunction *g, Options &options);
 void Zdense(RWGTree *t, RWGTree *s, ComplexDenseMatrix *dense, GreensFunction *gExt, Options &options);
 Real CalcHeatTransfer(Real a, Real eV, Real T1, Real T2, RWGTree *top, RWGTree *s, RWGTree *t, Options &options);

@@ -29,6 +29,30 @@ def test_compat_library_exports_reference_api():
     assert os.access(os.path.join(COMPAT, "transfer_sweep"), os.X_OK)
     assert os.access(os.path.join(COMPAT, "transfer_farm"), os.X_OK)
     assert os.access(os.path.join(COMPAT, "transfer"), os.X_OK)
+    assert "initReferenceCountingPointerLock()" in out
+
+
+REF_TRANSFER = "/root/reference/src/transfer.cpp"
+
+
+@pytest.mark.skipif(not os.path.exists(REF_TRANSFER), reason="the reference tree is not mounted")
+def test_reference_transfer_builds_unchanged(tmp_path):
+    """The reference's own single-frequency driver (src/transfer.cpp), its
+    text unchanged, compiles against the drop-in headers of
+    bem_amd/compat/include (HeatTransfer.h, Impedance.h, FrequencySweep.h,
+    ReferenceCountingPointer.h) and links against libbem_compat.so and
+    libbem_hip.so.  The text goes to the compiler on stdin from a scratch
+    directory, so the reference's own headers next to it are not picked up;
+    the binary stays in that directory.  Without arguments it prints the
+    reference's usage line and exits with its status (-2)."""
+    exe = str(tmp_path / "transfer_ref")
+    src = open(REF_TRANSFER, "rb").read()
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(COMPAT, "include"), "-x", "c++", "-",
+           "-x", "none", "-L", os.path.join(ROOT, "bem_amd"), "-lbem_compat", "-lbem_hip",
+           "-Wl,-rpath," + os.path.join(ROOT, "bem_amd"), "-o", exe]
+    subprocess.run(cmd, input=src, cwd=str(tmp_path), check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    assert r.returncode == 254 and r.stderr.startswith("Usage: transfer")
 
 
 def test_transfer_usage():
