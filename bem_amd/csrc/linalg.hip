@@ -22,6 +22,7 @@
 #include "bem_common.h"
 
 #include <algorithm>
+#include <type_traits>
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 typedef float flt4 __attribute__((ext_vector_type(4)));
@@ -757,6 +758,245 @@ __device__ __forceinline__ void lu_column(LuRow<T> &r, int w, int wave, int ri, 
   }
 }
 
+// One level of the triangular inverses of a 64 x 64 packed LU (sA, by
+// position) into sX: pairs of B-blocks at offsets o (multiples of 2B) with
+// their diagonal B-blocks' inverses already in sX.
+//  L: T = L21 Linv11 (rows o+B.., cols o..), Linv21 = -Linv22 T
+//  U: T = U12 Uinv22 (rows o.., cols o+B..), Uinv12 = -Uinv11 T
+// The L factor part of sA (strictly lower) and its U part are read-only until
+// phase 1 overwrites the L21/U12 blocks with T.  B is a template argument so
+// the output indexing is shifts and masks and the inner sums unroll.
+template <int B, typename T>
+__device__ __forceinline__ void inv_level(T *sA, T *sX, int tid) {
+  constexpr int PER = 32 * B;   // outputs per matrix: (32/B) pairs x B*B
+  constexpr int SLOTS = (2 * PER + 255) / 256;
+  T t[SLOTS];
+  auto index = [&](int q, bool &upper, int &o, int &i, int &jj) {
+    upper = q >= PER;
+    const int qq = upper ? q - PER : q;
+    const int pair = qq / (B * B), e = qq % (B * B);
+    o = pair * 2 * B;
+    i = e % B;
+    jj = e / B;
+  };
+  // phase 1: T into registers (slot s holds output q = tid + 256 s)
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl++) {
+    const int q = tid + 256 * sl;
+    T acc = Cplx<T>::mk(0.0, 0.0);
+    if (q < 2 * PER) {
+      bool upper;
+      int o, i, jj;
+      index(q, upper, o, i, jj);
+      if (!upper) {
+        // T[i][jj] = sum_{k >= jj} L21[i][k] Linv11[k][jj], Linv11 unit lower
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          if (k >= jj) {
+            const T l = sA[(o + k) * 64 + o + B + i];
+            const T x = (k == jj) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + jj) * 64 + o + k];
+            acc.x = std::fma(l.x, x.x, acc.x);
+            acc.x = std::fma(-l.y, x.y, acc.x);
+            acc.y = std::fma(l.x, x.y, acc.y);
+            acc.y = std::fma(l.y, x.x, acc.y);
+          }
+        }
+      } else {
+        // T[i][jj] = sum_{k <= jj} U12[i][k] Uinv22[k][jj]
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          if (k <= jj) {
+            const T u = sA[(o + B + k) * 64 + o + i];
+            const T x = sX[(o + B + jj) * 64 + o + B + k];
+            acc.x = std::fma(u.x, x.x, acc.x);
+            acc.x = std::fma(-u.y, x.y, acc.x);
+            acc.y = std::fma(u.x, x.y, acc.y);
+            acc.y = std::fma(u.y, x.x, acc.y);
+          }
+        }
+      }
+    }
+    t[sl] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl++) {
+    const int q = tid + 256 * sl;
+    if (q < 2 * PER) {
+      bool upper;
+      int o, i, jj;
+      index(q, upper, o, i, jj);
+      if (!upper) sA[(o + jj) * 64 + o + B + i] = t[sl];
+      else sA[(o + B + jj) * 64 + o + i] = t[sl];
+    }
+  }
+  __syncthreads();
+  // phase 2: Linv21 = -Linv22 T, Uinv12 = -Uinv11 T
+#pragma unroll
+  for (int sl = 0; sl < SLOTS; sl++) {
+    const int q = tid + 256 * sl;
+    if (q < 2 * PER) {
+      bool upper;
+      int o, i, jj;
+      index(q, upper, o, i, jj);
+      T acc = Cplx<T>::mk(0.0, 0.0);
+      if (!upper) {
+        // sum_{k <= i} Linv22[i][k] T[k][jj], Linv22 unit lower
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          if (k <= i) {
+            const T x = (k == i) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + B + k) * 64 + o + B + i];
+            const T tt = sA[(o + jj) * 64 + o + B + k];
+            acc.x = std::fma(x.x, tt.x, acc.x);
+            acc.x = std::fma(-x.y, tt.y, acc.x);
+            acc.y = std::fma(x.x, tt.y, acc.y);
+            acc.y = std::fma(x.y, tt.x, acc.y);
+          }
+        }
+        sX[(o + jj) * 64 + o + B + i] = Cplx<T>::mk(-acc.x, -acc.y);
+      } else {
+        // sum_{k >= i} Uinv11[i][k] T[k][jj]
+#pragma unroll
+        for (int k = 0; k < B; k++) {
+          if (k >= i) {
+            const T x = sX[(o + k) * 64 + o + i];
+            const T tt = sA[(o + B + jj) * 64 + o + k];
+            acc.x = std::fma(x.x, tt.x, acc.x);
+            acc.x = std::fma(-x.y, tt.y, acc.x);
+            acc.y = std::fma(x.x, tt.y, acc.y);
+            acc.y = std::fma(x.y, tt.x, acc.y);
+          }
+        }
+        sX[(o + B + jj) * 64 + o + i] = Cplx<T>::mk(-acc.x, -acc.y);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// The same level for B = 16 and 32 (complex128) on FP64 MFMA: each product
+// of the level (T = L21 Linv11, T = U12 Uinv22, then -Linv22 T and
+// -Uinv11 T) is split into 16 x 16 output tiles; a wave takes one tile per
+// round and runs v_mfma_f64_16x16x4f64 over k in steps of 4, four real
+// products per complex step.  The triangles' structure (zeros, the unit
+// diagonal of Linv) is applied while loading the fragments, so every lane
+// stays active.  Fragment layout as in zgemm_kernel: the right factor's
+// fragment is the instruction's first operand (k = lane / 16, column =
+// lane % 16), the left factor's the second (row = lane % 16, k = lane / 16),
+// and element e of lane l holds row l % 16, column 4e + l / 16.
+template <int B>
+__device__ __forceinline__ void inv_level_mfma(double2 *sA, double2 *sX, int tid) {
+  constexpr int NT = B / 16;                 // tiles per block side
+  constexpr int PAIRS = 32 / B;
+  constexpr int TILES = 2 * PAIRS * NT * NT; // per phase, both matrices
+  constexpr int ROUNDS = TILES / 4;
+  static_assert(TILES % 4 == 0, "whole rounds of four waves");
+  const int wave = tid >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+  const double2 one = make_double2(1.0, 0.0), zero = make_double2(0.0, 0.0);
+  auto tile_of = [&](int tt, bool &upper, int &o, int &ti, int &tj) {
+    upper = tt >= TILES / 2;
+    const int q = upper ? tt - TILES / 2 : tt;
+    const int pair = q / (NT * NT), e = q % (NT * NT);
+    o = pair * 2 * B;
+    ti = e % NT;
+    tj = e / NT;
+  };
+  auto mma = [&](dbl4 &cr, dbl4 &ci, double2 a, double2 b) {   // c += a b (a: left factor, b: right)
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(b.x, a.x, cr, 0, 0, 0);
+    cr = __builtin_amdgcn_mfma_f64_16x16x4f64(b.y, -a.y, cr, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(b.y, a.x, ci, 0, 0, 0);
+    ci = __builtin_amdgcn_mfma_f64_16x16x4f64(b.x, a.y, ci, 0, 0, 0);
+  };
+  // phase 1: T tiles into registers
+  dbl4 tr[ROUNDS], tim[ROUNDS];
+#pragma unroll
+  for (int rd = 0; rd < ROUNDS; rd++) {
+    bool upper;
+    int o, ti, tj;
+    tile_of(wave + 4 * rd, upper, o, ti, tj);
+    dbl4 cr = (dbl4){0, 0, 0, 0}, ci = (dbl4){0, 0, 0, 0};
+    const int i = ti * 16 + li, jj = tj * 16 + li;   // left-factor row, right-factor column of this lane
+#pragma unroll
+    for (int k0 = 0; k0 < B; k0 += 4) {
+      const int k = k0 + lk;
+      double2 a, b;
+      if (!upper) {   // L21[i][k] Linv11[k][jj] (unit lower)
+        a = sA[(o + k) * 64 + o + B + i];
+        b = k == jj ? one : (k > jj ? sX[(o + jj) * 64 + o + k] : zero);
+      } else {        // U12[i][k] Uinv22[k][jj] (upper)
+        a = sA[(o + B + k) * 64 + o + i];
+        b = k <= jj ? sX[(o + B + jj) * 64 + o + B + k] : zero;
+      }
+      mma(cr, ci, a, b);
+    }
+    tr[rd] = cr;
+    tim[rd] = ci;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int rd = 0; rd < ROUNDS; rd++) {
+    bool upper;
+    int o, ti, tj;
+    tile_of(wave + 4 * rd, upper, o, ti, tj);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i = ti * 16 + li, jj = tj * 16 + 4 * e + lk;
+      const double2 v = make_double2(tr[rd][e], tim[rd][e]);
+      if (!upper) sA[(o + jj) * 64 + o + B + i] = v;
+      else sA[(o + B + jj) * 64 + o + i] = v;
+    }
+  }
+  __syncthreads();
+  // phase 2: Linv21 = -Linv22 T, Uinv12 = -Uinv11 T (each tile reads only
+  // blocks no tile of this phase writes)
+#pragma unroll
+  for (int rd = 0; rd < ROUNDS; rd++) {
+    bool upper;
+    int o, ti, tj;
+    tile_of(wave + 4 * rd, upper, o, ti, tj);
+    dbl4 cr = (dbl4){0, 0, 0, 0}, ci = (dbl4){0, 0, 0, 0};
+    const int i = ti * 16 + li, jj = tj * 16 + li;
+#pragma unroll
+    for (int k0 = 0; k0 < B; k0 += 4) {
+      const int k = k0 + lk;
+      double2 a, b;
+      if (!upper) {   // Linv22[i][k] (unit lower) T[k][jj]
+        a = k == i ? one : (k < i ? sX[(o + B + k) * 64 + o + B + i] : zero);
+        b = sA[(o + jj) * 64 + o + B + k];
+      } else {        // Uinv11[i][k] (upper) T[k][jj]
+        a = k >= i ? sX[(o + k) * 64 + o + i] : zero;
+        b = sA[(o + B + jj) * 64 + o + k];
+      }
+      mma(cr, ci, a, b);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int i2 = ti * 16 + li, j2 = tj * 16 + 4 * e + lk;
+      const double2 v = make_double2(-cr[e], -ci[e]);
+      if (!upper) sX[(o + j2) * 64 + o + B + i2] = v;
+      else sX[(o + B + j2) * 64 + o + i2] = v;
+    }
+  }
+  __syncthreads();
+}
+
+// Levels 1 .. 32 of the triangular inverses: scalar up to B = 8, MFMA for
+// B = 16 and 32 in complex128.
+template <typename T>
+__device__ __forceinline__ void inv_levels(T *sA, T *sX, int tid) {
+  inv_level<1>(sA, sX, tid);
+  inv_level<2>(sA, sX, tid);
+  inv_level<4>(sA, sX, tid);
+  inv_level<8>(sA, sX, tid);
+  if constexpr (std::is_same<T, double2>::value) {
+    inv_level_mfma<16>(sA, sX, tid);
+    inv_level_mfma<32>(sA, sX, tid);
+  } else {
+    inv_level<16>(sA, sX, tid);
+    inv_level<32>(sA, sX, tid);
+  }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int64_t lda, int n, int k0,
                                                           T *__restrict__ inv, int *__restrict__ ipiv,
@@ -798,93 +1038,7 @@ __global__ void __launch_bounds__(256) diag_lu_piv_kernel(T *__restrict__ A, int
     sX[idx] = v;
   }
   __syncthreads();
-  // levels b = 1, 2, 4, 8, 16, 32: pairs of b-blocks at offset o (multiple of 2b).
-  //  L: T = L21 Linv11 (rows o+b.., cols o..), Linv21 = -Linv22 T
-  //  U: T = U12 Uinv22 (rows o.., cols o+b..), Uinv12 = -Uinv11 T
-  // The L factor part of sA (strictly lower) and its U part are read-only
-  // until phase 1 of the level overwrites the L21/U12 blocks with T.
-  for (int b = 1; b < 64; b <<= 1) {
-    const int per = 32 * b;   // outputs per matrix at this level: (32/b) pairs x b*b
-    T t[8];
-    // phase 1: T into registers (slot s holds output q = tid + 256 s)
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int q = tid + 256 * s;
-      if (q >= 2 * per) break;
-      const bool upper = q >= per;
-      const int qq = upper ? q - per : q;
-      const int pair = qq / (b * b), e = qq % (b * b);
-      const int o = pair * 2 * b, i = e % b, jj = e / b;
-      T acc = Cplx<T>::mk(0.0, 0.0);
-      if (!upper) {
-        // T[i][jj] = sum_k L21[i][k] Linv11[k][jj], L21 = sA[o+b+i][o+k], Linv11 (k >= jj)
-        for (int k = jj; k < b; k++) {
-          T l = sA[(o + k) * 64 + o + b + i];
-          T x = (k == jj) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + jj) * 64 + o + k];
-          acc.x = std::fma(l.x, x.x, acc.x);
-          acc.x = std::fma(-l.y, x.y, acc.x);
-          acc.y = std::fma(l.x, x.y, acc.y);
-          acc.y = std::fma(l.y, x.x, acc.y);
-        }
-      } else {
-        // T[i][jj] = sum_k U12[i][k] Uinv22[k][jj], U12 = sA[o+i][o+b+k], Uinv22 (k <= jj)
-        for (int k = 0; k <= jj; k++) {
-          T u = sA[(o + b + k) * 64 + o + i];
-          T x = sX[(o + b + jj) * 64 + o + b + k];
-          acc.x = std::fma(u.x, x.x, acc.x);
-          acc.x = std::fma(-u.y, x.y, acc.x);
-          acc.y = std::fma(u.x, x.y, acc.y);
-          acc.y = std::fma(u.y, x.x, acc.y);
-        }
-      }
-      t[s] = acc;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < 8; s++) {
-      const int q = tid + 256 * s;
-      if (q >= 2 * per) break;
-      const bool upper = q >= per;
-      const int qq = upper ? q - per : q;
-      const int pair = qq / (b * b), e = qq % (b * b);
-      const int o = pair * 2 * b, i = e % b, jj = e / b;
-      if (!upper) sA[(o + jj) * 64 + o + b + i] = t[s];
-      else sA[(o + b + jj) * 64 + o + i] = t[s];
-    }
-    __syncthreads();
-    // phase 2: Linv21 = -Linv22 T, Uinv12 = -Uinv11 T
-    for (int q = tid; q < 2 * per; q += 256) {
-      const bool upper = q >= per;
-      const int qq = upper ? q - per : q;
-      const int pair = qq / (b * b), e = qq % (b * b);
-      const int o = pair * 2 * b, i = e % b, jj = e / b;
-      T acc = Cplx<T>::mk(0.0, 0.0);
-      if (!upper) {
-        // sum_k Linv22[i][k] T[k][jj], Linv22 lower with unit diagonal (k <= i)
-        for (int k = 0; k <= i; k++) {
-          T x = (k == i) ? Cplx<T>::mk(1.0, 0.0) : sX[(o + b + k) * 64 + o + b + i];
-          T tt = sA[(o + jj) * 64 + o + b + k];
-          acc.x = std::fma(x.x, tt.x, acc.x);
-          acc.x = std::fma(-x.y, tt.y, acc.x);
-          acc.y = std::fma(x.x, tt.y, acc.y);
-          acc.y = std::fma(x.y, tt.x, acc.y);
-        }
-        sX[(o + jj) * 64 + o + b + i] = Cplx<T>::mk(-acc.x, -acc.y);
-      } else {
-        // sum_k Uinv11[i][k] T[k][jj], Uinv11 upper (k >= i)
-        for (int k = i; k < b; k++) {
-          T x = sX[(o + k) * 64 + o + i];
-          T tt = sA[(o + b + jj) * 64 + o + k];
-          acc.x = std::fma(x.x, tt.x, acc.x);
-          acc.x = std::fma(-x.y, tt.y, acc.x);
-          acc.y = std::fma(x.x, tt.y, acc.y);
-          acc.y = std::fma(x.y, tt.x, acc.y);
-        }
-        sX[(o + b + jj) * 64 + o + i] = Cplx<T>::mk(-acc.x, -acc.y);
-      }
-    }
-    __syncthreads();
-  }
+  inv_levels(sA, sX, tid);
   T *Li = inv, *Ui = inv + 64 * 64;
   for (int idx = tid; idx < 64 * 64; idx += 256) {
     int r = idx & 63, c = idx >> 6;
