@@ -839,6 +839,16 @@ int oracle_green(double eps_re, double eps_im, double mu_re, double mu_im, doubl
   return 0;
 }
 
+/* Threads of the following assembly calls (the CPU baseline times the
+   reference-faithful single-threaded assembly beside the all-cores one). */
+void oracle_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 int oracle_num_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

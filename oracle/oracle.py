@@ -61,6 +61,8 @@ def lib():
         L.oracle_eh_nonsmooth_tri.argtypes = [f64p, ctypes.c_int, ctypes.c_int, f64p, f64p]
         L.oracle_green.argtypes = [ctypes.c_double] * 5 + [f64p, f64p, ctypes.c_int, f64p]
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_set_threads.argtypes = [ctypes.c_int]
+        L.oracle_set_threads.restype = None
         L.oracle_dipole_rhs.argtypes = [ctypes.c_int, f64p, ctypes.c_int, i32p, ctypes.c_int, i32p] + \
             [ctypes.c_double] * 5 + [ctypes.c_int, f64p, f64p, i32p, ctypes.c_void_p]
         L.oracle_dipole_rhs.restype = ctypes.c_int
