@@ -15,6 +15,7 @@ from .runtime import (  # noqa: F401
     BEM_PREC_MIXED,
     BEM_SOLVE_LOWRANK,
     BEM_LOWRANK_ACA,
+    BEM_SOLVE_NORECOVER,
     BemError,
     Body,
     Context,

@@ -2245,9 +2245,10 @@ static int lane_collect(bem_ctx *c, Lane &L) {
 enum { STEP_DONE = 0, STEP_MORE = 1, STEP_RESTART = 2 };
 
 // Collects the lane's finished stage and decides what follows:
-//  - Recovery: a block-local solve that met an exactly zero pivot, whose
-//    growth exceeds BEM_GROWTH_LIMIT, or (FP64) whose probe estimates an
-//    error of Re S above BEM_PROBE_LIMIT is solved again with tournament
+//  - Recovery: a block-local solve that met an exactly zero pivot, or (FP64)
+//    whose probe estimates an error of Re S above BEM_PROBE_LIMIT (a tenth
+//    of it when the growth exceeds BEM_GROWTH_LIMIT), or (complex64) whose
+//    growth exceeds BEM_GROWTH_LIMIT, is solved again with tournament
 //    pivoting and refinement (STEP_RESTART: the lane is released and L.req
 //    holds the request with its new flags);
 //  - with refinement requested, a result whose relative residual is above
@@ -2261,17 +2262,19 @@ static int lane_step(bem_ctx *c, Lane &L, int *err) {
   bem_result *res = L.res;
   *err = r;
   const bool tourn = (L.req.flags & BEM_SOLVE_TOURNAMENT) != 0;
+  const bool recover = !(L.req.flags & BEM_SOLVE_NORECOVER);
+  // growth above the limit makes the probe's limit ten times tighter rather
+  // than forcing a re-solve: the probe measures the result itself
+  const double plimit = res->growth <= BEM_GROWTH_LIMIT ? BEM_PROBE_LIMIT : 0.1 * BEM_PROBE_LIMIT;
   bool restart = false;
-  if (L.stage == ST_SOLVE && L.rank > 0 && L.restarts < 2 &&
-      (r == BEM_E_SINGULAR ||
-       (!r && (!(res->growth <= BEM_GROWTH_LIMIT) || !(res->rel_err_est <= BEM_PROBE_LIMIT))))) {
-    // a low-rank result the probe (or a factorisation) rejects: the dense solve
+  if (recover && L.stage == ST_SOLVE && L.rank > 0 && L.restarts < 2 &&
+      (r == BEM_E_SINGULAR || (!r && !(res->rel_err_est <= plimit)))) {
+    // a low-rank result the probe rejects (or a singular factor): the dense solve
     L.req.flags &= ~BEM_SOLVE_LOWRANK;
     restart = true;
-  } else if (L.stage == ST_SOLVE && !tourn && L.restarts < 2 &&
+  } else if (recover && L.stage == ST_SOLVE && !tourn && L.restarts < 2 &&
       (r == BEM_E_SINGULAR ||
-       (!r && (!(res->growth <= BEM_GROWTH_LIMIT) ||
-               (L.prec == BEM_PREC_FP64 && !(res->rel_err_est <= BEM_PROBE_LIMIT)))))) {
+       (!r && (L.prec == BEM_PREC_FP64 ? !(res->rel_err_est <= plimit) : !(res->growth <= BEM_GROWTH_LIMIT))))) {
     L.req.flags |= BEM_SOLVE_TOURNAMENT | BEM_SOLVE_REFINE;
     L.req.precision = BEM_PREC_FP64;
     restart = true;
@@ -2283,7 +2286,7 @@ static int lane_step(bem_ctx *c, Lane &L, int *err) {
       L.busy = false;
       res->status = e;
       *err = e;
-    } else if (L.prec == BEM_PREC_MIXED && L.restarts < 2) {
+    } else if (recover && L.prec == BEM_PREC_MIXED && L.restarts < 2) {
       L.req.precision = BEM_PREC_FP64;
       restart = true;
     }

@@ -30,6 +30,7 @@ BEM_SOLVE_TOURNAMENT = 1
 BEM_SOLVE_REFINE = 2
 BEM_SOLVE_LOWRANK = 4
 BEM_LOWRANK_ACA = 8
+BEM_SOLVE_NORECOVER = 16
 BEM_GROWTH_LIMIT = 16.0
 BEM_PROBE_COLS = 8
 BEM_PROBE_LIMIT = 1e-9

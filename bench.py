@@ -398,7 +398,7 @@ def main():
                          "(randomized range finder) and eliminated in Woodbury form; lowrank-aca: compressed by "
                          "ACA with full pivoting and SVD truncation")
     ap.add_argument("--rank-tol", type=float, default=0.0,
-                    help="relative Frobenius tolerance of the compressed cross block (0: the library default, 1e-10)")
+                    help="relative Frobenius tolerance of the compressed cross block (0: the library default, 1e-11)")
     ap.add_argument("--roofline-solves", type=int, default=2, help="single-lane solves timed per kernel class")
     ap.add_argument("--tune", default="cache", choices=["cache", "force", "off"],
                     help="GEMM tiles: from the autotuner cache (measured when missing), re-measured, or built-in rule")

@@ -102,10 +102,17 @@ typedef struct {
  * by orders of magnitude while rel_err_est does not.
  *
  * Recovery.  A block-local FP64 solve is solved again with tournament
- * pivoting and BEM_SOLVE_REFINE when its growth exceeds BEM_GROWTH_LIMIT,
- * when it met an exactly zero pivot, or when rel_err_est exceeds
- * BEM_PROBE_LIMIT; `restarts` counts such re-solves. */
+ * pivoting and BEM_SOLVE_REFINE when it met an exactly zero pivot or when
+ * rel_err_est exceeds BEM_PROBE_LIMIT -- a tenth of it when the growth
+ * exceeds BEM_GROWTH_LIMIT, since large growth may hide an inaccurate
+ * factorisation; a complex64 solve is when its growth exceeds the limit.
+ * `restarts` counts such re-solves. */
 #define BEM_SOLVE_TOURNAMENT 1
+/* BEM_SOLVE_NORECOVER: the first result is final, with its growth, probe and
+ * rel_err_est as measured -- no re-solve (Recovery) and no switch from the
+ * low-rank to the dense solve; refinement steps still run when requested.
+ * For callers that take the error statement and decide themselves. */
+#define BEM_SOLVE_NORECOVER 16
 #define BEM_GROWTH_LIMIT 16.0
 #define BEM_PROBE_COLS 8
 #define BEM_PROBE_LIMIT 1e-9
@@ -146,7 +153,7 @@ typedef struct {
  * rank used (0: dense). */
 #define BEM_SOLVE_LOWRANK 4
 #define BEM_LOWRANK_ACA 8
-#define BEM_LOWRANK_TOL 1e-10
+#define BEM_LOWRANK_TOL 1e-11
 #define BEM_LOWRANK_MAX_FRAC 0.5
 
 typedef struct {

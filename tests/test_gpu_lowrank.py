@@ -109,7 +109,7 @@ def test_lowrank_spectral_power_vs_oracle(gpu_ctx, eV, aca):
     eps = lorentz_eps(eV)
     ref = O.spectral_power(s.translated(gap), s, eV, 1e-6, 300.0, 0.0, eps, eps, return_parts=True)
     flags = BEM_SOLVE_LOWRANK | (BEM_LOWRANK_ACA if aca else 0)
-    r = gpu_ctx.calc_heat_transfer(b1, b2, Request(eV=eV, gap=gap, eps1=eps, eps2=eps, flags=flags))
+    r = gpu_ctx.calc_heat_transfer(b1, b2, Request(eV=eV, gap=gap, eps1=eps, eps2=eps, flags=flags, rank_tol=1e-10))
     rel = abs(r["ReS"] - ref["ReS"]) / abs(ref["ReS"])
     say(f"lowrank{' aca' if aca else ''} eV {eV}: rank {r['rank']} rel {rel:.2e} est {r['rel_err_est']:.1e} "
         f"restarts {r['restarts']} compress {r['ms_compress']:.1f} ms")

@@ -305,8 +305,10 @@ def _tip_request(d):
 def test_recovery_through_sweep_and_serving(gpu_ctx):
     """The re-solve with tournament pivoting and refinement runs the same way
     in bem_sweep (several lanes) and in bem_submit/bem_poll: at the cone tip
-    at 0.118 eV (gap 0.2) the block-local factorisation grows by 34 (limit 16)
-    and is solved again with tournament pivoting, and every result comes back once, with status 0, pivoting 1, one restart,
+    at 0.118 eV (gap 0.2) the block-local factorisation grows by 34 (limit
+    16, so the probe's limit is 1e-10), its error estimate exceeds that and it
+    is solved again with tournament pivoting, and every result comes back
+    once, with status 0, pivoting 1, one restart,
     and Re S within 1e-8 of the oracle; the 0.105 eV point needs no re-solve."""
     import bem_amd
     pts = _tip_points([0.118, 0.105])
