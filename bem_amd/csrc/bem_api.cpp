@@ -232,7 +232,7 @@ struct ProbeOut {
 };
 
 // Stages of the request in flight on a lane.
-enum { ST_IDLE = 0, ST_SOLVE = 1, ST_REFINE = 2 };
+enum { ST_IDLE = 0, ST_SOLVE = 1, ST_REFINE = 2, ST_COMPRESS = 3 };
 
 // One stream and its workspaces; holds at most one solve in flight.
 struct Lane {
@@ -262,6 +262,15 @@ struct Lane {
   bool probe_async = false;       // the stage's probe runs on aux (probe_join pending)
   ProbeOut *h_probe = nullptr;    // pinned copy of the last stage's probe
   double *h_rfull = nullptr;      // pinned: squared norm of the last refinement residual
+  // range finder in flight (ST_COMPRESS): the block being compressed, its
+  // factors' workspaces, the blocks issued and those whose estimates were read
+  struct {
+    const double2 *B = nullptr;
+    double2 *Q = nullptr, *Vh = nullptr;
+    int m1 = 0, m2 = 0, rmax = 0, nbmax = 0, issued = 0, checked = 0;
+    double tol = 0.0;
+    std::chrono::steady_clock::time_point t0;
+  } lr;
   int rank = 0;                   // rank of the coupling block of a low-rank solve (0: dense)
   double ms_compress = 0.0;       // its compression time (host clock)
   double *h_lr = nullptr;         // pinned: norms read back by the low-rank compression
@@ -1583,91 +1592,98 @@ static int gemm_ta(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, cons
   return ev_end(c, L, r);
 }
 
-// Blocked randomized range finder of B (m1 x m2): Q (m1 x r, orthonormal
-// columns, r a multiple of LR_BLK, leading dimension m1) and Vh = Q^H B
-// (r x m2, leading dimension ldv) with B ~ Q Vh.  Block k draws LR_BLK seeded
-// test columns W_k (entries with real and imaginary parts uniform in
-// [-1, 1), E|w|^2 = 2/3), Y = B W_k, projects out the blocks before it
-// twice (Y -= Q (Q^H Y)), and orthonormalises Y by CholQR2 (G = Y^H Y = R^H
-// R, Y <- Y R^-1, twice).  E||Y_k||_F^2 = LR_BLK (2/3) ||B - Q_<k Q_<k^H B||_F^2,
-// so r ends with the first block whose estimate is at most tol^2 ||B||_F^2
-// (that block still joins Q, which only lowers the error); a block whose
-// Gram is singular ends it before that block when its estimate is under the
-// tolerance, else the block is not compressible here.  Blocks are issued in
-// chunks and their estimates read back once per chunk: the first chunk is
-// `hint` blocks (what the pair needed last time), later ones LR_CHUNK.
-// Block k depends only on the blocks before it, so r, Q and Vh do not depend
-// on the chunking.  *rank = 0 when B is zero; -1 when rmax columns do not
-// reach the tolerance (the caller then solves densely).
+// Blocked randomized range finder of B (m1 x m2), in chunks of blocks on the
+// lane's stream: Q (m1 x r, orthonormal columns, r a multiple of LR_BLK,
+// leading dimension m1) and then Vh = Q^H B (r x m2) with B ~ Q Vh.  Block k
+// draws LR_BLK seeded test columns W_k (entries with real and imaginary parts
+// uniform in [-1, 1), E|w|^2 = 2/3), Y = B W_k, projects out the blocks
+// before it twice (Y -= Q (Q^H Y)), and orthonormalises Y by CholQR2 (G =
+// Y^H Y = R^H R, Y <- Y R^-1, twice).  E||Y_k||_F^2 = LR_BLK (2/3)
+// ||B - Q_<k Q_<k^H B||_F^2, so r ends with the first block whose estimate is
+// at most tol^2 ||B||_F^2 (that block still joins Q, which only lowers the
+// error); a block whose Gram is singular ends it before that block when its
+// estimate is under the tolerance, else the block is not compressible here.
+// lr_rand_issue() queues `nblocks` more blocks and the copies of their
+// estimates; lr_rand_eval() reads them once the stream has drained.  Block k
+// depends only on the blocks before it, so r, Q and Vh do not depend on the
+// chunking (the first chunk is what the pair needed last time).
 static const int LR_CHUNK = 4;
-static int lr_compress_rand(bem_ctx *c, Lane &L, const double2 *B, int m1, int m2, double tol, int rmax, double2 *Q,
-                            double2 *Vh, int ldv, int hint, int *rank) {
+static int lr_rand_issue(bem_ctx *c, Lane &L, int nblocks) {
   const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-  *rank = -1;
-  const int nbmax = std::min(rmax / LR_BLK, LR_MAX_BLOCKS);
-  if (nbmax < 1) return 0;
-  double2 *W, *Cb, *G, *Rinv;
+  auto &S = L.lr;
+  const int m1 = S.m1, m2 = S.m2;
+  double2 *W, *Cb, *G, *Rinv, *tap;
   double *est, *part;
   int *binfo;
   CHK(ws_get(c, L, "lr_W", sizeof(double2) * (size_t)m2 * LR_BLK, (void **)&W));
-  CHK(ws_get(c, L, "lr_Cb", sizeof(double2) * (size_t)nbmax * LR_BLK * LR_BLK, (void **)&Cb));
+  CHK(ws_get(c, L, "lr_Cb", sizeof(double2) * (size_t)S.nbmax * LR_BLK * LR_BLK, (void **)&Cb));
   CHK(ws_get(c, L, "lr_G", sizeof(double2) * LR_BLK * LR_BLK, (void **)&G));
   CHK(ws_get(c, L, "lr_Rinv", sizeof(double2) * 64 * 64, (void **)&Rinv));
   CHK(ws_get(c, L, "lr_est", sizeof(double) * (LR_MAX_BLOCKS + 1), (void **)&est));
   CHK(ws_get(c, L, "lr_binfo", sizeof(int) * (LR_MAX_BLOCKS + 1), (void **)&binfo));
   const size_t np = std::max(bem_sumsq_partials(m1, m2), bem_sumsq_partials(m1, LR_BLK));
   CHK(ws_get(c, L, "lr_ssq", sizeof(double) * np, (void **)&part));
-  if (bem_sumsq(B, m1, m1, m2, part, 0, est, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");   // est[0] = ||B||^2
-  DEV(c, hipMemsetAsync(binfo, 0, sizeof(int) * (LR_MAX_BLOCKS + 1), L.st));
-  int issued = 0, r = -1;
-  bool done = false;
-  while (!done && issued < nbmax) {
-    const int chunk = std::min(nbmax - issued, issued == 0 ? std::max(hint, LR_CHUNK) : LR_CHUNK);
-    for (int k = issued; k < issued + chunk; k++) {
-      double2 *Y = Q + (int64_t)k * LR_BLK * m1;
-      const int rk = k * LR_BLK;
-      if (bem_fill_hash(W, sizeof(double2) * (size_t)m2 * LR_BLK, 0, LR_SEED + 0x9E37ull * k, L.st))
-        return fail(c, BEM_E_DEVICE, "test column fill launch");
-      CHK(gemm(c, L, m1, LR_BLK, m2, one, B, m1, W, m2, zero, Y, m1));
-      if (rk > 0)
-        for (int pass = 0; pass < 2; pass++) {
-          CHK(gemm_ta(c, L, rk, LR_BLK, m1, one, Q, m1, Y, m1, zero, Cb, rk));
-          CHK(gemm(c, L, m1, LR_BLK, rk, mone, Q, m1, Cb, rk, one, Y, m1));
-        }
-      if (bem_sumsq(Y, m1, m1, LR_BLK, part, 0, est + 1 + k, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
-      for (int pass = 0; pass < 2; pass++) {
-        CHK(gemm_ta(c, L, LR_BLK, LR_BLK, m1, one, Y, m1, Y, m1, zero, G, LR_BLK));
-        if (bem_chol_inv64(G, LR_BLK, LR_BLK, Rinv, binfo + k, L.st)) return fail(c, BEM_E_DEVICE, "cholesky launch");
-        CHK(gemm_inplace(c, L, 'R', m1, LR_BLK, Rinv, Y, m1));
-      }
-    }
-    DEV(c, hipMemcpyAsync(L.h_lr, est, sizeof(double) * (issued + chunk + 1), hipMemcpyDeviceToHost, L.st));
-    DEV(c, hipMemcpyAsync(L.h_lri, binfo, sizeof(int) * (issued + chunk), hipMemcpyDeviceToHost, L.st));
-    DEV(c, hipStreamSynchronize(L.st));
-    const double bn2 = L.h_lr[0];
-    if (!(bn2 > 0)) {
-      *rank = bn2 == 0 ? 0 : -1;
-      return 0;
-    }
-    for (int k = issued; k < issued + chunk; k++) {
-      const bool last = L.h_lr[1 + k] / (LR_BLK * (2.0 / 3.0)) <= tol * tol * bn2;
-      if (L.h_lri[k]) {   // dependent columns: the range is exhausted if the estimate is under the tolerance
-        done = true;
-        r = last ? k * LR_BLK : -1;
-        break;
-      }
-      if (last) {
-        done = true;
-        r = (k + 1) * LR_BLK;
-        break;
-      }
-    }
-    issued += chunk;
+  // the split-K partials of every product below, at their largest, so the
+  // buffer does not grow between launches
+  const size_t ntap = std::max(bem_zgemm_ta_partials(S.rmax, LR_BLK, m1), bem_zgemm_ta_partials(LR_BLK, LR_BLK, m1));
+  CHK(ws_get(c, L, "lr_tapart", sizeof(double2) * std::max<size_t>(ntap, 1), (void **)&tap));
+  if (S.issued == 0) {
+    if (bem_sumsq(S.B, m1, m1, m2, part, 0, est, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");   // ||B||^2
+    DEV(c, hipMemsetAsync(binfo, 0, sizeof(int) * (LR_MAX_BLOCKS + 1), L.st));
   }
-  if (!done || r <= 0) return 0;
-  CHK(gemm_ta(c, L, r, m2, m1, one, Q, m1, B, m1, zero, Vh, ldv));
-  *rank = r;
+  const int k1 = std::min(S.nbmax, S.issued + nblocks);
+  for (int k = S.issued; k < k1; k++) {
+    double2 *Y = S.Q + (int64_t)k * LR_BLK * m1;
+    const int rk = k * LR_BLK;
+    if (bem_fill_hash(W, sizeof(double2) * (size_t)m2 * LR_BLK, 0, LR_SEED + 0x9E37ull * k, L.st))
+      return fail(c, BEM_E_DEVICE, "test column fill launch");
+    CHK(gemm(c, L, m1, LR_BLK, m2, one, S.B, m1, W, m2, zero, Y, m1));
+    if (rk > 0)
+      for (int pass = 0; pass < 2; pass++) {
+        CHK(gemm_ta(c, L, rk, LR_BLK, m1, one, S.Q, m1, Y, m1, zero, Cb, rk));
+        CHK(gemm(c, L, m1, LR_BLK, rk, mone, S.Q, m1, Cb, rk, one, Y, m1));
+      }
+    if (bem_sumsq(Y, m1, m1, LR_BLK, part, 0, est + 1 + k, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");
+    for (int pass = 0; pass < 2; pass++) {
+      CHK(gemm_ta(c, L, LR_BLK, LR_BLK, m1, one, Y, m1, Y, m1, zero, G, LR_BLK));
+      if (bem_chol_inv64(G, LR_BLK, LR_BLK, Rinv, binfo + k, L.st)) return fail(c, BEM_E_DEVICE, "cholesky launch");
+      CHK(gemm_inplace(c, L, 'R', m1, LR_BLK, Rinv, Y, m1));
+    }
+  }
+  S.issued = k1;
+  DEV(c, hipMemcpyAsync(L.h_lr, est, sizeof(double) * (S.issued + 1), hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_lri, binfo, sizeof(int) * S.issued, hipMemcpyDeviceToHost, L.st));
   return 0;
+}
+
+// Reads the estimates of the blocks issued since the last call (the lane's
+// stream has drained): *done with *rank = r, 0 when B is zero or -1 when it
+// does not compress within rmax; otherwise more blocks are needed.
+static void lr_rand_eval(Lane &L, bool *done, int *rank) {
+  auto &S = L.lr;
+  *done = false;
+  *rank = -1;
+  const double bn2 = L.h_lr[0];
+  if (!(bn2 > 0)) {
+    *done = true;
+    *rank = bn2 == 0 ? 0 : -1;
+    return;
+  }
+  for (int k = S.checked; k < S.issued; k++) {
+    const bool last = L.h_lr[1 + k] / (LR_BLK * (2.0 / 3.0)) <= S.tol * S.tol * bn2;
+    if (L.h_lri[k]) {   // dependent columns: the range is exhausted if the estimate is under the tolerance
+      *done = true;
+      *rank = last && k > 0 ? k * LR_BLK : -1;
+      return;
+    }
+    if (last) {
+      *done = true;
+      *rank = (k + 1) * LR_BLK;
+      return;
+    }
+  }
+  S.checked = S.issued;
+  if (S.issued >= S.nbmax) *done = true;   // rmax columns without reaching the tolerance
 }
 
 // Adaptive cross approximation with full pivoting on the device (see
@@ -1783,7 +1799,7 @@ static int rk_truncate(bem_ctx *c, Lane &L, double2 *U, int64_t ldu, double2 *V,
 // and the truncation on singular values relative to the largest, neither a
 // bound on ||B - U Vh||_F, so both run a decade tighter than tol:
 // eps2 = (tol / 10)^2 and eps = tol / 10.  U (m1 x rmax) and Vh (rmax x m2,
-// leading dimension ldv) receive the factors; *rank as lr_compress_rand.
+// leading dimension ldv) receive the factors; *rank as lr_rand_eval's.
 static int lr_compress_aca(bem_ctx *c, Lane &L, const double2 *B, int m1, int m2, double tol, int rmax, double2 *U,
                            double2 *Vh, int ldv, int *rank) {
   *rank = -1;
@@ -1936,6 +1952,94 @@ static int solve_lowrank(bem_ctx *c, Lane &L, const SelfEntry &E1, const SelfEnt
   return probe_lowrank(c, L, A, D, E1.Araw, E2.Araw, Bm, Cm, E1.Gs, E2.Gs, P, Z, K, U2, r, pout);
 }
 
+// The low-rank coupling could not be used: the dense FP64 solve on the same
+// blocks needs its own copy of B for the probe, since it factors B in place.
+static int lr_dense_fallback(bem_ctx *c, Lane &L) {
+  const int m1 = L.e1->n, m2 = L.e2->n;
+  double2 *Bm, *Braw;
+  CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
+  CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
+  DEV(c, hipMemcpyAsync(Braw, Bm, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
+  return 0;
+}
+
+// Issues the elimination, products, trace and probe of the lane's request
+// once its blocks are assembled (lane_issue) -- dense, complex64, or, with
+// r > 0, on the compressed coupling (Vh = Q^H B first unless vh_ready) --
+// and the copies of the results; lane_step() collects them.
+static int issue_solve(bem_ctx *c, Lane &L, int r, bool vh_ready) {
+  const SelfEntry &E1 = *L.e1, &E2 = *L.e2;
+  const int m1 = E1.n, m2 = E2.n;
+  const int pivot = (L.req.flags & BEM_SOLVE_TOURNAMENT) ? 1 : 0;
+  const int prec = L.prec;
+  double2 *Bm, *Cm, *Braw, *trace_out;
+  int *info;
+  unsigned long long *gro;
+  ProbeOut *pout;
+  CHK(ws_get(c, L, "B", sizeof(double2) * (size_t)m1 * m2, (void **)&Bm));
+  CHK(ws_get(c, L, "C", sizeof(double2) * (size_t)m2 * m1, (void **)&Cm));
+  if (prec == BEM_PREC_FP64 && r == 0) CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
+  else Braw = Bm;   // the mixed and low-rank eliminations leave B unfactored
+  CHK(ws_get(c, L, "info", sizeof(int) * 4, (void **)&info));
+  CHK(ws_get(c, L, "growth", sizeof(unsigned long long) * 3, (void **)&gro));
+  CHK(ws_get(c, L, "trace_out", sizeof(double2) * 2, (void **)&trace_out));
+  CHK(ws_get(c, L, "probe_out", sizeof(ProbeOut), (void **)&pout));
+  L.stage = ST_SOLVE;
+  if (r > 0) {
+    L.rank = r;
+    if (!vh_ready)
+      CHK(gemm_ta(c, L, r, m2, m1, make_double2(1, 0), L.lr.Q, m1, Bm, m1, make_double2(0, 0), L.lr.Vh, L.lr.rmax));
+    CHK(solve_lowrank(c, L, E1, E2, Bm, Cm, L.lr.Q, L.lr.Vh, L.lr.rmax, r, info, gro, trace_out, pout));
+  } else if (prec == BEM_PREC_MIXED) {
+    CHK(solve_mixed(c, L, E1, E2, Bm, Cm, info, gro, trace_out, pout, L.refine));
+    DEV(c, hipEventRecord(L.ph[2], L.st));
+  } else {
+    double2 *Sc, *B1, *partial, *invS;
+    int *ipS;
+    CHK(ws_get(c, L, "Sc", sizeof(double2) * (size_t)m2 * m2, (void **)&Sc));
+    CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
+    CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
+    CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
+    size_t np = bem_htrace_partials(m2);
+    CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
+    DEV(c, hipMemcpyAsync(Sc, E2.Araw, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
+    // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
+    const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
+    CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2, !E1.pivot));   // B <- Y
+    CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
+    if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
+    else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
+    if (L.refine) {   // Y for the refinement's corrections
+      double2 *Y;
+      CHK(ws_get(c, L, "Y64", sizeof(double2) * (size_t)m1 * m2, (void **)&Y));
+      DEV(c, hipMemcpyAsync(Y, Bm, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
+    }
+    // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
+    CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1, !pivot));
+    double2 *X = Bm;
+    DEV(c, hipEventRecord(L.ph[2], L.st));
+    const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, S{Sc, m2, ipS, invS, !pivot};
+    CHK(probe_start<double2>(c, L, A, S, E1.Araw, Braw, Cm, E2.Araw, E1.Gs, E2.Gs, Bm, pout));
+    // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
+    // = 1/4 tr(sym(G2) P) with P = X^H sym(G1) X Hermitian (sym(G) is G + G^H),
+    // so only the upper half of P is formed
+    CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
+    EventRec *rec;
+    CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &rec));
+    int e = bem_zgemm_htrace(m2, m1, X, m1, B1, m1, E2.Gs, m2, partial, trace_out, L.st);
+    if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
+    CHK(ev_end(c, L, rec));
+  }
+  CHK(probe_join(c, L));
+  DEV(c, hipEventRecord(L.ph[3], L.st));
+  DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, L.st));
+  DEV(c, hipMemcpyAsync(L.h_probe, pout, sizeof(ProbeOut), hipMemcpyDeviceToHost, L.st));
+  L.busy = true;
+  return 0;
+}
+
 // Issues every kernel of the solve stage of one spectral-power request on the
 // lane's stream and returns without waiting; lane_step() collects the result
 // and decides what follows.
@@ -2009,80 +2113,38 @@ static int lane_issue(bem_ctx *c, Lane &L, PairGeom &g, bem_body *b1, bem_body *
   CHK(assemble_cross(c, L, g, mext, Bm, Cm, prec == BEM_PREC_FP64 && !lowrank ? Braw : nullptr, info + 1));
   DEV(c, hipEventRecord(L.ph[1], L.st));
 
-  bool solved = false;
   if (lowrank) {
     const double tol = qq.rank_tol > 0 ? qq.rank_tol : BEM_LOWRANK_TOL;
     const int rmax = ((int)(BEM_LOWRANK_MAX_FRAC * std::min(m1, m2)) / LR_BLK) * LR_BLK;
-    int r = -1;
     if (rmax >= LR_BLK) {
-      double2 *Q, *Vh;
-      CHK(ws_get(c, L, "lr_Q", sizeof(double2) * (size_t)m1 * rmax, (void **)&Q));
-      CHK(ws_get(c, L, "lr_Vh", sizeof(double2) * (size_t)rmax * m2, (void **)&Vh));
-      const auto tc = std::chrono::steady_clock::now();
-      if (qq.flags & BEM_LOWRANK_ACA) CHK(lr_compress_aca(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, &r));
-      else CHK(lr_compress_rand(c, L, Bm, m1, m2, tol, rmax, Q, Vh, rmax, g.lr_blocks, &r));
-      if (r > 0 && !(qq.flags & BEM_LOWRANK_ACA)) g.lr_blocks = r / LR_BLK;
-      L.ms_compress = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc).count();
-      if (r > 0) {
-        L.rank = r;
-        CHK(solve_lowrank(c, L, E1, E2, Bm, Cm, Q, Vh, rmax, r, info, gro, trace_out, pout));
-        solved = true;
+      auto &S = L.lr;
+      S = {};
+      S.B = Bm;
+      S.m1 = m1;
+      S.m2 = m2;
+      S.rmax = rmax;
+      S.nbmax = std::min(rmax / LR_BLK, LR_MAX_BLOCKS);
+      S.tol = tol;
+      S.t0 = std::chrono::steady_clock::now();
+      CHK(ws_get(c, L, "lr_Q", sizeof(double2) * (size_t)m1 * rmax, (void **)&S.Q));
+      CHK(ws_get(c, L, "lr_Vh", sizeof(double2) * (size_t)rmax * m2, (void **)&S.Vh));
+      if (qq.flags & BEM_LOWRANK_ACA) {   // the reference's route, run to its end here
+        int r = -1;
+        CHK(lr_compress_aca(c, L, Bm, m1, m2, tol, rmax, S.Q, S.Vh, rmax, &r));
+        L.ms_compress = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - S.t0).count();
+        if (r > 0) return issue_solve(c, L, r, true);
+      } else {
+        // first chunk: what the pair needed last time; lane_step() reads the
+        // estimates and issues more blocks or the solve
+        CHK(lr_rand_issue(c, L, std::max(g.lr_blocks, LR_CHUNK)));
+        L.stage = ST_COMPRESS;
+        L.busy = true;
+        return 0;
       }
     }
-    if (!solved) {   // not compressible within rmax: the dense solve on the same blocks
-      CHK(ws_get(c, L, "Braw", sizeof(double2) * (size_t)m1 * m2, (void **)&Braw));
-      DEV(c, hipMemcpyAsync(Braw, Bm, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
-    }
+    CHK(lr_dense_fallback(c, L));
   }
-  if (solved) {
-  } else if (prec == BEM_PREC_MIXED) {
-    CHK(solve_mixed(c, L, E1, E2, Bm, Cm, info, gro, trace_out, pout, L.refine));
-    DEV(c, hipEventRecord(L.ph[2], L.st));
-  } else {
-    double2 *Sc, *B1, *partial, *invS;
-    int *ipS;
-    CHK(ws_get(c, L, "Sc", sizeof(double2) * (size_t)m2 * m2, (void **)&Sc));
-    CHK(ws_get(c, L, "B1", sizeof(double2) * (size_t)m1 * m2, (void **)&B1));
-    CHK(ws_get(c, L, "ipiv", sizeof(int) * (size_t)(m2 + 64), (void **)&ipS));
-    CHK(ws_get(c, L, "invS", inv_bytes(m2), (void **)&invS));
-    size_t np = bem_htrace_partials(m2);
-    CHK(ws_get(c, L, "trace_partial", sizeof(double2) * (np + 1), (void **)&partial));
-    DEV(c, hipMemcpyAsync(Sc, E2.Araw, sizeof(double2) * (size_t)m2 * m2, hipMemcpyDeviceToDevice, L.st));
-    // ---- solve: Y = A^-1 B, Sc = D - C Y, X = -Y Sc^-1
-    const double2 one = make_double2(1, 0), mone = make_double2(-1, 0), zero = make_double2(0, 0);
-    CHK(lu_solve(c, L, E1.Z, m1, m1, E1.ipiv, E1.inv, Bm, m1, m2, !E1.pivot));   // B <- Y
-    CHK(gemm(c, L, m2, m2, m1, mone, Cm, m2, Bm, m1, one, Sc, m2));
-    if (pivot) CHK(lu_factor(c, L, Sc, m2, m2, ipS, info, invS));
-    else CHK(lu_factor_block(c, L, Sc, m2, m2, ipS, info, invS, gro));
-    if (L.refine) {   // Y for the refinement's corrections
-      double2 *Y;
-      CHK(ws_get(c, L, "Y64", sizeof(double2) * (size_t)m1 * m2, (void **)&Y));
-      DEV(c, hipMemcpyAsync(Y, Bm, sizeof(double2) * (size_t)m1 * m2, hipMemcpyDeviceToDevice, L.st));
-    }
-    // B <- Y Sc^-1 = -X; the trace below is even in X, so the sign is dropped
-    CHK(lu_solve_right(c, L, Sc, m2, m2, ipS, invS, Bm, m1, m1, !pivot));
-    double2 *X = Bm;
-    DEV(c, hipEventRecord(L.ph[2], L.st));
-    const LuRef<double2> A{E1.Z, m1, E1.ipiv, E1.inv, !E1.pivot}, S{Sc, m2, ipS, invS, !pivot};
-    CHK(probe_start<double2>(c, L, A, S, E1.Araw, Braw, Cm, E2.Araw, E1.Gs, E2.Gs, Bm, pout));
-    // ---- products and trace: S = 1/4 sum conj(X sym(G2)) .* (sym(G1) X)
-    // = 1/4 tr(sym(G2) P) with P = X^H sym(G1) X Hermitian (sym(G) is G + G^H),
-    // so only the upper half of P is formed
-    CHK(gemm(c, L, m1, m2, m1, one, E1.Gs, m1, X, m1, zero, B1, m1));
-    EventRec *r;
-    CHK(ev_begin(c, L, BEM_KS_ZTRACE, 4.0 * m1 * (double)m2 * m2, 16.0 * (2.0 * m1 * m2 + 0.5 * m2 * m2), &r));
-    int e = bem_zgemm_htrace(m2, m1, X, m1, B1, m1, E2.Gs, m2, partial, trace_out, L.st);
-    if (e) return fail(c, BEM_E_DEVICE, "trace gemm: %d", e);
-    CHK(ev_end(c, L, r));
-  }
-  CHK(probe_join(c, L));
-  DEV(c, hipEventRecord(L.ph[3], L.st));
-  DEV(c, hipMemcpyAsync(L.h_trace, trace_out, sizeof(double2) * 2, hipMemcpyDeviceToHost, L.st));
-  DEV(c, hipMemcpyAsync(L.h_info, info, sizeof(int) * 4, hipMemcpyDeviceToHost, L.st));
-  DEV(c, hipMemcpyAsync(L.h_growth, gro, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, L.st));
-  DEV(c, hipMemcpyAsync(L.h_probe, pout, sizeof(ProbeOut), hipMemcpyDeviceToHost, L.st));
-  L.busy = true;
-  return 0;
+  return issue_solve(c, L, 0, false);
 }
 
 // Issues one refinement step (refine_step) of the lane's request, whose solve
@@ -2258,6 +2320,44 @@ enum { STEP_DONE = 0, STEP_MORE = 1, STEP_RESTART = 2 };
 //  - otherwise the result is final and the lane is released (STEP_DONE, *err
 //    the request's status).
 static int lane_step(bem_ctx *c, Lane &L, int *err) {
+  if (L.stage == ST_COMPRESS) {
+    // range-finder blocks done: more blocks, the low-rank solve, or the
+    // dense solve when the block does not compress within rmax
+    L.busy = false;
+    hipError_t se = hipStreamSynchronize(L.st);
+    int e = 0;
+    if (se != hipSuccess) {
+      e = fail(c, BEM_E_DEVICE, "compression failed on the device: %s", hipGetErrorString(se));
+    } else {
+      bool done;
+      int rk;
+      lr_rand_eval(L, &done, &rk);
+      if (!done) {
+        e = lr_rand_issue(c, L, LR_CHUNK);
+        if (!e) {
+          L.busy = true;
+          return STEP_MORE;
+        }
+      } else {
+        L.ms_compress = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - L.lr.t0).count();
+        if (rk > 0) {
+          L.pg->lr_blocks = rk / LR_BLK;
+          e = issue_solve(c, L, rk, false);
+        } else {
+          e = lr_dense_fallback(c, L);
+          if (!e) e = issue_solve(c, L, 0, false);
+        }
+        if (!e) return STEP_MORE;
+      }
+    }
+    (void)hipStreamSynchronize(L.st);
+    L.busy = false;
+    L.res->status = e;
+    *err = e;
+    self_release(c, L);
+    L.stage = ST_IDLE;
+    return STEP_DONE;
+  }
   int r = lane_collect(c, L);
   bem_result *res = L.res;
   *err = r;
@@ -2265,7 +2365,8 @@ static int lane_step(bem_ctx *c, Lane &L, int *err) {
   const bool recover = !(L.req.flags & BEM_SOLVE_NORECOVER);
   // growth above the limit makes the probe's limit ten times tighter rather
   // than forcing a re-solve: the probe measures the result itself
-  const double plimit = res->growth <= BEM_GROWTH_LIMIT ? BEM_PROBE_LIMIT : 0.1 * BEM_PROBE_LIMIT;
+  const double accept = L.req.err_tol > 0 ? L.req.err_tol : BEM_PROBE_LIMIT;
+  const double plimit = res->growth <= BEM_GROWTH_LIMIT ? accept : 0.1 * accept;
   bool restart = false;
   if (recover && L.stage == ST_SOLVE && L.rank > 0 && L.restarts < 2 &&
       (r == BEM_E_SINGULAR || (!r && !(res->rel_err_est <= plimit)))) {

@@ -60,7 +60,7 @@ class _Req(ctypes.Structure):
     _fields_ = [("eV", ctypes.c_double), ("a", ctypes.c_double), ("T1", ctypes.c_double), ("T2", ctypes.c_double),
                 ("gap", ctypes.c_double * 3), ("eps1", ctypes.c_double * 2), ("mu1", ctypes.c_double * 2),
                 ("eps2", ctypes.c_double * 2), ("mu2", ctypes.c_double * 2), ("precision", ctypes.c_int),
-                ("flags", ctypes.c_int), ("rank_tol", ctypes.c_double)]
+                ("flags", ctypes.c_int), ("rank_tol", ctypes.c_double), ("err_tol", ctypes.c_double)]
 
 
 class _Res(ctypes.Structure):
@@ -193,6 +193,7 @@ class Request:
     precision: int = BEM_PREC_FP64
     flags: int = 0
     rank_tol: float = 0.0
+    err_tol: float = 0.0
 
     def to_c(self):
         r = _Req()
@@ -206,6 +207,7 @@ class Request:
         r.precision = self.precision
         r.flags = self.flags
         r.rank_tol = self.rank_tol
+        r.err_tol = self.err_tol
         return r
 
 

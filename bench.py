@@ -123,7 +123,8 @@ def run_precision(args):
     tune(ctx, args)
     b1, b2 = ctx.body(sph), ctx.body(plate)
     reqs = [Request(eV=f, gap=(0.0, 0.0, 1.0 + g), eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f),
-                    precision=prec, flags=flags, rank_tol=args.rank_tol) for f in freqs for g in gaps]
+                    precision=prec, flags=flags, rank_tol=args.rank_tol, err_tol=args.err_tol)
+            for f in freqs for g in gaps]
     n = args.steps if args.steps < len(reqs) else len(reqs)
     reqs = reqs[:n]
     t0 = time.perf_counter()
@@ -399,6 +400,8 @@ def main():
                          "ACA with full pivoting and SVD truncation")
     ap.add_argument("--rank-tol", type=float, default=0.0,
                     help="relative Frobenius tolerance of the compressed cross block (0: the library default, 1e-11)")
+    ap.add_argument("--err-tol", type=float, default=0.0,
+                    help="accuracy accepted per result (rel_err_est bound; 0: the library default, 1e-9)")
     ap.add_argument("--roofline-solves", type=int, default=2, help="single-lane solves timed per kernel class")
     ap.add_argument("--tune", default="cache", choices=["cache", "force", "off"],
                     help="GEMM tiles: from the autotuner cache (measured when missing), re-measured, or built-in rule")
@@ -459,7 +462,7 @@ def main():
 
     def req(f):
         return Request(eV=f, gap=wl["gap"], eps1=bem_amd.lorentz_eps(f), eps2=bem_amd.lorentz_eps(f),
-                       precision=prec, flags=flags, rank_tol=args.rank_tol)
+                       precision=prec, flags=flags, rank_tol=args.rank_tol, err_tol=args.err_tol)
 
     reqs = [req(f) for f in freqs]
     if freqs_w:
@@ -516,7 +519,8 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f64 assembly + c64 elimination",
         "data": "synthetic",
         "config": {"workload": wl["name"], "precision": args.precision, "solver": args.solver,
-                   "rank_tol": args.rank_tol or None, "lanes": args.lanes, "gemm_tiles": tiles,
+                   "rank_tol": args.rank_tol or None, "err_tol": args.err_tol or None, "lanes": args.lanes,
+                   "gemm_tiles": tiles,
                    "freqs_eV": [round(f, 6) for f in freqs], "T1": 300.0, "T2": 0.0,
                    "permittivity": "6.7(w^2-0.120^2+i0.00059w)/(w^2-0.0983^2+i0.00059w)"},
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),

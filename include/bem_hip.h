@@ -77,6 +77,8 @@ typedef struct {
   int flags;         /* BEM_SOLVE_* bits */
   double rank_tol;   /* BEM_SOLVE_LOWRANK: relative Frobenius tolerance of the compressed
                         coupling block (0: BEM_LOWRANK_TOL) */
+  double err_tol;    /* accuracy the caller accepts: a result whose rel_err_est is at most
+                        err_tol is final (0: BEM_PROBE_LIMIT); see Recovery */
 } bem_request;
 
 /* Pivoting of the dense solve.  Default: partial pivoting inside each 64 x 64
@@ -103,10 +105,10 @@ typedef struct {
  *
  * Recovery.  A block-local FP64 solve is solved again with tournament
  * pivoting and BEM_SOLVE_REFINE when it met an exactly zero pivot or when
- * rel_err_est exceeds BEM_PROBE_LIMIT -- a tenth of it when the growth
- * exceeds BEM_GROWTH_LIMIT, since large growth may hide an inaccurate
- * factorisation; a complex64 solve is when its growth exceeds the limit.
- * `restarts` counts such re-solves. */
+ * rel_err_est exceeds the request's err_tol (BEM_PROBE_LIMIT when 0) -- a
+ * tenth of it when the growth exceeds BEM_GROWTH_LIMIT, since large growth
+ * may hide an inaccurate factorisation; a complex64 solve is when its growth
+ * exceeds the limit.  `restarts` counts such re-solves. */
 #define BEM_SOLVE_TOURNAMENT 1
 /* BEM_SOLVE_NORECOVER: the first result is final, with its growth, probe and
  * rel_err_est as measured -- no re-solve (Recovery) and no switch from the

@@ -162,3 +162,30 @@ def test_lowrank_c2_full_size_vs_oracle(gpu_ctx):
             f"est {r['rel_err_est']:.1e} compress {r['ms_compress']:.1f} ms")
         assert r["status"] == 0 and rel <= S_TOL
         assert r["rank"] <= 1920
+
+
+def test_err_tol_and_norecover(gpu_ctx):
+    """The request's err_tol decides which estimated error is accepted: a
+    coupling compressed at 1e-2 is rejected by the default limit (re-solved
+    densely) but final when the caller accepts 1e-1; BEM_SOLVE_NORECOVER
+    returns the first result with its estimate and no re-solve."""
+    from bem_amd import Request, lorentz_eps, BEM_SOLVE_LOWRANK, BEM_SOLVE_NORECOVER
+    from oracle import oracle as O
+    s, b1, b2, gap = _pair(gpu_ctx, 2, 5.0)
+    eV = 0.1
+    eps = lorentz_eps(eV)
+    ref = O.spectral_power(s.translated(gap), s, eV, 1e-6, 300.0, 0.0, eps, eps, return_parts=True)
+    q = dict(eV=eV, gap=gap, eps1=eps, eps2=eps, flags=BEM_SOLVE_LOWRANK, rank_tol=1e-2)
+    strict = gpu_ctx.calc_heat_transfer(b1, b2, Request(**q))
+    loose = gpu_ctx.calc_heat_transfer(b1, b2, Request(**q, err_tol=1e-1))
+    first = gpu_ctx.calc_heat_transfer(b1, b2, Request(**dict(q, flags=BEM_SOLVE_LOWRANK | BEM_SOLVE_NORECOVER)))
+    for name, r in (("strict", strict), ("loose", loose), ("norecover", first)):
+        say(f"{name}: rank {r['rank']} restarts {r['restarts']} est {r['rel_err_est']:.1e} "
+            f"rel {abs(r['ReS'] - ref['ReS']) / abs(ref['ReS']):.1e}")
+    assert strict["rank"] == 0 and strict["restarts"] == 1
+    assert abs(strict["ReS"] - ref["ReS"]) <= S_TOL * abs(ref["ReS"])
+    assert loose["rank"] > 0 and loose["restarts"] == 0 and loose["rel_err_est"] <= 1e-1
+    assert first["rank"] == loose["rank"] and first["restarts"] == 0 and first["ReS"] == loose["ReS"]
+    assert first["rel_err_est"] > 1e-9
+    # the estimate states the error: the actual error of the loose result is below it
+    assert abs(loose["ReS"] - ref["ReS"]) / abs(ref["ReS"]) <= loose["rel_err_est"]
