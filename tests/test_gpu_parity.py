@@ -256,7 +256,9 @@ def test_full_size_anchor_vs_oracle(gpu_ctx, workload):
     oracle's refined value where the point carries one (the LU value after two
     FP64 refinement steps; at the cone tip at 0.120 eV, gap 0.05, W has a
     1-norm condition estimate of 2e10 and the plain LU value differs from the
-    refined one by 4.6e-11), else of its LU value."""
+    refined one by 4.6e-11), else of its LU value.  The sphere over the plate
+    at 0.08 eV, gap 0.05 is the point whose diagonal-block LU grows by 20:
+    the probe accepts it without a re-solve."""
     import json
     path = os.path.join(GOLDEN, f"{workload}_anchor.json")
     if not os.path.exists(path):
