@@ -117,7 +117,8 @@ def test_c3_mixed_refined_at_anchors():
     """Sphere over the plate (M = 33360) at the two oracle anchor points in
     the mixed precision with refinement: complex64 elimination on FP32 MFMA,
     FP64 refinement to a relative residual of 1e-10; Re S within 1e-8 of the
-    oracle."""
+    oracle.  A point whose complex64 factorisation grows past the limit is
+    re-solved in FP64, with the same bounds."""
     import bem_amd
     from bem_amd import Request, lorentz_eps, BEM_PREC_MIXED
     from bem_amd.runtime import BEM_SOLVE_REFINE, BEM_REFINE_TOL
@@ -137,5 +138,10 @@ def test_c3_mixed_refined_at_anchors():
             rel = abs(r["ReS"] - ref) / abs(ref)
             say(f"c3 mixed+refine eV {d['eV']} gap {d['gap']}: rel {rel:.2e}, {r['iters']} steps, "
                 f"residual {r['residual']:.1e}, precision {r['precision']}, restarts {r['restarts']}")
-            assert r["status"] == 0 and rel <= S_TOL
-            assert r["residual"] <= BEM_REFINE_TOL and r["iters"] >= 1
+            assert r["status"] == 0 and rel <= S_TOL and r["residual"] <= BEM_REFINE_TOL
+            if r["precision"] == BEM_PREC_MIXED:
+                assert r["iters"] >= 1 and r["restarts"] == 0
+            else:
+                # growth over the limit in complex64 (0.08 eV, gap 0.05: 20.5):
+                # the request is re-solved in FP64, once
+                assert r["restarts"] == 1
