@@ -23,6 +23,7 @@ from .runtime import (  # noqa: F401
     lib_path,
     load_library,
     omega_tilde,
+    rwg_tree,
     theta,
     lorentz_eps,
 )

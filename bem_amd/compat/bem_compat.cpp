@@ -2,6 +2,7 @@
 #include "bem_compat.h"
 #include "x3d.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -52,21 +53,41 @@ void GreensFunction::init(Complex e, Complex m, Complex w) {
 }
 
 RWGTree::RWGTree(const std::vector<double> &xyz, const std::vector<int> &tri, Dielectric *d, const Real s[3])
-    : rwgs(0), subs(0), g(new GreensFunction(false)), diel(d), body(nullptr), owns_g_(true) {
+    : rwgs(0), subs(0), g(new GreensFunction(false)), diel(d), reference_order(false), body(nullptr),
+      owns_g_(true) {
   sub[0] = sub[1] = nullptr;
   for (int c = 0; c < 3; c++) shift[c] = s ? s[c] : 0.0;
-  bem::check(bem_body_upload(bem::context(), xyz.data(), (int)(xyz.size() / 3), tri.data(), (int)(tri.size() / 3),
-                             &body));
+  const int nv = (int)(xyz.size() / 3), nt = (int)(tri.size() / 3);
+  bem::check(bem_body_upload(bem::context(), xyz.data(), nv, tri.data(), nt, &body));
   int ne = 0;
   bem::check(bem_body_edges(body, nullptr, &ne));
   rwgs = ne;
+  // the tree is built on the translated body, as the reference meshes it
+  std::vector<double> moved(xyz);
+  for (size_t i = 0; i < moved.size(); i++) moved[i] += shift[i % 3];
+  std::vector<int32_t> ord(ne), lv(ne);
+  int n = 0, nl = 0;
+  if (bem_rwg_tree(moved.data(), nv, tri.data(), nt, 32, 0.0, ord.data(), &n, lv.data(), &nl) == 0) {
+    order.assign(ord.begin(), ord.begin() + n);
+    leaves.assign(lv.begin(), lv.begin() + nl);
+    reference_order = n == ne;
+  }
+  std::vector<char> held(ne, 0);
+  for (int e : order) held[e] = 1;
+  for (int e = 0; e < ne; e++)
+    if (!held[e]) order.push_back(e);
 }
 
 RWGTree::RWGTree(RWGTree *s, RWGTree *t)
-    : rwgs(s->rwgs + t->rwgs), subs(2), g(nullptr), diel(nullptr), body(nullptr), owns_g_(false) {
+    : rwgs(s->rwgs + t->rwgs), subs(2), g(nullptr), diel(nullptr),
+      reference_order(s->reference_order && t->reference_order), body(nullptr), owns_g_(false) {
   sub[0] = s;
   sub[1] = t;
   shift[0] = shift[1] = shift[2] = 0.0;
+  order = s->order;
+  for (int e : t->order) order.push_back(e + s->rwgs);
+  leaves = s->leaves;
+  leaves.insert(leaves.end(), t->leaves.begin(), t->leaves.end());
 }
 
 RWGTree::~RWGTree() {
@@ -84,9 +105,48 @@ static void collect(RWGTree *t, std::vector<bem_body *> &b, std::vector<double> 
   sh.insert(sh.end(), t->shift, t->shift + 3);
 }
 
-// Zts (src/Impedance.cpp:388-413).  The hot path calls it with t == s (the
-// self-interaction of a tree); for t != s with both inside one two-body tree,
-// the (t, s) block of the union's matrix is returned.
+// Rows and columns from the library's mesh order into the tree's: unknown
+// 2i + a of the tree (a = 0 for J, 1 for M) is unknown 2 order[i] + a of the
+// mesh.  Rows are gathered one column at a time, columns are moved along the
+// permutation's cycles, so one column of scratch suffices.
+static void to_tree_order(ComplexDenseMatrix &Z, const std::vector<int> &order) {
+  const int n = Z.n;
+  std::vector<int> src(n);
+  bool identity = true;
+  for (int i = 0; i < n / 2; i++) {
+    src[2 * i] = 2 * order[i];
+    src[2 * i + 1] = 2 * order[i] + 1;
+    identity = identity && order[i] == i;
+  }
+  if (identity) return;
+  std::vector<Complex> col(n);
+  for (int j = 0; j < n; j++) {
+    Complex *c = &Z(0, j);
+    for (int i = 0; i < n; i++) col[i] = c[src[i]];
+    std::copy(col.begin(), col.end(), c);
+  }
+  std::vector<char> done(n, 0);
+  for (int j0 = 0; j0 < n; j0++) {
+    if (done[j0] || src[j0] == j0) continue;
+    std::copy(&Z(0, j0), &Z(0, j0) + n, col.begin());
+    int j = j0;
+    for (;;) {
+      done[j] = 1;
+      const int from = src[j];
+      if (from == j0) {
+        std::copy(col.begin(), col.end(), &Z(0, j));
+        break;
+      }
+      std::copy(&Z(0, from), &Z(0, from) + n, &Z(0, j));
+      j = from;
+    }
+  }
+}
+
+// Zts (src/Impedance.cpp:388-413), rows and columns in the tree's order.  The
+// hot path calls it with t == s (the self-interaction of a tree); for t != s
+// with both inside one two-body tree, the (t, s) block of the union's matrix
+// is returned.
 ComplexDenseMatrix *Zts(RWGTree *t, RWGTree *s, GreensFunction *g, Options &) {
   if (t == s) {
     std::vector<bem_body *> b;
@@ -101,6 +161,7 @@ ComplexDenseMatrix *Zts(RWGTree *t, RWGTree *s, GreensFunction *g, Options &) {
       delete Z;
       throw;
     }
+    to_tree_order(*Z, t->order);
     return Z;
   }
   if (t->subs || s->subs) throw bem::Error(BEM_E_UNSUPPORTED, "Zts(t, s): t and s must be single bodies");
@@ -173,6 +234,30 @@ void parseShapes(RWGTree *&top, RWGTree *&s, RWGTree *&t, const char *body1x3d, 
   s = new RWGTree(b1.coord, b1.face, diel(b1), shift);
   t = new RWGTree(b2.coord, b2.face, diel(b2));
   top = new RWGTree(s, t);
+}
+
+extern "C" int bemc_tree_zts(const double *xyz1, int nv1, const int *tri1, int nt1, const double *shift1,
+                             const double *xyz2, int nv2, const int *tri2, int nt2, double omega, double *Z,
+                             int *order, int *reference_order) {
+  try {
+    ConstantDielectric vac(1.0, 1.0);
+    RWGTree s(std::vector<double>(xyz1, xyz1 + 3 * (size_t)nv1), std::vector<int>(tri1, tri1 + 3 * (size_t)nt1), &vac,
+              shift1);
+    RWGTree t(std::vector<double>(xyz2, xyz2 + 3 * (size_t)nv2), std::vector<int>(tri2, tri2 + 3 * (size_t)nt2), &vac);
+    RWGTree top(&s, &t);
+    GreensFunction g(Complex(1.0, 0.0), Complex(1.0, 0.0), Complex(omega, 0.0), true);
+    Options o;
+    ComplexDenseMatrix *W = Zts(&top, &top, &g, o);
+    std::copy(W->p.begin(), W->p.end(), reinterpret_cast<Complex *>(Z));
+    delete W;
+    std::copy(top.order.begin(), top.order.end(), order);
+    *reference_order = top.reference_order ? 1 : 0;
+    top.sub[0] = top.sub[1] = nullptr;
+    return 0;
+  } catch (const bem::Error &e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return e.code;
+  }
 }
 
 // src/FrequencySweep.cpp:4-17

@@ -141,10 +141,13 @@ class ComplexDenseMatrix {
 class RWGTree {
  public:
   // One body: closed triangle mesh (xyz nv x 3, tri nt x 3), its medium and a
-  // translation (the reference applies the gap to body 1's shape).
+  // translation (the reference applies the gap to body 1's shape).  The
+  // unknowns take the reference's tree order (bem_rwg_tree on the translated
+  // mesh, minSegmentSize 32, minRadius 0 as parseShapes passes them).
   RWGTree(const std::vector<double> &xyz, const std::vector<int> &tri, Dielectric *diel,
           const Real shift[3] = nullptr);
-  // Two-body top tree over bodies s (body 1) and t (body 2).
+  // Two-body top tree over bodies s (body 1) and t (body 2): s's unknowns,
+  // then t's (src/RWGTree.cpp:748-761).
   RWGTree(RWGTree *s, RWGTree *t);
   ~RWGTree();
   int rwgs;
@@ -152,6 +155,16 @@ class RWGTree {
   int subs;
   GreensFunction *g;
   Dielectric *diel;
+  // The reference's rwg[] order: order[i] is the mesh edge index
+  // (bem_body_edges order, body 1's edges first in a two-body tree) of the
+  // RWG at tree position i; Zts rows and columns follow it.  leaves holds
+  // the RWG counts of the tree's leaves.  reference_order is false where the
+  // reference's tree would not hold every edge (a closed loop of edges
+  // sharper than 2 rad: its mesher keeps only the part holding triangle 0)
+  // or its plane search fails; the missing edges then follow the tree's in
+  // mesh order, so that every unknown stays in the problem.
+  std::vector<int> order, leaves;
+  bool reference_order;
   // library handles
   bem_body *body;
   Real shift[3];
@@ -170,8 +183,8 @@ struct Vector3 {
 // file without one is an error), the two-body tree in top.  The trees come
 // back through references (the reference's pointer arguments are by value).
 // Meshes are used as read: the reference's transfer drivers refine no edge
-// (refineEdgeFuncBody returns false, src/HeatTransfer.h:28-31), and its
-// segmentation and tree order only permute the unknowns.
+// (refineEdgeFuncBody returns false, src/HeatTransfer.h:28-31); the trees
+// carry the reference's unknown order.
 void parseShapes(RWGTree *&top, RWGTree *&s, RWGTree *&t, const char *body1x3d, const char *body2x3d,
                  const Vector3 &gap, Options &options);
 
@@ -182,6 +195,17 @@ void initReferenceCountingPointerLock();
 ComplexDenseMatrix *Zts(RWGTree *t, RWGTree *s, GreensFunction *g, Options &options);
 void Zdense(RWGTree *t, RWGTree *s, ComplexDenseMatrix *dense, GreensFunction *gExt, Options &options);
 Real CalcHeatTransfer(Real a, Real eV, Real T1, Real T2, RWGTree *top, RWGTree *s, RWGTree *t, Options &options);
+
+// C entry point for callers outside C++ (bindings, tests); needs a device.
+// Two bodies as parseShapes makes them (body 1 translated by shift1), the
+// exterior Zts(top, top, g) at dimensionless omega (vacuum) written to Z
+// (2E x 2E complex128 column-major, E = total RWGs) in the tree's order, and
+// the tree's order (mesh RWG index per position, body 1's edges first) to
+// order[0..E).  *reference_order is 1 where that order is the reference's.
+// Returns 0 or a BEM_E_* code.
+extern "C" int bemc_tree_zts(const double *xyz1, int nv1, const int *tri1, int nt1, const double *shift1,
+                             const double *xyz2, int nv2, const int *tri2, int nt2, double omega, double *Z,
+                             int *order, int *reference_order);
 
 class FrequencySweep {
  public:

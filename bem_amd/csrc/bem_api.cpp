@@ -3019,6 +3019,20 @@ int bem_body_edges(bem_body *b, int32_t *edges, int *ne) {
   return 0;
 }
 
+int bem_rwg_tree(const double *xyz, int nv, const int32_t *tri, int nt, int min_size, double min_radius,
+                 int32_t *order, int *n, int32_t *leaves, int *nleaves) {
+  if (!xyz || !tri || nv <= 0 || nt <= 0 || !order || !n || !leaves || !nleaves || min_size < 0)
+    return BEM_E_INVALID;
+  std::vector<int32_t> sizes;
+  const int rc = bemgeo::rwg_tree_order(nv, xyz, nt, tri, min_size, min_radius, order, *n, sizes);
+  if (rc == -1) return BEM_E_INVALID;
+  if (rc == -2) return BEM_E_MESH;
+  if (rc) return BEM_E_UNSUPPORTED;
+  std::memcpy(leaves, sizes.data(), sizeof(int32_t) * sizes.size());
+  *nleaves = (int)sizes.size();
+  return 0;
+}
+
 void bem_body_free(bem_body *b) {
   if (!b) return;
   bem_ctx *c = b->ctx;

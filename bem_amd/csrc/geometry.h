@@ -36,5 +36,12 @@ void build_tree(const std::vector<const BodyMesh *> &bodies, const double *shift
 // RWGs of its edges e12, e23, e31 with the signed edge length.  Returns -1
 // when an edge of a triangle has no RWG (an open surface).
 int flux_table(const BodyMesh &B, const double *shift, int rwg_off, std::vector<FluxTri> &out);
+// The reference's RWGTree order of one body (rwgtree.cpp): order[0..n) are
+// edge indices (build_edges order) leaf by leaf, leaf_sizes the RWG counts of
+// the leaves.  order needs room for every edge.  Returns 0, -1 on a bad
+// vertex id, -2 when the surface is not closed, -3 when the plane search
+// does not converge.
+int rwg_tree_order(int nv, const double *xyz, int nt, const int32_t *tri, int min_size, double min_radius,
+                   int32_t *order, int &n, std::vector<int32_t> &leaf_sizes);
 
 }  // namespace bemgeo

@@ -105,7 +105,7 @@ EXPORTS = ["bem_ctx_create", "bem_ctx_destroy", "bem_last_error", "bem_ctx_devic
            "bem_device_free", "bem_memcpy_h2d", "bem_memcpy_d2h", "bem_last_timings", "bem_kernel_stats",
            "bem_submit", "bem_poll", "bem_queue_depth", "bem_cgemm", "bem_gemm_tiles", "bem_autotune",
            "bem_synchronize", "bem_zgetrf_local_noalloc", "bem_dipole_rhs", "bem_flux", "bem_dipole_flux",
-           "bem_aca", "bem_rk_truncate"]
+           "bem_aca", "bem_rk_truncate", "bem_rwg_tree"]
 
 
 def load_library():
@@ -157,8 +157,33 @@ def load_library():
     L.bem_queue_depth.argtypes = [vp, P(i32), P(i32)]
     L.bem_aca.argtypes = [vp, vp, i64, i32, i32, dbl, i32, vp, i64, vp, i64, vp, vp, P(i32)]
     L.bem_rk_truncate.argtypes = [vp, vp, i64, vp, i64, i32, i32, i32, dbl, P(i32), vp]
+    L.bem_rwg_tree.argtypes = [vp, i32, vp, i32, i32, dbl, vp, P(i32), vp, P(i32)]
     _lib = L
     return L
+
+
+def rwg_tree(xyz, tri, shift=None, min_size=32, min_radius=0.0):
+    """The reference's order of a body's unknowns (bem_rwg_tree; the RWGTree
+    of src/RWGTree.cpp:602-761 as parseShapes builds it, with
+    Options::minSegmentSize = 32).  The body is translated by shift first,
+    as the reference translates body 1 before meshing.  Returns (order,
+    leaves): the mesh edge index at each tree position and the RWG count of
+    each leaf.  Host only: needs the library, no device."""
+    L = load_library()
+    X = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+    if shift is not None:
+        X = X + np.asarray(shift, dtype=np.float64)
+    T = np.ascontiguousarray(tri, dtype=np.int32).reshape(-1, 3)
+    cap = 3 * len(T)
+    order = np.zeros(cap, dtype=np.int32)
+    leaves = np.zeros(cap, dtype=np.int32)
+    n, nl = ctypes.c_int(0), ctypes.c_int(0)
+    rc = L.bem_rwg_tree(X.ctypes.data, len(X), T.ctypes.data, len(T), int(min_size), float(min_radius),
+                        order.ctypes.data, ctypes.byref(n), leaves.ctypes.data, ctypes.byref(nl))
+    if rc:
+        raise BemError(rc, {BEM_E_MESH: "rwg_tree: the surface is not closed",
+                            BEM_E_UNSUPPORTED: "rwg_tree: the plane search did not converge"}.get(rc, "rwg_tree: bad argument"))
+    return order[:n.value].copy(), leaves[:nl.value].copy()
 
 
 def omega_tilde(eV, a):

@@ -23,6 +23,8 @@
  *   bem_body_upload     mesh -> RWG construction of ShapeMesh + RWG
  *                       (src/ShapeMesh.cpp:821-947, src/RWG.cpp:5-39), with
  *                       the reference's edge order and orientation
+ *   bem_rwg_tree        RWGTree(meshDiel, edgesAtPoint, minSize, minRadius)
+ *                       src/RWGTree.cpp:602-761: the reference's unknown order
  *   bem_zgetrf / bem_zgetrf_local / bem_zgetrs / bem_zgemm
  *                       the LAPACK/BLAS FFI the reference binds
  *                       (zgetrf_/zgetri_ src/util/DenseMatrix.cpp:38-77,
@@ -199,6 +201,23 @@ int bem_body_upload(bem_ctx *ctx, const double *xyz, int nv, const int32_t *tri,
  * (ne x 4) when edges != NULL. */
 int bem_body_edges(bem_body *b, int32_t *edges, int *ne);
 void bem_body_free(bem_body *b);
+/* The order in which the reference lists a body's unknowns: the leaves of
+ * the RWGTree its drivers build (RWGTree(meshDiel, edgesAtPoint,
+ * minSegmentSize, minRadius), src/RWGTree.cpp:602-761, called from
+ * parseShapes src/HeatTransfer.cpp:98-105; its partition src/RWGTree.cpp:
+ * 21-559 and the mesher's segmentation src/ShapeMesh.cpp:930-970,
+ * 1917-1927).  Host only: no context, no device.  xyz, tri as
+ * bem_body_upload, with the body's translation already applied (the
+ * reference translates body 1 before meshing); min_size is
+ * Options::minSegmentSize (32), min_radius 0 as parseShapes passes.
+ * order[i] (room for every edge) is the edge index (bem_body_edges order) of
+ * tree position i < *n; leaves[k] (room for every edge) the RWG count of
+ * leaf k, k < *nleaves.  *n is below the edge count where the surface has a
+ * closed loop of edges sharper than 2 rad: the reference keeps only the part
+ * that holds triangle 0.  BEM_E_MESH for an open surface, BEM_E_UNSUPPORTED
+ * when the plane search does not converge (the reference aborts). */
+int bem_rwg_tree(const double *xyz, int nv, const int32_t *tri, int nt, int min_size, double min_radius,
+                 int32_t *order, int *n, int32_t *leaves, int *nleaves);
 
 /* ---- assembly (Zts) --------------------------------------------------- */
 /* Impedance matrix of the union of nb (1 or 2) bodies, body k translated by

@@ -254,6 +254,40 @@ def run_reference(meshes, eps_list, mu_list, omega, workdir):
     return edges, mats[0], mats[1:]
 
 
+def run_reference_tree(meshes, workdir):
+    """Run the reference's RWGTree construction (oracle/_ref/zts_ref --tree)
+    on the bodies as parseShapes meshes them (src/HeatTransfer.cpp:75-105).
+
+    Returns, per body, (nseg, order, leaves): the number of segments its
+    ShapeMesh keeps, the Edge::index of each RWG in the body's subtree in
+    tree order, and the RWG counts of the tree's leaves in that order.
+    Fixture generation only."""
+    if not os.path.exists(REF_DRIVER):
+        raise RuntimeError("reference driver not built (make -C oracle ref)")
+    lines = ["1.0", str(len(meshes))]
+    for m in meshes:
+        lines.append(f"1.0 0.0 1.0 0.0 {m.nv} {m.nt}")
+        lines += [f"{x!r} {y!r} {z!r}" for x, y, z in m.xyz.tolist()]
+        lines += [f"{a} {b} {c}" for a, b, c in m.tri.tolist()]
+    out = os.path.join(workdir, "tree_ref.bin")
+    subprocess.run([REF_DRIVER, out, "--tree"], input="\n".join(lines) + "\n", text=True, check=True,
+                   stdout=subprocess.DEVNULL)
+    raw = np.frombuffer(open(out, "rb").read(), dtype=np.int64)
+    pos = 0
+    for _ in meshes:
+        pos += 1 + 4 * int(raw[pos])
+    res = []
+    for _ in meshes:
+        nseg, n = int(raw[pos]), int(raw[pos + 1])
+        order = raw[pos + 2:pos + 2 + n].astype(np.int32)
+        pos += 2 + n
+        nl = int(raw[pos])
+        leaves = raw[pos + 1:pos + 1 + nl].astype(np.int32)
+        pos += 1 + nl
+        res.append((nseg, order, leaves))
+    return res
+
+
 def run_reference_source(meshes, eps_list, mu_list, omega, region, p, j, X, workdir):
     """Run the reference's Bdense and flux (oracle/_ref/zts_ref --source).
 

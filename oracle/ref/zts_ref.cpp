@@ -19,6 +19,11 @@
 //   then complex128 column-major matrices:
 //     Zts(top, top, exterior g)   (2E x 2E, E = total RWGs, body 1 first)
 //     Zts(body_b, body_b, g_b)    for each body b
+//   or, with --tree (edge tables as above, then per body):
+//     int64 nseg: segments the reference's ShapeMesh keeps (m->segment)
+//     int64 n, then n int64: Edge::index of each RWG in the body's subtree of
+//       the reference's RWGTree(meshDiel, ...) after unshuffle (tree order)
+//     int64 nleaf, then nleaf int64: RWG counts of the leaves in that order
 //   or, with --source:
 //     B (2E complex128): Bdense of each body's tree in body order, the
 //       leaf level of BH (src/Source.cpp:199-215)
@@ -28,7 +33,9 @@
 // RWG::RWG (src/RWG.cpp:5), RWGTree protected ctor (src/RWGTree.cpp:590),
 // GreensFunction (src/GreensFunction.cpp:11), Zts (src/Impedance.cpp:388),
 // DipolePointSource (src/Source.cpp:14), Bdense (src/Source.cpp:167),
-// flux (src/Flux.cpp:100).
+// flux (src/Flux.cpp:100), RWGTree(meshDiel, edgesAtPoint, minSize,
+// minRadius) (src/RWGTree.cpp:602) with the arguments parseShapes gives it
+// (src/HeatTransfer.cpp:98-105).
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -63,9 +70,19 @@ static void write_matrix(ComplexDenseMatrix *Z) {
   }
 }
 
+static void leaves(RWGTree *t, std::vector<int64_t> &sizes) {
+  if (!t->subs) {
+    sizes.push_back(t->rwgs);
+    return;
+  }
+  for (int i = 0; i < t->subs; i++) leaves(t->sub[i], sizes);
+}
+
 int main(int argc, char **argv) {
-  if (argc != 2 && !(argc == 3 && std::string(argv[2]) == "--source")) return 2;
-  const bool source_mode = argc == 3;
+  const std::string mode = argc == 3 ? argv[2] : "";
+  if (argc != 2 && !(argc == 3 && (mode == "--source" || mode == "--tree"))) return 2;
+  const bool source_mode = mode == "--source";
+  const bool tree_mode = mode == "--tree";
   out = fopen(argv[1], "wb");
   if (!out) return 2;
   initReferenceCountingPointerLock();
@@ -113,6 +130,38 @@ int main(int argc, char **argv) {
       fwrite(rec, sizeof(int64_t), 4, out);
       rwgs[b].push_back(new RWG(e));
     }
+  }
+  if (tree_mode) {
+    // the edges-at-point map of parseShapes (getEdgesAtPoints,
+    // src/ShapeMesh.cpp:1962-1973): every edge of every kept segment, in
+    // segment and edge-index order
+    ShapeMeshDielectricPairList meshDiel;
+    for (int b = 0; b < nbody; b++) {
+      for (SegmentSet::const_iterator i = meshes[b]->segment.begin(); i != meshes[b]->segment.end(); ++i)
+        for (EdgeSet::const_iterator k = (*i)->edge.begin(); k != (*i)->edge.end(); ++k) {
+          eap[(*k)->p1].push_back(*k);
+          eap[(*k)->p2].push_back(*k);
+        }
+      meshDiel.push_back(ShapeMeshDielectricPair(meshes[b], (Dielectric *)NULL));
+    }
+    RWGTree top(meshDiel, eap, options.minSegmentSize, 0.0);
+    for (int b = 0; b < nbody; b++) {
+      RWGTree *t = nbody > 1 ? top.sub[b] : &top;
+      int64_t nseg = meshes[b]->segment.size(), n = t->rwgs;
+      fwrite(&nseg, sizeof nseg, 1, out);
+      fwrite(&n, sizeof n, 1, out);
+      for (int i = 0; i < t->rwgs; i++) {
+        int64_t k = t->rwg[i]->e->index;
+        fwrite(&k, sizeof k, 1, out);
+      }
+      std::vector<int64_t> sizes;
+      leaves(t, sizes);
+      int64_t nl = sizes.size();
+      fwrite(&nl, sizeof nl, 1, out);
+      fwrite(sizes.data(), sizeof(int64_t), sizes.size(), out);
+    }
+    fclose(out);
+    return 0;
   }
   size_t total = 0;
   for (int b = 0; b < nbody; b++) total += rwgs[b].size();

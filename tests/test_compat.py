@@ -189,3 +189,50 @@ def test_transfer_farm_refuses_ambiguous_restart(tmp_path):
     out = subprocess.run([farm], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert out.returncode == 2
     assert "same x" in out.stderr and "move ./output away" in out.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["spheres", "cone"])
+def test_compat_zts_in_reference_tree_order(gpu_ctx, kind):
+    """Compat Zts lists its unknowns in the reference's tree order: the
+    library's mesh-order W with rows and columns taken in the order that
+    bem_rwg_tree gives (itself equal to the reference's RWGTree,
+    tests/test_rwgtree.py), J and M of each RWG kept together, bit for bit.
+    The cone's base rim is sharper than 2 rad: the reference's tree would
+    hold only part of its edges, so the order is flagged as not the
+    reference's and the rest follow in mesh order."""
+    import ctypes
+    from bem_amd import mesh, rwg_tree
+    if kind == "spheres":
+        b1, b2, shift = mesh.icosphere(2), mesh.icosphere(2), (2.2, 0.0, 0.0)
+    else:
+        b1, b2, shift = mesh.cone_tip(n_around=12, n_along=6, n_cap=2, n_base=2), mesh.cube(3), (0.0, 0.0, 2.6)
+    lib = ctypes.CDLL(os.path.join(ROOT, "bem_amd", "libbem_compat.so"))
+    vp, i32 = ctypes.c_void_p, ctypes.c_int
+    lib.bemc_tree_zts.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp, i32, ctypes.c_double, vp, vp,
+                                  ctypes.POINTER(i32)]
+    ne1, ne2 = 3 * b1.nt // 2, 3 * b2.nt // 2
+    E = ne1 + ne2
+    omega = 0.5
+    Z = np.zeros((2 * E, 2 * E), dtype=np.complex128, order="F")
+    order = np.zeros(E, dtype=np.int32)
+    ref = i32(0)
+    sh = np.array(shift, dtype=np.float64)
+    print(f"compat tree Zts: {kind}, E = {E}", flush=True)
+    rc = lib.bemc_tree_zts(b1.xyz.ctypes.data, b1.nv, b1.tri.ctypes.data, b1.nt, sh.ctypes.data,
+                           b2.xyz.ctypes.data, b2.nv, b2.tri.ctypes.data, b2.nt, omega, Z.ctypes.data,
+                           order.ctypes.data, ctypes.byref(ref))
+    assert rc == 0
+    o1, _ = rwg_tree(b1.xyz, b1.tri, shift=shift)
+    o2, _ = rwg_tree(b2.xyz, b2.tri)
+    if kind == "spheres":
+        assert ref.value == 1
+        assert np.array_equal(order, np.concatenate([o1, o2 + ne1]))
+    else:
+        assert ref.value == 0 and len(o1) < ne1
+        rest = np.setdiff1d(np.arange(ne1), o1)
+        assert np.array_equal(order, np.concatenate([o1, rest, o2 + ne1]))
+    W = gpu_ctx.zts([gpu_ctx.body(b1), gpu_ctx.body(b2)], [shift, (0.0, 0.0, 0.0)], 1.0, 1.0, omega)
+    idx = np.empty(2 * E, dtype=np.int64)
+    idx[0::2], idx[1::2] = 2 * order, 2 * order + 1
+    assert np.array_equal(Z, W[np.ix_(idx, idx)])
