@@ -521,6 +521,10 @@ def main():
         "config": {"workload": wl["name"], "precision": args.precision, "solver": args.solver,
                    "rank_tol": args.rank_tol or None, "err_tol": args.err_tol or None, "lanes": args.lanes,
                    "gemm_tiles": tiles,
+                   "twin_bodies": wl["m1"] is wl["m2"],
+                   "self_blocks": ("identical bodies and media: each frequency's exterior and interior self "
+                                   "blocks are assembled once and copied to the other body; a dissimilar pair "
+                                   "assembles both" if wl["m1"] is wl["m2"] else "assembled per body"),
                    "freqs_eV": [round(f, 6) for f in freqs], "T1": 300.0, "T2": 0.0,
                    "permittivity": "6.7(w^2-0.120^2+i0.00059w)/(w^2-0.0983^2+i0.00059w)"},
         "latency_ms": {"p50": statistics.median(lat), "p99": float(np.percentile(lat, 99)),
