@@ -1576,6 +1576,8 @@ static const int LR_MAX_BLOCKS = 512;
 // Seed of the range finder's test columns (fixed, so a request gives the same
 // factors alone and in a mix).
 static const unsigned long long LR_SEED = 0x10C0FFEEull;
+// Blocks whose products Y = B W_k are formed by one GEMM.
+static const int LR_YGROUP = 8;
 
 // C (m x n) <- alpha A^H B + beta C (A k x m, B k x n) on the lane's stream.
 static int gemm_ta(bem_ctx *c, Lane &L, int m, int n, int k, double2 alpha, const double2 *A, int64_t lda,
@@ -1615,7 +1617,7 @@ static int lr_rand_issue(bem_ctx *c, Lane &L, int nblocks) {
   double2 *W, *Cb, *G, *Rinv, *tap;
   double *est, *part;
   int *binfo;
-  CHK(ws_get(c, L, "lr_W", sizeof(double2) * (size_t)m2 * LR_BLK, (void **)&W));
+  CHK(ws_get(c, L, "lr_W", sizeof(double2) * (size_t)m2 * LR_BLK * LR_YGROUP, (void **)&W));
   CHK(ws_get(c, L, "lr_Cb", sizeof(double2) * (size_t)S.nbmax * LR_BLK * LR_BLK, (void **)&Cb));
   CHK(ws_get(c, L, "lr_G", sizeof(double2) * LR_BLK * LR_BLK, (void **)&G));
   CHK(ws_get(c, L, "lr_Rinv", sizeof(double2) * 64 * 64, (void **)&Rinv));
@@ -1625,7 +1627,8 @@ static int lr_rand_issue(bem_ctx *c, Lane &L, int nblocks) {
   CHK(ws_get(c, L, "lr_ssq", sizeof(double) * np, (void **)&part));
   // the split-K partials of every product below, at their largest, so the
   // buffer does not grow between launches
-  const size_t ntap = std::max(bem_zgemm_ta_partials(S.rmax, LR_BLK, m1), bem_zgemm_ta_partials(LR_BLK, LR_BLK, m1));
+  size_t ntap = bem_zgemm_ta_partials(LR_BLK, LR_BLK, m1);
+  for (int rk = LR_BLK; rk <= S.rmax; rk += LR_BLK) ntap = std::max(ntap, bem_zgemm_ta_partials(rk, LR_BLK, m1));
   CHK(ws_get(c, L, "lr_tapart", sizeof(double2) * std::max<size_t>(ntap, 1), (void **)&tap));
   if (S.issued == 0) {
     if (bem_sumsq(S.B, m1, m1, m2, part, 0, est, L.st)) return fail(c, BEM_E_DEVICE, "norm launch");   // ||B||^2
@@ -1635,9 +1638,18 @@ static int lr_rand_issue(bem_ctx *c, Lane &L, int nblocks) {
   for (int k = S.issued; k < k1; k++) {
     double2 *Y = S.Q + (int64_t)k * LR_BLK * m1;
     const int rk = k * LR_BLK;
-    if (bem_fill_hash(W, sizeof(double2) * (size_t)m2 * LR_BLK, 0, LR_SEED + 0x9E37ull * k, L.st))
-      return fail(c, BEM_E_DEVICE, "test column fill launch");
-    CHK(gemm(c, L, m1, LR_BLK, m2, one, S.B, m1, W, m2, zero, Y, m1));
+    if ((k - S.issued) % LR_YGROUP == 0) {
+      // Y = B W for up to LR_YGROUP blocks in one GEMM.  With the
+      // three-product tiles every output element sums over m2 in the same
+      // order whatever the product's width, so block k's columns are the
+      // numbers its own product gives and the grouping changes only the rate
+      const int g = std::min(LR_YGROUP, k1 - k);
+      for (int j = 0; j < g; j++)
+        if (bem_fill_hash(W + (size_t)j * m2 * LR_BLK, sizeof(double2) * (size_t)m2 * LR_BLK, 0,
+                          LR_SEED + 0x9E37ull * (k + j), L.st))
+          return fail(c, BEM_E_DEVICE, "test column fill launch");
+      CHK(gemm(c, L, m1, LR_BLK * g, m2, one, S.B, m1, W, m2, zero, Y, m1));
+    }
     if (rk > 0)
       for (int pass = 0; pass < 2; pass++) {
         CHK(gemm_ta(c, L, rk, LR_BLK, m1, one, S.Q, m1, Y, m1, zero, Cb, rk));

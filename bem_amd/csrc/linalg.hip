@@ -2305,10 +2305,15 @@ int bem_identity_minus(double2 *Cm, int64_t ldc, int n, hipStream_t st) {
   return (int)hipGetLastError();
 }
 
+// Tile of bem_zgemm_ta: 64 x 64 on eight waves when both output dimensions
+// reach 64 (the projections onto a range basis, rank x 64), else 32 x 32.
+static int ta_tile(int m, int n) { return (m >= 64 && n >= 64) ? 64 : 32; }
+
 // K splits of bem_zgemm_ta for an m x n output over k: enough workgroups to
 // fill the device (about 512), each split at least 64 long.
 static int ta_splits(int m, int n, int k) {
-  const int blocks = ((m + 31) / 32) * ((n + 31) / 32);
+  const int t = ta_tile(m, n);
+  const int blocks = ((m + t - 1) / t) * ((n + t - 1) / t);
   int nz = (512 + blocks - 1) / blocks;
   const int kmax = (k + 63) / 64;
   if (nz > kmax) nz = kmax;
@@ -2329,10 +2334,15 @@ int bem_zgemm_ta(int m, int n, int k, double2 alpha, const double2 *A, int64_t l
   int kper = (k + nz - 1) / nz;
   kper = (kper + 15) / 16 * 16;
   nz = (k + kper - 1) / kper;
-  if ((n + 31) / 32 > 65535 || nz > 65535) return -1;
-  dim3 grid((m + 31) / 32, (n + 31) / 32, nz);
-  hipLaunchKernelGGL((zgemm_ta_kernel<32, 32, 2, 2, 16>), grid, dim3(256), 0, st, m, n, k, kper, A, lda, B, ldb, alpha,
-                     beta, C, ldc, part);
+  const int t = ta_tile(m, n);
+  if ((n + t - 1) / t > 65535 || nz > 65535) return -1;
+  dim3 grid((m + t - 1) / t, (n + t - 1) / t, nz);
+  if (t == 64)
+    hipLaunchKernelGGL((zgemm_ta_kernel<64, 64, 2, 4, 16>), grid, dim3(512), 0, st, m, n, k, kper, A, lda, B, ldb,
+                       alpha, beta, C, ldc, part);
+  else
+    hipLaunchKernelGGL((zgemm_ta_kernel<32, 32, 2, 2, 16>), grid, dim3(256), 0, st, m, n, k, kper, A, lda, B, ldb,
+                       alpha, beta, C, ldc, part);
   if (nz > 1)
     hipLaunchKernelGGL(zgemm_ta_finish_kernel, dim3((m + 255) / 256, n), dim3(256), 0, st, m, n, nz,
                        (const double2 *)part, alpha, beta, C, ldc);

@@ -148,7 +148,7 @@ def run_precision(args):
                        for q, r in zip(reqs, res)]}
     if args.solver != "dense":
         out["rank"] = rank_summary(res)
-    if args.precision != "fp64" or args.solver != "dense":
+    if (args.precision != "fp64" or args.solver != "dense") and not args.no_reference:
         ref = []
         for q in reqs:
             ref.append(ctx.calc_heat_transfer(b1, b2, Request(eV=q.eV, gap=q.gap, eps1=q.eps1, eps2=q.eps2)))
