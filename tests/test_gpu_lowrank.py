@@ -189,3 +189,29 @@ def test_err_tol_and_norecover(gpu_ctx):
     assert first["rel_err_est"] > 1e-9
     # the estimate states the error: the actual error of the loose result is below it
     assert abs(loose["ReS"] - ref["ReS"]) / abs(ref["ReS"]) <= loose["rel_err_est"]
+
+
+def test_lowrank_c3_anchors_vs_oracle():
+    """The sphere over the plate (M = 33360) with the low-rank coupling at the
+    oracle's full-size anchor points (tests/golden/c3_anchor.json, LAPACK on
+    the CPU): Re S within 1e-8, the coupling compressed (rank below half of
+    15360) and not sent to the dense solve."""
+    import bem_amd
+    from bem_amd import Request, lorentz_eps, BEM_SOLVE_LOWRANK
+    sys.path.insert(0, GOLDEN)
+    from make_anchor import WORKLOADS
+    _, body1, body2, _ = WORKLOADS["c3"]
+    points = json.load(open(os.path.join(GOLDEN, "c3_anchor.json")))["points"]
+    with bem_amd.Context(0, lanes=2) as ctx:
+        b1, b2 = ctx.body(body1()), ctx.body(body2())
+        for d in points:
+            eps = lorentz_eps(d["eV"])
+            r = ctx.calc_heat_transfer(b1, b2, Request(eV=d["eV"], a=d["a"], T1=d["T1"], T2=d["T2"],
+                                                       gap=tuple(d["shift"]), eps1=eps, eps2=eps,
+                                                       flags=BEM_SOLVE_LOWRANK))
+            ref = d.get("ReS_refined", d["ReS"])
+            rel = abs(r["ReS"] - ref) / abs(ref)
+            say(f"c3 lowrank eV {d['eV']} gap {d['gap']}: rank {r['rank']} rel {rel:.2e} "
+                f"est {r['rel_err_est']:.1e} restarts {r['restarts']} compress {r['ms_compress']:.0f} ms")
+            assert r["status"] == 0 and rel <= S_TOL
+            assert 0 < r["rank"] <= 7680 and r["restarts"] == 0
