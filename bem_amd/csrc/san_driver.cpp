@@ -84,7 +84,49 @@ Mesh icosphere(int level) {
   return m;
 }
 
+// Octahedron flattened to half-height hz and subdivided `level` times
+// (midpoints, no projection): its equator is a closed loop of edges sharper
+// than 2 rad when hz is small.
+Mesh flat_octahedron(int level, double hz) {
+  Mesh m;
+  const double v0[6][3] = {{1, 0, 0}, {-1, 0, 0}, {0, 1, 0}, {0, -1, 0}, {0, 0, hz}, {0, 0, -hz}};
+  const int f0[8][3] = {{0, 2, 4}, {2, 1, 4}, {1, 3, 4}, {3, 0, 4}, {2, 0, 5}, {1, 2, 5}, {3, 1, 5}, {0, 3, 5}};
+  for (auto &v : v0) m.xyz.insert(m.xyz.end(), v, v + 3);
+  for (auto &f : f0) m.tri.insert(m.tri.end(), f, f + 3);
+  for (int l = 0; l < level; l++) {
+    std::map<std::pair<int, int>, int> mid;
+    auto middle = [&](int a, int b) {
+      auto k = std::make_pair(std::min(a, b), std::max(a, b));
+      auto it = mid.find(k);
+      if (it != mid.end()) return it->second;
+      for (int c = 0; c < 3; c++) m.xyz.push_back(0.5 * (m.xyz[3 * a + c] + m.xyz[3 * b + c]));
+      return mid[k] = m.nv() - 1;
+    };
+    std::vector<int32_t> next;
+    for (int f = 0; f < m.nt(); f++) {
+      int a = m.tri[3 * f], b = m.tri[3 * f + 1], c = m.tri[3 * f + 2];
+      int ab = middle(a, b), bc = middle(b, c), ca = middle(c, a);
+      const int q[12] = {a, ab, ca, b, bc, ab, c, ca, bc, ab, bc, ca};
+      next.insert(next.end(), q, q + 12);
+    }
+    m.tri.swap(next);
+  }
+  return m;
+}
+
 void host_checks() {
+  {
+    // a closed loop of sharp edges: the tree holds only the part with
+    // triangle 0 (the reference's mesher keeps one segment)
+    Mesh o = flat_octahedron(3, 0.2);
+    const int ne = 3 * o.nt() / 2;
+    std::vector<int32_t> order(ne), leaves(ne);
+    int n = 0, nl = 0;
+    EXPECT(bem_rwg_tree(o.xyz.data(), o.nv(), o.tri.data(), o.nt(), 32, 0.0, order.data(), &n, leaves.data(), &nl) ==
+           0);
+    EXPECT(n > 0 && n < ne && nl >= 1);
+    std::printf("host checks: flattened octahedron, tree of %d of %d RWGs in %d leaves\n", n, ne, nl);
+  }
   Mesh s = icosphere(2);
   const int ne = 3 * s.nt() / 2;
   std::vector<int32_t> order(ne), leaves(ne);

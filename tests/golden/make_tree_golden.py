@@ -23,6 +23,30 @@ from bem_amd import mesh  # noqa: E402
 
 OUT = os.path.dirname(os.path.abspath(__file__))
 
+
+def flat_octahedron(level, hz):
+    """Octahedron of half-height hz subdivided level times (midpoints, not
+    projected): for small hz its equator is a closed loop of edges sharper
+    than 2 rad, so the reference's mesher keeps only the half holding
+    triangle 0."""
+    xyz = [[1, 0, 0], [-1, 0, 0], [0, 1, 0], [0, -1, 0], [0, 0, hz], [0, 0, -hz]]
+    tri = [[0, 2, 4], [2, 1, 4], [1, 3, 4], [3, 0, 4], [2, 0, 5], [1, 2, 5], [3, 1, 5], [0, 3, 5]]
+    for _ in range(level):
+        mid, nxt = {}, []
+
+        def middle(a, b):
+            k = (min(a, b), max(a, b))
+            if k not in mid:
+                xyz.append([0.5 * (xyz[a][c] + xyz[b][c]) for c in range(3)])
+                mid[k] = len(xyz) - 1
+            return mid[k]
+        for a, b, c in tri:
+            ab, bc, ca = middle(a, b), middle(b, c), middle(c, a)
+            nxt += [[a, ab, ca], [b, bc, ab], [c, ca, bc], [ab, bc, ca]]
+        tri = nxt
+    return mesh.Mesh(np.array(xyz, dtype=np.float64), np.array(tri, dtype=np.int32), "flat_octahedron")
+
+
 # name: (body, translation)
 CASES = {
     # C2: 1280-triangle spheres, body 1 at centre gap 2.2
@@ -41,7 +65,11 @@ CASES = {
     "c4_cuboid": (lambda: mesh.cuboid(1.0, 1.0, 0.5, 12, 12, 6), (0.0, 0.0, 0.0)),
     "ico2": (lambda: mesh.icosphere(2), (0.0, 0.0, 0.0)),
     "cube2": (lambda: mesh.cube(2), (0.0, 0.0, 0.0)),
+    # a closed loop of sharp edges other than the cone's base rim
+    "flat_octahedron": (lambda: flat_octahedron(3, 0.2), (0.0, 0.0, 0.0)),
 }
+# bodies whose reference tree holds only part of the edges
+SHARP = {"tip_cone", "cone_default", "flat_octahedron"}
 
 
 def body(name):
