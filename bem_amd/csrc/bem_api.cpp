@@ -1019,12 +1019,15 @@ static int body_tree(bem_ctx *c, bem_body *b, const DevTree **out) {
   return 0;
 }
 
-// Algorithmic work per triangle pair (transcendentals counted as one flop):
-// a 6-point far pair is 6 outer x (6 inner x 166 + 342) + 280 = 8308 FP64
-// flops; a close pair 105 outer x (91 inner x 180 + 980) + neighbour edge
-// terms ~ 1.83e6.  Each pair writes one 36-double tile (288 bytes).
-static const double F_FAR_PAIR = 8308.0;
-static const double F_CLOSE_PAIR = 1.83e6;
+// Algorithmic work per triangle pair of the reference's assembly, counted
+// operation by operation on its restatement (additions, multiplications,
+// divisions, and sqrt / exp / log / atan2 / sincos as one each; the
+// classification included, the scatter into Z not): 8638 for a 6-point far
+// pair, and 1939966 for a close pair on average over the 1280-triangle
+// sphere's self block (1.89e6 self, 1.94e6 neighbours).  Each pair writes one
+// 36-double tile (288 bytes).
+static const double F_FAR_PAIR = 8638.0;
+static const double F_CLOSE_PAIR = 1939966.0;
 // close pairs whose inner sums share one pass of the moment workspace
 // (20 doubles per (pair, outer point): 69 MB at 4096 pairs)
 static const size_t CLOSE_CHUNK = 4096;

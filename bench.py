@@ -60,11 +60,14 @@ def elimination_flops(m1, m2):
     return 8.0 / 3.0 * (m1 ** 3 + m2 ** 3) + 16.0 * m1 * m1 * m2 + 20.0 * m1 * m2 * m2
 
 
-# flops per triangle pair by class {close, g6, g10, g15, g21, g36}: the close
-# pair's 105 x 91 smooth kernel + analytic terms, and points_out x
-# (points_in x 166 + 342) + 280 for the far rules
-PAIR_FLOPS = np.array([1.83e6, 8308.0, 10 * (10 * 166 + 342) + 280.0, 15 * (15 * 166 + 342) + 280.0,
-                       21 * (21 * 166 + 342) + 280.0, 36 * (36 * 166 + 342) + 280.0])
+# operations per triangle pair of the reference's assembly by class {close,
+# g6, g10, g15, g21, g36}, counted on its restatement operation by operation
+# (sqrt, exp, log, atan2 and sincos one each): the pair's classification and
+# integrals (the close figure is the mean over the 1280-triangle sphere's
+# self block) plus SCATTER_FLOPS for its scatter into Z
+SCATTER_FLOPS = 234.0
+PAIR_INTEGRAL_FLOPS = np.array([1939966.2, 8638.0, 21574.0, 45844.0, 86848.0, 246058.0])
+PAIR_FLOPS = PAIR_INTEGRAL_FLOPS + SCATTER_FLOPS
 
 
 def assembly_flops(ctx, b1, b2, gap, twins):
@@ -548,11 +551,12 @@ def main():
                            "achieved": rate(cg, 1e12), "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                            "frac": rate(cg, 1e12) / PEAK_FP32_TFLOPS, "launches": cg["launches"]},
         "assembly_roofline": {"bound": "fp64-valu", "kernel": "far_g6_kernel",
-                              "how": "8308 flops per 6-point far pair (transcendentals = 1 flop) / HIP-event durations",
+                              "how": ("8638 operations per 6-point far pair, counted on the reference algorithm "
+                                      "(sqrt/exp/log/atan2/sincos = 1 each) / HIP-event durations"),
                               "achieved": rate(far, 1e12), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                               "frac": rate(far, 1e12) / PEAK_FP64_TFLOPS, "launches": far["launches"],
                               "avg_launch_us": far["ms"] * 1e3 / max(far["launches"], 1),
-                              "close_kernel": {"kernel": "close_inner_kernel + pair_block_kernel<105,91,PRE> (chunks of 4096 pairs)", "flops_per_pair": 1.83e6,
+                              "close_kernel": {"kernel": "close_inner_kernel + pair_block_kernel<105,91,PRE> (chunks of 4096 pairs)", "flops_per_pair": float(PAIR_INTEGRAL_FLOPS[0]),
                                                "achieved": rate(close, 1e12),
                                                "frac": rate(close, 1e12) / PEAK_FP64_TFLOPS,
                                                "launches": close["launches"]},

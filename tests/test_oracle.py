@@ -385,3 +385,41 @@ def test_lowrank_composition_matches_dense():
     keep = int(np.sum(sv >= 1e-10 * sv[0]))
     S1 = O.heat_transfer_lowrank(A, D, Uu[:, :keep] * sv[:keep], Vh[:keep], G1, G2)
     assert keep < n1 and abs(S1 - S0) <= 1e-10 * abs(S0)
+
+
+def test_operation_counts(tmp_path):
+    """The per-pair operation counts the benchmark and the library's kernel
+    statistics use (oracle/flopcount.json, from the operation-counting build
+    of the oracle, oracle/flopcount/): that build computes the oracle's Zts
+    bit for bit (test row 0 of a 320-triangle sphere), its 6-point far pair
+    costs the same on that mesh as on the 1280-triangle sphere the committed
+    counts come from, and bench.py carries the committed figures."""
+    import json
+    import subprocess
+    import sys
+    from bem_amd import mesh
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = os.path.join(root, "oracle", "_ref", "flopcount")
+    if not os.path.exists(exe):
+        pytest.fail("the operation-counting build is missing (make -C oracle flopcount)")
+    counts = json.load(open(os.path.join(root, "oracle", "flopcount.json")))
+    m = mesh.icosphere(2)
+    text = f"{m.nv} {m.nt}\n" + "".join(f"{x!r} {y!r} {z!r}\n" for x, y, z in m.xyz.tolist()) + \
+        "".join(f"{a} {b} {c}\n" for a, b, c in m.tri.tolist())
+    zfile = str(tmp_path / "z.bin")
+    out = subprocess.run([exe, zfile], input=text, capture_output=True, text=True, check=True)
+    small = json.loads(out.stdout)
+    Z = O.Tree([m]).zts(3 + 0.4j, 1.0, 0.5, ta_range=(0, 1))
+    Zc = np.fromfile(zfile, dtype=np.complex128).reshape(Z.shape, order="F")
+    assert np.count_nonzero(Z) > 0 and np.array_equal(Z, Zc)
+    per = lambda d, k: d[k]["ops"] / d[k]["pairs"]
+    assert per(small, "far6") == per(counts, "far6") == 8638
+    assert per(small, "scatter") == per(counts, "scatter") == 234
+    # every kind of close pair costs within 3% of the mean the benchmark uses
+    for k in ("close_self", "close_edge", "close_vertex"):
+        assert abs(per(counts, k) / per(counts, "close") - 1) < 0.03
+    sys.path.insert(0, root)
+    import bench
+    want = [per(counts, k) for k in ("close", "far6", "far10", "far15", "far21", "far36")]
+    assert np.allclose(bench.PAIR_INTEGRAL_FLOPS, want, rtol=1e-9, atol=0.05)
+    assert bench.SCATTER_FLOPS == per(counts, "scatter")
