@@ -321,6 +321,18 @@ def pick_freqs(freqs, n, offset=0):
     return [float(freqs[i]) for i in idx]
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or None."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(wl, eV, budget_rows=None):
     """Oracle (C restatement, all host threads via OpenMP) timed on a bounded
     sample: assembly of a stripe of test-triangle rows of the three impedance
@@ -371,7 +383,8 @@ def cpu_baseline(wl, eV, budget_rows=None):
     finally:
         O.lib().oracle_set_threads(threads)
     faithful = t_top1 + t_int1 + t_solve
-    return dict(value=1.0 / total, unit="solves/s", cores=threads, kind="port",
+    return dict(value=1.0 / total, unit="solves/s", cores=threads, kind="port", cpu=cpu_model(),
+                host_cpus=os.cpu_count(),
                 sample=(f"one solve at {eV:.4f} eV: assembly timed on {rows}/{T} exterior and {r1}/{T1} interior "
                         f"test-triangle rows (scaled linearly), dense inverse+products+trace at full M={M}"),
                 seconds=dict(assembly_ext=t_top, assembly_int=t_int, dense=t_solve),
