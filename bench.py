@@ -308,6 +308,61 @@ def run_serving(args):
 TUNE_CACHE = os.path.join(ROOT, "bem_amd", "tuning", "gemm_tiles.json")
 
 
+def run_gemm_shapes(args):
+    """The FP64 GEMM (bem_zgemm through the C ABI, with the autotuner's tile
+    table) alone at the shapes one C2 solve (m = 3840 per body) issues: the
+    two K = m products (C Y and sym(G1) X), the LU's rank-256 trailing updates
+    as the factorisation shrinks, the substitutions' rank-256 updates (r =
+    3840 right-hand sides) and their rank-64 inner updates.  Each rate is the
+    algorithmic 8mnk flops over the launch's HIP-event duration (the mean of
+    the second half of 12 or 24 launches: the clocks ramp up over the first).  `tiles` counts the C tiles of the launch and `waves`
+    how many rounds of workgroup slots they take (256 CUs with two 512-thread
+    workgroups each for the 64 x 64 tile): tiles / (waves x slots) bounds the
+    rate a launch can reach when nothing else runs beside it."""
+    import bem_amd
+    ctx = bem_amd.Context(0, lanes=1)
+    tiles = tune(ctx, args)
+    m = 3840
+    rng = np.random.default_rng(0)
+
+    def dev_random():
+        a = (rng.standard_normal(m * m) + 1j * rng.standard_normal(m * m)).astype(np.complex128)
+        return ctx._dev(a)
+    dA, dB, dC = dev_random(), dev_random(), dev_random()
+    al = np.array([-1.0, 0.0])
+    be = np.array([1.0, 0.0])
+    shapes = [(3840, 3840, 3840), (3584, 3584, 256), (2816, 2816, 256), (1792, 1792, 256), (1024, 1024, 256),
+              (512, 512, 256), (3584, 3840, 256), (1792, 3840, 256), (512, 3840, 256), (192, 3840, 64)]
+    slots = 512
+    rows = []
+    try:
+        for (mm, nn, kk) in shapes:
+            reps = 12 if kk == m else 24
+            ms = []
+            for _ in range(reps):
+                ctx._check(ctx.lib.bem_zgemm(ctx.h, mm, nn, kk, al.ctypes.data, dA, mm, dB, kk, be.ctypes.data, dC,
+                                             mm))
+                ms.append(ctx.kernel_stats()["zgemm"]["ms"])
+            t = float(np.mean(ms[reps // 2:]))   # the clocks ramp up over the first launches
+            nt = -(-mm // 64) * -(-nn // 64)
+            waves = -(-nt // slots)
+            tf = 8.0 * mm * nn * kk / (t * 1e-3) / 1e12
+            rows.append(dict(m=mm, n=nn, k=kk, us=round(t * 1e3, 1), tflops=round(tf, 2),
+                             frac=round(tf / PEAK_FP64_TFLOPS, 3), tiles=nt, waves=waves,
+                             quantization=round(nt / (waves * slots), 3)))
+            print(f"# zgemm {mm} x {nn} x {kk}: {t * 1e3:8.1f} us {tf:6.2f} TF/s "
+                  f"({tf / PEAK_FP64_TFLOPS:.3f} of peak), {nt} tiles in {waves} round(s) of {slots}",
+                  file=sys.stderr, flush=True)
+    finally:
+        for p in (dA, dB, dC):
+            ctx.lib.bem_device_free(ctx.h, p)
+        ctx.close()
+    return {"metric": "zgemm TF/s by shape", "unit": "TFLOP/s", "peak": PEAK_FP64_TFLOPS,
+            "kernel": zgemm_kernel_name(tiles), "gemm_tiles": tiles, "slots": slots,
+            "how": "8mnk / HIP-event duration of bem_zgemm's launch, mean of the second half of the launches",
+            "rows": rows}
+
+
 def tune(ctx, args):
     """GEMM tile table from the autotuner (bem_autotune) per --tune."""
     if args.tune == "off":
@@ -429,6 +484,8 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy three times: the reference kernel that scales "
                          "the FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--gemm-shapes", action="store_true",
+                    help="only time the FP64 GEMM alone at the shapes one C2 solve issues")
     args = ap.parse_args()
     if args.steps is None:
         args.steps = {"c2": 64, "c3": 8}.get(args.workload, 16)
@@ -450,6 +507,9 @@ def main():
         print(json.dumps({"copy_bytes_each_way": n * 8, "copies": 3}))
         return
 
+    if args.gemm_shapes:
+        print(json.dumps(run_gemm_shapes(args)))
+        return
     if args.workload == "c5":
         print(json.dumps(run_serving(args)))
         return
