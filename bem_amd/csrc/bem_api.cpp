@@ -37,6 +37,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <list>
+#include <map>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -216,10 +217,29 @@ struct SelfEntry {
   unsigned long long *dgrowth = nullptr;
   SelfStatus *h = nullptr;   // pinned
   hipEvent_t ready = nullptr;
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;   // device blocks and their sizes
   size_t bytes = 0;
   int pins = 0;
   uint64_t stamp = 0;
+};
+
+// Device arena of the self-block cache in a context without an HBM budget.
+// Entries come and go with every frequency of a sweep, and hipFree
+// synchronises the whole device: once the cache reached its cap, each
+// eviction made the next request wait for every lane to drain (a 4-lane sweep
+// of 1280-triangle sphere pairs fell from 24 to 14.5 solves/s).  The arena
+// takes device memory in segments of at least ARENA_SEG bytes and keeps it
+// until the context ends or a body's release leaves a segment empty; blocks
+// are carved first fit from per-segment free
+// lists (ARENA_ALIGN granules, neighbours coalesced on release), so an
+// eviction is host bookkeeping only.  A context with a budget allocates and
+// frees each block instead, so that its footprint is exactly what it holds.
+static const size_t ARENA_SEG = (size_t)2 << 30;
+static const size_t ARENA_ALIGN = 256;
+struct ArenaSeg {
+  char *base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free;   // offset -> length, disjoint, coalesced
 };
 
 // Error probe of a solve, written on the device (see bem_hip.h, Error probe):
@@ -309,6 +329,10 @@ struct bem_ctx {
   std::list<PairGeom> pairs;                    // most recently used first
   std::list<SelfEntry> selfs;
   size_t self_bytes = 0, self_cap = 0;
+  std::vector<ArenaSeg> arena;              // self-entry memory without a budget
+  size_t arena_bytes = 0;
+  std::vector<SelfStatus *> spare_status;   // pinned slots and events of evicted entries
+  std::vector<hipEvent_t> spare_events;
   uint64_t clock = 0;
   bool profile = true;   // bracket timed kernel classes with events
   bool lookahead = false;   // factor the next LU diagonal block on the lane's aux stream
@@ -1105,25 +1129,127 @@ static size_t self_entry_bytes(int n, int role, int prec) {
   return b;
 }
 
+static size_t arena_round(size_t bytes) { return (std::max<size_t>(bytes, 1) + ARENA_ALIGN - 1) / ARENA_ALIGN * ARENA_ALIGN; }
+
+// Best fit over the segments' free lists (the smallest free block that
+// holds the request, so small blocks fill leftovers and large holes stay
+// whole); nullptr when no block is large enough.
+static void *arena_take(bem_ctx *c, size_t bytes) {
+  bytes = arena_round(bytes);
+  ArenaSeg *best = nullptr;
+  std::map<size_t, size_t>::iterator bit;
+  for (auto &sg : c->arena)
+    for (auto it = sg.free.begin(); it != sg.free.end(); ++it)
+      if (it->second >= bytes && (!best || it->second < bit->second)) {
+        best = &sg;
+        bit = it;
+      }
+  if (!best) return nullptr;
+  const size_t off = bit->first, len = bit->second;
+  best->free.erase(bit);
+  if (len > bytes) best->free.emplace(off + bytes, len - bytes);
+  return best->base + off;
+}
+
+// Returns a block to its segment, merged with free neighbours.
+static void arena_give(bem_ctx *c, void *p, size_t bytes) {
+  bytes = arena_round(bytes);
+  char *q = (char *)p;
+  for (auto &sg : c->arena) {
+    if (q < sg.base || q >= sg.base + sg.size) continue;
+    size_t off = (size_t)(q - sg.base);
+    auto nx = sg.free.lower_bound(off);
+    if (nx != sg.free.end() && nx->first == off + bytes) {
+      bytes += nx->second;
+      nx = sg.free.erase(nx);
+    }
+    if (nx != sg.free.begin()) {
+      auto pv = std::prev(nx);
+      if (pv->first + pv->second == off) {
+        pv->second += bytes;
+        return;
+      }
+    }
+    sg.free.emplace(off, bytes);
+    return;
+  }
+}
+
+// Segments with no block in use go back to the runtime.  Called when a body
+// is freed (which already waits for the lanes and frees the body's device
+// memory), so that blocks of bodies that are gone do not stay held.
+static void arena_release_empty(bem_ctx *c) {
+  for (auto it = c->arena.begin(); it != c->arena.end();) {
+    const bool empty = it->free.size() == 1 && it->free.begin()->first == 0 && it->free.begin()->second == it->size;
+    if (empty) {
+      (void)hipFree(it->base);
+      c->bytes -= it->size;
+      c->arena_bytes -= it->size;
+      it = c->arena.erase(it);
+    } else {
+      ++it;
+    }
+  }
+}
+
+// A new segment that holds at least `bytes` (ARENA_SEG, or exactly the
+// request when the device cannot give a whole segment).
+static int arena_grow(bem_ctx *c, size_t bytes) {
+  size_t sz = std::max(ARENA_SEG, arena_round(bytes));
+  void *p = nullptr;
+  hipError_t r = hipMalloc(&p, sz);
+  if (r != hipSuccess && sz > arena_round(bytes)) {
+    (void)hipGetLastError();
+    sz = arena_round(bytes);
+    r = hipMalloc(&p, sz);
+  }
+  if (r != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for the self-block arena: %s", sz, hipGetErrorString(r));
+  ArenaSeg sg;
+  sg.base = (char *)p;
+  sg.size = sz;
+  sg.free.emplace(0, sz);
+  c->arena.push_back(std::move(sg));
+  c->arena_bytes += sz;
+  c->bytes += sz;
+  return 0;
+}
+
+// Releases an entry's memory: to the arena without a budget, to the runtime
+// with one.  Its pinned status slot and event are kept for the next entry
+// (pinned-memory frees synchronise the device like hipFree).  Only unpinned
+// entries come here, and a lane unpins its entries after its stream has
+// drained, so no kernel still reads the blocks.
 static void self_free(bem_ctx *c, SelfEntry &e) {
   if (e.ready) (void)hipEventSynchronize(e.ready);
-  for (void *p : e.allocs) (void)hipFree(p);
+  for (auto &a : e.allocs) {
+    if (c->budget) {
+      (void)hipFree(a.first);
+      c->bytes -= a.second;
+    } else {
+      arena_give(c, a.first, a.second);
+    }
+  }
   e.allocs.clear();
-  if (e.h) (void)hipHostFree(e.h);
+  if (e.h) c->spare_status.push_back(e.h);
   e.h = nullptr;
-  if (e.ready) (void)hipEventDestroy(e.ready);
+  if (e.ready) c->spare_events.push_back(e.ready);
   e.ready = nullptr;
-  c->bytes -= e.bytes;
   c->self_bytes -= e.bytes;
   e.bytes = 0;
+}
+
+// The least recently used unpinned entry, or end().
+static std::list<SelfEntry>::iterator self_lru(bem_ctx *c) {
+  auto victim = c->selfs.end();
+  for (auto it = c->selfs.begin(); it != c->selfs.end(); ++it)
+    if (it->pins == 0 && (victim == c->selfs.end() || it->stamp < victim->stamp)) victim = it;
+  return victim;
 }
 
 // Drop least recently used unpinned entries until `need` more bytes fit.
 static void self_evict(bem_ctx *c, size_t need) {
   while (c->self_bytes + need > c->self_cap) {
-    auto victim = c->selfs.end();
-    for (auto it = c->selfs.begin(); it != c->selfs.end(); ++it)
-      if (it->pins == 0 && (victim == c->selfs.end() || it->stamp < victim->stamp)) victim = it;
+    auto victim = self_lru(c);
     if (victim == c->selfs.end()) return;   // everything in use: the cache runs over its cap
     self_free(c, *victim);
     c->selfs.erase(victim);
@@ -1131,13 +1257,34 @@ static void self_evict(bem_ctx *c, size_t need) {
 }
 
 static int self_alloc(bem_ctx *c, SelfEntry &e, void **p, size_t bytes) {
-  if (c->budget && c->bytes + bytes > c->budget)
-    return fail(c, BEM_E_NOMEM, "self-block cache entry (%zu bytes) exceeds the HBM budget", bytes);
-  hipError_t r = hipMalloc(p, bytes ? bytes : 1);
-  if (r != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for a self block: %s", bytes, hipGetErrorString(r));
-  e.allocs.push_back(*p);
+  if (c->budget) {
+    if (c->bytes + bytes > c->budget)
+      return fail(c, BEM_E_NOMEM, "self-block cache entry (%zu bytes) exceeds the HBM budget", bytes);
+    hipError_t r = hipMalloc(p, bytes ? bytes : 1);
+    if (r != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for a self block: %s", bytes, hipGetErrorString(r));
+    c->bytes += bytes;
+  } else {
+    // a free block, else a new segment while the arena is under the cache's
+    // cap, else room made by evicting (every entry pinned: a segment beyond
+    // the cap)
+    *p = arena_take(c, bytes);
+    while (!*p) {
+      if (c->arena_bytes + std::max(ARENA_SEG, arena_round(bytes)) <= c->self_cap) {
+        CHK(arena_grow(c, bytes));
+      } else {
+        auto victim = self_lru(c);
+        if (victim == c->selfs.end()) {
+          CHK(arena_grow(c, bytes));
+        } else {
+          self_free(c, *victim);
+          c->selfs.erase(victim);
+        }
+      }
+      *p = arena_take(c, bytes);
+    }
+  }
+  e.allocs.emplace_back(*p, bytes);
   e.bytes += bytes;
-  c->bytes += bytes;
   c->self_bytes += bytes;
   return 0;
 }
@@ -1199,10 +1346,18 @@ static int self_get(bem_ctx *c, Lane &L, bem_body *b, const double *eps, const d
   if (!r && mixed) r = self_alloc(c, e, (void **)&e.Z32, nn / 2);
   if (!r && mixed) r = self_alloc(c, e, (void **)&e.Gs32, nn / 2);
   if (!r && mixed && role == 0) r = self_alloc(c, e, (void **)&e.inv32, inv_bytes<float2>(n));
-  if (!r && hipHostMalloc((void **)&e.h, sizeof(SelfStatus), hipHostMallocDefault) != hipSuccess)
+  if (!r && !c->spare_status.empty()) {
+    e.h = c->spare_status.back();
+    c->spare_status.pop_back();
+  } else if (!r && hipHostMalloc((void **)&e.h, sizeof(SelfStatus), hipHostMallocDefault) != hipSuccess) {
     r = fail(c, BEM_E_NOMEM, "pinned status slot");
-  if (!r && hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess)
+  }
+  if (!r && !c->spare_events.empty()) {
+    e.ready = c->spare_events.back();
+    c->spare_events.pop_back();
+  } else if (!r && hipEventCreateWithFlags(&e.ready, hipEventDisableTiming) != hipSuccess) {
     r = fail(c, BEM_E_DEVICE, "event creation");
+  }
   if (r) return undo(r);
   e.dgrowth = (unsigned long long *)(e.dinfo + 4);
   // Z_ext(b, b) (vacuum) then + Z_int(b), whose copy is G
@@ -2947,6 +3102,9 @@ void bem_ctx_destroy(bem_ctx *c) {
   for (auto &g : c->pairs)
     for (void *p : g.allocs) (void)hipFree(p);
   for (auto &e : c->selfs) self_free(c, e);
+  for (auto &sg : c->arena) (void)hipFree(sg.base);
+  for (SelfStatus *h : c->spare_status) (void)hipHostFree(h);
+  for (hipEvent_t ev : c->spare_events) (void)hipEventDestroy(ev);
   for (bem_body *b : c->bodies) {
     for (void *p : b->allocs) (void)hipFree(p);
     b->allocs.clear();
@@ -3064,6 +3222,7 @@ void bem_body_free(bem_body *b) {
       }
     }
     self_drop_body(c, b);
+    arena_release_empty(c);
     if (!b->allocs.empty()) free_allocs(c, b->allocs, b->bytes);
     c->bodies.erase(b);
     std::vector<uint64_t> dropped;

@@ -63,7 +63,12 @@ typedef struct {
   size_t hbm_budget; /* bytes a context may hold, 0 = no limit */
   int lanes;         /* solves kept in flight, one stream each (0 = one per hardware queue,
                         GPU_MAX_HW_QUEUES, at most 4; max 8) */
-  size_t cache_bytes;/* device bytes for cached self blocks (0 = 40% of the device) */
+  size_t cache_bytes;/* device bytes for cached self blocks (0 = 40% of the device).  Without
+                        an hbm_budget the cache keeps its memory in segments of 2 GiB or more
+                        and reuses it for new blocks (no hipFree, which would synchronise
+                        the device, at an eviction); segments left empty when a body is
+                        freed go back to the runtime.  With a budget each block is
+                        allocated and freed on its own */
   int lookahead;     /* LU look-ahead on a second stream per lane: 1 on, -1 off, 0 on when the
                         2 x lanes streams fit the hardware queues (GPU_MAX_HW_QUEUES, default 4) */
 } bem_cfg;

@@ -197,7 +197,9 @@ def test_self_block_cache_across_gaps(gpu_ctx):
 def test_body_free_releases_device_memory(gpu_ctx):
     """Geometry and cached self blocks of a body go when the body is freed:
     two identical upload-solve-free rounds end at the same device footprint
-    (lane workspaces are grow-only and sized by the first round)."""
+    (lane workspaces are grow-only and sized by the first round; the freed
+    self blocks return to the cache's arena, whose segment the second round
+    reuses)."""
     from bem_amd import mesh, Request, lorentz_eps
     s = mesh.icosphere(1)
     held = []

@@ -78,7 +78,9 @@ def test_budget_admission_serialises_and_rejects():
     from bem_amd import mesh, Request, lorentz_eps, BEM_E_NOMEM
     s = mesh.icosphere(2)   # 320 triangles, m = 960 per body
     reqs = [Request(eV=f, gap=(2.4, 0, 0), eps1=lorentz_eps(f), eps2=lorentz_eps(f)) for f in (0.09, 0.1, 0.11, 0.12)]
-    with bem_amd.Context(0, lanes=1, lookahead=-1) as ref_ctx:
+    # any budget makes the context allocate each block exactly (no arena
+    # segments), so its device bytes are the requests' own footprint
+    with bem_amd.Context(0, lanes=1, lookahead=-1, hbm_budget=1 << 50) as ref_ctx:
         b1, b2 = ref_ctx.body(s), ref_ctx.body(s)
         alone = [ref_ctx.calc_heat_transfer(b1, b2, q) for q in reqs]
         one = ref_ctx.device_bytes()
@@ -128,3 +130,32 @@ def test_poll_pending_then_done():
             r = ctx.poll(t, wait_ms=50)
         assert r is not None and r["status"] == 0
         assert first is None or first["status"] == 0
+
+
+def test_cache_arena_reuses_evicted_memory():
+    """A sweep over more frequencies than the self-block cache holds, in a
+    context without an HBM budget: evicted blocks return to the cache's arena
+    and new ones are carved from it, so the device footprint stops growing
+    once the first requests have sized the lanes and the arena, and every
+    result equals the same request solved on a fresh context bit for bit
+    (blocks carved from reused memory hold exactly what fresh ones would)."""
+    import bem_amd
+    from bem_amd import mesh, Request, lorentz_eps
+    s = mesh.icosphere(2)   # m = 960: about 76 MB of self entries per frequency
+    freqs = np.linspace(0.09, 0.13, 24)
+    reqs = [Request(eV=float(f), gap=(2.4, 0, 0), eps1=lorentz_eps(float(f)), eps2=lorentz_eps(float(f)))
+            for f in freqs]
+    got, held = [], []
+    with bem_amd.Context(0, lanes=2, cache_bytes=200 << 20, lookahead=-1) as ctx:
+        b1, b2 = ctx.body(s), ctx.body(s)
+        for k in range(3):
+            got += ctx.sweep(b1, b2, reqs[8 * k:8 * k + 8])
+            held.append(ctx.device_bytes())
+            say(f"after {8 * (k + 1)} requests: {held[-1] / 2**20:.0f} MiB held")
+    assert held[1] == held[0] and held[2] == held[0]
+    with bem_amd.Context(0, lanes=1, cache_bytes=1, lookahead=-1) as alone_ctx:
+        a1, a2 = alone_ctx.body(s), alone_ctx.body(s)
+        for q, g in zip(reqs, got):
+            a = alone_ctx.calc_heat_transfer(a1, a2, q)
+            assert g["status"] == 0 and a["status"] == 0
+            assert g["ReS"] == a["ReS"], (q.eV, g["ReS"], a["ReS"])
