@@ -182,7 +182,7 @@ struct PairGeom {
   double gap[3] = {0, 0, 0};
   DevTree top;
   int T1 = 0, E1 = 0;   // triangles / RWGs of body 1 (the first ones of top)
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;
   size_t bytes = 0;
   int pins = 0;         // lanes whose solve in flight reads this tree
   int lr_blocks = 0;    // range-finder blocks the last low-rank solve on this pair needed (issue hint)
@@ -362,7 +362,7 @@ struct bem_body {
   uint64_t fp = 0;              // FNV-1a of the vertex and triangle arrays
   DevTree dt;                   // the body alone, untranslated (self blocks)
   bool up = false;
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;
   size_t bytes = 0;
 };
 
@@ -955,19 +955,22 @@ static Medium make_medium(double er, double ei, double mr, double mi, double ome
 }
 
 // Persistent upload of a tree (owned by a PairGeom, or by a one-off holder).
-static int upload_tree(bem_ctx *c, const bemgeo::Tree &t, DevTree &d, std::vector<void *> &allocs, size_t &bytes) {
+static int dev_take(bem_ctx *c, size_t bytes, void **p, const char *what);
+static void dev_give(bem_ctx *c, void *p, size_t bytes);
+
+static int upload_tree(bem_ctx *c, const bemgeo::Tree &t, DevTree &d, std::vector<std::pair<void *, size_t>> &allocs,
+                       size_t &bytes) {
   std::vector<int2> cl(t.close.size());
   for (size_t i = 0; i < cl.size(); i++) cl[i] = make_int2(t.close[i].first, t.close[i].second);
   size_t b1 = sizeof(TriGeom) * t.tg.size(), b2 = sizeof(RwgRec) * t.rw.size(), b3 = sizeof(int2) * (cl.size() + 1);
   void *p1, *p2, *p3;
-  DEV(c, hipMalloc(&p1, b1));
-  allocs.push_back(p1);
-  DEV(c, hipMalloc(&p2, b2));
-  allocs.push_back(p2);
-  DEV(c, hipMalloc(&p3, b3));
-  allocs.push_back(p3);
+  CHK(dev_take(c, b1, &p1, "triangle geometry"));
+  allocs.emplace_back(p1, b1);
+  CHK(dev_take(c, b2, &p2, "RWG records"));
+  allocs.emplace_back(p2, b2);
+  CHK(dev_take(c, b3, &p3, "close-pair list"));
+  allocs.emplace_back(p3, b3);
   bytes += b1 + b2 + b3;
-  c->bytes += b1 + b2 + b3;
   DEV(c, hipMemcpy(p1, t.tg.data(), b1, hipMemcpyHostToDevice));
   DEV(c, hipMemcpy(p2, t.rw.data(), b2, hipMemcpyHostToDevice));
   if (!cl.empty()) DEV(c, hipMemcpy(p3, cl.data(), sizeof(int2) * cl.size(), hipMemcpyHostToDevice));
@@ -980,11 +983,14 @@ static int upload_tree(bem_ctx *c, const bemgeo::Tree &t, DevTree &d, std::vecto
   return 0;
 }
 
-static void free_allocs(bem_ctx *c, std::vector<void *> &allocs, size_t bytes) {
-  (void)hipDeviceSynchronize();
-  for (void *p : allocs) (void)hipFree(p);
+// Releases geometry blocks nothing in flight reads (an unpinned pair tree,
+// or a body whose lanes have been retired): back to the arena without a
+// budget, hipFree'd (after the device drains) with one.
+static void free_allocs(bem_ctx *c, std::vector<std::pair<void *, size_t>> &allocs, size_t &bytes) {
+  if (c->budget && !allocs.empty()) (void)hipDeviceSynchronize();
+  for (auto &a : allocs) dev_give(c, a.first, a.second);
   allocs.clear();
-  c->bytes -= bytes;
+  bytes = 0;
 }
 
 static const size_t PAIR_CACHE = 16;   // more than the lanes (at most 8), so one is always unpinned
@@ -1221,14 +1227,7 @@ static int arena_grow(bem_ctx *c, size_t bytes) {
 // drained, so no kernel still reads the blocks.
 static void self_free(bem_ctx *c, SelfEntry &e) {
   if (e.ready) (void)hipEventSynchronize(e.ready);
-  for (auto &a : e.allocs) {
-    if (c->budget) {
-      (void)hipFree(a.first);
-      c->bytes -= a.second;
-    } else {
-      arena_give(c, a.first, a.second);
-    }
-  }
+  for (auto &a : e.allocs) dev_give(c, a.first, a.second);
   e.allocs.clear();
   if (e.h) c->spare_status.push_back(e.h);
   e.h = nullptr;
@@ -1256,33 +1255,49 @@ static void self_evict(bem_ctx *c, size_t need) {
   }
 }
 
-static int self_alloc(bem_ctx *c, SelfEntry &e, void **p, size_t bytes) {
+// Device memory for geometry and self entries.  Without a budget: a free
+// block of the arena, else a new segment while the arena is under the
+// cache's cap, else room made by evicting cached entries (every entry
+// pinned: a segment beyond the cap).  With a budget: hipMalloc, counted
+// against it.
+static int dev_take(bem_ctx *c, size_t bytes, void **p, const char *what) {
   if (c->budget) {
     if (c->bytes + bytes > c->budget)
-      return fail(c, BEM_E_NOMEM, "self-block cache entry (%zu bytes) exceeds the HBM budget", bytes);
+      return fail(c, BEM_E_NOMEM, "%s (%zu bytes) exceeds the HBM budget", what, bytes);
     hipError_t r = hipMalloc(p, bytes ? bytes : 1);
-    if (r != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for a self block: %s", bytes, hipGetErrorString(r));
+    if (r != hipSuccess) return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for %s: %s", bytes, what, hipGetErrorString(r));
     c->bytes += bytes;
-  } else {
-    // a free block, else a new segment while the arena is under the cache's
-    // cap, else room made by evicting (every entry pinned: a segment beyond
-    // the cap)
-    *p = arena_take(c, bytes);
-    while (!*p) {
-      if (c->arena_bytes + std::max(ARENA_SEG, arena_round(bytes)) <= c->self_cap) {
+    return 0;
+  }
+  *p = arena_take(c, bytes);
+  while (!*p) {
+    if (c->arena_bytes + std::max(ARENA_SEG, arena_round(bytes)) <= c->self_cap) {
+      CHK(arena_grow(c, bytes));
+    } else {
+      auto victim = self_lru(c);
+      if (victim == c->selfs.end()) {
         CHK(arena_grow(c, bytes));
       } else {
-        auto victim = self_lru(c);
-        if (victim == c->selfs.end()) {
-          CHK(arena_grow(c, bytes));
-        } else {
-          self_free(c, *victim);
-          c->selfs.erase(victim);
-        }
+        self_free(c, *victim);
+        c->selfs.erase(victim);
       }
-      *p = arena_take(c, bytes);
     }
+    *p = arena_take(c, bytes);
   }
+  return 0;
+}
+
+static void dev_give(bem_ctx *c, void *p, size_t bytes) {
+  if (c->budget) {
+    (void)hipFree(p);
+    c->bytes -= bytes;
+  } else {
+    arena_give(c, p, bytes);
+  }
+}
+
+static int self_alloc(bem_ctx *c, SelfEntry &e, void **p, size_t bytes) {
+  CHK(dev_take(c, bytes, p, "self-block cache entry"));
   e.allocs.emplace_back(*p, bytes);
   e.bytes += bytes;
   c->self_bytes += bytes;
@@ -3099,18 +3114,16 @@ void bem_ctx_destroy(bem_ctx *c) {
   for (auto &kv : c->ws)
     if (kv.second.p) (void)hipFree(kv.second.p);
   for (auto &kv : c->user_allocs) (void)hipFree(kv.first);
-  for (auto &g : c->pairs)
-    for (void *p : g.allocs) (void)hipFree(p);
+  for (auto &g : c->pairs) free_allocs(c, g.allocs, g.bytes);
   for (auto &e : c->selfs) self_free(c, e);
-  for (auto &sg : c->arena) (void)hipFree(sg.base);
   for (SelfStatus *h : c->spare_status) (void)hipHostFree(h);
   for (hipEvent_t ev : c->spare_events) (void)hipEventDestroy(ev);
   for (bem_body *b : c->bodies) {
-    for (void *p : b->allocs) (void)hipFree(p);
-    b->allocs.clear();
+    free_allocs(c, b->allocs, b->bytes);
     b->up = false;
     b->ctx = nullptr;   // bem_body_free then only releases the host object
   }
+  for (auto &sg : c->arena) (void)hipFree(sg.base);
   if (c->h_noalloc) (void)hipHostFree(c->h_noalloc);
   for (auto &L : c->lanes) {
     for (auto &r : L.ev) {
@@ -3222,8 +3235,8 @@ void bem_body_free(bem_body *b) {
       }
     }
     self_drop_body(c, b);
-    arena_release_empty(c);
     if (!b->allocs.empty()) free_allocs(c, b->allocs, b->bytes);
+    arena_release_empty(c);
     c->bodies.erase(b);
     std::vector<uint64_t> dropped;
     {
@@ -3286,7 +3299,7 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   int M = 2 * t.E;
   if (ldz < M) return fail(c, BEM_E_INVALID, "ldz %lld < %d", (long long)ldz, M);
   DevTree d;
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;
   size_t tb = 0;
   int e = upload_tree(c, t, d, allocs, tb);
   Medium md = make_medium(eps[0], eps[1], mu[0], mu[1], omega);
@@ -3313,7 +3326,9 @@ int bem_assemble(bem_ctx *c, int nb, bem_body *const *bodies, const double *shif
   int hflag[4] = {0, 0, 0, 0};
   if (!e && hipMemcpyAsync(hflag, flag, sizeof hflag, hipMemcpyDeviceToHost, L.st) != hipSuccess)
     e = fail(c, BEM_E_DEVICE, "flag copy");
-  if (!e && hipStreamSynchronize(L.st) != hipSuccess) e = fail(c, BEM_E_DEVICE, "assembly failed on the device");
+  // drained before the tree's blocks go back, also after a failed launch
+  const hipError_t se = hipStreamSynchronize(L.st);
+  if (!e && se != hipSuccess) e = fail(c, BEM_E_DEVICE, "assembly failed on the device");
   if (!e && hflag[1]) e = fail(c, BEM_E_DEVICE, "36-point pair queue overflowed");
   free_allocs(c, allocs, tb);
   return e;
@@ -3337,7 +3352,7 @@ int bem_dipole_rhs(bem_ctx *c, int nb, bem_body *const *bodies, const double *sh
   if (ns == 0) return 0;
   t.close.clear();
   DevTree d;
-  std::vector<void *> allocs;
+  std::vector<std::pair<void *, size_t>> allocs;
   size_t tb = 0;
   int e = upload_tree(c, t, d, allocs, tb);
   std::vector<int> toff(t.tri_off);
@@ -3350,7 +3365,9 @@ int bem_dipole_rhs(bem_ctx *c, int nb, bem_body *const *bodies, const double *sh
   if (!e && !dev && hipMemcpy2DAsync(B, sizeof(double2) * ldb, Bd, sizeof(double2) * M, sizeof(double2) * M, ns,
                                      hipMemcpyDeviceToHost, L.st) != hipSuccess)
     e = fail(c, BEM_E_DEVICE, "copy of B failed");
-  if (!e && hipStreamSynchronize(L.st) != hipSuccess) e = fail(c, BEM_E_DEVICE, "dipole sources failed on the device");
+  // drained before the tree's blocks go back, also after a failed launch
+  const hipError_t se = hipStreamSynchronize(L.st);
+  if (!e && se != hipSuccess) e = fail(c, BEM_E_DEVICE, "dipole sources failed on the device");
   free_allocs(c, allocs, tb);
   return e;
 }
