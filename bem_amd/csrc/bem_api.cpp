@@ -396,6 +396,11 @@ static int fail(bem_ctx *c, int code, const char *fmt, ...) {
 static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **out) {
   DevBuf &b = c->ws[L.tag + name];
   if (b.cap < bytes) {
+    // a workspace that has to grow again takes half as much more without a
+    // budget: requests served shortest first grow in size, and each regrowth
+    // frees the old buffer (hipFree synchronises the device)
+    const size_t want = bytes;
+    if (b.p && !c->budget) bytes = std::max(bytes, b.cap + b.cap / 2);
     if (b.p) {
       DEV(c, hipStreamSynchronize(L.st));
       DEV(c, hipFree(b.p));
@@ -407,6 +412,11 @@ static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **ou
     if (c->budget && c->bytes + bytes > c->budget)
       return fail(c, BEM_E_NOMEM, "workspace '%s%s' (%zu bytes) exceeds the HBM budget", L.tag.c_str(), name, bytes);
     hipError_t e = hipMalloc(&b.p, bytes);
+    if (e != hipSuccess && bytes > want) {   // no room for the headroom: exactly what is asked
+      (void)hipGetLastError();
+      bytes = want;
+      e = hipMalloc(&b.p, bytes);
+    }
     if (e != hipSuccess)
       return fail(c, BEM_E_NOMEM, "hipMalloc(%zu) for '%s%s': %s", bytes, L.tag.c_str(), name, hipGetErrorString(e));
     b.cap = bytes;
