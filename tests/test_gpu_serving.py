@@ -159,3 +159,28 @@ def test_cache_arena_reuses_evicted_memory():
             a = alone_ctx.calc_heat_transfer(a1, a2, q)
             assert g["status"] == 0 and a["status"] == 0
             assert g["ReS"] == a["ReS"], (q.eV, g["ReS"], a["ReS"])
+
+
+def test_cache_arena_mixed_sizes():
+    """Self entries of three sizes through a 40 MB cache, below one 2 GiB
+    arena segment, on two lanes: the first block grows a segment past the
+    cap, the cache evicts entries of one size, and the arena carves blocks of
+    other sizes from their freed memory.  Each served result equals the
+    request solved alone on a fresh context, bit for bit."""
+    import bem_amd
+    from bem_amd import mesh, Request, lorentz_eps
+    bodies = [(mesh.icosphere(1), (2.4, 0, 0)), (mesh.icosphere(2), (2.4, 0, 0)), (mesh.cube(3), (1.3, 0, 0))]
+    reqs = []
+    for i, f in enumerate(np.linspace(0.09, 0.13, 12)):
+        k = i % 3
+        reqs.append((k, Request(eV=float(f), gap=bodies[k][1], eps1=lorentz_eps(float(f)), eps2=lorentz_eps(float(f)))))
+    with bem_amd.Context(0, lanes=2, cache_bytes=40 << 20, lookahead=-1) as ctx:
+        hs = [(ctx.body(m), ctx.body(m)) for m, _ in bodies]
+        ts = [ctx.submit(hs[k][0], hs[k][1], q) for k, q in reqs]
+        got = [ctx.poll(t, wait_ms=-1) for t in ts]
+    with bem_amd.Context(0, lanes=1, cache_bytes=1, lookahead=-1) as alone_ctx:
+        ha = [(alone_ctx.body(m), alone_ctx.body(m)) for m, _ in bodies]
+        for (k, q), g in zip(reqs, got):
+            a = alone_ctx.calc_heat_transfer(ha[k][0], ha[k][1], q)
+            assert g["status"] == 0 and a["status"] == 0
+            assert g["ReS"] == a["ReS"], (k, q.eV, g["ReS"], a["ReS"])
