@@ -396,11 +396,12 @@ static int fail(bem_ctx *c, int code, const char *fmt, ...) {
 static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **out) {
   DevBuf &b = c->ws[L.tag + name];
   if (b.cap < bytes) {
-    // a workspace that has to grow again takes half as much more without a
-    // budget: requests served shortest first grow in size, and each regrowth
-    // frees the old buffer (hipFree synchronises the device)
+    // a workspace that has to grow again takes headroom without a budget
+    // (three times its old size, at most 1 GiB more): requests served
+    // shortest first grow in size, and each regrowth frees the old buffer
+    // (hipFree synchronises the device)
     const size_t want = bytes;
-    if (b.p && !c->budget) bytes = std::max(bytes, b.cap + b.cap / 2);
+    if (b.p && !c->budget) bytes = std::max(bytes, std::min(3 * b.cap, b.cap + ((size_t)1 << 30)));
     if (b.p) {
       DEV(c, hipStreamSynchronize(L.st));
       DEV(c, hipFree(b.p));
