@@ -331,6 +331,8 @@ struct bem_ctx {
   size_t self_bytes = 0, self_cap = 0;
   std::vector<ArenaSeg> arena;              // self-entry memory without a budget
   size_t arena_bytes = 0;
+  std::vector<std::pair<void *, size_t>> ws_spare;   // workspaces outgrown by their lane (no budget)
+  size_t ws_spare_bytes = 0;
   std::vector<SelfStatus *> spare_status;   // pinned slots and events of evicted entries
   std::vector<hipEvent_t> spare_events;
   uint64_t clock = 0;
@@ -393,25 +395,51 @@ static int fail(bem_ctx *c, int code, const char *fmt, ...) {
 
 // Named workspace of a lane (grow-only).  Growing frees the old buffer after
 // the lane's stream has drained.
+static const size_t WS_SPARE_MAX = (size_t)1 << 30, WS_SPARE_TOTAL = (size_t)8 << 30;
+
 static int ws_get(bem_ctx *c, Lane &L, const char *name, size_t bytes, void **out) {
   DevBuf &b = c->ws[L.tag + name];
   if (b.cap < bytes) {
     // a workspace that has to grow again takes headroom without a budget
     // (three times its old size, at most 1 GiB more): requests served
-    // shortest first grow in size, and each regrowth frees the old buffer
-    // (hipFree synchronises the device)
+    // shortest first grow in size, and a regrowth that frees the old buffer
+    // synchronises the device (hipFree)
     const size_t want = bytes;
     if (b.p && !c->budget) bytes = std::max(bytes, std::min(3 * b.cap, b.cap + ((size_t)1 << 30)));
     if (b.p) {
       DEV(c, hipStreamSynchronize(L.st));
-      DEV(c, hipFree(b.p));
-      c->bytes -= b.cap;
+      // idle now that the lane has drained: without a budget a buffer under
+      // WS_SPARE_MAX is kept for any lane's next workspace instead of
+      // hipFree'd (it stays in c->bytes), up to WS_SPARE_TOTAL in all
+      if (!c->budget && b.cap < WS_SPARE_MAX && c->ws_spare_bytes + b.cap <= WS_SPARE_TOTAL) {
+        c->ws_spare.emplace_back(b.p, b.cap);
+        c->ws_spare_bytes += b.cap;
+      } else {
+        DEV(c, hipFree(b.p));
+        c->bytes -= b.cap;
+      }
       L.ws_bytes -= b.cap;
       b.p = nullptr;
       b.cap = 0;
     }
     if (c->budget && c->bytes + bytes > c->budget)
       return fail(c, BEM_E_NOMEM, "workspace '%s%s' (%zu bytes) exceeds the HBM budget", L.tag.c_str(), name, bytes);
+    if (!c->budget) {   // the smallest outgrown workspace that holds the request
+      size_t best = c->ws_spare.size();
+      for (size_t i = 0; i < c->ws_spare.size(); i++)
+        if (c->ws_spare[i].second >= want &&
+            (best == c->ws_spare.size() || c->ws_spare[i].second < c->ws_spare[best].second))
+          best = i;
+      if (best < c->ws_spare.size()) {
+        b.p = c->ws_spare[best].first;
+        b.cap = c->ws_spare[best].second;
+        c->ws_spare_bytes -= b.cap;
+        c->ws_spare.erase(c->ws_spare.begin() + (long)best);
+        L.ws_bytes += b.cap;
+        *out = b.p;
+        return 0;
+      }
+    }
     hipError_t e = hipMalloc(&b.p, bytes);
     if (e != hipSuccess && bytes > want) {   // no room for the headroom: exactly what is asked
       (void)hipGetLastError();
@@ -3124,6 +3152,7 @@ void bem_ctx_destroy(bem_ctx *c) {
   (void)hipDeviceSynchronize();
   for (auto &kv : c->ws)
     if (kv.second.p) (void)hipFree(kv.second.p);
+  for (auto &w : c->ws_spare) (void)hipFree(w.first);
   for (auto &kv : c->user_allocs) (void)hipFree(kv.first);
   for (auto &g : c->pairs) free_allocs(c, g.allocs, g.bytes);
   for (auto &e : c->selfs) self_free(c, e);
